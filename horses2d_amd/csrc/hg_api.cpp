@@ -1,0 +1,598 @@
+// C ABI of the MI355X DGSEM library (declared in include/horses2d_gpu.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hg_device.h"
+#include "hg_mesh.h"
+#include "horses2d_gpu.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(const std::string &m) {
+    g_err = m;
+    return 1;
+}
+
+#define HG_CHECK(call)                                                                          \
+    do {                                                                                        \
+        hipError_t _e = (call);                                                                 \
+        if (_e != hipSuccess) return set_err(std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct hg_handle {
+    hg::Mesh mesh;
+    int K = 0, N = 0, P = 0;
+    int64_t nodes = 0, ndof = 0;
+    bool mixed = false;
+    hg_options opt{};
+    hg_physics phys{};
+    hg::PhysParams ph{};
+    hipStream_t stream = nullptr;
+    std::vector<DevBuf> bufs;
+    // device arrays
+    double *Q = nullptr, *G = nullptr, *QDot = nullptr;
+    void *tr[2] = {nullptr, nullptr};
+    void *dQ = nullptr, *dtr = nullptr;
+    void *metric = nullptr, *fmet = nullptr;
+    int4 *face = nullptr;
+    int *forder = nullptr;
+    double *egeo = nullptr, *einvh = nullptr, *zone = nullptr;
+    double *hatD = nullptr, *D = nullptr, *lb = nullptr, *lbw = nullptr, *dlbw = nullptr, *w = nullptr;
+    double *jac64 = nullptr;
+    double *dx = nullptr, *umax = nullptr, *amax = nullptr, *dtstate = nullptr, *dtout = nullptr;
+    double *cons_partial = nullptr, *cons_out = nullptr;
+    int cons_nb = 0;
+    unsigned long long *resid = nullptr;
+    int cur = 0;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    int64_t prof_n[2] = {0, 0};
+    double prof_ms[2] = {0.0, 0.0};
+
+    template <typename T>
+    hipError_t alloc(T **p, size_t count) {
+        void *raw = nullptr;
+        hipError_t e = hipMalloc(&raw, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return e;
+        bufs.push_back({raw, count * sizeof(T)});
+        *p = (T *)raw;
+        return hipSuccess;
+    }
+    hipError_t alloc_raw(void **p, size_t bytes) {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+        if (e != hipSuccess) return e;
+        bufs.push_back({*p, bytes});
+        return hipSuccess;
+    }
+};
+
+namespace {
+
+template <typename T>
+hipError_t upload(hg_handle *h, T **dst, const std::vector<T> &src) {
+    hipError_t e = h->alloc(dst, src.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+hipError_t upload_real(hg_handle *h, void **dst, const std::vector<double> &src, bool as_float) {
+    if (!as_float) {
+        double *d;
+        hipError_t e = upload(h, &d, src);
+        *dst = d;
+        return e;
+    }
+    std::vector<float> f(src.begin(), src.end());
+    float *d;
+    hipError_t e = upload(h, &d, f);
+    *dst = d;
+    return e;
+}
+
+hg::StageArgs base_args(hg_handle *h) {
+    hg::StageArgs a{};
+    a.K = h->K;
+    a.nodes = h->nodes;
+    a.Q = h->Q;
+    a.G = h->G;
+    a.QDot = h->QDot;
+    a.dQ = h->dQ;
+    a.dQ_out = h->dQ;
+    a.dtr = h->dtr;
+    a.dtr_out = h->dtr;
+    a.metric = h->metric;
+    a.face = h->face;
+    a.forder = h->forder;
+    a.egeo = h->egeo;
+    a.einvh = h->einvh;
+    a.zone = h->zone;
+    a.fmet = h->fmet;
+    a.hatD = h->hatD;
+    a.D = h->D;
+    a.lb = h->lb;
+    a.lbw = h->lbw;
+    a.dlbw = h->dlbw;
+    a.resid = h->resid;
+    a.ph = h->ph;
+    return a;
+}
+
+hipEvent_t take_event(hg_handle *h) {
+    if (!h->ev_pool.empty()) {
+        hipEvent_t e = h->ev_pool.back();
+        h->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Launch with optional event timing (kernel id 0: stage, 1: gradient).
+template <typename F>
+hipError_t timed(hg_handle *h, int which, F &&launch) {
+    if (!h->prof) return launch();
+    hipEvent_t a = take_event(h), b = take_event(h);
+    if (!a || !b) return hipErrorOutOfMemory;
+    hipEventRecord(a, h->stream);
+    hipError_t e = launch();
+    hipEventRecord(b, h->stream);
+    h->ev_pending.push_back({which, {a, b}});
+    return e;
+}
+
+int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode) {
+    if (h->ph.ns) {
+        hg::StageArgs g = base_args(h);
+        g.tr_in = h->tr[h->cur];
+        HG_CHECK(timed(h, 1, [&] { return hg::launch_gradient(h->P, h->mixed, g, h->stream); }));
+    }
+    hg::StageArgs a = base_args(h);
+    a.tr_in = h->tr[h->cur];
+    a.tr_out = h->tr[h->cur ^ 1];
+    a.am = am;
+    a.cq = cq;
+    a.cg = cg;
+    a.first_stage = first ? 1 : 0;
+    a.mode = mode;
+    a.do_resid = last ? 1 : 0;
+    if (last) HG_CHECK(hg::launch_clear_resid(h->resid, h->stream));
+    HG_CHECK(timed(h, 0, [&] { return hg::launch_stage(h->P, h->mixed, a, h->stream); }));
+    if (mode == 0) h->cur ^= 1;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hg_last_error(char *buf, int32_t n) {
+    if (!buf || n <= 0) return 1;
+    std::snprintf(buf, (size_t)n, "%s", g_err.c_str());
+    return 0;
+}
+
+int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options *opt, hg_handle **out) {
+    if (!mesh || !phys || !opt || !out) return set_err("hg_create: null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err("hg_create: no HIP device available");
+    if (opt->riemann < 0 || opt->riemann > 3) return set_err("hg_create: unknown Riemann solver");
+    if (opt->navier_stokes && (opt->viscous < 1 || opt->viscous > 2))
+        return set_err("hg_create: Navier-Stokes needs viscous = BR1 or IP");
+    if (opt->precision != HG_PRECISION_FP64 && opt->precision != HG_PRECISION_MIXED)
+        return set_err("hg_create: unknown precision");
+    if (opt->scheme < 0 || opt->scheme > 2) return set_err("hg_create: unknown scheme");
+    std::unique_ptr<hg_handle> h(new hg_handle());
+    std::string err = hg::build_mesh(*mesh, h->mesh);
+    if (!err.empty()) return set_err("hg_create: " + err);
+    hg::Mesh &m = h->mesh;
+    h->K = (int)m.el.size();
+    h->N = m.el[0].N;
+    for (auto &e : m.el)
+        if (e.N != h->N) return set_err("hg_create: the GPU path needs one polynomial order per mesh");
+    if (!hg::supported_order(h->N)) return set_err("hg_create: polynomial order not compiled (1..10)");
+    h->P = h->N + 1;
+    const int P = h->P, PP = P * P, K = h->K;
+    h->nodes = (int64_t)K * PP;
+    h->ndof = 4 * h->nodes;
+    h->mixed = opt->precision == HG_PRECISION_MIXED;
+    h->opt = *opt;
+    h->phys = *phys;
+    // PhysicsNS.f90:405-481 (air)
+    hg::PhysParams &ph = h->ph;
+    ph.gamma = 1.4; ph.gm1 = 0.4; ph.cp = 3.5; ph.cv = 2.5; ph.lambda = -2.0 / 3.0;
+    ph.invSGM = 1.0 / (std::sqrt(1.4) * phys->mach);
+    ph.mach_sqrtg = phys->mach * std::sqrt(1.4);
+    ph.mu = phys->reynolds == 0.0 ? 0.0 : 1.0 / phys->reynolds;
+    ph.cpPr = 3.5 / phys->prandtl;
+    ph.sigma0 = opt->sigma0;
+    ph.riemann = opt->riemann;
+    ph.viscous = opt->navier_stokes ? opt->viscous : 0;
+    ph.ns = opt->navier_stokes ? 1 : 0;
+    if (opt->ip_method == HG_IP_SIPG) { ph.ip_eps = -1.0; ph.ip_gradflux = 1; }
+    else if (opt->ip_method == HG_IP_NIPG) { ph.ip_eps = 1.0; ph.ip_gradflux = 1; }
+    else { ph.ip_eps = 0.0; ph.ip_gradflux = 0; }
+    if (ph.viscous != 2) ph.ip_gradflux = 0;
+
+    HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    // ---- host-side tables
+    const int E = (int)m.ed.size();
+    std::vector<int> geo_off(E);
+    std::vector<double> egeo;
+    std::vector<double> einvh(E);
+    for (int k = 0; k < E; ++k) {
+        const hg::Edge &ed = m.ed[k];
+        geo_off[k] = (int)(egeo.size() / 3);
+        int np = ed.curved ? ed.N + 1 : 1;
+        for (int i = 0; i < np; ++i) {
+            egeo.push_back(ed.n[2 * i]);
+            egeo.push_back(ed.n[2 * i + 1]);
+            egeo.push_back(ed.dS[i]);
+        }
+        einvh[k] = ed.invh;
+    }
+    std::vector<int4> face(4 * (size_t)K);
+    std::vector<int> forder(K);
+    std::vector<double> zone(8 * std::max<size_t>(m.zones.size(), 1), 0.0);
+    for (size_t z = 0; z < m.zones.size(); ++z) {
+        const hg::ZoneDesc &Z = m.zones[z];
+        if (Z.type == HG_BC_FARFIELD)
+            for (int c = 0; c < 4; ++c) zone[8 * z + c] = Z.q[c];
+        else { zone[8 * z] = Z.Tw; zone[8 * z + 1] = Z.vw[0]; zone[8 * z + 2] = Z.vw[1]; }
+    }
+    for (int e = 0; e < K; ++e) {
+        const hg::Elem &el = m.el[e];
+        int ord[4] = {0, 1, 2, 3};
+        std::sort(ord, ord + 4, [&](int a, int b) { return el.edges[a] < el.edges[b]; });
+        forder[e] = ord[0] | (ord[1] << 2) | (ord[2] << 4) | (ord[3] << 6);
+        for (int f = 0; f < 4; ++f) {
+            const int k = el.edges[f];
+            const hg::Edge &ed = m.ed[k];
+            const int side = el.qpos[f];
+            int flags = 0, y = 0;
+            if (!ed.bdry) {
+                flags = side == 0 ? hg::FK_INT_L : hg::FK_INT_R;
+                int other = ed.quads[1 - side];
+                y = other * 4 + (ed.loc[1 - side] - 1);
+                if (ed.inverse) flags |= hg::FF_REV;
+                if (side == 1 && ed.inverse) flags |= hg::FF_SELFREV;
+            } else {
+                const hg::ZoneDesc &Z = m.zones[ed.marker - 1];
+                if (Z.type == HG_BC_PERIODIC) {
+                    const hg::Edge &pe = m.ed[ed.partner];
+                    flags = hg::FK_PERIODIC;
+                    y = pe.quads[0] * 4 + (pe.loc[0] - 1);
+                    if (ed.inverse) flags |= hg::FF_REV;
+                } else {
+                    flags = Z.type == HG_BC_FARFIELD ? hg::FK_FARFIELD : hg::FK_WALL;
+                    y = ed.marker - 1;
+                    if (Z.type == HG_BC_VISCOUSWALL && Z.isothermal) flags |= hg::FF_ISOTHERMAL;
+                    if (!ph.ns && Z.type == HG_BC_VISCOUSWALL)
+                        return set_err("hg_create: viscous walls need the Navier-Stokes form");
+                }
+                if (Z.riemann >= 0) flags |= (Z.riemann + 1) << hg::FF_RS_SHIFT;
+            }
+            if (ed.curved) flags |= hg::FF_CURVED;
+            face[4 * (size_t)e + f] = make_int4(flags, y, geo_off[k], k);
+        }
+    }
+    // metric planes and per-face-node metric for the IP gradient face terms
+    std::vector<double> metric(5 * (size_t)h->nodes), jac(h->nodes), dxe(K);
+    std::vector<double> fmet;
+    const hg::Spectral &sp = hg::spectral(h->N);
+    if (ph.ip_gradflux) fmet.assign((size_t)K * 4 * P * 5, 0.0);
+    for (int e = 0; e < K; ++e) {
+        const hg::Elem &el = m.el[e];
+        for (int n = 0; n < PP; ++n) {
+            size_t g = (size_t)e * PP + n;
+            for (int c = 0; c < 4; ++c) metric[c * h->nodes + g] = el.Ja[c * PP + n];
+            metric[4 * h->nodes + g] = el.jac[n];
+            jac[g] = el.jac[n];
+        }
+        double best = 1e300;
+        for (int c = 0; c < 4; ++c) {
+            int a = el.nodes[c], b = el.nodes[(c + 1) % 4];
+            double ddx = m.pts[2 * a] - m.pts[2 * b], ddy = m.pts[2 * a + 1] - m.pts[2 * b + 1];
+            best = std::fmin(best, std::sqrt(ddx * ddx + ddy * ddy));
+        }
+        dxe[e] = best;
+        if (!fmet.empty())
+            for (int f = 0; f < 4; ++f) {
+                const double *lbv = (f == 0 || f == 3) ? sp.lb[0] : sp.lb[1];
+                bool horiz = (f == 0 || f == 2);
+                for (int k = 0; k < P; ++k) {
+                    double v[5] = {0, 0, 0, 0, 0};
+                    for (int l = 0; l < P; ++l) {
+                        int n = horiz ? (k + P * l) : (l + P * k);
+                        v[0] += el.Ja[0 * PP + n] * lbv[l];  // Ja(IX,IX) -> Jxi_x
+                        v[1] += el.Ja[2 * PP + n] * lbv[l];  // Ja(IY,IX) -> Jxi_y
+                        v[2] += el.Ja[1 * PP + n] * lbv[l];  // Ja(IX,IY) -> Jeta_x
+                        v[3] += el.Ja[3 * PP + n] * lbv[l];  // Ja(IY,IY) -> Jeta_y
+                        v[4] += el.jac[n] * lbv[l];
+                    }
+                    for (int c = 0; c < 5; ++c) fmet[(((size_t)e * 4 + f) * P + k) * 5 + c] = v[c];
+                }
+            }
+    }
+    std::vector<double> hat(PP), Dm(PP), lb(2 * P), lbw(2 * P), dlbw(2 * P), wv(P);
+    for (int i = 0; i < P; ++i) {
+        for (int l = 0; l < P; ++l) { hat[i * P + l] = sp.hatD[i][l]; Dm[i * P + l] = sp.D[i][l]; }
+        for (int s = 0; s < 2; ++s) { lb[s * P + i] = sp.lb[s][i]; lbw[s * P + i] = sp.lbw[s][i]; dlbw[s * P + i] = sp.dlbw[s][i]; }
+        wv[i] = sp.w[i];
+    }
+    // ---- device allocations
+    HG_CHECK(h->alloc(&h->Q, h->ndof));
+    HG_CHECK(h->alloc(&h->G, h->ndof));
+    HG_CHECK(h->alloc(&h->QDot, h->ndof));
+    const size_t tsz = (size_t)K * 4 * P * 4 * (h->mixed ? sizeof(float) : sizeof(double));
+    HG_CHECK(h->alloc_raw(&h->tr[0], tsz));
+    HG_CHECK(h->alloc_raw(&h->tr[1], tsz));
+    if (ph.ns) {
+        const size_t rs = h->mixed ? sizeof(float) : sizeof(double);
+        HG_CHECK(h->alloc_raw(&h->dQ, (size_t)h->nodes * 8 * rs));
+        HG_CHECK(h->alloc_raw(&h->dtr, (size_t)K * 4 * P * 8 * rs));
+    }
+    HG_CHECK(upload_real(h.get(), &h->metric, metric, h->mixed));
+    if (!fmet.empty()) HG_CHECK(upload_real(h.get(), &h->fmet, fmet, h->mixed));
+    HG_CHECK(upload(h.get(), &h->face, face));
+    HG_CHECK(upload(h.get(), &h->forder, forder));
+    HG_CHECK(upload(h.get(), &h->egeo, egeo));
+    HG_CHECK(upload(h.get(), &h->einvh, einvh));
+    HG_CHECK(upload(h.get(), &h->zone, zone));
+    HG_CHECK(upload(h.get(), &h->hatD, hat));
+    HG_CHECK(upload(h.get(), &h->D, Dm));
+    HG_CHECK(upload(h.get(), &h->lb, lb));
+    HG_CHECK(upload(h.get(), &h->lbw, lbw));
+    HG_CHECK(upload(h.get(), &h->dlbw, dlbw));
+    HG_CHECK(upload(h.get(), &h->w, wv));
+    HG_CHECK(upload(h.get(), &h->jac64, jac));
+    HG_CHECK(upload(h.get(), &h->dx, dxe));
+    HG_CHECK(h->alloc(&h->umax, K));
+    HG_CHECK(h->alloc(&h->amax, K));
+    HG_CHECK(h->alloc(&h->dtstate, 2));
+    HG_CHECK(h->alloc(&h->dtout, 1));
+    HG_CHECK(hipMemset(h->dtstate, 0, 2 * sizeof(double)));
+    h->cons_nb = (K + std::max(1, 256 / PP) - 1) / std::max(1, 256 / PP);
+    HG_CHECK(h->alloc(&h->cons_partial, 4 * (size_t)h->cons_nb));
+    HG_CHECK(h->alloc(&h->cons_out, 4));
+    HG_CHECK(h->alloc(&h->resid, 4));
+    HG_CHECK(hipMemset(h->resid, 0, 4 * sizeof(unsigned long long)));
+    HG_CHECK(hipMemset(h->Q, 0, h->ndof * sizeof(double)));
+    HG_CHECK(hipMemset(h->G, 0, h->ndof * sizeof(double)));
+    HG_CHECK(hipMemset(h->QDot, 0, h->ndof * sizeof(double)));
+    HG_CHECK(hipDeviceSynchronize());
+    *out = h.release();
+    return 0;
+}
+
+int hg_destroy(hg_handle *h) {
+    if (!h) return 0;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto &b : h->bufs) hipFree(b.p);
+    for (auto &e : h->ev_pool) hipEventDestroy(e);
+    for (auto &p : h->ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int64_t hg_ndof(const hg_handle *h) { return h ? h->ndof : -1; }
+
+int hg_get_node_coords(hg_handle *h, double *X, int64_t nnodes) {
+    if (!h || !X) return set_err("hg_get_node_coords: null argument");
+    if (nnodes != h->nodes) return set_err("hg_get_node_coords: size mismatch");
+    for (const auto &el : h->mesh.el)
+        std::memcpy(X + 2 * el.off, el.X.data(), sizeof(double) * el.X.size());
+    return 0;
+}
+
+int hg_get_connectivity(hg_handle *h, int32_t *edges, int32_t *elems, int32_t *partner) {
+    if (!h) return set_err("hg_get_connectivity: null handle");
+    const hg::Mesh &m = h->mesh;
+    for (size_t k = 0; k < m.ed.size(); ++k) {
+        const hg::Edge &ed = m.ed[k];
+        if (edges) {
+            int32_t *o = edges + 8 * k;
+            o[0] = ed.nodes[0]; o[1] = ed.nodes[1]; o[2] = ed.quads[0]; o[3] = ed.quads[1];
+            o[4] = ed.loc[0]; o[5] = ed.bdry ? 0 : ed.loc[1]; o[6] = ed.inverse; o[7] = ed.marker;
+        }
+        if (partner) partner[k] = ed.partner;
+    }
+    if (elems)
+        for (size_t e = 0; e < m.el.size(); ++e)
+            for (int c = 0; c < 4; ++c) {
+                elems[12 * e + c] = m.el[e].edges[c];
+                elems[12 * e + 4 + c] = m.el[e].dir[c];
+                elems[12 * e + 8 + c] = m.el[e].qpos[c];
+            }
+    return 0;
+}
+
+int hg_num_edges(hg_handle *h) { return h ? (int)h->mesh.ed.size() : -1; }
+
+int hg_build_connectivity(const hg_mesh_desc *mesh, int32_t *nedges, int32_t *edges, int32_t *elems,
+                          int32_t *partner) {
+    if (!mesh || !nedges) return set_err("hg_build_connectivity: null argument");
+    hg::Mesh m;
+    std::string err = hg::build_mesh(*mesh, m);
+    if (!err.empty()) return set_err("hg_build_connectivity: " + err);
+    *nedges = (int32_t)m.ed.size();
+    if (!edges) return 0;
+    for (size_t k = 0; k < m.ed.size(); ++k) {
+        const hg::Edge &ed = m.ed[k];
+        int32_t *o = edges + 8 * k;
+        o[0] = ed.nodes[0]; o[1] = ed.nodes[1]; o[2] = ed.quads[0]; o[3] = ed.quads[1];
+        o[4] = ed.loc[0]; o[5] = ed.bdry ? 0 : ed.loc[1]; o[6] = ed.inverse; o[7] = ed.marker;
+        if (partner) partner[k] = ed.partner;
+    }
+    if (elems)
+        for (size_t e = 0; e < m.el.size(); ++e)
+            for (int c = 0; c < 4; ++c) {
+                elems[12 * e + c] = m.el[e].edges[c];
+                elems[12 * e + 4 + c] = m.el[e].dir[c];
+                elems[12 * e + 8 + c] = m.el[e].qpos[c];
+            }
+    return 0;
+}
+
+int hg_build_geometry(const hg_mesh_desc *mesh, int64_t nnodes, double *X, double *Ja, double *jac) {
+    if (!mesh) return set_err("hg_build_geometry: null argument");
+    hg::Mesh m;
+    std::string err = hg::build_mesh(*mesh, m);
+    if (!err.empty()) return set_err("hg_build_geometry: " + err);
+    if (nnodes != m.nodes_total) return set_err("hg_build_geometry: node count mismatch");
+    for (const auto &el : m.el) {
+        const int PP = el.P * el.P;
+        for (int n = 0; n < PP; ++n) {
+            const int64_t g = el.off + n;
+            if (X) { X[2 * g] = el.X[2 * n]; X[2 * g + 1] = el.X[2 * n + 1]; }
+            if (Ja) for (int c = 0; c < 4; ++c) Ja[4 * g + c] = el.Ja[c * PP + n];
+            if (jac) jac[g] = el.jac[n];
+        }
+    }
+    return 0;
+}
+
+int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
+    if (!h || !Q) return set_err("hg_upload_state: null argument");
+    if (ndof != h->ndof) return set_err("hg_upload_state: ndof mismatch");
+    HG_CHECK(hipMemcpyAsync(h->Q, Q, ndof * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    hg::StageArgs a = base_args(h);
+    h->cur = 0;
+    a.tr_out = h->tr[0];
+    HG_CHECK(hg::launch_prolong(h->P, h->mixed, a, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_download_state(hg_handle *h, double *Q, int64_t ndof) {
+    if (!h || !Q) return set_err("hg_download_state: null argument");
+    if (ndof != h->ndof) return set_err("hg_download_state: ndof mismatch");
+    HG_CHECK(hipMemcpyAsync(Q, h->Q, ndof * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_download_qdot(hg_handle *h, double *QDot, int64_t ndof) {
+    if (!h || !QDot) return set_err("hg_download_qdot: null argument");
+    if (ndof != h->ndof) return set_err("hg_download_qdot: ndof mismatch");
+    HG_CHECK(hipMemcpyAsync(QDot, h->QDot, ndof * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_time_derivative(hg_handle *h, double t) {
+    (void)t;  // no time-dependent boundary data on the GPU path
+    if (!h) return set_err("hg_time_derivative: null handle");
+    return run_stage(h, 0.0, 1.0, 0.0, true, true, 1);
+}
+
+int hg_rk_step(hg_handle *h, double t, double dt) {
+    (void)t;
+    if (!h) return set_err("hg_rk_step: null handle");
+    if (h->opt.scheme == HG_SCHEME_RK3) {
+        // TimeIntegrator_WilliamsonRK3 (DGTimeIntegrator.f90:282-317)
+        const double am[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+        const double gm[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+        for (int s = 0; s < 3; ++s)
+            if (run_stage(h, am[s], 1.0, gm[s] * dt, s == 0, s == 2, 0)) return 1;
+    } else if (h->opt.scheme == HG_SCHEME_RK5) {
+        // TimeIntegrator_WilliamsonRK5 (DGTimeIntegrator.f90:319-365): G = am G + dt QDot, Q += gm G
+        const double am[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+        const double gm[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+        for (int s = 0; s < 5; ++s)
+            if (run_stage(h, am[s], dt, gm[s], s == 0, s == 4, 0)) return 1;
+    } else {
+        // TimeIntegrator_ExplicitEuler (DGTimeIntegrator.f90:263-280)
+        if (run_stage(h, 0.0, 1.0, dt, true, true, 0)) return 1;
+    }
+    return 0;
+}
+
+int hg_residuals(hg_handle *h, double res[4]) {
+    if (!h || !res) return set_err("hg_residuals: null argument");
+    unsigned long long bits[4];
+    HG_CHECK(hipMemcpyAsync(bits, h->resid, sizeof bits, hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < 4; ++c) std::memcpy(&res[c], &bits[c], sizeof(double));
+    return 0;
+}
+
+int hg_estimate_dt(hg_handle *h, double dt_max, double cfl, double *dt) {
+    if (!h || !dt) return set_err("hg_estimate_dt: null argument");
+    HG_CHECK(hg::launch_dt_elem(h->P, h->Q, h->K, h->ph.gamma, h->ph.gm1, h->umax, h->amax, h->stream));
+    const double coef = h->phys.mach * std::sqrt(h->ph.gamma) * cfl;
+    HG_CHECK(hg::launch_dt_scan(h->K, h->umax, h->amax, h->dx, coef, (double)(h->N + 1), dt_max, h->dtstate, h->dtout,
+                                h->stream));
+    HG_CHECK(hipMemcpyAsync(dt, h->dtout, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_reset_dt_state(hg_handle *h) {
+    if (!h) return set_err("hg_reset_dt_state: null handle");
+    HG_CHECK(hipMemsetAsync(h->dtstate, 0, 2 * sizeof(double), h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_conservation_sums(hg_handle *h, double out[4]) {
+    if (!h || !out) return set_err("hg_conservation_sums: null argument");
+    HG_CHECK(hg::launch_cons_sums(h->P, h->Q, h->jac64, h->w, h->K, h->cons_partial, h->cons_nb, h->cons_out, h->stream));
+    HG_CHECK(hipMemcpyAsync(out, h->cons_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_synchronize(hg_handle *h) {
+    if (!h) return set_err("hg_synchronize: null handle");
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_profile_enable(hg_handle *h, int32_t on) {
+    if (!h) return set_err("hg_profile_enable: null handle");
+    h->prof = on != 0;
+    h->prof_n[0] = h->prof_n[1] = 0;
+    h->prof_ms[0] = h->prof_ms[1] = 0.0;
+    return 0;
+}
+
+int hg_profile_read(hg_handle *h, int32_t which, int64_t *launches, double *ms) {
+    if (!h || which < 0 || which > 1) return set_err("hg_profile_read: bad argument");
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    for (auto &p : h->ev_pending) {
+        float t = 0.0f;
+        HG_CHECK(hipEventElapsedTime(&t, p.second.first, p.second.second));
+        h->prof_n[p.first] += 1;
+        h->prof_ms[p.first] += t;
+        h->ev_pool.push_back(p.second.first);
+        h->ev_pool.push_back(p.second.second);
+    }
+    h->ev_pending.clear();
+    if (launches) *launches = h->prof_n[which];
+    if (ms) *ms = h->prof_ms[which];
+    return 0;
+}
+
+}  // extern "C"

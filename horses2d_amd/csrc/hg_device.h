@@ -1,0 +1,74 @@
+// Device-side data model and kernel launchers of the DGSEM stage pipeline.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hg {
+
+// Face-record flag bits (per element face, element-local orientation).
+enum FaceKind : int {
+    FK_INT_L = 0,     // this element is the LEFT (lowest id) side of an interior edge
+    FK_INT_R = 1,     // RIGHT side: Riemann arguments swapped, flux negated
+    FK_PERIODIC = 2,  // periodic boundary edge: partner trace is the outer state
+    FK_FARFIELD = 3,  // characteristic far field (Farfield.incf:138-376)
+    FK_WALL = 4,      // viscous wall (ViscousWall.incf:158-254)
+};
+constexpr int FF_CURVED = 1 << 4;    // per-node normals and dS
+constexpr int FF_REV = 1 << 5;       // neighbor trace index runs backwards
+constexpr int FF_SELFREV = 1 << 6;   // edge geometry index runs backwards (RIGHT side of an inverse edge)
+constexpr int FF_ISOTHERMAL = 1 << 7;
+constexpr int FF_RS_SHIFT = 8;       // (zone Riemann solver + 1), 0 = interior method
+
+struct PhysParams {
+    double gamma, gm1, cp, cv, lambda, invSGM, mu, cpPr, sigma0, ip_eps, mach_sqrtg;
+    int riemann, viscous, ip_gradflux, ns;
+};
+
+// Everything a stage or gradient kernel reads or writes.  Uniform P per handle.
+struct StageArgs {
+    int K;                      // elements
+    int64_t nodes;              // K*P*P
+    const double *Q;            // state (FP64), updated in place by the stage kernel
+    double *G;                  // low-storage RK register (FP64)
+    double *QDot;               // derivative output (derivative-only mode) or nullptr
+    const void *tr_in;          // Q traces [(e*4+f)*P + k][4] (FP64 or FP32)
+    void *tr_out;               // next-stage Q traces
+    const void *dQ;             // gradient [node][2][4] (NS)
+    void *dQ_out;               // gradient output (gradient kernels)
+    const void *dtr;            // gradient traces [(e*4+f)*P + k][2][4]
+    void *dtr_out;
+    const void *metric;         // 5 planes of nodes: Ja11, Ja12, Ja21, Ja22, jac
+    const int4 *face;           // per element face record
+    const int *forder;          // per element: faces sorted by global edge id (2 bits each)
+    const double *egeo;         // edge geometry triplets (nx, ny, dS)
+    const double *einvh;        // per edge 1/h for the IP penalty
+    const double *zone;         // per zone 8 doubles
+    const void *fmet;           // IP: per face node (Jxi_x, Jxi_y, Jeta_x, Jeta_y, jac)
+    const double *hatD, *D, *lb, *lbw, *dlbw;  // spectral operators (P*P, P*P, 2P, 2P, 2P)
+    double am, cq, cg;          // G = am*G + cq*QDot ; Q = Q + cg*G
+    int first_stage;            // G is not read (am = 0)
+    int mode;                   // 0: RK update + next traces, 1: derivative only
+    unsigned long long *resid;  // 4 slots, max|QDot| bit patterns
+    int do_resid;
+    PhysParams ph;
+};
+
+// Launchers (return hipError_t of the launch).
+hipError_t launch_prolong(int P, bool mixed, const StageArgs &a, hipStream_t s);
+hipError_t launch_stage(int P, bool mixed, const StageArgs &a, hipStream_t s);
+hipError_t launch_gradient(int P, bool mixed, const StageArgs &a, hipStream_t s);
+hipError_t launch_clear_resid(unsigned long long *r, hipStream_t s);
+
+// Time-step estimate: per element maxima, then an ordered prefix-max scan.
+hipError_t launch_dt_elem(int P, const double *Q, int K, double gamma, double gm1, double *umax, double *amax,
+                          hipStream_t s);
+hipError_t launch_dt_scan(int K, const double *umax, const double *amax, const double *dx, double coef, double N1,
+                          double dt_max, double *state /* [umax, amax] */, double *dt_out, hipStream_t s);
+// Conservation sums (deterministic two-pass).
+hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const double *w, int K, double *partial,
+                            int nblocks, double *out, hipStream_t s);
+
+bool supported_order(int N);
+
+}  // namespace hg

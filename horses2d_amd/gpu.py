@@ -1,0 +1,306 @@
+"""Host-side mirror of HORSES2D's solver objects over the C ABI of ``libhorses2d_gpu.so``.
+
+``DGSEM`` plays the role of ``DGSEM_t`` (Approximation/DGSEM_class.f90:64-78): it builds
+the mesh from a HiOMesh description, applies the initial condition on the host (the
+``libproblemfile`` hook stays on the host) and owns the device state.  ``integrate``
+restates ``TimeIntegrator_Integrate`` (Approximation/DGTimeIntegrator.f90:159-255):
+final-time clamp, one ``TimeScheme`` call per iteration, residual monitor, and the
+time-step re-estimate every 100 iterations.
+
+The HIP library is required: nothing here falls back to a CPU path.  A missing or
+unloadable library raises ``RuntimeError`` at first use.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import cases as C
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhorses2d_gpu.so")
+
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+class _MeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("nnodes", ctypes.c_int32), ("points", ctypes.c_void_p),
+        ("nelements", ctypes.c_int32), ("quads", ctypes.c_void_p), ("order", ctypes.c_void_p),
+        ("nbedges", ctypes.c_int32), ("bedges", ctypes.c_void_p), ("bmarker", ctypes.c_void_p),
+        ("ncurved", ctypes.c_int32), ("np1", ctypes.c_int32), ("cedges", ctypes.c_void_p), ("cxy", ctypes.c_void_p),
+        ("nmarkers", ctypes.c_int32),
+        ("bc_type", ctypes.c_void_p), ("bc_int", ctypes.c_void_p), ("bc_real", ctypes.c_void_p),
+        ("bc_riemann", ctypes.c_void_p),
+    ]
+
+
+class _Physics(ctypes.Structure):
+    _fields_ = [("mach", ctypes.c_double), ("reynolds", ctypes.c_double), ("prandtl", ctypes.c_double)]
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("navier_stokes", ctypes.c_int32), ("riemann", ctypes.c_int32), ("viscous", ctypes.c_int32),
+                ("ip_method", ctypes.c_int32), ("sigma0", ctypes.c_double), ("precision", ctypes.c_int32),
+                ("scheme", ctypes.c_int32)]
+
+
+class _BatchOptions(ctypes.Structure):
+    _fields_ = [("tile_elems_override", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+
+# every symbol declared in include/horses2d_gpu.h
+EXPORTS = [
+    "hg_create", "hg_destroy", "hg_ndof", "hg_get_node_coords", "hg_get_connectivity", "hg_num_edges",
+    "hg_build_connectivity", "hg_build_geometry",
+    "hg_upload_state", "hg_download_state", "hg_download_qdot", "hg_time_derivative", "hg_rk_step",
+    "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_synchronize",
+    "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_batch_create", "hg_batch_rk_step",
+    "hg_batch_synchronize", "hg_batch_destroy",
+]
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads the HIP library and declares the C signatures (raises if missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library {path} is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(path)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    sig = {
+        "hg_create": ([vp, vp, vp, vp], ctypes.c_int),
+        "hg_destroy": ([vp], ctypes.c_int),
+        "hg_ndof": ([vp], i64),
+        "hg_get_node_coords": ([vp, vp, i64], ctypes.c_int),
+        "hg_get_connectivity": ([vp, vp, vp, vp], ctypes.c_int),
+        "hg_num_edges": ([vp], ctypes.c_int),
+        "hg_build_connectivity": ([vp, vp, vp, vp, vp], ctypes.c_int),
+        "hg_build_geometry": ([vp, i64, vp, vp, vp], ctypes.c_int),
+        "hg_upload_state": ([vp, vp, i64], ctypes.c_int),
+        "hg_download_state": ([vp, vp, i64], ctypes.c_int),
+        "hg_download_qdot": ([vp, vp, i64], ctypes.c_int),
+        "hg_time_derivative": ([vp, dbl], ctypes.c_int),
+        "hg_rk_step": ([vp, dbl, dbl], ctypes.c_int),
+        "hg_residuals": ([vp, vp], ctypes.c_int),
+        "hg_estimate_dt": ([vp, dbl, dbl, vp], ctypes.c_int),
+        "hg_reset_dt_state": ([vp], ctypes.c_int),
+        "hg_conservation_sums": ([vp, vp], ctypes.c_int),
+        "hg_synchronize": ([vp], ctypes.c_int),
+        "hg_last_error": ([vp, i32], ctypes.c_int),
+        "hg_profile_enable": ([vp, i32], ctypes.c_int),
+        "hg_profile_read": ([vp, i32, vp, vp], ctypes.c_int),
+        "hg_batch_create": ([vp, i32, vp, vp], ctypes.c_int),
+        "hg_batch_rk_step": ([vp, vp, vp], ctypes.c_int),
+        "hg_batch_synchronize": ([vp], ctypes.c_int),
+        "hg_batch_destroy": ([vp], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    load_library().hg_last_error(buf, 1024)
+    return buf.value.decode(errors="replace")
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {last_error()}")
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def mesh_desc(case: C.Case, keep: list) -> _MeshDesc:
+    """Fills the ``hg_mesh_desc`` of a case; ``keep`` holds the arrays alive."""
+    m = case.mesh
+
+    def arr(a, dt):
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a
+
+    pts = arr(m.points, np.float64)
+    quads = arr(m.quads, np.int32)
+    order = arr(case.N, np.int32)
+    be = arr(m.bedges, np.int32)
+    bm = arr(m.bmarker, np.int32)
+    md = _MeshDesc()
+    md.nnodes = pts.shape[0]; md.points = _ptr(pts)
+    md.nelements = quads.shape[0]; md.quads = _ptr(quads); md.order = _ptr(order)
+    md.nbedges = be.shape[0]; md.bedges = _ptr(be); md.bmarker = _ptr(bm)
+    if m.cedges is not None and len(m.cedges):
+        ce = arr(m.cedges, np.int32); cx = arr(m.cxy, np.float64)
+        md.ncurved = ce.shape[0]; md.np1 = cx.shape[1]; md.cedges = _ptr(ce); md.cxy = _ptr(cx)
+    zones = m.zones
+    md.nmarkers = len(zones)
+    md.bc_type = _ptr(arr([z.kind for z in zones], np.int32))
+    md.bc_int = _ptr(arr([z.ints() for z in zones], np.int32))
+    md.bc_real = _ptr(arr([z.reals() for z in zones], np.float64))
+    md.bc_riemann = _ptr(arr([z.riemann for z in zones], np.int32))
+    return md
+
+
+def build_connectivity(case: C.Case):
+    """Host-only connectivity of the product's mesh builder (no GPU needed)."""
+    L = load_library()
+    keep = []
+    md = mesh_desc(case, keep)
+    ne = ctypes.c_int32()
+    _check(L.hg_build_connectivity(ctypes.byref(md), ctypes.byref(ne), None, None, None), "hg_build_connectivity")
+    E = ne.value
+    edges = np.zeros((E, 8), np.int32)
+    elems = np.zeros((case.mesh.K, 12), np.int32)
+    partner = np.zeros(E, np.int32)
+    _check(L.hg_build_connectivity(ctypes.byref(md), ctypes.byref(ne), _ptr(edges), _ptr(elems), _ptr(partner)),
+           "hg_build_connectivity")
+    return edges, elems, partner
+
+
+def build_geometry(case: C.Case):
+    """Host-only node coordinates, metric terms and Jacobian of the product's builder."""
+    L = load_library()
+    keep = []
+    md = mesh_desc(case, keep)
+    nodes = case.ndof // 4
+    X = np.zeros((nodes, 2)); Ja = np.zeros((nodes, 4)); jac = np.zeros(nodes)
+    _check(L.hg_build_geometry(ctypes.byref(md), nodes, _ptr(X), _ptr(Ja), _ptr(jac)), "hg_build_geometry")
+    return X, Ja, jac
+
+
+class DGSEM:
+    """One case on the GPU (the HORSES2D ``DGSEM_t`` object, device side)."""
+
+    def __init__(self, case: C.Case, precision: str = "fp64", apply_ic: bool = True):
+        L = load_library()
+        keep = []
+        md = mesh_desc(case, keep)
+
+        ph = _Physics(case.Mach, case.Re, case.Pr)
+        op = _Options(int(case.navier_stokes), C.RIEMANN[case.riemann],
+                      C.VISCOUS[case.viscous] if case.navier_stokes else 0, C.IP_METHOD[case.ip_method],
+                      case.sigma0, PRECISION[precision], C.SCHEME[case.scheme])
+        h = ctypes.c_void_p()
+        _check(L.hg_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h)), "hg_create")
+        self._h = h
+        self._L = L
+        self.case = case
+        self.precision = precision
+        self.ndof = int(L.hg_ndof(h))
+        self.nodes = self.ndof // 4
+        self.t = 0.0
+        self.iter = 0
+        if apply_ic and case.ic is not None:
+            self.upload(case.initial_state(self.coords()))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.hg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- data movement
+    def coords(self) -> np.ndarray:
+        X = np.zeros((self.nodes, 2))
+        _check(self._L.hg_get_node_coords(self._h, _ptr(X), self.nodes), "hg_get_node_coords")
+        return X
+
+    def connectivity(self):
+        E = int(self._L.hg_num_edges(self._h))
+        edges = np.zeros((E, 8), np.int32)
+        elems = np.zeros((self.case.mesh.K, 12), np.int32)
+        partner = np.zeros(E, np.int32)
+        _check(self._L.hg_get_connectivity(self._h, _ptr(edges), _ptr(elems), _ptr(partner)), "hg_get_connectivity")
+        return edges, elems, partner
+
+    def upload(self, Q: np.ndarray):
+        Q = np.ascontiguousarray(Q, np.float64)
+        _check(self._L.hg_upload_state(self._h, _ptr(Q), Q.size), "hg_upload_state")
+
+    def state(self) -> np.ndarray:
+        Q = np.zeros(self.ndof)
+        _check(self._L.hg_download_state(self._h, _ptr(Q), self.ndof), "hg_download_state")
+        return Q
+
+    def qdot(self) -> np.ndarray:
+        Q = np.zeros(self.ndof)
+        _check(self._L.hg_download_qdot(self._h, _ptr(Q), self.ndof), "hg_download_qdot")
+        return Q
+
+    # ---- hot path
+    def time_derivative(self, t: float = 0.0) -> np.ndarray:
+        _check(self._L.hg_time_derivative(self._h, t), "hg_time_derivative")
+        return self.qdot()
+
+    def rk_step(self, t: float, dt: float):
+        _check(self._L.hg_rk_step(self._h, t, dt), "hg_rk_step")
+
+    def residuals(self) -> np.ndarray:
+        r = np.zeros(4)
+        _check(self._L.hg_residuals(self._h, _ptr(r)), "hg_residuals")
+        return r
+
+    def estimate_dt(self, dt_max: float, cfl: float) -> float:
+        out = np.zeros(1)
+        _check(self._L.hg_estimate_dt(self._h, dt_max, cfl, _ptr(out)), "hg_estimate_dt")
+        return float(out[0])
+
+    def reset_dt_state(self):
+        _check(self._L.hg_reset_dt_state(self._h), "hg_reset_dt_state")
+
+    def conservation_sums(self) -> np.ndarray:
+        r = np.zeros(4)
+        _check(self._L.hg_conservation_sums(self._h, _ptr(r)), "hg_conservation_sums")
+        return r
+
+    def synchronize(self):
+        _check(self._L.hg_synchronize(self._h), "hg_synchronize")
+
+    def profile(self, on: bool = True):
+        _check(self._L.hg_profile_enable(self._h, 1 if on else 0), "hg_profile_enable")
+
+    def profile_read(self, which: int = 0):
+        n = ctypes.c_int64(); ms = ctypes.c_double()
+        _check(self._L.hg_profile_read(self._h, which, ctypes.byref(n), ctypes.byref(ms)), "hg_profile_read")
+        return int(n.value), float(ms.value)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def integrate(self, steps: int, dt_max: Optional[float] = None, t_end: float = 1.0e30,
+                  monitor: bool = False):
+        """TimeIntegrator_Integrate (DGTimeIntegrator.f90:159-255), transient mode."""
+        dt_max = self.case.dt_max if dt_max is None else dt_max
+        dt = self.estimate_dt(dt_max, self.case.cfl)
+        history = []
+        for it in range(self.iter + 1, self.iter + steps + 1):
+            if self.t + dt >= t_end:
+                dt = t_end - self.t
+            self.rk_step(self.t, dt)
+            self.t = self.t + dt
+            self.iter = it
+            if monitor:
+                history.append(self.residuals())
+            if it % 100 == 0:
+                dt = self.estimate_dt(dt_max, self.case.cfl)
+        self.dt = dt
+        return np.array(history)

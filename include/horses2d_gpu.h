@@ -1,0 +1,207 @@
+/*
+ * horses2d_gpu.h -- C ABI of the MI355X DGSEM residual + explicit Runge-Kutta library.
+ *
+ * Drop-in boundary for HORSES2D's hot path.  Each entry point names the
+ * reference interface it replaces (paths relative to the HORSES2D source tree).
+ * All functions return 0 on success and a nonzero code on failure; the message
+ * is available through hg_last_error().  No function exits or aborts.
+ *
+ * Data layout contract (same as the reference's flat Storage%Q,
+ * Generic/Storage.f90:54-73, QuadElement_QuadProcedures.incf:22-40):
+ *   element e occupies Q[4*off_e .. 4*off_e + 4*P_e^2), off_e = sum_{k<e} P_k^2,
+ *   Q(eq,i,j) = Q[4*off_e + eq + 4*(i + P_e*j)], eq fastest, then xi, then eta.
+ * All state buffers passed in or out are host memory in that layout (FP64).
+ *
+ * Threading: a handle is used from one host thread; it owns one HIP stream.
+ */
+#ifndef HORSES2D_GPU_H
+#define HORSES2D_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hg_handle hg_handle;
+typedef struct hg_batch hg_batch;
+
+/* Boundary condition kinds (Generic/Defines.h:108-117 numbering). */
+#define HG_BC_PERIODIC 1
+#define HG_BC_VISCOUSWALL 4
+#define HG_BC_FARFIELD 5
+
+/* Riemann solvers: Roe, HLL, exact (Physics/RiemannSolvers.f90:33-399) and the
+ * Lax-Friedrichs flux, which the reference does not have. */
+#define HG_RIEMANN_ROE 0
+#define HG_RIEMANN_HLL 1
+#define HG_RIEMANN_EXACT 2
+#define HG_RIEMANN_LF 3
+
+#define HG_VISCOUS_NONE 0
+#define HG_VISCOUS_BR1 1
+#define HG_VISCOUS_IP 2
+
+#define HG_IP_SIPG 0
+#define HG_IP_NIPG 1
+#define HG_IP_IIPG 2
+
+#define HG_PRECISION_FP64 0
+#define HG_PRECISION_MIXED 1 /* FP64 state/RK/reductions, FP32 fluxes and contractions */
+
+#define HG_SCHEME_EULER 0
+#define HG_SCHEME_RK3 1
+#define HG_SCHEME_RK5 2
+
+/*
+ * Mesh description: the content of a .HiOMesh file plus the boundary-zone
+ * blocks of the .HiOCase file.  Mirrors MeshFile_t (Mesh/MeshFile.f90:31-64)
+ * after ReadMesh (:77-279): coordinates already divided by the Reynolds
+ * length, ids 0-based.  The library rebuilds connectivity with the
+ * reference's numbering (edges in first-appearance order over
+ * (element, face 1..4), LEFT = lowest element id) and the geometry
+ * (QuadMesh_Construct.incf:22-305), so a Fortran caller passes the arrays it
+ * read from the mesh file.
+ */
+typedef struct {
+    int32_t nnodes;
+    const double *points;     /* [nnodes][2] */
+    int32_t nelements;
+    const int32_t *quads;     /* [K][4] counter-clockwise node ids */
+    const int32_t *order;     /* [K] polynomial order (uniform per mesh on the GPU path) */
+    int32_t nbedges;
+    const int32_t *bedges;    /* [nb][2] */
+    const int32_t *bmarker;   /* [nb] 1-based marker */
+    int32_t ncurved;          /* number of curvilinear edges (0 if none) */
+    int32_t np1;              /* points per curvilinear edge */
+    const int32_t *cedges;    /* [nc][2] */
+    const double *cxy;        /* [nc][np1][2] at Chebyshev-Gauss-Lobatto points */
+    int32_t nmarkers;
+    const int32_t *bc_type;   /* [nmarkers] HG_BC_* */
+    const int32_t *bc_int;    /* [nmarkers][2] periodic: (direction 1=x|2=y, partner marker);
+                                 wall: (isothermal 0|1, 0) */
+    const double *bc_real;    /* [nmarkers][4] farfield: q_inf (dimensionless);
+                                 wall: (Tw, vx, vy, 0) */
+    const int32_t *bc_riemann;/* [nmarkers] per-zone Riemann solver or -1 (may be NULL) */
+} hg_mesh_desc;
+
+/* PhysicsNS.f90:405-481 inputs (air: gamma = 1.4). */
+typedef struct {
+    double mach;
+    double reynolds;  /* 0 => inviscid */
+    double prandtl;
+} hg_physics;
+
+typedef struct {
+    int32_t navier_stokes; /* 0: HORSES2D.Euler, 1: HORSES2D.NS (Solver/Makefile:163,180-186) */
+    int32_t riemann;       /* HG_RIEMANN_* ("Inviscid Riemann solver", Setup.f90:280-281) */
+    int32_t viscous;       /* HG_VISCOUS_* ("Viscous discretization", Setup.f90:295-296) */
+    int32_t ip_method;     /* HG_IP_* */
+    double sigma0;         /* "Jumps penalty parameter" */
+    int32_t precision;     /* HG_PRECISION_* */
+    int32_t scheme;        /* HG_SCHEME_* ("Integration scheme", Setup.f90:354-355) */
+} hg_options;
+
+/* Builds connectivity/geometry on the host, uploads it, allocates device state.
+ * Replaces the device-side part of DGSEM_construct (Approximation/DGSEM_class.f90:104-185)
+ * and DGSpatial_Initialization (DGSpatialDiscretizationMethods.f90:67-86). */
+int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options *opt, hg_handle **out);
+
+/* Frees device memory and the stream owned by the handle. */
+int hg_destroy(hg_handle *h);
+
+/* Number of scalar unknowns, size(Storage%Q) (DGSEM_class.f90:339-340). */
+int64_t hg_ndof(const hg_handle *h);
+
+/* Node coordinates X[node][2] (node = off_e + i + P j), the element mapping of
+ * QuadElement_QuadMappings.incf:325-334; used by the host to evaluate the
+ * libproblemfile initial condition (QuadMesh.f90:192-213). */
+int hg_get_node_coords(hg_handle *h, double *X, int64_t nnodes);
+
+/* Connectivity as built (for bit-exact comparison with the Fortran mesh):
+ *   edges[k][8] = {node0, node1, quadLEFT, quadRIGHT, locLEFT, locRIGHT, inverse, marker}
+ *   elems[e][12] = {edges[4], edgesDirection[4], quadPosition[4]}
+ *   partner[k] = periodic partner edge or -1.
+ * (MeshFile.f90:287-618, QuadElement_EdgeProcedures.incf:30-199,357-424,
+ *  QuadMesh_Zones.incf:22-131). */
+int hg_get_connectivity(hg_handle *h, int32_t *edges, int32_t *elems, int32_t *partner);
+
+/* Host-only mesh build (no device needed): connectivity tables as in
+ * hg_get_connectivity.  Call with edges == NULL to query *nedges first.
+ * Lets the caller check the numbering against its own mesh (MeshFile.f90:287-618). */
+int hg_build_connectivity(const hg_mesh_desc *mesh, int32_t *nedges, int32_t *edges, int32_t *elems,
+                          int32_t *partner);
+
+/* Host-only geometry of the built mesh: node coordinates X[node][2], metric
+ * Ja[node][4] = (Ja11, Ja12, Ja21, Ja22) and jac[node]
+ * (QuadElement_QuadMappings.incf:22-367).  Arrays sized by the total node count. */
+int hg_build_geometry(const hg_mesh_desc *mesh, int64_t nnodes, double *X, double *Ja, double *jac);
+
+/* Number of edges of the built mesh (size of the hg_get_connectivity edge table). */
+int hg_num_edges(hg_handle *h);
+
+/* Host <-> device state (Storage%Q layout).  Upload replaces the state after the
+ * initial condition or a restart (DGSEM_class.f90:201-205, 263-291). */
+int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof);
+int hg_download_state(hg_handle *h, double *Q, int64_t ndof);
+int hg_download_qdot(hg_handle *h, double *QDot, int64_t ndof);
+
+/* QDot = residual of the current state: DGSpatial_computeTimeDerivative
+ * (DGSpatialDiscretizationMethods.f90:94-116). */
+int hg_time_derivative(hg_handle *h, double t);
+
+/* One time step of the configured scheme (Williamson RK3 by default):
+ * the TimeScheme procedure pointer (DGTimeIntegrator.f90:71-83) bound to
+ * TimeIntegrator_WilliamsonRK3 (:282-317), RK5 (:319-365) or ExplicitEuler (:263-280). */
+int hg_rk_step(hg_handle *h, double t, double dt);
+
+/* max|QDot| per equation of the last stage (Mesh_ComputeResiduals,
+ * Mesh/QuadMesh_Auxiliars.incf:22-38), as used by Monitors%UpdateValues. */
+int hg_residuals(hg_handle *h, double res[4]);
+
+/* TimeIntegrator_EstimateTimeStep (DGTimeIntegrator.f90:462-511), including the
+ * running maxima umax/amax that the reference keeps across calls. */
+int hg_estimate_dt(hg_handle *h, double dt_max, double cfl, double *dt);
+
+/* Forgets the running maxima kept by hg_estimate_dt, as a fresh process would
+ * (the reference's SAVEd umax/amax start at zero in every run). */
+int hg_reset_dt_state(hg_handle *h);
+
+/* FP64 conservation sums  sum_e sum_ij w_i w_j J Q  (no reference counterpart;
+ * a monitor for the mixed-precision form). */
+int hg_conservation_sums(hg_handle *h, double out[4]);
+
+/* Blocks until the handle's stream is idle. */
+int hg_synchronize(hg_handle *h);
+
+/* Copies the last error message (NUL-terminated, truncated to n bytes). */
+int hg_last_error(char *buf, int32_t n);
+
+/* Kernel-level timing of the handle's stage kernels over the last hg_rk_step
+ * calls recorded with HIP events on the handle's stream: returns the number
+ * of recorded launches and their total milliseconds for kernel id `which`
+ * (0: stage kernel, 1: gradient kernel). Recording is enabled by
+ * hg_profile_enable(h, 1).  No reference counterpart (measurement only). */
+int hg_profile_enable(hg_handle *h, int32_t on);
+int hg_profile_read(hg_handle *h, int32_t which, int64_t *launches, double *ms);
+
+/* ------------------------------------------------------------------------ */
+/* Batched form: many independent cases advanced together, one persistent
+ * launch per RK stage over a (case, element-tile) work queue, captured into a
+ * HIP graph.  No reference counterpart: the reference runs one case per
+ * process; each case keeps its own dt (TimeIntegrator_t%dt). */
+typedef struct {
+    int32_t tile_elems_override; /* 0: autotuned tile size per polynomial order */
+    int32_t use_graph;           /* 1: capture the three stages in a hipGraph */
+} hg_batch_options;
+
+int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_options *opt, hg_batch **out);
+/* One RK step of every case; t[c], dt[c] per case. */
+int hg_batch_rk_step(hg_batch *b, const double *t, const double *dt);
+int hg_batch_synchronize(hg_batch *b);
+int hg_batch_destroy(hg_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HORSES2D_GPU_H */
