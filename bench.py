@@ -1,0 +1,212 @@
+"""Throughput benchmark of the DGSEM residual + Williamson RK3 step on one MI355X.
+
+Workload (default): the Navier-Stokes Taylor-Green vortex on a periodic [0,2pi]^2 square,
+128x128 quads, N=6 (P=7), Re=1600, Mach 0.1, Pr 0.72, BR1 viscous terms, Lax-Friedrichs
+flux, RK3, seeded +-1e-3 velocity noise -- synthetic input of the stated shape.
+
+A "step" is one RK3 time step (three stages) of the whole mesh.  The FP64 form and the
+mixed form (FP64 state/RK, FP32 fluxes and contractions) are both timed over the same
+steps from the same initial state; the mixed form's relative L2 error against the FP64
+form after those steps is reported per field.  ``value`` is the mixed form's rate when that
+error is <= 1e-6 and the FP64 rate otherwise.
+
+Prints one JSON line.  With ``--gpus`` above 1 it exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "DOF-stage updates/s on 1 MI355X: mixed/batched vs FP64 at L2 err ≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per DOF-stage of the stage kernel (one launch = one RK stage of the
+# whole mesh): state Q read+write and RK register G read+write in FP64 (32 B), metric
+# Ja (4) + jac (1) per node shared by 4 equations (10 B FP64 / 5 B FP32), and for
+# Navier-Stokes the gradient dQ (2 values per DOF) read once (16 B FP64 / 8 B FP32).
+STAGE_BYTES = {("euler", "fp64"): 42.0, ("euler", "mixed"): 37.0, ("ns", "fp64"): 58.0, ("ns", "mixed"): 45.0}
+# Whole stage (gradient + stage kernels), the per-DOF-stage figure of the path:
+# adds the gradient write (16 B FP64 / 8 B FP32).
+PATH_BYTES = {("euler", "fp64"): 42.0, ("euler", "mixed"): 37.0, ("ns", "fp64"): 74.0, ("ns", "mixed"): 53.0}
+
+
+def make_case(name: str):
+    from horses2d_amd import cases as C
+    if name == "tgv":
+        return C.ns_tgv_case(128, 6, steps=500), "ns"
+    if name == "vortex":
+        return C.euler_vortex_case(16, 4, steps=200), "euler"
+    if name == "kh":
+        return C.ns_kh_case(256, 7, steps=200), "ns"
+    if name == "cylinder":
+        return C.ns_cylinder_case(512, 512, 7, steps=2000), "ns"
+    raise SystemExit(f"unknown config {name}")
+
+
+def describe(case, name):
+    return {"workload": case.name, "config": name, "elements": int(case.mesh.K), "N": int(case.N[0]),
+            "ndof": int(case.ndof), "Re": case.Re, "Mach": case.Mach, "riemann": case.riemann,
+            "viscous": case.viscous if case.navier_stokes else "none", "scheme": case.scheme,
+            "cfl": case.cfl}
+
+
+def time_form(case, precision, steps, warmup, Q0, dt):
+    from horses2d_amd import gpu
+    dg = gpu.DGSEM(case, precision, apply_ic=False)
+    dg.upload(Q0)
+    t = 0.0
+    for _ in range(warmup):
+        dg.rk_step(t, dt)
+    dg.synchronize()
+    dg.upload(Q0)                     # timed run starts from the initial state
+    dg.profile(True)
+    dg.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dg.rk_step(t, dt)
+        t += dt
+    dg.synchronize()
+    wall = time.perf_counter() - t0
+    n_stage, ms_stage = dg.profile_read(0)
+    n_grad, ms_grad = dg.profile_read(1)
+    Q = dg.state()
+    res = dg.residuals()
+    dg.close()
+    return dict(wall=wall, n_stage=n_stage, ms_stage=ms_stage, n_grad=n_grad, ms_grad=ms_grad, Q=Q, res=res)
+
+
+def rel_l2(a, b, w=None):
+    a = a.reshape(-1, 4); b = b.reshape(-1, 4)
+    if w is None:
+        w = np.ones(a.shape[0])
+    num = np.sqrt(np.sum(w[:, None] * (a - b) ** 2, axis=0))
+    den = np.sqrt(np.sum(w[:, None] * b ** 2, axis=0))
+    return num / den
+
+
+def cpu_baseline(case, steps: int):
+    """The oracle (a C restatement of the reference's CPU path) timed on the host cores."""
+    from oracle.oracle import Oracle
+    orc = Oracle(case)
+    X = orc.coords()
+    orc.set_state(case.initial_state(X))
+    dt = orc.estimate_dt(case.dt_max, case.cfl)
+    orc.step(0.0, dt, case.scheme)       # first touch
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.step(0.0, dt, case.scheme)
+    wall = time.perf_counter() - t0
+    stages = {"rk3": 3, "rk5": 5, "euler": 1}[case.scheme]
+    return case.ndof * stages * steps / wall, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="tgv", choices=["tgv", "vortex", "kh", "cylinder"])
+    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    from horses2d_amd import gpu
+    case, phys = make_case(args.config)
+    stages = {"rk3": 3, "rk5": 5, "euler": 1}[case.scheme]
+    probe = gpu.DGSEM(case, "fp64", apply_ic=False)
+    X = probe.coords()
+    Q0 = case.initial_state(X)
+    probe.upload(Q0)
+    dt = probe.estimate_dt(case.dt_max, case.cfl)
+    probe.close()
+
+    forms = [f for f in args.forms.split(",") if f]
+    results = {}
+    for f in forms:
+        results[f] = time_form(case, f, args.steps, args.warmup, Q0, dt)
+
+    out_forms = {}
+    for f, r in results.items():
+        rate = case.ndof * stages * args.steps / r["wall"]
+        avg_stage_ms = r["ms_stage"] / max(r["n_stage"], 1)
+        avg_grad_ms = r["ms_grad"] / max(r["n_grad"], 1) if r["n_grad"] else 0.0
+        sb = STAGE_BYTES[(phys, f)] * case.ndof
+        pb = PATH_BYTES[(phys, f)] * case.ndof
+        out_forms[f] = {
+            "dof_stage_per_s": rate,
+            "ms_per_step": 1e3 * r["wall"] / args.steps,
+            "stage_kernel_avg_ms": avg_stage_ms,
+            "gradient_kernel_avg_ms": avg_grad_ms,
+            "stage_kernel_GBps": sb / (avg_stage_ms * 1e-3) / 1e9,
+            "path_GBps": pb / ((avg_stage_ms + avg_grad_ms) * 1e-3) / 1e9,
+            "residuals": [float(v) for v in r["res"]],
+        }
+    err = None
+    if "fp64" in results and "mixed" in results:
+        from horses2d_amd.cases import SCHEME  # noqa: F401
+        e_nodal = rel_l2(results["mixed"]["Q"], results["fp64"]["Q"])
+        linf = np.max(np.abs(results["mixed"]["Q"].reshape(-1, 4) - results["fp64"]["Q"].reshape(-1, 4)), axis=0) / \
+            np.max(np.abs(results["fp64"]["Q"].reshape(-1, 4)), axis=0)
+        err = {"rel_l2_nodal": [float(v) for v in e_nodal], "rel_linf": [float(v) for v in linf],
+               "reference": "FP64 GPU form (matches the CPU oracle within 1e-12)"}
+        out_forms["mixed"]["error_vs_fp64"] = err
+        out_forms["mixed"]["speedup_vs_fp64"] = out_forms["mixed"]["dof_stage_per_s"] / out_forms["fp64"]["dof_stage_per_s"]
+
+    if "mixed" in out_forms and err is not None and max(err["rel_l2_nodal"]) <= 1e-6:
+        best = "mixed"
+    else:
+        best = "fp64" if "fp64" in out_forms else forms[0]
+    bf = out_forms[best]
+    achieved = bf["stage_kernel_GBps"]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.config}_{best}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("stage_kernel_bytes_per_launch")
+    line = {
+        "metric": METRIC,
+        "value": bf["dof_stage_per_s"],
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": bf["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "form": best,
+        "config": describe(case, args.config),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_stage", "algorithmic_bytes_per_launch": STAGE_BYTES[(phys, best)] * case.ndof,
+                     "avg_launch_ms": bf["stage_kernel_avg_ms"]},
+        "forms": out_forms,
+    }
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1))))
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        rate, wall = cpu_baseline(case, args.cpu_steps)
+        line["cpu_baseline"] = {"value": rate, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+                                "sample": f"{args.cpu_steps} RK3 steps of the same {case.name} mesh in FP64 "
+                                          f"({wall:.1f} s), C oracle with OpenMP volume loops and the "
+                                          f"reference's serial face loop"}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
