@@ -415,6 +415,8 @@ __global__ __launch_bounds__(kThreads) void k_gradient(StageArgs a) {
     const bool br1 = a.ph.viscous == 1;
     if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; }
     for (int x = t; x < PP; x += kThreads) sHat[x] = (R)a.hatD[x];
+    if (!br1)
+        for (int x = t; x < PP; x += kThreads) sD[x] = (R)a.D[x];
     const int ne = min(EPB, a.K - e0);
     const int64_t nb0 = (int64_t)e0 * PP;
     for (int x = t; x < ne * PP * 4; x += kThreads) sQ[x] = a.Q[nb0 * 4 + x];
@@ -423,8 +425,6 @@ __global__ __launch_bounds__(kThreads) void k_gradient(StageArgs a) {
         sJa[(s * PP + n) * 4 + c] = ld<R>(a.metric, c * a.nodes + nb0 + s * PP + n);
     }
     __syncthreads();
-    if (!br1)
-        for (int x = t; x < PP; x += kThreads) sD[x] = (R)a.D[x];
     // face phase (BR1 only): solution Riemann flux uStar * dS * n
     if (br1) {
         for (int w = t; w < C::NF; w += kThreads) {
