@@ -1,0 +1,126 @@
+"""CPU tests of the product's host side: the library loads and exports the C ABI, and its
+O(E) mesh builder reproduces the oracle's brute-force restatement of the reference
+numbering (MeshFile.f90:287-618, QuadElement_EdgeProcedures.incf:30-424,
+QuadMesh_Zones.incf:22-131) bit-exactly, on generated meshes and on the reference's own
+HiOMesh files (fixtures under tests/golden)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from horses2d_amd import cases as C
+from horses2d_amd import gpu
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "horses2d_gpu.h")).read()
+    return sorted(set(re.findall(r"\b(hg_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(gpu_lib):
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(gpu_lib, s), s
+    assert sorted(gpu.EXPORTS) == syms
+
+
+def test_product_refuses_to_run_without_the_library(tmp_path, monkeypatch):
+    monkeypatch.setattr(gpu, "_lib", None)
+    with pytest.raises(RuntimeError):
+        gpu.load_library(str(tmp_path / "missing.so"))
+
+
+def reference_mesh_case(name, N=3):
+    d = np.load(os.path.join(GOLDEN, f"mesh_{name}.npz"), allow_pickle=False)
+    nm = len(d["marker_names"])
+    q = C.freestream_state(0.3, 0.1)
+    zones = [C.Zone(C.BC_FARFIELD, q=tuple(q)) for _ in range(nm)]
+    mesh = C.HiOMesh(d["points"], d["quads"], d["bedges"], d["bmarker"], zones,
+                     d["cedges"] if "cedges" in d else None, d["cxy"] if "cxy" in d else None)
+    return C.Case(f"ref_{name}", mesh, np.full(mesh.K, N, np.int32), 0.3, ic=C.uniform(q))
+
+
+def periodic_reference_vortex_mesh():
+    d = np.load(os.path.join(GOLDEN, "mesh_vortex_transport_16x16.npz"), allow_pickle=False)
+    # TestCases/Cases/VortexTransport/SETUP/VortexTransport.HiOCase:58-84 zones
+    zones = [C.Zone(C.BC_PERIODIC, direction=2, partner=2), C.Zone(C.BC_PERIODIC, direction=2, partner=1),
+             C.Zone(C.BC_PERIODIC, direction=1, partner=4), C.Zone(C.BC_PERIODIC, direction=1, partner=3)]
+    mesh = C.HiOMesh(d["points"] / 0.1, d["quads"], d["bedges"], d["bmarker"], zones)   # Reynolds length 0.1
+    return C.Case("ref_vortex_transport", mesh, np.full(mesh.K, 4, np.int32), 0.5,
+                  ic=C.isentropic_vortex(0.5, 1.0, 0.5, 0.5))
+
+
+CASES = {
+    "periodic_square": lambda: C.euler_vortex_case(7, 3),
+    "tgv": lambda: C.ns_tgv_case(5, 4),
+    "annulus": lambda: C.ns_cylinder_case(3, 10, N=4),
+    "ref_vortex_transport": periodic_reference_vortex_mesh,
+    "ref_cylinder_very_coarse": lambda: reference_mesh_case("cylinder_very_coarse"),
+    "ref_kelvin_helmholtz": lambda: reference_mesh_case("kelvin_helmholtz"),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_connectivity_and_geometry_match_the_oracle(name, gpu_lib):
+    case = CASES[name]()
+    orc = O.Oracle(case)
+    e1, el1, p1 = orc.connectivity()
+    e2, el2, p2 = gpu.build_connectivity(case)
+    np.testing.assert_array_equal(e1, e2)
+    np.testing.assert_array_equal(el1, el2)
+    np.testing.assert_array_equal(p1, p2)
+    X1 = orc.coords(); Ja1, jac1 = orc.metrics()
+    X2, Ja2, jac2 = gpu.build_geometry(case)
+    np.testing.assert_array_equal(X1, X2)
+    np.testing.assert_array_equal(Ja1, Ja2)
+    np.testing.assert_array_equal(jac1, jac2)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mesh_invariants(name, gpu_lib):
+    """CheckMesh invariants (Generic/Checks.f90:109-306): positive Jacobian, each element
+    owns four distinct edges, LEFT is the lower element id, every interior edge is seen
+    from both sides."""
+    case = CASES[name]()
+    edges, elems, partner = gpu.build_connectivity(case)
+    _, _, jac = gpu.build_geometry(case)
+    assert np.all(jac > 0)
+    for e in range(case.mesh.K):
+        assert len(set(elems[e, :4])) == 4
+    interior = edges[:, 3] >= 0
+    assert np.all(edges[interior, 2] < edges[interior, 3])
+    counts = np.bincount(elems[:, :4].ravel(), minlength=edges.shape[0])
+    np.testing.assert_array_equal(counts, np.where(interior, 2, 1))
+    per = partner >= 0
+    np.testing.assert_array_equal(partner[partner[per]], np.nonzero(per)[0])
+
+
+def test_reference_vortex_mesh_has_the_generated_topology(gpu_lib):
+    """The generator follows the reference mesh convention: the shipped 16x16 periodic mesh and
+    the generated one have identical connectivity tables."""
+    ref = periodic_reference_vortex_mesh()
+    gen = C.euler_vortex_case(16, 4)
+    a = gpu.build_connectivity(ref)
+    b = gpu.build_connectivity(gen)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_free_stream_on_reference_meshes():
+    """Uniform far-field state on the reference's straight-sided KH mesh: QDot vanishes.
+    The shipped cylinder mesh's curved-edge end points sit ~1e-6 away from its corner nodes
+    (the file's own data), so there the residual is only as small as that gap allows."""
+    case = reference_mesh_case("kelvin_helmholtz", N=5)
+    orc = O.Oracle(case)
+    orc.set_state(case.initial_state(orc.coords()))
+    assert np.max(np.abs(orc.time_derivative())) < 1e-11
+    case = reference_mesh_case("cylinder_very_coarse", N=5)
+    orc = O.Oracle(case)
+    orc.set_state(case.initial_state(orc.coords()))
+    assert np.max(np.abs(orc.time_derivative())) < 1e-3

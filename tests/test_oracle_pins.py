@@ -1,0 +1,303 @@
+"""Pins of the CPU oracle against checks that do not restate its formulas.
+
+Each test names the reference function it pins.  The reference's own end-to-end goldens
+(TestCases/Benchmark/*/SETUP/ProblemFile.f90 ``Finalize``) need RK5 with p-non-conforming and
+subdivided mortars, which are not on this path; see the oracle module docstring.
+"""
+import numpy as np
+import pytest
+
+from horses2d_amd import cases as C
+from oracle import oracle as O
+
+GAMMA = 1.4
+
+
+# ------------------------------------------------------------------------------------------
+# Spectral operators: NodesAndWeightsClass.f90:251-332, LegendreAlgorithms.f90:106-182
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("N", [1, 2, 3, 4, 5, 6, 7, 8, 9])
+def test_gauss_nodes_weights_match_numpy(N):
+    s = O.spectral(N)
+    x, w = np.polynomial.legendre.leggauss(N + 1)
+    np.testing.assert_allclose(s["xi"], 0.5 + 0.5 * x, atol=4e-16)
+    np.testing.assert_allclose(s["w"], 0.5 * w, atol=4e-16)
+
+
+@pytest.mark.parametrize("N", [2, 4, 6, 8])
+def test_derivative_and_boundary_operators_are_exact_on_polynomials(N):
+    s = O.spectral(N)
+    x = s["xi"]
+    for k in range(N + 1):           # D differentiates degree <= N exactly
+        np.testing.assert_allclose(s["D"] @ x ** k, k * x ** max(k - 1, 0) * (k > 0), atol=1e-11)
+        np.testing.assert_allclose(s["lb"][0] @ x ** k, 0.0 ** k, atol=1e-12)   # value at 0
+        np.testing.assert_allclose(s["lb"][1] @ x ** k, 1.0, atol=1e-12)        # value at 1
+        # dlb = D^T lb: dlb(side) . f = (D f) evaluated at the boundary
+        np.testing.assert_allclose(s["dlb"][1] @ x ** k, k * 1.0, atol=1e-10)
+    # hatD = M^-1 (M D)^T: weak derivative, (M hatD)_il = (M D)_li
+    M = np.diag(s["w"])
+    np.testing.assert_allclose(M @ s["hatD"], (M @ s["D"]).T, atol=1e-13)
+    np.testing.assert_allclose(s["lbw"], s["lb"] / s["w"], rtol=1e-15)
+    np.testing.assert_allclose(s["dlbw"], s["dlb"] / s["w"], rtol=1e-15)
+    # Gauss quadrature integrates degree 2N+1 exactly on [0,1]
+    np.testing.assert_allclose(s["w"] @ x ** (2 * N + 1), 1.0 / (2 * N + 2), rtol=1e-13)
+
+
+# ------------------------------------------------------------------------------------------
+# Point fluxes: RiemannSolvers.f90:33-399, InviscidFluxes.f90:104-147, ViscousFluxes.f90:39-80
+# ------------------------------------------------------------------------------------------
+def prim_to_cons(rho, u, v, p):
+    return np.stack([rho, rho * u, rho * v, p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v)], axis=-1)
+
+
+def random_states(n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rho = rng.uniform(0.5, 2.0, n); p = rng.uniform(0.5, 2.0, n)
+    u = rng.uniform(-1.5, 1.5, n); v = rng.uniform(-1.5, 1.5, n)
+    return prim_to_cons(rho, u, v, p)
+
+
+def physical_normal_flux(q, nrm, Mach):
+    """Independent numpy evaluation of (f n_x + g n_y) / (sqrt(gamma) M) from primitives."""
+    rho = q[:, 0]; u = q[:, 1] / rho; v = q[:, 2] / rho
+    p = (GAMMA - 1) * (q[:, 3] - 0.5 * rho * (u * u + v * v))
+    un = u * nrm[:, 0] + v * nrm[:, 1]
+    F = np.stack([rho * un, rho * u * un + p * nrm[:, 0], rho * v * un + p * nrm[:, 1], (q[:, 3] + p) * un], -1)
+    return F / (np.sqrt(GAMMA) * Mach)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+def test_riemann_consistency(solver):
+    q = random_states(64, 1)
+    th = np.random.Generator(np.random.PCG64(2)).uniform(0, 2 * np.pi, 64)
+    nrm = np.stack([np.cos(th), np.sin(th)], -1)
+    F = O.riemann(solver, 0.3, q, q, nrm)
+    np.testing.assert_allclose(F, physical_normal_flux(q, nrm, 0.3), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2, 3])
+def test_riemann_rotational_invariance(solver):
+    qL = random_states(32, 3); qR = random_states(32, 4)
+    th = np.random.Generator(np.random.PCG64(5)).uniform(0, 2 * np.pi, 32)
+    nrm = np.stack([np.cos(th), np.sin(th)], -1)
+    F = O.riemann(solver, 0.5, qL, qR, nrm)
+    # rotate states so that the normal becomes +x, solve, rotate the flux back
+    def rot(q, c, s):
+        return np.stack([q[:, 0], c * q[:, 1] + s * q[:, 2], -s * q[:, 1] + c * q[:, 2], q[:, 3]], -1)
+    c, s = nrm[:, 0], nrm[:, 1]
+    Fx = O.riemann(solver, 0.5, rot(qL, c, s), rot(qR, c, s), np.tile([1.0, 0.0], (32, 1)))
+    Fb = np.stack([Fx[:, 0], c * Fx[:, 1] - s * Fx[:, 2], s * Fx[:, 1] + c * Fx[:, 2], Fx[:, 3]], -1)
+    np.testing.assert_allclose(F, Fb, rtol=1e-11, atol=1e-11)
+
+
+@pytest.mark.parametrize("solver", [0, 1, 2])
+def test_supersonic_upwinding(solver):
+    """Supersonic flow along +n: the interface flux is the left state's physical flux."""
+    qL = prim_to_cons(np.array([1.0]), np.array([3.0]), np.array([0.2]), np.array([1.0]))
+    qR = prim_to_cons(np.array([0.7]), np.array([2.9]), np.array([-0.1]), np.array([0.8]))
+    n = np.array([[1.0, 0.0]])
+    F = O.riemann(solver, 1.0, qL, qR, n)
+    np.testing.assert_allclose(F, physical_normal_flux(qL, n, 1.0), rtol=1e-12)
+
+
+def toro_exact_interface_flux(WL, WR):
+    """Textbook exact Riemann solver (Toro, ch. 4): sample x/t = 0 and return the flux."""
+    g = GAMMA
+    def fK(p, rho, pk):
+        a = np.sqrt(g * pk / rho)
+        if p > pk:
+            A = 2 / ((g + 1) * rho); B = (g - 1) / (g + 1) * pk
+            return (p - pk) * np.sqrt(A / (p + B))
+        return 2 * a / (g - 1) * ((p / pk) ** ((g - 1) / (2 * g)) - 1)
+    (rL, uL, pL), (rR, uR, pR) = WL, WR
+    lo, hi = 1e-10, 100.0
+    for _ in range(200):                      # bisection on f(p) = fL + fR + du
+        mid = 0.5 * (lo + hi)
+        if fK(mid, rL, pL) + fK(mid, rR, pR) + uR - uL > 0:
+            hi = mid
+        else:
+            lo = mid
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (uL + uR) + 0.5 * (fK(ps, rR, pR) - fK(ps, rL, pL))
+    aL = np.sqrt(g * pL / rL)
+    assert us >= 0 and ps <= pL               # the cases below sample the left rarefaction
+    aLs = aL * (ps / pL) ** ((g - 1) / (2 * g))
+    if uL - aL >= 0:                           # head moves right: left state
+        r, u, p = rL, uL, pL
+    elif us - aLs < 0:                         # tail moves left: left star state
+        r, u, p = rL * (ps / pL) ** (1 / g), us, ps
+    else:                                      # x/t = 0 inside the fan (sonic rarefaction)
+        r = rL * (2 / (g + 1) + (g - 1) / ((g + 1) * aL) * uL) ** (2 / (g - 1))
+        u = 2 / (g + 1) * (aL + (g - 1) / 2 * uL)
+        p = pL * (2 / (g + 1) + (g - 1) / ((g + 1) * aL) * uL) ** (2 * g / (g - 1))
+    E = p / (g - 1) + 0.5 * r * u * u
+    return np.array([r * u, r * u * u + p, 0.0, u * (E + p)]), ps, us
+
+
+@pytest.mark.parametrize("WL,WR,pstar", [
+    ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), 0.30313),        # Toro test 1 (Sod): star region
+    ((1.0, 0.75, 1.0), (0.125, 0.0, 0.1), None),          # modified Sod: sonic rarefaction fan
+    ((1.0, 1.2, 1.0), (0.3, 0.5, 0.4), None),
+])
+def test_exact_solver_textbook(WL, WR, pstar):
+    Fref, ps, _ = toro_exact_interface_flux(WL, WR)
+    if pstar is not None:
+        assert ps == pytest.approx(pstar, abs=2e-5)
+    qL = prim_to_cons(*[np.array([c]) for c in (WL[0], WL[1], 0.0, WL[2])])
+    qR = prim_to_cons(*[np.array([c]) for c in (WR[0], WR[1], 0.0, WR[2])])
+    F = O.riemann(2, 1.0 / np.sqrt(GAMMA), qL, qR, np.array([[1.0, 0.0]]))   # invSGM = 1
+    np.testing.assert_allclose(F[0], Fref, rtol=2e-5, atol=2e-6)  # Newton stops at 1e-6 relative
+
+
+def test_lax_friedrichs_properties():
+    """LF is not in the reference (parity unpinned): consistency (above), exact antisymmetry,
+    and non-negative entropy-free dissipation of the density jump."""
+    qL = random_states(50, 7); qR = random_states(50, 8)
+    th = np.random.Generator(np.random.PCG64(9)).uniform(0, 2 * np.pi, 50)
+    n = np.stack([np.cos(th), np.sin(th)], -1)
+    F1 = O.riemann(3, 0.4, qL, qR, n)
+    F2 = O.riemann(3, 0.4, qR, qL, -n)
+    np.testing.assert_array_equal(F1, -F2)
+    central = 0.5 * (physical_normal_flux(qL, n, 0.4) + physical_normal_flux(qR, n, 0.4))
+    diss = (central - F1)[:, 0] * (qR[:, 0] - qL[:, 0])
+    assert np.all(diss >= -1e-14)
+
+
+def test_viscous_flux_against_primitive_gradients():
+    """Navier-Stokes stresses and heat flux built independently from primitive gradients."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    n = 20
+    rho, u, v, p = rng.uniform(0.8, 1.2, n), rng.uniform(-1, 1, n), rng.uniform(-1, 1, n), rng.uniform(0.8, 1.2, n)
+    q = prim_to_cons(rho, u, v, p)
+    # gradients of primitives, then of conservatives by the chain rule
+    gr = rng.normal(size=(n, 4, 2))                   # d(rho,u,v,T)/d(x,y)
+    T = p / rho
+    dq = np.zeros((n, 2, 4))
+    for d in range(2):
+        drho, du, dv, dT = gr[:, 0, d], gr[:, 1, d], gr[:, 2, d], gr[:, 3, d]
+        dp = drho * T + rho * dT
+        dq[:, d, 0] = drho
+        dq[:, d, 1] = drho * u + rho * du
+        dq[:, d, 2] = drho * v + rho * dv
+        dq[:, d, 3] = dp / (GAMMA - 1) + 0.5 * drho * (u * u + v * v) + rho * (u * du + v * dv)
+    Re, Pr = 100.0, 0.72
+    f, g = O.viscous_flux(Re, Pr, q, dq.reshape(n, 8))
+    ux, uy, vx, vy = gr[:, 1, 0], gr[:, 1, 1], gr[:, 2, 0], gr[:, 2, 1]
+    div = ux + vy
+    txx = 2 * ux - 2.0 / 3.0 * div; tyy = 2 * vy - 2.0 / 3.0 * div; txy = uy + vx
+    k = GAMMA / (GAMMA - 1) / Pr                      # cp/Pr in the reference's dimensionless units
+    fx = np.stack([0 * u, txx, txy, u * txx + v * txy + k * gr[:, 3, 0]], -1)
+    gy = np.stack([0 * u, txy, tyy, u * txy + v * tyy + k * gr[:, 3, 1]], -1)
+    np.testing.assert_allclose(f, fx, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(g, gy, rtol=1e-11, atol=1e-11)
+
+
+# ------------------------------------------------------------------------------------------
+# Whole residual: DGSpatialDiscretizationMethods.f90:94-235
+# ------------------------------------------------------------------------------------------
+def oracle_for(case):
+    o = O.Oracle(case)
+    X = o.coords()
+    o.set_state(case.initial_state(X))
+    return o, X
+
+
+def test_free_stream_preservation_periodic_euler_and_ns():
+    for ns in (False, True):
+        case = C.ns_tgv_case(5, 5) if ns else C.euler_vortex_case(5, 5)
+        case.ic = C.uniform(C.freestream_state(0.4, 0.7))
+        o, _ = oracle_for(case)
+        assert np.max(np.abs(o.time_derivative())) < 1e-12
+
+
+def test_free_stream_preservation_curved_annulus():
+    """Curved (quadratic) O-grid with far-field zones on both boundaries: QDot of a uniform
+    state vanishes (metric identities of the transfinite map, QuadElement_QuadMappings.incf)."""
+    q = C.freestream_state(0.3, 0.2)
+    mesh = C.annulus(4, 12, 0.5, 3.0, q)
+    mesh.zones[0] = C.Zone(C.BC_FARFIELD, q=tuple(q))
+    case = C.Case("annulus_fs", mesh, np.full(mesh.K, 5, np.int32), 0.3, ic=C.uniform(q))
+    o, _ = oracle_for(case)
+    assert np.max(np.abs(o.time_derivative())) < 1e-11
+
+
+def test_discrete_conservation_periodic():
+    for case in (C.euler_vortex_case(6, 4), C.ns_tgv_case(4, 4), C.ns_kh_case(4, 4)):
+        o, _ = oracle_for(case)
+        qd = o.time_derivative().reshape(-1, 4)
+        Ja, jac = o.metrics()
+        s = O.spectral(4)
+        w = np.outer(s["w"], s["w"]).T.reshape(-1)          # node i + P j -> w_i w_j
+        W = np.tile(w, case.mesh.K) * jac
+        total = W @ qd
+        scale = W @ np.abs(qd)
+        assert np.all(np.abs(total) <= 1e-11 * scale), (case.name, total, scale)
+
+
+def test_qdot_converges_spectrally_to_the_vortex_translation():
+    """The isentropic vortex translates rigidly: dQ/dt = -dQ/dx away from the periodic seam."""
+    errs = []
+    for N in (3, 5, 7, 9):
+        case = C.euler_vortex_case(16, N)
+        o, X = oracle_for(case)
+        qd = o.time_derivative().reshape(-1, 4)
+        h = 1e-5
+        ex = (case.ic(X[:, 0] - h, X[:, 1]) - case.ic(X[:, 0] + h, X[:, 1])) / (2 * h)
+        inner = (np.abs(X[:, 0]) < 3.5) & (np.abs(X[:, 1]) < 3.5)
+        errs.append(np.abs(qd - ex)[inner].max())
+    assert errs[-1] < 5e-6
+    assert all(b < a / 8 for a, b in zip(errs, errs[1:])), errs
+
+
+def test_br1_gradient_is_exact_for_linear_fields():
+    """BR1 lifting (DGViscous_BR1.f90:48-111) reproduces the exact gradient of a continuous
+    linear field in every element not touching the periodic seam."""
+    case = C.ns_tgv_case(6, 4)
+    a = np.array([[1.0, 0.01, -0.02], [0.1, 0.02, 0.03], [0.05, -0.01, 0.02], [2.5, 0.03, 0.01]])
+    case.ic = lambda x, y: np.stack([a[k, 0] + a[k, 1] * x + a[k, 2] * y for k in range(4)], -1)
+    o, X = oracle_for(case)
+    o.time_derivative()
+    g = o.grad()
+    nodes = X.shape[0]
+    PP = 25
+    dq = g.reshape(case.mesh.K, 2, PP, 4)                 # [e][dim][node][eq]
+    Xe = X.reshape(case.mesh.K, PP, 2)
+    L = 2 * np.pi
+    h = L / 6
+    inner = np.all((Xe[:, :, 0] > h) & (Xe[:, :, 0] < L - h) & (Xe[:, :, 1] > h) & (Xe[:, :, 1] < L - h), axis=1)
+    assert inner.sum() > 0
+    for k in range(4):
+        np.testing.assert_allclose(dq[inner, 0, :, k], a[k, 1], atol=1e-12)
+        np.testing.assert_allclose(dq[inner, 1, :, k], a[k, 2], atol=1e-12)
+    assert nodes == case.ndof // 4
+
+
+def test_rk3_is_third_order_in_time():
+    """Williamson RK3 (DGTimeIntegrator.f90:282-317): self-convergence ratio ~8 under dt halving."""
+    case = C.euler_vortex_case(8, 6)
+    sols = []
+    for k in range(3):
+        o, _ = oracle_for(case)
+        dt = 0.004 / 2 ** k
+        for _ in range(8 * 2 ** k):
+            o.step(0.0, dt, "rk3")
+        sols.append(o.state())
+    r = np.max(np.abs(sols[0] - sols[1])) / np.max(np.abs(sols[1] - sols[2]))
+    assert 6.5 < r < 11.0, r           # first/second order would give ~2/~4
+
+
+def test_time_step_estimate_closed_form_and_saved_maxima():
+    """EstimateTimeStep (DGTimeIntegrator.f90:462-511) on a uniform state:
+    dt = M sqrt(gamma) CFL dx / (|u| + a) / (N+1); the maxima persist across calls."""
+    M, N, n = 0.4, 4, 5
+    q = C.freestream_state(M, 0.3)
+    case = C.euler_vortex_case(n, N, Mach=M)
+    case.ic = C.uniform(q)
+    o, _ = oracle_for(case)
+    dt = o.estimate_dt(1e30, 0.3)
+    a = np.sqrt(GAMMA)
+    dx = 10.0 / n
+    assert dt == pytest.approx(M * np.sqrt(GAMMA) * 0.3 * dx / (M * np.sqrt(GAMMA) + a) / (N + 1), rel=1e-14)
+    # a slower, cooler state afterwards does not raise dt: the running maxima are kept
+    o.set_state(case.initial_state(o.coords()) * np.array([1.0, 0.5, 0.5, 0.5] * (o.ndof // 4)))
+    assert o.estimate_dt(1e30, 0.3) == pytest.approx(dt, rel=1e-15)
