@@ -32,6 +32,33 @@ def _compile(src: str) -> str:
     return obj
 
 
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+FORTRAN = os.path.join(PKG, "fortran")
+DRIVER = os.path.join(PKG, "bin", "horses_gpu_driver")
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """Compiles the ISO_C_BINDING module and the Fortran host driver against the library."""
+    fb = os.path.join(BUILD, "fortran")
+    os.makedirs(fb, exist_ok=True)
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    objs = []
+    for src in ("horses_gpu_bindings.f90", "horses_gpu_driver.f90"):
+        obj = os.path.join(fb, src.replace(".f90", ".o"))
+        cmd = [FLANG, "-O2", "-module-dir", fb, "-c", os.path.join(FORTRAN, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    cmd = [FLANG, "-o", DRIVER, objs[1], objs[0], "-L" + PKG, "-lhorses2d_gpu", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"flang link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -45,6 +72,8 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    if os.path.exists(FLANG):
+        build_fortran(verbose)
     return LIB
 
 

@@ -1,0 +1,180 @@
+!
+!  horses_gpu_bindings.f90 -- ISO_C_BINDING interface of libhorses2d_gpu.so for the
+!  HORSES2D Fortran solver (declarations mirror include/horses2d_gpu.h).
+!
+!  The solver keeps its HiOCase, HiOMesh, libproblemfile and HORSES2D.NS/.Euler entry
+!  points; a TimeScheme procedure built on hgf_rk_step replaces
+!  TimeIntegrator_WilliamsonRK3 (Approximation/DGTimeIntegrator.f90:282-317).  The
+!  adapter that binds it is shown in INTEGRATION.md.
+!
+module horses_gpu_bindings
+   use iso_c_binding
+   implicit none
+   private
+
+   integer(c_int32_t), parameter, public :: HG_BC_PERIODIC = 1, HG_BC_VISCOUSWALL = 4, HG_BC_FARFIELD = 5
+   integer(c_int32_t), parameter, public :: HG_RIEMANN_ROE = 0, HG_RIEMANN_HLL = 1, HG_RIEMANN_EXACT = 2, &
+                                            HG_RIEMANN_LF = 3
+   integer(c_int32_t), parameter, public :: HG_VISCOUS_NONE = 0, HG_VISCOUS_BR1 = 1, HG_VISCOUS_IP = 2
+   integer(c_int32_t), parameter, public :: HG_IP_SIPG = 0, HG_IP_NIPG = 1, HG_IP_IIPG = 2
+   integer(c_int32_t), parameter, public :: HG_PRECISION_FP64 = 0, HG_PRECISION_MIXED = 1
+   integer(c_int32_t), parameter, public :: HG_SCHEME_EULER = 0, HG_SCHEME_RK3 = 1, HG_SCHEME_RK5 = 2
+
+   type, bind(C), public :: hg_mesh_desc
+      integer(c_int32_t) :: nnodes
+      type(c_ptr)        :: points
+      integer(c_int32_t) :: nelements
+      type(c_ptr)        :: quads
+      type(c_ptr)        :: order
+      integer(c_int32_t) :: nbedges
+      type(c_ptr)        :: bedges
+      type(c_ptr)        :: bmarker
+      integer(c_int32_t) :: ncurved
+      integer(c_int32_t) :: np1
+      type(c_ptr)        :: cedges
+      type(c_ptr)        :: cxy
+      integer(c_int32_t) :: nmarkers
+      type(c_ptr)        :: bc_type
+      type(c_ptr)        :: bc_int
+      type(c_ptr)        :: bc_real
+      type(c_ptr)        :: bc_riemann
+   end type hg_mesh_desc
+
+   type, bind(C), public :: hg_physics
+      real(c_double) :: mach
+      real(c_double) :: reynolds
+      real(c_double) :: prandtl
+   end type hg_physics
+
+   type, bind(C), public :: hg_options
+      integer(c_int32_t) :: navier_stokes
+      integer(c_int32_t) :: riemann
+      integer(c_int32_t) :: viscous
+      integer(c_int32_t) :: ip_method
+      real(c_double)     :: sigma0
+      integer(c_int32_t) :: precision
+      integer(c_int32_t) :: scheme
+   end type hg_options
+
+   public :: hg_create, hg_destroy, hg_ndof, hg_get_node_coords, hg_upload_state, hg_download_state
+   public :: hg_download_qdot, hg_time_derivative, hg_rk_step, hg_residuals, hg_estimate_dt
+   public :: hg_reset_dt_state, hg_conservation_sums, hg_synchronize, hg_last_error
+   public :: hgf_check
+
+   interface
+      integer(c_int) function hg_create(mesh, phys, opt, handle) bind(C, name="hg_create")
+         import :: c_int, c_ptr, hg_mesh_desc, hg_physics, hg_options
+         type(hg_mesh_desc), intent(in) :: mesh
+         type(hg_physics),   intent(in) :: phys
+         type(hg_options),   intent(in) :: opt
+         type(c_ptr),        intent(out) :: handle
+      end function hg_create
+
+      integer(c_int) function hg_destroy(handle) bind(C, name="hg_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function hg_destroy
+
+      integer(c_int64_t) function hg_ndof(handle) bind(C, name="hg_ndof")
+         import :: c_int64_t, c_ptr
+         type(c_ptr), value :: handle
+      end function hg_ndof
+
+      integer(c_int) function hg_get_node_coords(handle, X, nnodes) bind(C, name="hg_get_node_coords")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value         :: handle
+         real(c_double)             :: X(*)
+         integer(c_int64_t), value  :: nnodes
+      end function hg_get_node_coords
+
+      integer(c_int) function hg_upload_state(handle, Q, ndof) bind(C, name="hg_upload_state")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value          :: handle
+         real(c_double), intent(in)  :: Q(*)
+         integer(c_int64_t), value   :: ndof
+      end function hg_upload_state
+
+      integer(c_int) function hg_download_state(handle, Q, ndof) bind(C, name="hg_download_state")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value          :: handle
+         real(c_double), intent(out) :: Q(*)
+         integer(c_int64_t), value   :: ndof
+      end function hg_download_state
+
+      integer(c_int) function hg_download_qdot(handle, QDot, ndof) bind(C, name="hg_download_qdot")
+         import :: c_int, c_ptr, c_double, c_int64_t
+         type(c_ptr), value          :: handle
+         real(c_double), intent(out) :: QDot(*)
+         integer(c_int64_t), value   :: ndof
+      end function hg_download_qdot
+
+      integer(c_int) function hg_time_derivative(handle, t) bind(C, name="hg_time_derivative")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value    :: handle
+         real(c_double), value :: t
+      end function hg_time_derivative
+
+      integer(c_int) function hg_rk_step(handle, t, dt) bind(C, name="hg_rk_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value    :: handle
+         real(c_double), value :: t, dt
+      end function hg_rk_step
+
+      integer(c_int) function hg_residuals(handle, res) bind(C, name="hg_residuals")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value          :: handle
+         real(c_double), intent(out) :: res(4)
+      end function hg_residuals
+
+      integer(c_int) function hg_estimate_dt(handle, dt_max, cfl, dt) bind(C, name="hg_estimate_dt")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value          :: handle
+         real(c_double), value       :: dt_max, cfl
+         real(c_double), intent(out) :: dt
+      end function hg_estimate_dt
+
+      integer(c_int) function hg_reset_dt_state(handle) bind(C, name="hg_reset_dt_state")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function hg_reset_dt_state
+
+      integer(c_int) function hg_conservation_sums(handle, sums) bind(C, name="hg_conservation_sums")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value          :: handle
+         real(c_double), intent(out) :: sums(4)
+      end function hg_conservation_sums
+
+      integer(c_int) function hg_synchronize(handle) bind(C, name="hg_synchronize")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: handle
+      end function hg_synchronize
+
+      integer(c_int) function hg_last_error(buf, n) bind(C, name="hg_last_error")
+         import :: c_int, c_char, c_int32_t
+         character(kind=c_char) :: buf(*)
+         integer(c_int32_t), value :: n
+      end function hg_last_error
+   end interface
+
+contains
+
+   ! Maps a nonzero status to the solver's usual `stop "..."` (Generic/Defines.h:156 style),
+   ! printing the library's message; the C side itself never exits.
+   subroutine hgf_check(rc, what)
+      integer(c_int), intent(in)   :: rc
+      character(len=*), intent(in) :: what
+      character(kind=c_char)       :: buf(512)
+      character(len=512)           :: msg
+      integer                      :: i, ierr
+      if (rc == 0) return
+      ierr = hg_last_error(buf, 512_c_int32_t)
+      msg = ""
+      do i = 1, 512
+         if (buf(i) == c_null_char) exit
+         msg(i:i) = buf(i)
+      end do
+      write(*, '(A,A,A,A)') "HORSES2D GPU: ", trim(what), " failed: ", trim(msg)
+      error stop "Stopped."
+   end subroutine hgf_check
+
+end module horses_gpu_bindings
