@@ -259,18 +259,18 @@ def euler_vortex_case(n: int = 16, N: int = 4, steps: int = 200, beta: float = 5
     """Plain FP64 Euler: isentropic vortex on a periodic [-L,L]^2 square, Roe, RK3, CFL 0.4."""
     mesh = periodic_rectangle(n, n, -L, L, -L, L)
     return Case(f"euler_vortex_{n}x{n}_N{N}", mesh, np.full(mesh.K, N, np.int32), Mach, 0.0,
-                riemann=riemann, cfl=0.4 if N <= 4 else 0.2, steps=steps,
+                riemann=riemann, cfl=0.4 if N <= 4 else 0.1, steps=steps,
                 ic=isentropic_vortex(Mach, beta, xc, yc))
 
 
 def stable_cfl(N: int) -> float:
-    """CFL used with the reference's time-step estimate (DGTimeIntegrator.f90:499-500).
+    """CFL used with the reference's time-step estimate (DGTimeIntegrator.f90:499-500) when a
+    configuration does not state one: the reference default 0.1 (Setup.f90:359-360).
 
-    That estimate divides by (N+1) while the spectral radius of the Gauss-node DG operator
-    grows like (N+1)^2, so RK3 at CFL 0.4 diverges for N >= 5 on the vortex and for N >= 4
-    on the Taylor-Green vortex (measured with the oracle).  0.4 is kept for N <= 3 and 0.2
-    is used above; the Euler vortex configuration keeps its stated CFL 0.4 at N = 4."""
-    return 0.4 if N <= 3 else 0.2
+    The estimate divides by (N+1) while the spectral radius of the Gauss-node DG operator
+    grows like (N+1)^2; measured with the oracle, RK3 diverges at CFL 0.4 for N >= 5 on the
+    vortex and at CFL 0.2 for N = 6 on the Taylor-Green vortex, and is stable at 0.1."""
+    return 0.4 if N <= 3 else 0.1
 
 
 def ns_tgv_case(n: int = 128, N: int = 6, steps: int = 500, Re: float = 1600.0, Mach: float = 0.1,
