@@ -17,15 +17,23 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libhorses2d_gpu.so")
 SOURCES = ["hg_mesh.cpp", "hg_api.cpp", "hg_device.hip", "hg_batch.hip"]
+ORDERS = range(2, 12)   # points per direction P = N+1: one object per order from hg_inst.hip
 ARCH = os.environ.get("HG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj]
+def _units():
+    units = [(s, os.path.splitext(s)[0] + ".o", []) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    units += [("hg_inst.hip", f"hg_inst_p{p}.o", [f"-DHG_P={p}"]) for p in ORDERS]
+    return units
+
+
+def _compile(unit) -> str:
+    src, objname, defs = unit
+    obj = os.path.join(BUILD, objname)
+    cmd = [HIPCC] + COMMON + DEVICE + defs + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -61,9 +69,10 @@ def build_fortran(verbose: bool = True) -> str:
 
 def build(verbose: bool = True) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    units = _units()
+    jobs = int(os.environ.get("HG_BUILD_JOBS", str(min(8, os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(_compile, units))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lamdhip64"]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -59,8 +59,9 @@ struct hg_handle {
     double *dx = nullptr, *umax = nullptr, *amax = nullptr, *dtstate = nullptr, *dtout = nullptr;
     double *cons_partial = nullptr, *cons_out = nullptr;
     int cons_nb = 0;
-    unsigned long long *resid = nullptr;
+    double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
     int cur = 0;
+    int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -131,7 +132,7 @@ hg::StageArgs base_args(hg_handle *h) {
     a.lb = h->lb;
     a.lbw = h->lbw;
     a.dlbw = h->dlbw;
-    a.resid = h->resid;
+    a.resid_part = h->resid_part;
     a.ph = h->ph;
     return a;
 }
@@ -160,11 +161,57 @@ hipError_t timed(hg_handle *h, int which, F &&launch) {
     return e;
 }
 
+// Chooses, for each kernel whose tile is not fixed yet, the faster of the two workgroup
+// tiles by timing derivative-only evaluations of the current state.  Both tiles produce
+// bit-identical results; only QDot and the gradient scratch arrays are written.
+int autotune(hg_handle *h) {
+    const bool need_g = h->ph.ns && h->tile[1] < 0, need_s = h->tile[0] < 0;
+    if (!need_g && !need_s) return 0;
+    hipEvent_t ev[2];
+    HG_CHECK(hipEventCreate(&ev[0]));
+    HG_CHECK(hipEventCreate(&ev[1]));
+    auto time_of = [&](auto launch, float *ms) -> hipError_t {
+        hipError_t e = launch();
+        if (e != hipSuccess) return e;
+        hipEventRecord(ev[0], h->stream);
+        for (int r = 0; r < 3; ++r)
+            if ((e = launch()) != hipSuccess) return e;
+        hipEventRecord(ev[1], h->stream);
+        if ((e = hipEventSynchronize(ev[1])) != hipSuccess) return e;
+        return hipEventElapsedTime(ms, ev[0], ev[1]);
+    };
+    hipError_t err = hipSuccess;
+    hg::StageArgs g = base_args(h);
+    g.tr_in = h->tr[h->cur];
+    if (need_g) {
+        float t[2] = {0.f, 0.f};
+        for (int k = 0; k < 2 && err == hipSuccess; ++k)
+            err = time_of([&] { return hg::launch_gradient(h->P, k, h->mixed, g, h->stream); }, &t[k]);
+        if (err == hipSuccess) h->tile[1] = t[0] <= t[1] ? 0 : 1;
+    }
+    if (need_s && err == hipSuccess) {
+        hg::StageArgs a = base_args(h);
+        a.tr_in = h->tr[h->cur];
+        a.cq = 1.0;
+        a.first_stage = 1;
+        a.mode = 1;
+        float t[2] = {0.f, 0.f};
+        for (int k = 0; k < 2 && err == hipSuccess; ++k)
+            err = time_of([&] { return hg::launch_stage(h->P, k, h->mixed, a, h->stream); }, &t[k]);
+        if (err == hipSuccess) h->tile[0] = t[0] <= t[1] ? 0 : 1;
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    HG_CHECK(err);
+    return 0;
+}
+
 int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode) {
+    if (autotune(h)) return 1;
     if (h->ph.ns) {
         hg::StageArgs g = base_args(h);
         g.tr_in = h->tr[h->cur];
-        HG_CHECK(timed(h, 1, [&] { return hg::launch_gradient(h->P, h->mixed, g, h->stream); }));
+        HG_CHECK(timed(h, 1, [&] { return hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream); }));
     }
     hg::StageArgs a = base_args(h);
     a.tr_in = h->tr[h->cur];
@@ -175,8 +222,9 @@ int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool la
     a.first_stage = first ? 1 : 0;
     a.mode = mode;
     a.do_resid = last ? 1 : 0;
-    if (last) HG_CHECK(hg::launch_clear_resid(h->resid, h->stream));
-    HG_CHECK(timed(h, 0, [&] { return hg::launch_stage(h->P, h->mixed, a, h->stream); }));
+    HG_CHECK(timed(h, 0, [&] { return hg::launch_stage(h->P, h->tile[0], h->mixed, a, h->stream); }));
+    if (last)
+        HG_CHECK(hg::launch_resid_final(h->resid_part, hg::stage_blocks(h->P, h->tile[0], h->K), h->resid, h->stream));
     if (mode == 0) h->cur ^= 1;
     return 0;
 }
@@ -376,7 +424,8 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(h->alloc(&h->cons_partial, 4 * (size_t)h->cons_nb));
     HG_CHECK(h->alloc(&h->cons_out, 4));
     HG_CHECK(h->alloc(&h->resid, 4));
-    HG_CHECK(hipMemset(h->resid, 0, 4 * sizeof(unsigned long long)));
+    HG_CHECK(h->alloc(&h->resid_part, 4 * (size_t)K));   // at least one element per workgroup
+    HG_CHECK(hipMemset(h->resid, 0, 4 * sizeof(double)));
     HG_CHECK(hipMemset(h->Q, 0, h->ndof * sizeof(double)));
     HG_CHECK(hipMemset(h->G, 0, h->ndof * sizeof(double)));
     HG_CHECK(hipMemset(h->QDot, 0, h->ndof * sizeof(double)));
@@ -531,10 +580,8 @@ int hg_rk_step(hg_handle *h, double t, double dt) {
 
 int hg_residuals(hg_handle *h, double res[4]) {
     if (!h || !res) return set_err("hg_residuals: null argument");
-    unsigned long long bits[4];
-    HG_CHECK(hipMemcpyAsync(bits, h->resid, sizeof bits, hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipMemcpyAsync(res, h->resid, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
-    for (int c = 0; c < 4; ++c) std::memcpy(&res[c], &bits[c], sizeof(double));
     return 0;
 }
 
@@ -567,6 +614,22 @@ int hg_conservation_sums(hg_handle *h, double out[4]) {
 int hg_synchronize(hg_handle *h) {
     if (!h) return set_err("hg_synchronize: null handle");
     HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile) {
+    if (!h) return set_err("hg_set_tiles: null handle");
+    if (stage_tile < -1 || stage_tile > 1 || gradient_tile < -1 || gradient_tile > 1)
+        return set_err("hg_set_tiles: tile must be -1 (autotune), 0 (narrow) or 1 (wide)");
+    h->tile[0] = stage_tile;
+    h->tile[1] = gradient_tile;
+    return 0;
+}
+
+int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile) {
+    if (!h) return set_err("hg_get_tiles: null handle");
+    if (stage_tile) *stage_tile = h->tile[0];
+    if (gradient_tile) *gradient_tile = h->tile[1];
     return 0;
 }
 

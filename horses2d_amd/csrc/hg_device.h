@@ -49,16 +49,21 @@ struct StageArgs {
     double am, cq, cg;          // G = am*G + cq*QDot ; Q = Q + cg*G
     int first_stage;            // G is not read (am = 0)
     int mode;                   // 0: RK update + next traces, 1: derivative only
-    unsigned long long *resid;  // 4 slots, max|QDot| bit patterns
+    double *resid_part;         // per workgroup max|QDot| per equation (4 doubles per workgroup)
     int do_resid;
     PhysParams ph;
 };
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_prolong(int P, bool mixed, const StageArgs &a, hipStream_t s);
-hipError_t launch_stage(int P, bool mixed, const StageArgs &a, hipStream_t s);
-hipError_t launch_gradient(int P, bool mixed, const StageArgs &a, hipStream_t s);
-hipError_t launch_clear_resid(unsigned long long *r, hipStream_t s);
+// tile: 0 = narrow workgroup (the widest element tile that fits one wavefront),
+//       1 = wide workgroup (up to four wavefronts).  Results are bit-identical.
+hipError_t launch_stage(int P, int tile, bool mixed, const StageArgs &a, hipStream_t s);
+hipError_t launch_gradient(int P, int tile, bool mixed, const StageArgs &a, hipStream_t s);
+// Workgroups of the stage kernel (= residual partials it writes) for a tile choice.
+int stage_blocks(int P, int tile, int K);
+// Reduces nb residual partials to out[4] (NaN propagates).
+hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream_t s);
 
 // Time-step estimate: per element maxima, then an ordered prefix-max scan.
 hipError_t launch_dt_elem(int P, const double *Q, int K, double gamma, double gm1, double *umax, double *amax,
@@ -70,5 +75,16 @@ hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const dou
                             int nblocks, double *out, hipStream_t s);
 
 bool supported_order(int N);
+
+// Per-order launchers, explicitly instantiated once per P (one object file per order).
+template <int P> hipError_t stage_p(int tile, bool mixed, const StageArgs &a, hipStream_t s);
+template <int P> int stage_blocks_p(int tile, int K);
+template <int P> hipError_t gradient_p(int tile, bool mixed, const StageArgs &a, hipStream_t s);
+template <int P> hipError_t prolong_p(bool mixed, const StageArgs &a, hipStream_t s);
+template <int P> hipError_t dt_elem_p(const double *Q, int K, double gamma, double gm1, double *umax, double *amax,
+                                      hipStream_t s);
+template <int P> int cons_blocks_p(int K);
+template <int P> hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, int K,
+                                           double *partial, hipStream_t s);
 
 }  // namespace hg

@@ -1,0 +1,672 @@
+// Stage, gradient and trace kernels of the DGSEM residual + low-storage RK for gfx950.
+//
+// Data layout in HBM (uniform polynomial order P = N+1 per handle):
+//   Q, G      : FP64, reference layout  Q[(e*P*P + i + P*j)*4 + eq]
+//   traces    : [(e*4 + f)*P + k][4]  element-local face orientation, FP64 or FP32
+//   gradient  : dQ[(e*P*P + n)*8 + d*4 + eq], FP64 or FP32
+//   metric    : five planes of e*P*P+n: Ja11, Ja12, Ja21, Ja22, jac (FP64 or FP32)
+//
+// Tiling: a workgroup owns EPB whole elements (one lane per node, one lane per face node);
+// two tile shapes per order are compiled (the widest that fits one wavefront, and the widest
+// that fits four) and the faster is chosen per handle at creation.  Every global load a lane
+// needs is issued before the first barrier so that its latency overlaps with the other
+// lanes' loads; the element state is staged in LDS for the tensor-product contractions.
+//
+// Each element computes the Riemann fluxes of its own four faces: the neighbor reads the
+// same trace values and evaluates the identical expression, so both sides see
+// bit-identical fluxes (the reference's FL = F, FR = -F), and the face lifts are subtracted
+// in ascending global edge order (DGSpatialDiscretizationMethods.f90:196-220).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hg_device.h"
+#include "hg_physics.h"
+
+namespace hg {
+
+template <typename T>
+__device__ inline T ld(const void *p, int64_t i) { return ((const T *)p)[i]; }
+
+__device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
+template <int P, int EPB_>
+struct Tile {
+    static constexpr int PP = P * P;
+    static constexpr int EPB = EPB_;
+    static constexpr int NF = EPB * 4 * P;
+    static constexpr int NN = EPB * PP;
+    static constexpr int TPB = (((NN > NF ? NN : NF) + 63) / 64) * 64;
+};
+
+// Tile choices: 0 = widest tile within one wavefront, 1 = widest within four wavefronts.
+template <int P>
+struct Tiles {
+    static constexpr int a64 = 64 / (P * P), b64 = 64 / (4 * P);
+    static constexpr int a256 = 256 / (P * P), b256 = 256 / (4 * P);
+    static constexpr int small = (a64 < b64 ? a64 : b64) > 0 ? (a64 < b64 ? a64 : b64) : 1;
+    static constexpr int large = (a256 < b256 ? a256 : b256) > 0 ? (a256 < b256 ? a256 : b256) : 1;
+};
+
+// ------------------------------------------------------------------ prolongation
+template <int P, int EPB, typename TR>
+__global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_prolong(StageArgs a) {
+    using T = Tile<P, EPB>;
+    __shared__ double sQ[T::NN * 4];
+    __shared__ double sLb[2 * P];
+    const int t = threadIdx.x, e0 = blockIdx.x * EPB;
+    if (t < 2 * P) sLb[t] = a.lb[t];
+    const int64_t base = (int64_t)e0 * T::PP * 4;
+    const int nval = min(EPB, a.K - e0) * T::PP * 4;
+    for (int x = t; x < nval; x += T::TPB) sQ[x] = a.Q[base + x];
+    __syncthreads();
+    if (t < T::NF) {
+        const int s = t / (4 * P), f = (t / P) % 4, k = t % P, e = e0 + s;
+        if (e < a.K) {
+            double tq[4];
+            face_trace<P, 4, 4>(sQ + s * T::PP * 4, sLb, f, k, tq);
+            TR *o = (TR *)a.tr_out + ((int64_t)(e * 4 + f) * P + k) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) o[c] = (TR)tq[c];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ gradient kernel
+// BR1: dQ = ( -Vector_StdVolumeGreen(Q) + sum_faces Vector_StdFace(uStar n dS) ) / jac
+// (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622, DGViscousMethods.f90:538-917).
+// IP: interior gradient (QuadElement_QuadProcedures.incf:42-88).
+// Writes dQ at the nodes and the gradient traces of the four faces.
+template <int P, int EPB, bool BR1, typename R, typename TR>
+__global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
+    using T = Tile<P, EPB>;
+    constexpr int PP = T::PP, NN = T::NN, NF = T::NF, TPB = T::TPB;
+    __shared__ double sQ[NN * 4];
+    __shared__ R sJa[BR1 ? NN * 4 : 1];
+    __shared__ R sFu[BR1 ? NF * 8 : 1];
+    __shared__ double sDQ[NN * 8];
+    __shared__ double sLb[2 * P];
+    __shared__ R sHat[PP], sLbw[2 * P], sD[PP];
+    const int t = threadIdx.x, e0 = blockIdx.x * EPB;
+    const Phys<R> ph(a.ph);
+    if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; }
+    for (int x = t; x < PP; x += TPB) { sHat[x] = (R)a.hatD[x]; sD[x] = (R)a.D[x]; }
+    const int ne = min(EPB, a.K - e0);
+    const int64_t nb0 = (int64_t)e0 * PP;
+    {
+        const double2 *src = (const double2 *)(a.Q + nb0 * 4);
+        double2 *dst = (double2 *)sQ;
+        for (int x = t; x < ne * PP * 2; x += TPB) dst[x] = src[x];
+    }
+    // ---- hoisted global loads
+    const int s = t / PP, n = t % PP, e = e0 + s;
+    const bool node_active = (t < NN) && (e < a.K);
+    R jac = R(1), j11 = R(0), j12 = R(0), j21 = R(0), j22 = R(0);
+    if (node_active) {
+        const int64_t g = nb0 + s * PP + n;
+        j11 = ld<R>(a.metric, 0 * a.nodes + g); j12 = ld<R>(a.metric, 1 * a.nodes + g);
+        j21 = ld<R>(a.metric, 2 * a.nodes + g); j22 = ld<R>(a.metric, 3 * a.nodes + g);
+        jac = ld<R>(a.metric, 4 * a.nodes + g);
+        if (BR1) {
+            sJa[t * 4 + 0] = j11; sJa[t * 4 + 1] = j12; sJa[t * 4 + 2] = j21; sJa[t * 4 + 3] = j22;
+        }
+    }
+    const int fs = t / (4 * P), ff = (t / P) % 4, fk = t % P, fe = e0 + fs;
+    const bool face_active = BR1 && (t < NF) && (fe < a.K);
+    int4 fr = make_int4(0, 0, 0, 0);
+    R nx = R(0), ny = R(0), dS = R(0), qo[4] = {R(0), R(0), R(0), R(0)};
+    if (face_active) {
+        fr = a.face[fe * 4 + ff];
+        const bool curved = fr.x & FF_CURVED, selfrev = fr.x & FF_SELFREV, rev = fr.x & FF_REV;
+        const int gi = curved ? (selfrev ? P - 1 - fk : fk) : 0;
+        const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
+        nx = (R)geo[0]; ny = (R)geo[1]; dS = (R)geo[2];
+        if ((fr.x & 15) <= FK_PERIODIC) {
+            const TR *nbt = (const TR *)a.tr_in + ((int64_t)fr.y * P + (rev ? P - 1 - fk : fk)) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
+        }
+    }
+    __syncthreads();
+    // ---- face phase (BR1): solution Riemann flux uStar dS n
+    if (face_active) {
+        const int kind = fr.x & 15;
+        double tq[4];
+        face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
+        R qs[4] = {(R)(TR)tq[0], (R)(TR)tq[1], (R)(TR)tq[2], (R)(TR)tq[3]};
+        R us[4];
+        if (kind <= FK_PERIODIC) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) us[c] = R(0.5) * (qs[c] + qo[c]);
+            if (kind == FK_INT_R) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) us[c] = -us[c];
+            }
+        } else {
+            R uB[4];
+            int neu;
+            ghost_state<R>(kind, fr.x, a.zone + 8 * fr.y, ph, qs, nx, ny, uB, us, neu);
+        }
+        R *o = sFu + t * 8;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) { o[c] = us[c] * dS * nx; o[4 + c] = us[c] * dS * ny; }
+    }
+    if (BR1) __syncthreads();
+    if (node_active) {
+        const int i = n % P, j = n / P;
+        const double *q = sQ + s * PP * 4;
+        R dq[8];
+        if (BR1) {
+            const R *Ja = sJa + s * PP * 4;
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    R acc = R(0);
+#pragma unroll
+                    for (int l = 0; l < P; ++l) acc += (R)q[(l + P * j) * 4 + c] * Ja[(l + P * j) * 4 + d * 2 + 0] * sHat[i * P + l];
+#pragma unroll
+                    for (int l = 0; l < P; ++l) acc += (R)q[(i + P * l) * 4 + c] * Ja[(i + P * l) * 4 + d * 2 + 1] * sHat[j * P + l];
+                    dq[d * 4 + c] = -acc;
+                }
+            const int ord = a.forder[e];
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                const int f = (ord >> (2 * o)) & 3;
+                const R *Fu = sFu + ((s * 4 + f) * P + ((f == 0 || f == 2) ? i : j)) * 8;
+                const R lw = (f == 0) ? sLbw[j] : (f == 1) ? sLbw[P + i] : (f == 2) ? sLbw[P + j] : sLbw[i];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) dq[c] += Fu[c] * lw;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) dq[c] = dq[c] / jac;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                R dxi = R(0), deta = R(0);
+#pragma unroll
+                for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * (R)q[(l + P * j) * 4 + c];
+#pragma unroll
+                for (int l = 0; l < P; ++l) deta += sD[j * P + l] * (R)q[(i + P * l) * 4 + c];
+                dq[c] = (dxi * j11 + deta * j12) / jac;
+                dq[4 + c] = (dxi * j21 + deta * j22) / jac;
+            }
+        }
+        R *go = (R *)a.dQ_out + (nb0 + s * PP + n) * 8;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) { go[c] = dq[c]; sDQ[t * 8 + c] = (double)dq[c]; }
+    }
+    __syncthreads();
+    if (t < NF) {
+        const int s2 = t / (4 * P), f = (t / P) % 4, k = t % P, e2 = e0 + s2;
+        if (e2 < a.K) {
+            double tg[8];
+            face_trace<P, 8, 8>(sDQ + s2 * PP * 8, sLb, f, k, tg);
+            TR *o = (TR *)a.dtr_out + ((int64_t)(e2 * 4 + f) * P + k) * 8;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) o[c] = (TR)tg[c];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ the stage kernel
+template <int P, int EPB, bool NS, bool IP, typename R, typename TR>
+__global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
+    using T = Tile<P, EPB>;
+    constexpr int PP = T::PP, NN = T::NN, NF = T::NF, TPB = T::TPB;
+    __shared__ double sQ[NN * 4];
+    __shared__ R sCF[NN * 4];
+    __shared__ R sCG[NN * 4];
+    __shared__ R sF[NF * 4];
+    __shared__ R sU[IP ? NF * 4 : 1];
+    __shared__ R sV[IP ? NF * 4 : 1];
+    __shared__ R sU2[IP ? NF * 4 : 1];
+    __shared__ R sV2[IP ? NF * 4 : 1];
+    __shared__ double sLb[2 * P];
+    __shared__ R sHat[PP], sLbw[2 * P], sDlbw[2 * P];
+    const int t = threadIdx.x, e0 = blockIdx.x * EPB;
+    const Phys<R> ph(a.ph);
+    const bool ipgrad = IP && a.ph.ip_gradflux;
+    if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; sDlbw[t] = (R)a.dlbw[t]; }
+    for (int x = t; x < PP; x += TPB) sHat[x] = (R)a.hatD[x];
+    const int ne = min(EPB, a.K - e0);
+    const int64_t nb0 = (int64_t)e0 * PP;
+    {
+        const double2 *src = (const double2 *)(a.Q + nb0 * 4);
+        double2 *dst = (double2 *)sQ;
+        for (int x = t; x < ne * PP * 2; x += TPB) dst[x] = src[x];
+    }
+
+    // ---- hoisted global loads: node lane
+    const int s = t / PP, n = t % PP, e = e0 + s;
+    const bool node_active = (t < NN) && (e < a.K);
+    const int64_t g = nb0 + s * PP + n;
+    R j11 = R(0), j12 = R(0), j21 = R(0), j22 = R(0), jac = R(1);
+    R dqn[NS ? 8 : 1];
+    double gold[4] = {0.0, 0.0, 0.0, 0.0};
+    if (node_active) {
+        j11 = ld<R>(a.metric, 0 * a.nodes + g); j12 = ld<R>(a.metric, 1 * a.nodes + g);
+        j21 = ld<R>(a.metric, 2 * a.nodes + g); j22 = ld<R>(a.metric, 3 * a.nodes + g);
+        jac = ld<R>(a.metric, 4 * a.nodes + g);
+        if (NS) {
+            const TR *dqp = (const TR *)a.dQ + g * 8;
+#pragma unroll
+            for (int c = 0; c < (NS ? 8 : 1); ++c) dqn[c] = (R)dqp[c];
+        }
+        if (a.mode == 0 && !a.first_stage) {
+            const double2 *gp = (const double2 *)(a.G + g * 4);
+            double2 g01 = gp[0], g23 = gp[1];
+            gold[0] = g01.x; gold[1] = g01.y; gold[2] = g23.x; gold[3] = g23.y;
+        }
+    }
+    // ---- hoisted global loads: face lane
+    const int fs = t / (4 * P), ff = (t / P) % 4, fk = t % P, fe = e0 + fs;
+    const bool face_active = (t < NF) && (fe < a.K);
+    int4 fr = make_int4(0, 0, 0, 0);
+    R nx = R(0), ny = R(0), dS = R(0), qo[4] = {R(0), R(0), R(0), R(0)};
+    R ds[NS ? 8 : 1], dn[NS ? 8 : 1];
+    R invh = R(0);
+    if (face_active) {
+        fr = a.face[fe * 4 + ff];
+        const bool curved = fr.x & FF_CURVED, selfrev = fr.x & FF_SELFREV, rev = fr.x & FF_REV;
+        const int gi = curved ? (selfrev ? P - 1 - fk : fk) : 0;
+        const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
+        nx = (R)geo[0]; ny = (R)geo[1]; dS = (R)geo[2];
+        const bool inner = (fr.x & 15) <= FK_PERIODIC;
+        const int64_t nslot = (int64_t)fr.y * P + (rev ? P - 1 - fk : fk);
+        if (inner) {
+            const TR *nbt = (const TR *)a.tr_in + nslot * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
+        }
+        if (NS) {
+            const int64_t oslot = (int64_t)(fe * 4 + ff) * P + fk;
+#pragma unroll
+            for (int c = 0; c < (NS ? 8 : 1); ++c) ds[c] = (R)((const TR *)a.dtr)[oslot * 8 + c];
+            if (inner) {
+#pragma unroll
+                for (int c = 0; c < (NS ? 8 : 1); ++c) dn[c] = (R)((const TR *)a.dtr)[nslot * 8 + c];
+            }
+            if (IP) invh = (R)a.einvh[fr.w];
+        }
+    }
+    __syncthreads();
+
+    // ---- node phase: contravariant fluxes (DGSpatial_QDotVolumeLoop, DGWeakIntegrals.f90:111-171)
+    if (node_active) {
+        R q[4], f[4], gg[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) q[c] = (R)sQ[t * 4 + c];
+        inviscid_flux(ph, q, f, gg);
+        if (NS) {
+            R fv[4], gv[4];
+            viscous_flux(ph, q, dqn, fv, gv);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { f[c] = (f[c] - ph.mu * fv[c]) - R(0); gg[c] = (gg[c] - ph.mu * gv[c]) - R(0); }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            sCF[t * 4 + c] = f[c] * j11 + gg[c] * j21;
+            sCG[t * 4 + c] = f[c] * j12 + gg[c] * j22;
+        }
+    }
+
+    // ---- face phase: Riemann fluxes of the element's own four faces
+    if (face_active) {
+        const int kind = fr.x & 15;
+        double tq[4];
+        face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
+        R qs[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) qs[c] = (R)(TR)tq[c];   // the same rounding the neighbor reads
+        int rs = a.ph.riemann;
+        R uSB[4] = {R(0), R(0), R(0), R(0)};
+        int neumann = 0;
+        if (kind > FK_PERIODIC) {
+            ghost_state<R>(kind, fr.x, a.zone + 8 * fr.y, ph, qs, nx, ny, qo, uSB, neumann);
+            const int zr = (fr.x >> FF_RS_SHIFT) & 15;
+            if (zr) rs = zr - 1;
+        }
+        R Fi[4];
+        if (kind == FK_INT_R) riemann<R>(rs, ph, qo, qs, nx, ny, Fi);
+        else riemann<R>(rs, ph, qs, qo, nx, ny, Fi);
+        R F[4];
+        if (!NS) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) F[c] = Fi[c] * dS;
+        } else {
+            R Fv[4] = {R(0), R(0), R(0), R(0)};
+            R G8[8] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+            const R pen = IP ? ph.sigma0 * ph.mu * (R)(P - 1) * (R)(P - 1) * invh : R(0);
+            if (kind <= FK_PERIODIC) {
+                const R *uL = (kind == FK_INT_R) ? qo : qs, *uR = (kind == FK_INT_R) ? qs : qo;
+                const R *dL = (kind == FK_INT_R) ? dn : ds, *dR = (kind == FK_INT_R) ? ds : dn;
+                R fL[4], gL[4], fR[4], gR[4];
+                viscous_flux(ph, uL, dL, fL, gL);
+                viscous_flux(ph, uR, dR, fR, gR);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    R acc = R(0);
+                    acc = acc + R(0.5) * (ph.mu * fL[c] + ph.mu * fR[c]) * nx;
+                    acc = acc + R(0.5) * (ph.mu * gL[c] + ph.mu * gR[c]) * ny;
+                    Fv[c] = acc;
+                }
+                if (IP) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (uL[c] - uR[c]);
+                }
+                if (ipgrad) {
+                    // IP_GradientRiemannSolver (DGViscous_IP.f90:216-249): this side's share
+                    R fgr[8], f4[4], g4[4];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) { fgr[c] = (uL[c] - uR[c]) * nx; fgr[4 + c] = (uL[c] - uR[c]) * ny; }
+                    const R *own = (kind == FK_INT_R) ? uR : uL;
+                    viscous_flux(ph, own, fgr, f4, g4);
+                    const R cf = (kind == FK_INT_R ? -R(0.5) : R(0.5)) * ph.ip_eps * ph.mu;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
+                }
+            } else if (!neumann) {
+                const bool adiab = kind == FK_WALL && !(fr.x & FF_ISOTHERMAL);
+                R fv4[4], gv4[4];
+                if (adiab) adiabatic_flux(ph, qs, ds, fv4, gv4);
+                else viscous_flux(ph, qs, ds, fv4, gv4);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    R acc = R(0);
+                    acc = acc + ph.mu * fv4[c] * nx;
+                    acc = acc + ph.mu * gv4[c] * ny;
+                    Fv[c] = acc;
+                }
+                if (IP) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (qs[c] - uSB[c]);
+                }
+                if (ipgrad) {
+                    R fgr[8], f4[4], g4[4];
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) { fgr[c] = (qs[c] - uSB[c]) * nx; fgr[4 + c] = (qs[c] - uSB[c]) * ny; }
+                    if (adiab) adiabatic_flux(ph, qs, fgr, f4, g4);
+                    else viscous_flux(ph, qs, fgr, f4, g4);
+                    const R cf = ph.ip_eps * ph.mu;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
+                }
+            }
+            if (kind <= FK_INT_R) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] - Fv[c]) - R(0)) * dS;
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] * dS - Fv[c] * dS) - R(0));
+            }
+            if (ipgrad) {
+                // contravariant gradient-flux components at the face (DGWeakIntegrals.f90:268-495)
+                const R *fm = (const R *)a.fmet + ((int64_t)(fe * 4 + ff) * P + fk) * 5;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    sU[t * 4 + c] = (G8[c] * fm[0] + G8[4 + c] * fm[1]) / fm[4];
+                    sV[t * 4 + c] = (G8[c] * fm[2] + G8[4 + c] * fm[3]) / fm[4];
+                }
+            }
+        }
+        if (kind == FK_INT_R) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) F[c] = -F[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sF[t * 4 + c] = F[c];
+    }
+    __syncthreads();
+
+    // ---- IP gradient face terms: U (lbw coefficient) and V (dlbw coefficient) per face node;
+    // horizontal faces: U = hatD . Fc, V = Gc ; vertical faces: U = hatD . Gc, V = Fc
+    if (ipgrad) {
+        if (face_active) {
+            const R *Fc = sU + (fs * 4 + ff) * P * 4, *Gc = sV + (fs * 4 + ff) * P * 4;
+            const bool horiz = (ff == 0 || ff == 2);
+            const R *src = horiz ? Fc : Gc;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                R acc = R(0);
+#pragma unroll
+                for (int l = 0; l < P; ++l) acc += sHat[fk * P + l] * src[l * 4 + c];
+                sU2[t * 4 + c] = acc;
+                sV2[t * 4 + c] = horiz ? Gc[fk * 4 + c] : Fc[fk * 4 + c];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- node phase 2: volume contraction, ordered face lifts, /jac, RK update
+    double qd[4] = {0.0, 0.0, 0.0, 0.0};
+    if (node_active) {
+        const int i = n % P, j = n / P;
+        R acc[4] = {R(0), R(0), R(0), R(0)};
+        const R *cF = sCF + s * PP * 4, *cG = sCG + s * PP * 4;
+#pragma unroll
+        for (int l = 0; l < P; ++l) {
+            const R hi = sHat[i * P + l], hj = sHat[j * P + l];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[c] = acc[c] + hi * cF[(l + P * j) * 4 + c] + hj * cG[(i + P * l) * 4 + c];
+        }
+        const int ord = a.forder[e];
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int f = (ord >> (2 * o)) & 3;
+            const int idx = (f == 0 || f == 2) ? i : j;
+            const int oth = (f == 0 || f == 2) ? j : i;
+            const int side = (f == 0 || f == 3) ? 0 : P;
+            const R lw = sLbw[side + oth];
+            const R *Ff = sF + ((s * 4 + f) * P + idx) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[c] = acc[c] - Ff[c] * lw;
+            if (ipgrad) {
+                const R dlw = sDlbw[side + oth];
+                const R *Uf = sU2 + ((s * 4 + f) * P + idx) * 4, *Vf = sV2 + ((s * 4 + f) * P + idx) * 4;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[c] = acc[c] - (Uf[c] * lw + Vf[c] * dlw);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) qd[c] = (double)(acc[c] / jac);
+        if (a.mode == 1) {
+            double2 *o = (double2 *)(a.QDot + g * 4);
+            o[0] = make_double2(qd[0], qd[1]);
+            o[1] = make_double2(qd[2], qd[3]);
+        } else {
+            double gv[4];
+            if (a.first_stage) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) gv[c] = a.cq * qd[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) gv[c] = fma(a.am, gold[c], a.cq * qd[c]);
+            }
+            double2 *go = (double2 *)(a.G + g * 4);
+            go[0] = make_double2(gv[0], gv[1]);
+            go[1] = make_double2(gv[2], gv[3]);
+            double qn[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) qn[c] = fma(a.cg, gv[c], sQ[t * 4 + c]);
+            double2 *qo2 = (double2 *)(a.Q + g * 4);
+            qo2[0] = make_double2(qn[0], qn[1]);
+            qo2[1] = make_double2(qn[2], qn[3]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) sQ[t * 4 + c] = qn[c];
+        }
+    }
+    if (a.do_resid) {
+        // max|QDot| per equation: wavefront shuffle, then across the workgroup's wavefronts in
+        // LDS; one partial per workgroup (reduced by k_resid_final).  NaN wins every compare.
+        __shared__ double sRes[TPB / 64][4];
+        double m[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            double v = node_active ? fabs(qd[c]) : 0.0;
+            for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_xor(v, off, 64));
+            m[c] = v;
+        }
+        if ((t & 63) == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) sRes[t >> 6][c] = m[c];
+        }
+        __syncthreads();
+        if (t < 4) {
+            double v = sRes[0][t];
+#pragma unroll
+            for (int w = 1; w < TPB / 64; ++w) v = nan_max(v, sRes[w][t]);
+            a.resid_part[(int64_t)blockIdx.x * 4 + t] = v;
+        }
+    }
+    if (a.mode == 0) {
+        __syncthreads();
+        if (face_active) {
+            double tq[4];
+            face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
+            TR *o = (TR *)a.tr_out + ((int64_t)(fe * 4 + ff) * P + fk) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) o[c] = (TR)tq[c];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ time step and sums
+template <int P>
+__global__ __launch_bounds__(256) void k_dt_elem(const double *Q, int K, double gamma, double gm1, double *umax,
+                                                 double *amax) {
+    constexpr int PP = P * P, EPB = 256 / PP > 0 ? 256 / PP : 1;
+    __shared__ double su[256], sa[256];
+    const int t = threadIdx.x, s = t / PP, n = t % PP, e = blockIdx.x * EPB + s;
+    double u = 0.0, aa = 0.0;
+    if (t < EPB * PP && e < K) {
+        const double *q = Q + ((int64_t)e * PP + n) * 4;
+        u = sqrt(q[1] * q[1] + q[2] * q[2]) / q[0];
+        const double ur = q[1] / q[0], vr = q[2] / q[0];
+        aa = sqrt(gamma * gm1 * (q[3] / q[0] - 0.5 * (ur * ur) - 0.5 * (vr * vr)));
+    }
+    su[t] = u; sa[t] = aa;
+    __syncthreads();
+    if (t < EPB * PP && e < K && n == 0) {
+        double mu = 0.0, ma = 0.0;
+        for (int x = 0; x < PP; ++x) { mu = fmax(mu, su[s * PP + x]); ma = fmax(ma, sa[s * PP + x]); }
+        umax[e] = mu; amax[e] = ma;
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_cons_partial(const double *Q, const double *jac, const double *w, int K,
+                                                      double *partial) {
+    constexpr int PP = P * P, EPB = 256 / PP > 0 ? 256 / PP : 1;
+    __shared__ double sv[256][4];
+    const int t = threadIdx.x, s = t / PP, n = t % PP, e = blockIdx.x * EPB + s;
+    double v[4] = {0, 0, 0, 0};
+    if (t < EPB * PP && e < K) {
+        const int i = n % P, j = n / P;
+        const int64_t g = (int64_t)e * PP + n;
+        const double c = w[i] * w[j] * jac[g];
+        for (int k = 0; k < 4; ++k) v[k] = c * Q[g * 4 + k];
+    }
+    for (int k = 0; k < 4; ++k) sv[t][k] = v[k];
+    __syncthreads();
+    for (int wd = 128; wd > 0; wd >>= 1) {
+        if (t < wd)
+            for (int k = 0; k < 4; ++k) sv[t][k] += sv[t + wd][k];
+        __syncthreads();
+    }
+    if (t < 4) partial[blockIdx.x * 4 + t] = sv[0][t];
+}
+
+// ------------------------------------------------------------------ per-order launchers
+template <int P, int EPB>
+inline int tile_blocks(int K) { return (K + EPB - 1) / EPB; }
+
+template <int P, int EPB>
+hipError_t stage_tile(bool mixed, const StageArgs &a, hipStream_t st) {
+    constexpr int TPB = Tile<P, EPB>::TPB;
+    const int nb = tile_blocks<P, EPB>(a.K);
+    const bool ns = a.ph.ns != 0, ip = a.ph.viscous == 2;
+    if (mixed) {
+        if (!ns) hipLaunchKernelGGL((k_stage<P, EPB, false, false, float, float>), dim3(nb), dim3(TPB), 0, st, a);
+        else if (ip) hipLaunchKernelGGL((k_stage<P, EPB, true, true, float, float>), dim3(nb), dim3(TPB), 0, st, a);
+        else hipLaunchKernelGGL((k_stage<P, EPB, true, false, float, float>), dim3(nb), dim3(TPB), 0, st, a);
+    } else {
+        if (!ns) hipLaunchKernelGGL((k_stage<P, EPB, false, false, double, double>), dim3(nb), dim3(TPB), 0, st, a);
+        else if (ip) hipLaunchKernelGGL((k_stage<P, EPB, true, true, double, double>), dim3(nb), dim3(TPB), 0, st, a);
+        else hipLaunchKernelGGL((k_stage<P, EPB, true, false, double, double>), dim3(nb), dim3(TPB), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+template <int P, int EPB>
+hipError_t gradient_tile(bool mixed, const StageArgs &a, hipStream_t st) {
+    constexpr int TPB = Tile<P, EPB>::TPB;
+    const int nb = tile_blocks<P, EPB>(a.K);
+    const bool br1 = a.ph.viscous == 1;
+    if (mixed) {
+        if (br1) hipLaunchKernelGGL((k_gradient<P, EPB, true, float, float>), dim3(nb), dim3(TPB), 0, st, a);
+        else hipLaunchKernelGGL((k_gradient<P, EPB, false, float, float>), dim3(nb), dim3(TPB), 0, st, a);
+    } else {
+        if (br1) hipLaunchKernelGGL((k_gradient<P, EPB, true, double, double>), dim3(nb), dim3(TPB), 0, st, a);
+        else hipLaunchKernelGGL((k_gradient<P, EPB, false, double, double>), dim3(nb), dim3(TPB), 0, st, a);
+    }
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t stage_p(int tile, bool mixed, const StageArgs &a, hipStream_t st) {
+    return tile == 0 ? stage_tile<P, Tiles<P>::small>(mixed, a, st) : stage_tile<P, Tiles<P>::large>(mixed, a, st);
+}
+
+template <int P>
+int stage_blocks_p(int tile, int K) {
+    return tile == 0 ? tile_blocks<P, Tiles<P>::small>(K) : tile_blocks<P, Tiles<P>::large>(K);
+}
+
+template <int P>
+hipError_t gradient_p(int tile, bool mixed, const StageArgs &a, hipStream_t st) {
+    return tile == 0 ? gradient_tile<P, Tiles<P>::small>(mixed, a, st)
+                     : gradient_tile<P, Tiles<P>::large>(mixed, a, st);
+}
+
+template <int P>
+hipError_t prolong_p(bool mixed, const StageArgs &a, hipStream_t st) {
+    constexpr int EPB = Tiles<P>::large, TPB = Tile<P, EPB>::TPB;
+    const int nb = tile_blocks<P, EPB>(a.K);
+    if (mixed) hipLaunchKernelGGL((k_prolong<P, EPB, float>), dim3(nb), dim3(TPB), 0, st, a);
+    else hipLaunchKernelGGL((k_prolong<P, EPB, double>), dim3(nb), dim3(TPB), 0, st, a);
+    return hipGetLastError();
+}
+
+template <int P>
+hipError_t dt_elem_p(const double *Q, int K, double gamma, double gm1, double *umax, double *amax, hipStream_t st) {
+    constexpr int EPB = 256 / (P * P) > 0 ? 256 / (P * P) : 1;
+    hipLaunchKernelGGL((k_dt_elem<P>), dim3((K + EPB - 1) / EPB), dim3(256), 0, st, Q, K, gamma, gm1, umax, amax);
+    return hipGetLastError();
+}
+
+template <int P>
+int cons_blocks_p(int K) {
+    constexpr int EPB = 256 / (P * P) > 0 ? 256 / (P * P) : 1;
+    return (K + EPB - 1) / EPB;
+}
+
+template <int P>
+hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, int K, double *partial,
+                          hipStream_t st) {
+    hipLaunchKernelGGL((k_cons_partial<P>), dim3(cons_blocks_p<P>(K)), dim3(256), 0, st, Q, jac, w, K, partial);
+    return hipGetLastError();
+}
+
+#define HG_INSTANTIATE_ORDER(PV)                                                                              \
+    template hipError_t stage_p<PV>(int, bool, const StageArgs &, hipStream_t);                               \
+    template hipError_t gradient_p<PV>(int, bool, const StageArgs &, hipStream_t);                            \
+    template int stage_blocks_p<PV>(int, int);                                                                \
+    template hipError_t prolong_p<PV>(bool, const StageArgs &, hipStream_t);                                  \
+    template hipError_t dt_elem_p<PV>(const double *, int, double, double, double *, double *, hipStream_t);  \
+    template int cons_blocks_p<PV>(int);                                                                      \
+    template hipError_t cons_partial_p<PV>(const double *, const double *, const double *, int, double *,     \
+                                           hipStream_t);
+
+}  // namespace hg

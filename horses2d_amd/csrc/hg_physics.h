@@ -1,0 +1,349 @@
+// Point-wise physics on the device: fluxes, Riemann solvers, boundary ghost states and the
+// face-trace helper shared by every kernel.  Templated on the compute type R (double for the
+// FP64 form, float for the mixed form).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hg_device.h"
+
+namespace hg {
+
+template <typename R>
+struct Phys {
+    R gamma, gm1, cp, cv, lambda, invSGM, mu, cpPr, sigma0, ip_eps;
+    __device__ explicit Phys(const PhysParams &p)
+        : gamma((R)p.gamma), gm1((R)p.gm1), cp((R)p.cp), cv((R)p.cv), lambda((R)p.lambda),
+          invSGM((R)p.invSGM), mu((R)p.mu), cpPr((R)p.cpPr), sigma0((R)p.sigma0), ip_eps((R)p.ip_eps) {}
+};
+
+__device__ inline double dsqrt(double x) { return sqrt(x); }
+__device__ inline float dsqrt(float x) { return sqrtf(x); }
+__device__ inline double dpow(double a, double b) { return pow(a, b); }
+__device__ inline float dpow(float a, float b) { return powf(a, b); }
+__device__ inline double dabs(double x) { return fabs(x); }
+__device__ inline float dabs(float x) { return fabsf(x); }
+
+// ------------------------------------------------------------------ point fluxes
+// InviscidFluxes.f90:104-147
+template <typename R>
+__device__ inline void inviscid_flux(const Phys<R> &ph, const R *u, R *f, R *g) {
+    R invRho = R(1) / u[0];
+    R vx = u[1] * invRho, vy = u[2] * invRho;
+    R p = ph.gm1 * (u[3] - R(0.5) * u[1] * vx - R(0.5) * u[2] * vy);
+    f[0] = u[1]; f[1] = u[1] * vx + p; f[2] = u[1] * vy; f[3] = (u[3] + p) * vx;
+    g[0] = u[2]; g[1] = f[2]; g[2] = u[2] * vy + p; g[3] = (u[3] + p) * vy;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) { f[k] *= ph.invSGM; g[k] *= ph.invSGM; }
+}
+
+// RiemannSolvers.f90:189-305
+template <typename R>
+__device__ void roe_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
+    R rhoL = qL[0], invRhoL = R(1) / rhoL, sqrtRhoL = dsqrt(rhoL);
+    R uL = (qL[1] * nx + qL[2] * ny) * invRhoL, vL = (-qL[1] * ny + qL[2] * nx) * invRhoL;
+    R pL = ph.gm1 * (qL[3] - R(0.5) * rhoL * (uL * uL + vL * vL));
+    R HL = ph.cp * pL * invRhoL + R(0.5) * (uL * uL + vL * vL);
+    R rhoR = qR[0], invRhoR = R(1) / rhoR, sqrtRhoR = dsqrt(rhoR);
+    R uR = (qR[1] * nx + qR[2] * ny) * invRhoR, vR = (-qR[1] * ny + qR[2] * nx) * invRhoR;
+    R pR = ph.gm1 * (qR[3] - R(0.5) * rhoR * (uR * uR + vR * vR));
+    R HR = ph.cp * pR * invRhoR + R(0.5) * (uR * uR + vR * vR);
+    R invrho = R(1) / (sqrtRhoL + sqrtRhoR);
+    R u = (sqrtRhoL * uL + sqrtRhoR * uR) * invrho;
+    R v = (sqrtRhoL * vL + sqrtRhoR * vR) * invrho;
+    R H = (sqrtRhoL * HL + sqrtRhoR * HR) * invrho;
+    R a = dsqrt(ph.gm1 * (H - R(0.5) * (u * u + v * v)));
+    R d0 = rhoR - rhoL, d1 = rhoR * uR - rhoL * uL, d2 = rhoR * vR - rhoL * vL, d3 = qR[3] - qL[3];
+    R al2 = d2 - v * d0;
+    R al1 = ph.gm1 * (d0 * (H - u * u) + u * d1 - d3 + (d2 - v * d0) * v) / (a * a);
+    R al0 = (d0 * (u + a) - d1 - a * al1) / (R(2) * a);
+    R al3 = d0 - (al0 + al1);
+    R Fs0 = rhoL * uL, Fs1 = Fs0 * uL + pL, Fs2 = Fs0 * vL, Fs3 = HL * Fs0;
+    // waves with negative speed, counted like RiemannSolvers.f90:253-261
+    if (!(u - a > R(0))) {
+        R c = al0 * (u - a);
+        Fs0 += c; Fs1 += c * (u - a); Fs2 += c * v; Fs3 += c * (H - u * a);
+        if (!(u > R(0))) {
+            R c1 = al1 * u;
+            Fs0 += c1; Fs1 += c1 * u; Fs2 += c1 * v; Fs3 += c1 * (R(0.5) * (u * u + v * v));
+            R c2 = al2 * u;
+            Fs2 += c2; Fs3 += c2 * v;
+            if (!(u + a > R(0))) {
+                R c3 = al3 * (u + a);
+                Fs0 += c3; Fs1 += c3 * (u + a); Fs2 += c3 * v; Fs3 += c3 * (H + u * a);
+            }
+        }
+    }
+    R Ft = Fs1 * nx - Fs2 * ny;
+    Fs2 = Fs1 * ny + Fs2 * nx;
+    F[0] = Fs0 * ph.invSGM; F[1] = Ft * ph.invSGM; F[2] = Fs2 * ph.invSGM; F[3] = Fs3 * ph.invSGM;
+}
+
+// RiemannSolvers.f90:307-399
+template <typename R>
+__device__ void hll_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
+    R rhoL = qL[0], invRhoL = R(1) / rhoL;
+    R uL = (qL[1] * nx + qL[2] * ny) * invRhoL, vL = (-qL[1] * ny + qL[2] * nx) * invRhoL;
+    R pL = ph.gm1 * (qL[3] - R(0.5) * rhoL * (uL * uL + vL * vL));
+    R HL = ph.cp * pL * invRhoL + R(0.5) * (uL * uL + vL * vL);
+    R aL = dsqrt(ph.gamma * pL * invRhoL), rhoeL = qL[3];
+    R rhoR = qR[0], invRhoR = R(1) / rhoR;
+    R uR = (qR[1] * nx + qR[2] * ny) * invRhoR, vR = (-qR[1] * ny + qR[2] * nx) * invRhoR;
+    R pR = ph.gm1 * (qR[3] - R(0.5) * rhoR * (uR * uR + vR * vR));
+    R HR = ph.cp * pR * invRhoR + R(0.5) * (uR * uR + vR * vR);
+    R aR = dsqrt(ph.gamma * pR * invRhoR), rhoeR = qR[3];
+    R sL = dsqrt(rhoL), sR = dsqrt(rhoR), invrho = R(1) / (sL + sR);
+    R u = (sL * uL + sR * uR) * invrho, v = (sL * vL + sR * vR) * invrho, H = (sL * HL + sR * HR) * invrho;
+    R a = dsqrt(ph.gm1 * (H - R(0.5) * (u * u + v * v)));
+    R SL = fmin(uL - aL, u - a), SR = fmax(uR + aR, u + a);
+    R Fs[4] = {R(0), R(0), R(0), R(0)};
+    if (SL >= R(0)) {
+        Fs[0] = rhoL * uL; Fs[1] = Fs[0] * uL + pL; Fs[2] = Fs[0] * vL; Fs[3] = HL * Fs[0];
+    } else if (SR <= R(0)) {
+        Fs[0] = rhoR * uR; Fs[1] = Fs[0] * uR + pR; Fs[2] = Fs[0] * vR; Fs[3] = HR * Fs[0];
+    } else if ((SL < R(0)) || (SR > R(0))) {
+        R idS = R(1) / (SR - SL);
+        Fs[0] = (SR * rhoL * uL - SL * rhoR * uR + SL * SR * (rhoR - rhoL)) * idS;
+        Fs[1] = (SR * (rhoL * uL * uL + pL) - SL * (rhoR * uR * uR + pR) + SL * SR * (rhoR * uR - rhoL * uL)) * idS;
+        Fs[2] = (SR * rhoL * uL * vL - SL * rhoR * uR * vR + SR * SL * (rhoR * vR - rhoL * vL)) * idS;
+        Fs[3] = (SR * uL * rhoL * HL - SL * uR * rhoR * HR + SR * SL * (rhoeR - rhoeL)) * idS;
+    }
+    R Ft = Fs[1] * nx - Fs[2] * ny;
+    Fs[2] = Fs[1] * ny + Fs[2] * nx;
+    F[0] = Fs[0] * ph.invSGM; F[1] = Ft * ph.invSGM; F[2] = Fs[2] * ph.invSGM; F[3] = Fs[3] * ph.invSGM;
+}
+
+// RiemannSolvers.f90:461-484
+template <typename R>
+__device__ inline void exact_F(const Phys<R> &ph, R p, const R *W, R &F, R &dF) {
+    if (p > W[3]) {
+        R A = R(2) / (W[0] * (ph.gamma + R(1)));
+        R B = ph.gm1 * W[3] / (ph.gamma + R(1));
+        R s = dsqrt(A / (p + B));
+        F = (p - W[3]) * s;
+        dF = s * (R(1) - R(0.5) * (p - W[3]) / (B + p));
+    } else {
+        F = R(2) * ph.cv * W[5] * (dpow(p / W[3], R(0.5) / ph.cp) - R(1));
+        dF = R(1) / (W[0] * W[5]) * dpow(p / W[3], -R(0.5) * (ph.gamma + R(1)) / ph.gamma);
+    }
+}
+
+// RiemannSolvers.f90:33-187 with ExactRiemann_ComputePStar :405-459 (at most 50 Newton
+// iterations; the star velocity is taken from the last iterate when not converged).
+template <typename R>
+__device__ void exact_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
+    const R g = ph.gamma, gm1 = ph.gm1, cp = ph.cp, cv = ph.cv;
+    R qnL[4] = {qL[0], qL[1] * nx + qL[2] * ny, -qL[1] * ny + qL[2] * nx, qL[3]};
+    R qnR[4] = {qR[0], qR[1] * nx + qR[2] * ny, -qR[1] * ny + qR[2] * nx, qR[3]};
+    R wL[6], wR[6];
+    wL[0] = qnL[0]; wL[1] = qnL[1] / qnL[0]; wL[2] = qnL[2] / qnL[0];
+    wL[3] = gm1 * (qnL[3] - R(0.5) * (qnL[1] * wL[1] + qnL[2] * wL[2])); wL[4] = R(0); wL[5] = dsqrt(g * wL[3] / wL[0]);
+    wR[0] = qnR[0]; wR[1] = qnR[1] / qnR[0]; wR[2] = qnR[2] / qnR[0];
+    wR[3] = gm1 * (qnR[3] - R(0.5) * (qnR[1] * wR[1] + qnR[2] * wR[2])); wR[4] = R(0); wR[5] = dsqrt(g * wR[3] / wR[0]);
+    R FL, dFL, FR, dFR;
+    R ps = dpow((wL[5] + wR[5] - R(0.5) * gm1 * (wR[1] - wL[1])) /
+                    (wL[5] / dpow(wL[3], R(0.5) / cp) + wR[5] / dpow(wR[3], R(0.5) / cp)),
+                R(2) * cp);
+    for (int it = 1; it <= 50; ++it) {
+        exact_F(ph, ps, wL, FL, dFL);
+        exact_F(ph, ps, wR, FR, dFR);
+        R Fv = FL + FR + wR[1] - wL[1], dF = dFL + dFR, pold = ps;
+        ps = pold - Fv / dF;
+        R cha = R(0.5) * dabs(ps - pold) / (ps + pold);
+        if (dabs(cha) < R(1.0e-6)) break;
+    }
+    exact_F(ph, ps, wL, FL, dFL);
+    exact_F(ph, ps, wR, FR, dFR);
+    R us = R(0.5) * (wL[1] + wR[1] + FR - FL), pstar = ps, rs, uF, pF;
+    R Fs[4] = {R(0), R(0), R(0), R(0)};
+    if (us >= R(0)) {
+        R sh = wL[5] * dsqrt((g + R(1)) / (R(2) * g) * pstar / wL[3] + R(0.5) / cp);
+        if ((pstar <= wL[3]) && (wL[1] >= wL[5])) {
+            Fs[0] = qnL[1]; Fs[1] = qnL[1] * wL[1] + wL[3]; Fs[2] = qnL[2] * wL[1];
+            Fs[3] = wL[1] * (cp * wL[3] + R(0.5) * (qnL[1] * wL[1] + qnL[2] * wL[2]));
+        } else if ((pstar <= wL[3]) && (wL[1] < wL[5]) && (us < wL[5] * dpow(pstar / wL[3], R(0.5) / cp))) {
+            rs = wL[0] * dpow(pstar / wL[3], R(1) / g);
+            Fs[0] = rs * us; Fs[1] = rs * us * us + pstar; Fs[2] = Fs[0] * wL[2];
+            Fs[3] = us * (cp * pstar + R(0.5) * rs * (us * us + wL[2] * wL[2]));
+        } else if ((pstar <= wL[3]) && (wL[1] < wL[5]) && (us >= wL[5] * dpow(pstar / wL[3], R(0.5) / cp))) {
+            uF = R(2) * wL[5] / (g + R(1)) + (g - R(1)) / (g + R(1)) * wL[1];
+            rs = wL[0] * dpow(uF / wL[5], R(2) * cv);
+            pF = wL[3] * dpow(rs / wL[0], g);
+            Fs[0] = rs * uF; Fs[1] = rs * uF * uF + pF; Fs[2] = rs * uF * wL[2];
+            Fs[3] = uF * (cp * pF + R(0.5) * rs * (uF * uF + wL[2] * wL[2]));
+        } else if ((pstar > wL[3]) && (wL[1] >= sh)) {
+            Fs[0] = qnL[1]; Fs[1] = qnL[1] * wL[1] + wL[3]; Fs[2] = qnL[2] * wL[1];
+            Fs[3] = wL[1] * (cp * wL[3] + R(0.5) * (qnL[1] * wL[1] + qnL[2] * wL[2]));
+        } else if ((pstar > wL[3]) && (wL[1] < sh)) {
+            rs = wL[0] * ((pstar / wL[3] + (g - R(1)) / (g + R(1))) / (pstar / wL[3] * (g - R(1)) / (g + R(1)) + R(1)));
+            Fs[0] = rs * us; Fs[1] = rs * us * us + pstar; Fs[2] = rs * us * wL[2];
+            Fs[3] = us * (cp * pstar + R(0.5) * rs * (us * us + wL[2] * wL[2]));
+        }
+    } else {
+        if ((pstar <= wR[3]) && (wR[1] + wR[5] <= R(0))) {
+            Fs[0] = qnR[1]; Fs[1] = qnR[1] * wR[1] + wR[3]; Fs[2] = qnR[2] * wR[1];
+            Fs[3] = wR[1] * (cp * wR[3] + R(0.5) * (qnR[1] * wR[1] + qnR[2] * wR[2]));
+        } else if ((pstar <= wR[3]) && (wR[1] + wR[5] >= R(0)) && (us + wR[5] * dpow(pstar / wR[3], R(0.5) / cp) >= R(0))) {
+            rs = wR[0] * dpow(pstar / wR[3], R(1) / g);
+            Fs[0] = rs * us; Fs[1] = rs * us * us + pstar; Fs[2] = Fs[0] * wR[2];
+            Fs[3] = us * (cp * pstar + R(0.5) * rs * (us * us + wR[2] * wR[2]));
+        } else if ((pstar <= wR[3]) && (wR[1] + wR[5] > R(0)) && (us + wR[5] * dpow(pstar / wR[3], R(0.5) / cp) < R(0))) {
+            uF = -R(2) * wR[5] / (g + R(1)) + (g - R(1)) / (g + R(1)) * wR[1];
+            rs = wR[0] * dpow(-uF / wR[5], R(2) * cv);
+            pF = wR[3] * dpow(rs / wR[0], g);
+            Fs[0] = rs * uF; Fs[1] = rs * uF * uF + pF; Fs[2] = rs * uF * wR[2];
+            Fs[3] = uF * (cp * pF + R(0.5) * rs * (uF * uF + wR[2] * wR[2]));
+        } else if ((pstar > wR[3]) && (wR[1] + wR[5] * (R(0.5) * (g + R(1)) / g * pstar / wR[3] + R(0.5) / cp) <= R(0))) {
+            Fs[0] = qnR[1]; Fs[1] = qnR[1] * wR[1] + wR[3]; Fs[2] = qnR[2] * wR[1];
+            Fs[3] = wR[1] * (cp * wR[3] + R(0.5) * (qnR[1] * wR[1] + qnR[2] * wR[2]));
+        } else if ((pstar > wR[3]) && (wR[1] + wR[5] * ((g + R(1)) / (R(2) * g) * pstar / wR[3] + R(0.5) / cp) > R(0))) {
+            rs = wR[0] * ((pstar / wR[3] + (g - R(1)) / (g + R(1))) / (pstar / wR[3] * (g - R(1)) / (g + R(1)) + R(1)));
+            Fs[0] = rs * us; Fs[1] = rs * us * us + pstar; Fs[2] = rs * us * wR[2];
+            Fs[3] = us * (cp * pstar + R(0.5) * rs * (us * us + wR[2] * wR[2]));
+        }
+    }
+    R Ft = Fs[1] * nx - Fs[2] * ny;
+    Fs[2] = Fs[1] * ny + Fs[2] * nx;
+    F[0] = Fs[0] * ph.invSGM; F[1] = Ft * ph.invSGM; F[2] = Fs[2] * ph.invSGM; F[3] = Fs[3] * ph.invSGM;
+}
+
+// Lax-Friedrichs (Rusanov) flux; not present in the reference, defined as
+// 1/2 (Fn(qL) + Fn(qR)) - 1/2 max(|un|+a) (qR - qL), scaled by 1/(sqrt(gamma) M).
+template <typename R>
+__device__ void lf_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
+    R fL[4], gL[4], fR[4], gR[4];
+    R iL = R(1) / qL[0], iR = R(1) / qR[0];
+    R uL = qL[1] * iL, vL = qL[2] * iL, uR = qR[1] * iR, vR = qR[2] * iR;
+    R pL = ph.gm1 * (qL[3] - R(0.5) * qL[1] * uL - R(0.5) * qL[2] * vL);
+    R pR = ph.gm1 * (qR[3] - R(0.5) * qR[1] * uR - R(0.5) * qR[2] * vR);
+    fL[0] = qL[1]; fL[1] = qL[1] * uL + pL; fL[2] = qL[1] * vL; fL[3] = (qL[3] + pL) * uL;
+    gL[0] = qL[2]; gL[1] = fL[2]; gL[2] = qL[2] * vL + pL; gL[3] = (qL[3] + pL) * vL;
+    fR[0] = qR[1]; fR[1] = qR[1] * uR + pR; fR[2] = qR[1] * vR; fR[3] = (qR[3] + pR) * uR;
+    gR[0] = qR[2]; gR[1] = fR[2]; gR[2] = qR[2] * vR + pR; gR[3] = (qR[3] + pR) * vR;
+    R lL = dabs(uL * nx + vL * ny) + dsqrt(ph.gamma * pL * iL);
+    R lR = dabs(uR * nx + vR * ny) + dsqrt(ph.gamma * pR * iR);
+    R lam = lL > lR ? lL : lR;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        R fnL = fL[k] * nx + gL[k] * ny, fnR = fR[k] * nx + gR[k] * ny;
+        F[k] = (R(0.5) * (fnL + fnR) - R(0.5) * lam * (qR[k] - qL[k])) * ph.invSGM;
+    }
+}
+
+template <typename R>
+__device__ inline void riemann(int id, const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
+    switch (id) {
+    case 1: hll_flux(ph, qL, qR, nx, ny, F); break;
+    case 2: exact_flux(ph, qL, qR, nx, ny, F); break;
+    case 3: lf_flux(ph, qL, qR, nx, ny, F); break;
+    default: roe_flux(ph, qL, qR, nx, ny, F); break;
+    }
+}
+
+// ViscousFluxes.f90:39-80; dq[d*4 + eq]
+template <typename R>
+__device__ inline void viscous_flux(const Phys<R> &ph, const R *q, const R *dq, R *f, R *g) {
+    R invRho = R(1) / q[0], u = q[1] * invRho, v = q[2] * invRho, uDR = u * invRho, vDR = v * invRho;
+    const R *dx = dq, *dy = dq + 4;
+    R divV = -uDR * dx[0] + invRho * dx[1] - vDR * dy[0] + invRho * dy[2];
+    R dxT = ph.gm1 * invRho * (-q[3] * invRho * dx[0] + dx[3] + u * (u * dx[0] - dx[1]) + v * (v * dx[0] - dx[2]));
+    R dyT = ph.gm1 * invRho * (-q[3] * invRho * dy[0] + dy[3] + u * (u * dy[0] - dy[1]) + v * (v * dy[0] - dy[2]));
+    f[0] = R(0);
+    f[1] = R(2) * (-uDR * dx[0] + invRho * dx[1]) + ph.lambda * divV;
+    f[2] = -vDR * dx[0] - uDR * dy[0] + invRho * dy[1] + invRho * dx[2];
+    f[3] = u * f[1] + v * f[2] + ph.cpPr * dxT;
+    g[0] = R(0);
+    g[1] = f[2];
+    g[2] = R(2) * (-vDR * dy[0] + invRho * dy[2]) + ph.lambda * divV;
+    g[3] = u * g[1] + v * g[2] + ph.cpPr * dyT;
+}
+
+// ViscousFluxes.f90:279-318 with qB = q (the BR1/IP adiabatic wall uses the interior state)
+template <typename R>
+__device__ inline void adiabatic_flux(const Phys<R> &ph, const R *q, const R *dq, R *f, R *g) {
+    R invRho = R(1) / q[0], u = q[1] * invRho, v = q[2] * invRho, uDR = u * invRho, vDR = v * invRho;
+    const R *dx = dq, *dy = dq + 4;
+    R divV = -uDR * dx[0] + invRho * dx[1] - vDR * dy[0] + invRho * dy[2];
+    f[0] = R(0);
+    f[1] = R(2) * (-uDR * dx[0] + invRho * dx[1]) + ph.lambda * divV;
+    f[2] = -vDR * dx[0] - uDR * dy[0] + invRho * dy[1] + invRho * dx[2];
+    g[0] = R(0);
+    g[1] = f[2];
+    g[2] = R(2) * (-vDR * dy[0] + invRho * dy[2]) + ph.lambda * divV;
+    f[3] = (q[1] * f[1] + q[2] * g[1]) / q[0];
+    g[3] = (q[1] * f[2] + q[2] * g[2]) / q[0];
+}
+
+// Ghost states of the non-periodic boundaries, computed from the own trace.
+// Farfield.incf:138-376 (uB = uSB; viscous type DIRICHLET, or NEUMANN at supersonic
+// outflow -- the reference leaves it unset) and ViscousWall.incf:158-254.
+template <typename R>
+__device__ inline void ghost_state(int kind, int flags, const double *zd, const Phys<R> &ph, const R *Qt, R nx, R ny,
+                                   R *uB, R *uSB, int &neumann) {
+    neumann = 0;
+    if (kind == FK_FARFIELD) {
+        R q[4] = {(R)zd[0], (R)zd[1], (R)zd[2], (R)zd[3]};
+        R rhoI = q[0], uI = q[1] / rhoI, vI = q[2] / rhoI;
+        R pI = ph.gm1 * (q[3] - R(0.5) * q[1] * uI - R(0.5) * q[2] * vI);
+        R rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
+        R vnL = uL * nx + vL * ny;
+        R pL = ph.gm1 * (Qt[3] - R(0.5) * Qt[1] * uL - R(0.5) * Qt[2] * vL);
+        R aL = dsqrt(ph.gamma * pL / rhoL);
+        R ML = dsqrt(uL * uL + vL * vL) / aL;
+        if (vnL < R(0)) {
+            if (ML > R(1)) {
+                for (int k = 0; k < 4; ++k) uB[k] = q[k];
+            } else {
+                R pR = R(0.5) * (pL + pI - rhoL * aL * (nx * (uL - uI) + ny * (vL - vI)));
+                R rhoR = rhoI + (pR - pI) / (aL * aL);
+                R uR = uI + nx * (pR - pI) / (aL * rhoL), vR = vI + ny * (pR - pI) / (aL * rhoL);
+                uB[0] = rhoR; uB[1] = rhoR * uR; uB[2] = rhoR * vR;
+                uB[3] = ph.cv * pR + R(0.5) * (uB[1] * uR + uB[2] * vR);
+            }
+        } else {
+            if (ML > R(1)) {
+                for (int k = 0; k < 4; ++k) uB[k] = Qt[k];
+                neumann = 1;
+            } else {
+                R pR = pI;
+                R rhoR = rhoL + (pI - pL) / (aL * aL);
+                R uR = uL - nx * (pI - pL) / (rhoL * aL), vR = vL - ny * (pI - pL) / (rhoL * aL);
+                uB[0] = rhoR; uB[1] = rhoR * uR; uB[2] = rhoR * vR;
+                uB[3] = ph.cv * pR + R(0.5) * (uB[1] * uR + uB[2] * vR);
+            }
+        }
+        for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
+    } else {  // viscous wall
+        R Tw = (R)zd[0], vwx = (R)zd[1], vwy = (R)zd[2];
+        bool iso = (flags & FF_ISOTHERMAL) != 0;
+        uB[0] = Qt[0];
+        uB[1] = R(2) * uB[0] * vwx - Qt[1];
+        uB[2] = R(2) * uB[0] * vwy - Qt[2];
+        R Ek = R(0.5) * (uB[1] * uB[1] + uB[2] * uB[2]) / uB[0];
+        uB[3] = iso ? ph.cv * uB[0] * R(1) * Tw + Ek : Qt[3];
+        uSB[0] = Qt[0]; uSB[1] = uSB[0] * vwx; uSB[2] = uSB[0] * vwy;
+        uSB[3] = iso ? ph.cv * uSB[0] * R(1) * Tw + R(0.5) * (uSB[1] * uSB[1] + uSB[2] * uSB[2]) / uSB[0]
+                     : Qt[3] - Ek + R(0.5) * uSB[0] * (vwx * vwx + vwy * vwy);
+    }
+}
+
+// ------------------------------------------------------------------ spectral helpers
+// Trace of a face from the staged element state (DGSpatialDiscretizationMethods.f90:1520-1574).
+// Always accumulated in FP64 with explicit fma in l order so that every kernel that forms
+// a trace (prolong, stage epilogue, stage recomputation) produces the same bits.
+template <int P, int STRIDE, int NV>
+__device__ inline void face_trace(const double *sX /* element block, [n*STRIDE + c] */, const double *sLb, int f, int k,
+                                  double *out) {
+    const double *lb = (f == 0 || f == 3) ? sLb : sLb + P;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) out[c] = 0.0;
+#pragma unroll
+    for (int l = 0; l < P; ++l) {
+        int n = (f == 0 || f == 2) ? (k + P * l) : (l + P * k);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) out[c] = fma(sX[n * STRIDE + c], lb[l], out[c]);
+    }
+}
+
+}  // namespace hg

@@ -185,6 +185,13 @@ int hg_last_error(char *buf, int32_t n);
 int hg_profile_enable(hg_handle *h, int32_t on);
 int hg_profile_read(hg_handle *h, int32_t which, int64_t *launches, double *ms);
 
+/* Workgroup tile of the stage and gradient kernels: -1 = choose the faster at the
+ * first evaluation (timed derivative-only launches), 0 = narrow (elements that fit
+ * one wavefront), 1 = wide (up to four wavefronts).  Results are bit-identical for
+ * every tile.  No reference counterpart (performance control only). */
+int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile);
+int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile);
+
 /* ------------------------------------------------------------------------ */
 /* Batched form: many independent cases advanced together, one persistent
  * launch per RK stage over a (case, element-tile) work queue, captured into a

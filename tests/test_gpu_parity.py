@@ -59,10 +59,12 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("tile", [0, 1])
 @pytest.mark.parametrize("name", list(CASES))
-def test_time_derivative_matches_oracle(name):
+def test_time_derivative_matches_oracle(name, tile):
     case = CASES[name]()
     dg, orc = pair(case)
+    dg.set_tiles(tile, tile)
     qd_g = dg.time_derivative(0.0)
     qd_o = orc.time_derivative(0.0)
     assert rel(qd_g, qd_o) <= TOL, name
@@ -76,6 +78,25 @@ def test_rk3_steps_match_oracle(name):
     run_both(dg, orc, case, 10)
     assert rel(dg.state(), orc.state()) <= TOL, name
     np.testing.assert_allclose(dg.residuals(), orc.residuals(), rtol=1e-10)
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_workgroup_tiles_are_bit_identical(name):
+    """The narrow and wide workgroup tiles evaluate the same expressions per node and face."""
+    from horses2d_amd import gpu
+    case = CASES[name]()
+    runs = []
+    for tile in (0, 1):
+        dg = gpu.DGSEM(case, "fp64")
+        dg.set_tiles(tile, tile)
+        dg.upload(case.initial_state(dg.coords()))
+        dt = dg.estimate_dt(case.dt_max, case.cfl)
+        for it in range(3):
+            dg.rk_step(it * dt, dt)
+        runs.append((dg.state(), dg.residuals()))
+        assert dg.tiles() == (tile, tile)
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
 
 
 def test_euler_vortex_full_config_200_steps():
