@@ -301,3 +301,94 @@ def test_time_step_estimate_closed_form_and_saved_maxima():
     # a slower, cooler state afterwards does not raise dt: the running maxima are kept
     o.set_state(case.initial_state(o.coords()) * np.array([1.0, 0.5, 0.5, 0.5] * (o.ndof // 4)))
     assert o.estimate_dt(1e30, 0.3) == pytest.approx(dt, rel=1e-15)
+
+
+def _trig_state(x, y):
+    """A smooth periodic conservative state on [0, 2pi]^2 and its exact gradient."""
+    s, c = np.sin, np.cos
+    q = np.stack([1.0 + 0.1 * s(x) * c(y),
+                  0.3 + 0.1 * s(y) + 0.05 * c(x),
+                  -0.2 + 0.1 * c(x) * s(y),
+                  2.5 + 0.1 * c(x + y) + 0.05 * s(x)], -1)
+    qx = np.stack([0.1 * c(x) * c(y), -0.05 * s(x), -0.1 * s(x) * s(y), -0.1 * s(x + y) + 0.05 * c(x)], -1)
+    qy = np.stack([-0.1 * s(x) * s(y), 0.1 * c(y), 0.1 * c(x) * c(y), -0.1 * s(x + y)], -1)
+    return q, qx, qy
+
+
+def _total_flux(x, y, Mach, Re, Pr):
+    q, qx, qy = _trig_state(x, y)
+    fi, gi = O.inviscid_flux(Mach, q)
+    dq = np.concatenate([qx, qy], -1)                    # [d/dx (4), d/dy (4)] per node
+    fv, gv = O.viscous_flux(Re, Pr, q, dq)
+    return fi - fv / Re, gi - gv / Re
+
+
+@pytest.mark.parametrize("viscous,ip", [("br1", "sipg"), ("ip", "sipg"), ("ip", "nipg"), ("ip", "iipg")])
+def test_navier_stokes_residual_converges_to_the_analytic_divergence(viscous, ip):
+    """Whole Navier-Stokes residual (volume, Riemann, BR1 or IP viscous faces, lifts, /jac)
+    against the analytic -div(F - mu Fv) of a smooth periodic state, built from the pinned
+    point fluxes; the viscous term is O(1) at Re = 10, so a wrong sign, factor or lift of any
+    viscous contribution leaves an error that does not converge."""
+    Mach, Re, Pr = 0.3, 10.0, 0.72
+    errs = []
+    for N in (3, 5, 7):
+        mesh = C.periodic_rectangle(4, 4, 0.0, 2.0 * np.pi, 0.0, 2.0 * np.pi)
+        case = C.Case("trig", mesh, np.full(mesh.K, N, np.int32), Mach, Re, Pr, True, riemann="roe",
+                      viscous=viscous, ip_method=ip, sigma0=1.0,
+                      ic=lambda x, y: _trig_state(x, y)[0])
+        o, X = oracle_for(case)
+        qd = o.time_derivative().reshape(-1, 4)
+        h = 1e-4
+        x, y = X[:, 0], X[:, 1]
+        fp, _ = _total_flux(x + h, y, Mach, Re, Pr)
+        fm, _ = _total_flux(x - h, y, Mach, Re, Pr)
+        _, gp = _total_flux(x, y + h, Mach, Re, Pr)
+        _, gm = _total_flux(x, y - h, Mach, Re, Pr)
+        exact = -((fp - fm) + (gp - gm)) / (2 * h)
+        errs.append(np.abs(qd - exact).max() / np.abs(exact).max())
+    assert errs[-1] < 1e-5, errs
+    assert errs[0] / errs[-1] > 100.0, errs
+
+
+def _viscous_operator(viscous, ip, mom, N=4, n=3):
+    """Momentum part of QDot(Re=10) - QDot(Re=0) at rho = 1, rhoe = 2.5: exactly linear in the
+    momentum field (u = m at unit density) and equal to mu times the discrete viscous operator."""
+    mesh = C.periodic_rectangle(n, n, 0.0, 2.0 * np.pi, 0.0, 2.0 * np.pi)
+    out = []
+    for Re in (10.0, 0.0):
+        case = C.Case("sym", mesh, np.full(mesh.K, N, np.int32), 0.3, Re, 0.72, True, riemann="roe",
+                      viscous=viscous, ip_method=ip)
+        o = O.Oracle(case)
+        Q = np.zeros((mom.shape[0], 4)); Q[:, 0] = 1.0; Q[:, 3] = 2.5; Q[:, 1:3] = mom
+        o.set_state(Q.reshape(-1))
+        out.append(o.time_derivative().reshape(-1, 4)[:, 1:3].copy())
+        _, jac = o.metrics()
+    s = O.spectral(N)
+    W = np.tile(np.outer(s["w"], s["w"]).T.reshape(-1), n * n) * jac
+    return out[0] - out[1], W
+
+
+def test_viscous_operator_symmetry_and_ip_linearity_in_epsilon():
+    """BR1 (central uStar and central viscous flux) is a symmetric discretisation of the stress
+    divergence: <a, V b> = <b, V a> in the quadrature inner product.  The IP family is linear in
+    epsilon (SIPG -1, NIPG +1, IIPG 0: V_sipg + V_nipg = 2 V_iipg).  The IP gradient-face term
+    follows the reference, whose RIGHT-side coefficient is -epsilon/2 (DGViscous_IP.f90:245-246)
+    while both sides subtract their lifted term (DGSpatialDiscretizationMethods.f90:307,312),
+    so its SIPG is not symmetric; flipping that one sign makes it symmetric to rounding."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    nn = 9 * 25
+    a = 1e-3 * rng.normal(size=(nn, 2)); b = 1e-3 * rng.normal(size=(nn, 2))
+
+    def asym(viscous, ip):
+        Va, W = _viscous_operator(viscous, ip, a)
+        Vb, _ = _viscous_operator(viscous, ip, b)
+        sab, sba = np.sum(W[:, None] * a * Vb), np.sum(W[:, None] * b * Va)
+        return abs(sab - sba) / (abs(sab) + abs(sba)), Va
+
+    br1, _ = asym("br1", "sipg")
+    assert br1 < 1e-11
+    s, Vs = asym("ip", "sipg")
+    ni, Vn = asym("ip", "nipg")
+    ii, Vi = asym("ip", "iipg")
+    np.testing.assert_allclose(Vs + Vn, 2 * Vi, rtol=0, atol=1e-10 * np.abs(Vi).max())
+    assert ii > 1e-2 and ni > 1e-2 and s > 1e-3, (ii, ni, s)
