@@ -27,6 +27,8 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 def _units():
     units = [(s, os.path.splitext(s)[0] + ".o", []) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     units += [("hg_inst.hip", f"hg_inst_p{p}.o", [f"-DHG_P={p}"]) for p in ORDERS]
+    # batched kernels: 2 * physics (Euler, NS BR1, NS IP) + mixed
+    units += [("hg_batch_inst.hip", f"hg_batch_v{v}.o", [f"-DHG_BV={v}"]) for v in range(6)]
     return units
 
 

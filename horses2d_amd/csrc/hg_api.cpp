@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "hg_device.h"
+#include "hg_handle.h"
 #include "hg_mesh.h"
 #include "horses2d_gpu.h"
 
@@ -29,62 +30,9 @@ int set_err(const std::string &m) {
         if (_e != hipSuccess) return set_err(std::string(#call) + ": " + hipGetErrorString(_e)); \
     } while (0)
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
 
 }  // namespace
 
-struct hg_handle {
-    hg::Mesh mesh;
-    int K = 0, N = 0, P = 0;
-    int64_t nodes = 0, ndof = 0;
-    bool mixed = false;
-    hg_options opt{};
-    hg_physics phys{};
-    hg::PhysParams ph{};
-    hipStream_t stream = nullptr;
-    std::vector<DevBuf> bufs;
-    // device arrays
-    double *Q = nullptr, *G = nullptr, *QDot = nullptr;
-    void *tr[2] = {nullptr, nullptr};
-    void *dQ = nullptr, *dtr = nullptr;
-    void *metric = nullptr, *fmet = nullptr;
-    int4 *face = nullptr;
-    int *forder = nullptr;
-    double *egeo = nullptr, *einvh = nullptr, *zone = nullptr;
-    double *hatD = nullptr, *D = nullptr, *lb = nullptr, *lbw = nullptr, *dlbw = nullptr, *w = nullptr;
-    double *jac64 = nullptr;
-    double *dx = nullptr, *umax = nullptr, *amax = nullptr, *dtstate = nullptr, *dtout = nullptr;
-    double *cons_partial = nullptr, *cons_out = nullptr;
-    int cons_nb = 0;
-    double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
-    int cur = 0;
-    int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet
-    // profiling
-    bool prof = false;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
-    int64_t prof_n[2] = {0, 0};
-    double prof_ms[2] = {0.0, 0.0};
-
-    template <typename T>
-    hipError_t alloc(T **p, size_t count) {
-        void *raw = nullptr;
-        hipError_t e = hipMalloc(&raw, std::max<size_t>(count, 1) * sizeof(T));
-        if (e != hipSuccess) return e;
-        bufs.push_back({raw, count * sizeof(T)});
-        *p = (T *)raw;
-        return hipSuccess;
-    }
-    hipError_t alloc_raw(void **p, size_t bytes) {
-        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
-        if (e != hipSuccess) return e;
-        bufs.push_back({*p, bytes});
-        return hipSuccess;
-    }
-};
 
 namespace {
 
@@ -231,6 +179,11 @@ int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool la
 
 }  // namespace
 
+namespace hg {
+StageArgs handle_args(hg_handle *h) { return base_args(h); }
+int handle_error(const std::string &m) { return set_err(m); }
+}  // namespace hg
+
 extern "C" {
 
 int hg_last_error(char *buf, int32_t n) {
@@ -274,6 +227,8 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     ph.mu = phys->reynolds == 0.0 ? 0.0 : 1.0 / phys->reynolds;
     ph.cpPr = 3.5 / phys->prandtl;
     ph.sigma0 = opt->sigma0;
+    ph.pref = 0.0;
+    ph.Eref = 0.0;
     ph.riemann = opt->riemann;
     ph.viscous = opt->navier_stokes ? opt->viscous : 0;
     ph.ns = opt->navier_stokes ? 1 : 0;
@@ -525,6 +480,17 @@ int hg_build_geometry(const hg_mesh_desc *mesh, int64_t nnodes, double *X, doubl
 int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     if (!h || !Q) return set_err("hg_upload_state: null argument");
     if (ndof != h->ndof) return set_err("hg_upload_state: ndof mismatch");
+    if (h->mixed) {
+        // Reference pressure of the FP32 form: the mean nodal pressure of the uploaded state.
+        double sum = 0.0;
+        for (int64_t n = 0; n < h->nodes; ++n) {
+            const double *q = Q + 4 * n;
+            sum += h->ph.gm1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+        }
+        const double pref = h->nodes > 0 ? sum / (double)h->nodes : 0.0;
+        h->ph.pref = std::isfinite(pref) ? pref : 0.0;
+        h->ph.Eref = h->ph.pref / h->ph.gm1;
+    }
     HG_CHECK(hipMemcpyAsync(h->Q, Q, ndof * sizeof(double), hipMemcpyHostToDevice, h->stream));
     hg::StageArgs a = base_args(h);
     h->cur = 0;

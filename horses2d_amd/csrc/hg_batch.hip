@@ -1,43 +1,331 @@
-// Batched form: many independent cases advanced together.
-// First version: per-stage launches of every case on the batch's own stream.
+// Batched form: many independent cases advanced together (host side).
+//
+// Every RK stage of every case is one entry of a per-case slot table (StageArgs with the
+// case's buffers, the stage's RK coefficients and the trace buffers of the current parity).
+// A work list holds one entry per element tile (case, first element, order, residual slot);
+// one launch per stage evaluates all of them, and the step (all stages, the gradient passes
+// and the per-case residual reduction) is captured once per trace parity into a HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
+#include "hg_device.h"
+#include "hg_handle.h"
 #include "horses2d_gpu.h"
+
+namespace {
+
+#define HB_CHECK(call)                                                                                     \
+    do {                                                                                                   \
+        hipError_t _e = (call);                                                                            \
+        if (_e != hipSuccess) return hg::handle_error(std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+// Williamson coefficients (DGTimeIntegrator.f90:263-365), as in hg_rk_step.
+struct Scheme {
+    int nst;
+    double am[5], cq[5], cg[5];
+    int dt_cq, dt_cg;
+};
+
+Scheme scheme_of(int s) {
+    Scheme r{};
+    if (s == HG_SCHEME_RK3) {
+        r.nst = 3;
+        const double am[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0}, gm[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+        for (int i = 0; i < 3; ++i) { r.am[i] = am[i]; r.cq[i] = 1.0; r.cg[i] = gm[i]; }
+        r.dt_cq = 0; r.dt_cg = 1;
+    } else if (s == HG_SCHEME_RK5) {
+        r.nst = 5;
+        const double am[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+        const double gm[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+        for (int i = 0; i < 5; ++i) { r.am[i] = am[i]; r.cq[i] = 1.0; r.cg[i] = gm[i]; }
+        r.dt_cq = 1; r.dt_cg = 0;
+    } else {
+        r.nst = 1;
+        r.am[0] = 0.0; r.cq[0] = 1.0; r.cg[0] = 1.0;
+        r.dt_cq = 0; r.dt_cg = 1;
+    }
+    return r;
+}
+
+}  // namespace
 
 struct hg_batch {
     std::vector<hg_handle *> cases;
     hg_batch_options opt{};
+    Scheme sch{};
+    int variant = 0, nslots = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> bufs;
+    hg::StageArgs *slots = nullptr;        // [case][nslots]
+    int4 *items[2] = {nullptr, nullptr};   // per tile
+    int nitems[2] = {0, 0};
+    int block[2] = {0, 0};
+    int *sched[2][3] = {};                 // per tile, per order
+    int2 *ranges[2] = {nullptr, nullptr};  // per tile: per case residual partial range
+    double **resid_out = nullptr;          // per case: the handle's residual slots
+    double *dt = nullptr;                  // per case
+    double *resid_part = nullptr;
+    int tile = -1, order = -1;
+    int parity = 0;
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+
+    template <typename T>
+    hipError_t upload(T **dst, const std::vector<T> &src) {
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(src.size(), 1) * sizeof(T));
+        if (e != hipSuccess) return e;
+        bufs.push_back(p);
+        *dst = (T *)p;
+        return src.empty() ? hipSuccess : hipMemcpy(p, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    }
 };
+
+namespace {
+
+hg::BatchArgs args_of(const hg_batch *b, int tile, int order, int slot) {
+    hg::BatchArgs a{};
+    a.slots = b->slots;
+    a.nslots = b->nslots;
+    a.slot = slot;
+    a.items = b->items[tile];
+    a.sched = b->sched[tile][order];
+    a.nitems = b->nitems[tile];
+    a.dt = b->dt;
+    a.dt_cq = b->sch.dt_cq;
+    a.dt_cg = b->sch.dt_cg;
+    return a;
+}
+
+// One RK step of all cases for the given trace parity (launches only).
+hipError_t enqueue_step(hg_batch *b, int parity) {
+    const bool ns = b->cases[0]->ph.ns != 0;
+    for (int s = 0; s < b->sch.nst; ++s) {
+        const int slot = parity * b->sch.nst + s;
+        hg::BatchArgs a = args_of(b, b->tile, b->order, slot);
+        hipError_t e;
+        if (ns && (e = hg::launch_batch_gradient(b->variant, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess)
+            return e;
+        if ((e = hg::launch_batch_stage(b->variant, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess) return e;
+    }
+    return hg::launch_batch_resid(b->resid_part, b->ranges[b->tile], b->resid_out, (int)b->cases.size(), b->stream);
+}
+
+// Times derivative-only evaluations (plus the gradient pass) of every (tile, order)
+// candidate not fixed by the options and keeps the fastest.
+int autotune(hg_batch *b) {
+    const bool ns = b->cases[0]->ph.ns != 0;
+    const int dslot = 2 * b->sch.nst;
+    hipEvent_t ev[2];
+    HB_CHECK(hipEventCreate(&ev[0]));
+    HB_CHECK(hipEventCreate(&ev[1]));
+    float best = 1e30f;
+    int bt = 1, bo = 0;
+    for (int t = 0; t < 2; ++t) {
+        if (b->opt.tile >= 0 && t != b->opt.tile) continue;
+        for (int o = 0; o < 3; ++o) {
+            if (b->opt.order >= 0 && o != b->opt.order) continue;
+            hg::BatchArgs a = args_of(b, t, o, dslot);
+            auto run = [&]() -> hipError_t {
+                hipError_t e = hipSuccess;
+                if (ns) e = hg::launch_batch_gradient(b->variant, t, a, b->block[t], b->stream);
+                if (e == hipSuccess) e = hg::launch_batch_stage(b->variant, t, a, b->block[t], b->stream);
+                return e;
+            };
+            HB_CHECK(run());
+            HB_CHECK(hipEventRecord(ev[0], b->stream));
+            for (int r = 0; r < 3; ++r) HB_CHECK(run());
+            HB_CHECK(hipEventRecord(ev[1], b->stream));
+            HB_CHECK(hipEventSynchronize(ev[1]));
+            float ms = 0.f;
+            HB_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            if (ms < best) { best = ms; bt = t; bo = o; }
+        }
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    b->tile = bt;
+    b->order = bo;
+    return 0;
+}
+
+int build_graphs(hg_batch *b) {
+    for (int p = 0; p < 2; ++p) {
+        HB_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = enqueue_step(b, p);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+        HB_CHECK(e);
+        HB_CHECK(e2);
+        b->graph[p] = g;
+        HB_CHECK(hipGraphInstantiate(&b->exec[p], g, nullptr, nullptr, 0));
+    }
+    return 0;
+}
+
+}  // namespace
 
 extern "C" {
 
 int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_options *opt, hg_batch **out) {
-    if (!cases || ncases <= 0 || !out) return 1;
-    hg_batch *b = new hg_batch();
+    if (!cases || ncases <= 0 || !out) return hg::handle_error("hg_batch_create: bad argument");
+    *out = nullptr;
+    hg_batch_options o{-1, -1, 1};
+    if (opt) o = *opt;
+    if (o.tile < -1 || o.tile > 1 || o.order < -1 || o.order > 2)
+        return hg::handle_error("hg_batch_create: tile must be -1..1 and order -1..2");
+    hg_handle *h0 = cases[0];
+    for (int c = 0; c < ncases; ++c) {
+        hg_handle *h = cases[c];
+        if (!h) return hg::handle_error("hg_batch_create: null case handle");
+        if (h->ph.ns != h0->ph.ns || h->ph.viscous != h0->ph.viscous || h->mixed != h0->mixed ||
+            h->opt.scheme != h0->opt.scheme)
+            return hg::handle_error("hg_batch_create: the cases of a batch must share equations, viscous method, "
+                                    "precision and time scheme");
+        for (int k = 0; k < c; ++k)
+            if (cases[k] == h) return hg::handle_error("hg_batch_create: a case appears twice");
+    }
+    std::unique_ptr<hg_batch> b(new hg_batch());
     b->cases.assign(cases, cases + ncases);
-    if (opt) b->opt = *opt;
-    *out = b;
+    b->opt = o;
+    b->sch = scheme_of(h0->opt.scheme);
+    b->variant = 2 * (h0->ph.ns ? (h0->ph.viscous == 2 ? 2 : 1) : 0) + (h0->mixed ? 1 : 0);
+    b->nslots = 2 * b->sch.nst + 1;
+    HB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto *h : b->cases) HB_CHECK(hipStreamSynchronize(h->stream));
+
+    // work lists per tile: (case, first element, P, residual slot); residual ranges per case
+    size_t max_items = 0;
+    std::vector<int4> items[2];
+    std::vector<int2> ranges[2];
+    for (int t = 0; t < 2; ++t) {
+        int blk = 64;
+        for (int c = 0; c < ncases; ++c) {
+            const hg_handle *h = cases[c];
+            const int epb = hg::tile_elems(h->P, t);
+            if (epb <= 0) return hg::handle_error("hg_batch_create: polynomial order not compiled");
+            blk = std::max(blk, hg::tile_threads(h->P, t));
+            const int first = (int)items[t].size();
+            for (int e0 = 0; e0 < h->K; e0 += epb) items[t].push_back(make_int4(c, e0, h->P, (int)items[t].size()));
+            ranges[t].push_back(make_int2(first, (int)items[t].size()));
+        }
+        b->nitems[t] = (int)items[t].size();
+        b->block[t] = blk;
+        max_items = std::max(max_items, items[t].size());
+        HB_CHECK(b->upload(&b->items[t], items[t]));
+        HB_CHECK(b->upload(&b->ranges[t], ranges[t]));
+        // orders: case-major; costliest tiles first (work ~ elements * P^3); cases interleaved
+        const int n = b->nitems[t];
+        std::vector<int> s0(n), s1(n), s2;
+        std::iota(s0.begin(), s0.end(), 0);
+        s1 = s0;
+        auto cost = [&](int i) {
+            const int4 v = items[t][i];
+            const hg_handle *h = cases[v.x];
+            const int ne = std::min(hg::tile_elems(h->P, t), h->K - v.y);
+            return (double)ne * h->P * h->P * h->P;
+        };
+        std::stable_sort(s1.begin(), s1.end(), [&](int a, int c) { return cost(a) > cost(c); });
+        s2.reserve(n);
+        std::vector<int> pos(ncases);
+        for (int c = 0; c < ncases; ++c) pos[c] = ranges[t][c].x;
+        for (bool more = true; more;) {
+            more = false;
+            for (int c = 0; c < ncases; ++c)
+                if (pos[c] < ranges[t][c].y) { s2.push_back(pos[c]++); more = true; }
+        }
+        HB_CHECK(b->upload(&b->sched[t][0], s0));
+        HB_CHECK(b->upload(&b->sched[t][1], s1));
+        HB_CHECK(b->upload(&b->sched[t][2], s2));
+    }
+    std::vector<double> zeros(4 * std::max<size_t>(max_items, 1), 0.0);
+    HB_CHECK(b->upload(&b->resid_part, zeros));
+    std::vector<double> dt0(ncases, 0.0);
+    HB_CHECK(b->upload(&b->dt, dt0));
+    std::vector<double *> rout(ncases);
+    for (int c = 0; c < ncases; ++c) rout[c] = cases[c]->resid;
+    HB_CHECK(b->upload(&b->resid_out, rout));
+
+    // slot table: parity p, stage s -> p*nst + s ; derivative-only slot 2*nst
+    std::vector<hg::StageArgs> slots((size_t)ncases * b->nslots);
+    for (int c = 0; c < ncases; ++c) {
+        hg_handle *h = cases[c];
+        for (int p = 0; p < 2; ++p)
+            for (int s = 0; s < b->sch.nst; ++s) {
+                hg::StageArgs a = hg::handle_args(h);
+                const int in = h->cur ^ p ^ (s & 1);
+                a.tr_in = h->tr[in];
+                a.tr_out = h->tr[in ^ 1];
+                a.am = b->sch.am[s];
+                a.cq = b->sch.cq[s];
+                a.cg = b->sch.cg[s];
+                a.first_stage = s == 0 ? 1 : 0;
+                a.mode = 0;
+                a.do_resid = s == b->sch.nst - 1 ? 1 : 0;
+                a.resid_part = b->resid_part;
+                slots[(size_t)c * b->nslots + p * b->sch.nst + s] = a;
+            }
+        hg::StageArgs d = hg::handle_args(h);
+        d.tr_in = h->tr[h->cur];
+        d.tr_out = h->tr[h->cur ^ 1];
+        d.cq = 1.0;
+        d.first_stage = 1;
+        d.mode = 1;
+        d.do_resid = 0;
+        d.resid_part = b->resid_part;
+        slots[(size_t)c * b->nslots + 2 * b->sch.nst] = d;
+    }
+    HB_CHECK(b->upload(&b->slots, slots));
+    HB_CHECK(hipDeviceSynchronize());
+    *out = b.release();
     return 0;
 }
 
 int hg_batch_rk_step(hg_batch *b, const double *t, const double *dt) {
-    if (!b || !t || !dt) return 1;
-    for (size_t c = 0; c < b->cases.size(); ++c)
-        if (hg_rk_step(b->cases[c], t[c], dt[c])) return 1;
+    (void)t;  // no time-dependent boundary data on the GPU path
+    if (!b || !dt) return hg::handle_error("hg_batch_rk_step: bad argument");
+    for (auto *h : b->cases) HB_CHECK(hipStreamSynchronize(h->stream));
+    HB_CHECK(hipMemcpyAsync(b->dt, dt, b->cases.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    if (b->tile < 0) {
+        if (b->parity != 0) return hg::handle_error("hg_batch_rk_step: internal parity error");
+        if (autotune(b)) return 1;
+        if (b->opt.use_graph && build_graphs(b)) return 1;
+    }
+    if (b->opt.use_graph) HB_CHECK(hipGraphLaunch(b->exec[b->parity], b->stream));
+    else HB_CHECK(enqueue_step(b, b->parity));
+    b->parity ^= 1;
+    for (auto *h : b->cases) h->cur ^= 1;   // every scheme has an odd number of stages
     return 0;
 }
 
 int hg_batch_synchronize(hg_batch *b) {
-    if (!b) return 1;
-    for (auto *h : b->cases)
-        if (hg_synchronize(h)) return 1;
+    if (!b) return hg::handle_error("hg_batch_synchronize: null batch");
+    HB_CHECK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int hg_batch_schedule(hg_batch *b, int32_t *tile, int32_t *order) {
+    if (!b) return hg::handle_error("hg_batch_schedule: null batch");
+    if (tile) *tile = b->tile;
+    if (order) *order = b->order;
     return 0;
 }
 
 int hg_batch_destroy(hg_batch *b) {
+    if (!b) return 0;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (int p = 0; p < 2; ++p) {
+        if (b->exec[p]) hipGraphExecDestroy(b->exec[p]);
+        if (b->graph[p]) hipGraphDestroy(b->graph[p]);
+    }
+    for (void *p : b->bufs) hipFree(p);
+    if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return 0;
 }

@@ -1,0 +1,35 @@
+// One physics/precision variant of the batched kernels: compiled once per variant with
+// -DHG_BV=<2*physics + mixed> (physics 0 Euler, 1 Navier-Stokes BR1, 2 Navier-Stokes IP).
+#include <type_traits>
+
+#include "hg_batch_kernels.h"
+
+#ifndef HG_BV
+#error "compile with -DHG_BV=<variant>"
+#endif
+
+#define HG_CAT2(a, b) a##b
+#define HG_CAT(a, b) HG_CAT2(a, b)
+
+namespace hg {
+
+namespace {
+using BvR = std::conditional_t<(HG_BV % 2) == 1, float, double>;
+constexpr bool kBvNS = (HG_BV / 2) >= 1;
+constexpr bool kBvIP = (HG_BV / 2) == 2;
+}  // namespace
+
+hipError_t HG_CAT(batch_stage_v, HG_BV)(int tile, const BatchArgs &b, int block, hipStream_t st) {
+    return batch_stage_variant<kBvNS, kBvIP, BvR, BvR>(tile, b, block, st);
+}
+
+hipError_t HG_CAT(batch_gradient_v, HG_BV)(int tile, const BatchArgs &b, int block, hipStream_t st) {
+    if constexpr (!kBvNS) {
+        (void)tile; (void)b; (void)block; (void)st;
+        return hipErrorInvalidValue;
+    } else {
+        return batch_gradient_variant<!kBvIP, BvR, BvR>(tile, b, block, st);
+    }
+}
+
+}  // namespace hg

@@ -1,0 +1,146 @@
+// Batched form: one launch per RK stage (and per gradient pass) over a work list of element
+// tiles drawn from many independent cases of different mesh size and polynomial order.
+// Workgroup k evaluates work item sched[k] with the same stage_body / gradient_body as the
+// single-case kernels, so the per-element arithmetic, and therefore every result bit, is the
+// one of the single-case path.  The LDS of the workgroup is the union of the per-order tile
+// layouts.  A grid-stride (persistent) loop over the list was measured to raise the register
+// allocation from 149 to 213-256 VGPRs (occupancy 3 -> 2 or 1 wavefronts per SIMD): the
+// compiler keeps every order's loop-invariant lane index math live across iterations.  So
+// the launch has one workgroup per work item and the hardware dispatcher walks the list in
+// the order sched gives it.
+#pragma once
+#include "hg_kernels.h"
+
+namespace hg {
+
+template <int P, int TILE>
+struct TileOf {
+    static constexpr int EPB = TILE == 0 ? Tiles<P>::small : Tiles<P>::large;
+};
+
+template <int TILE, bool IP, typename R>
+union BatchStageSmem {
+    StageSmem<2, TileOf<2, TILE>::EPB, IP, R> p2;
+    StageSmem<3, TileOf<3, TILE>::EPB, IP, R> p3;
+    StageSmem<4, TileOf<4, TILE>::EPB, IP, R> p4;
+    StageSmem<5, TileOf<5, TILE>::EPB, IP, R> p5;
+    StageSmem<6, TileOf<6, TILE>::EPB, IP, R> p6;
+    StageSmem<7, TileOf<7, TILE>::EPB, IP, R> p7;
+    StageSmem<8, TileOf<8, TILE>::EPB, IP, R> p8;
+    StageSmem<9, TileOf<9, TILE>::EPB, IP, R> p9;
+    StageSmem<10, TileOf<10, TILE>::EPB, IP, R> p10;
+    StageSmem<11, TileOf<11, TILE>::EPB, IP, R> p11;
+};
+
+template <int TILE, bool BR1, typename R>
+union BatchGradSmem {
+    GradSmem<2, TileOf<2, TILE>::EPB, BR1, R> p2;
+    GradSmem<3, TileOf<3, TILE>::EPB, BR1, R> p3;
+    GradSmem<4, TileOf<4, TILE>::EPB, BR1, R> p4;
+    GradSmem<5, TileOf<5, TILE>::EPB, BR1, R> p5;
+    GradSmem<6, TileOf<6, TILE>::EPB, BR1, R> p6;
+    GradSmem<7, TileOf<7, TILE>::EPB, BR1, R> p7;
+    GradSmem<8, TileOf<8, TILE>::EPB, BR1, R> p8;
+    GradSmem<9, TileOf<9, TILE>::EPB, BR1, R> p9;
+    GradSmem<10, TileOf<10, TILE>::EPB, BR1, R> p10;
+    GradSmem<11, TileOf<11, TILE>::EPB, BR1, R> p11;
+};
+
+#define HG_BATCH_CASE(PV, CALL) \
+    case PV: CALL(PV, p##PV); break;
+
+// The work item is uniform across the workgroup: read it into scalar registers, so that the
+// case's StageArgs is addressed uniformly and its fields load through the scalar cache.
+__device__ inline int4 uniform_item(const int4 *__restrict__ items, const int *__restrict__ sched, int k) {
+    const int idx = __builtin_amdgcn_readfirstlane(sched[k]);
+    const int4 v = items[idx];
+    return make_int4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                     __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+}
+
+template <int TILE, bool NS, bool IP, typename R, typename TR>
+__global__ __launch_bounds__(256) void k_batch_stage(const StageArgs *__restrict__ slots, const int4 *__restrict__ items,
+                                                     const int *__restrict__ sched, const double *__restrict__ dt,
+                                                     BatchArgs b) {
+    __shared__ BatchStageSmem<TILE, IP, R> S;
+    const int k = blockIdx.x;
+    if (k < b.nitems) {
+        const int4 it = uniform_item(items, sched, k);
+        const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
+        double cq = a.cq, cg = a.cg;
+        if (dt) {
+            const double d = dt[it.x];
+            if (b.dt_cq) cq *= d;
+            if (b.dt_cg) cg *= d;
+        }
+#define HG_CALL(PV, M) stage_body<PV, TileOf<PV, TILE>::EPB, NS, IP, R, TR>(a, it.y, it.w, cq, cg, S.M)
+        switch (it.z) {
+            HG_BATCH_CASE(2, HG_CALL)
+            HG_BATCH_CASE(3, HG_CALL)
+            HG_BATCH_CASE(4, HG_CALL)
+            HG_BATCH_CASE(5, HG_CALL)
+            HG_BATCH_CASE(6, HG_CALL)
+            HG_BATCH_CASE(7, HG_CALL)
+            HG_BATCH_CASE(8, HG_CALL)
+            HG_BATCH_CASE(9, HG_CALL)
+            HG_BATCH_CASE(10, HG_CALL)
+            HG_BATCH_CASE(11, HG_CALL)
+            default: break;
+        }
+#undef HG_CALL
+    }
+}
+
+template <int TILE, bool BR1, typename R, typename TR>
+__global__ __launch_bounds__(256) void k_batch_gradient(const StageArgs *__restrict__ slots,
+                                                        const int4 *__restrict__ items,
+                                                        const int *__restrict__ sched, BatchArgs b) {
+    __shared__ BatchGradSmem<TILE, BR1, R> S;
+    const int k = blockIdx.x;
+    if (k < b.nitems) {
+        const int4 it = uniform_item(items, sched, k);
+        const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
+#define HG_CALL(PV, M) gradient_body<PV, TileOf<PV, TILE>::EPB, BR1, R, TR>(a, it.y, S.M)
+        switch (it.z) {
+            HG_BATCH_CASE(2, HG_CALL)
+            HG_BATCH_CASE(3, HG_CALL)
+            HG_BATCH_CASE(4, HG_CALL)
+            HG_BATCH_CASE(5, HG_CALL)
+            HG_BATCH_CASE(6, HG_CALL)
+            HG_BATCH_CASE(7, HG_CALL)
+            HG_BATCH_CASE(8, HG_CALL)
+            HG_BATCH_CASE(9, HG_CALL)
+            HG_BATCH_CASE(10, HG_CALL)
+            HG_BATCH_CASE(11, HG_CALL)
+            default: break;
+        }
+#undef HG_CALL
+    }
+}
+
+#undef HG_BATCH_CASE
+
+// Launchers of one physics/precision variant (instantiated by hg_batch_inst.hip); grid = nitems.
+template <bool NS, bool IP, typename R, typename TR>
+hipError_t batch_stage_variant(int tile, const BatchArgs &b, int block, hipStream_t st) {
+    if (tile == 0)
+        hipLaunchKernelGGL((k_batch_stage<0, NS, IP, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
+                           b.sched, b.dt, b);
+    else
+        hipLaunchKernelGGL((k_batch_stage<1, NS, IP, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
+                           b.sched, b.dt, b);
+    return hipGetLastError();
+}
+
+template <bool BR1, typename R, typename TR>
+hipError_t batch_gradient_variant(int tile, const BatchArgs &b, int block, hipStream_t st) {
+    if (tile == 0)
+        hipLaunchKernelGGL((k_batch_gradient<0, BR1, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
+                           b.sched, b);
+    else
+        hipLaunchKernelGGL((k_batch_gradient<1, BR1, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
+                           b.sched, b);
+    return hipGetLastError();
+}
+
+}  // namespace hg

@@ -22,6 +22,8 @@ constexpr int FF_RS_SHIFT = 8;       // (zone Riemann solver + 1), 0 = interior 
 
 struct PhysParams {
     double gamma, gm1, cp, cv, lambda, invSGM, mu, cpPr, sigma0, ip_eps, mach_sqrtg;
+    // Reference pressure of the FP32 form (pref, Eref = pref/(gamma-1)); zero in the FP64 form.
+    double pref, Eref;
     int riemann, viscous, ip_gradflux, ns;
 };
 
@@ -54,6 +56,37 @@ struct StageArgs {
     PhysParams ph;
 };
 
+// The batched form: one launch per RK stage over a work list of element tiles of many cases.
+struct BatchArgs {
+    const StageArgs *slots;  // per case, nslots StageArgs (parity x stage, then derivative-only)
+    int nslots, slot;        // slot of this launch
+    const int4 *items;       // (case, first element, points per direction P, residual slot)
+    const int *sched;        // processing order of the items
+    int nitems;
+    const double *dt;        // per case time step
+    int dt_cq, dt_cg;        // multiply the slot's cq / cg by the case's dt
+};
+
+// variant = 2 * physics (0 Euler, 1 Navier-Stokes BR1, 2 Navier-Stokes IP) + (1 if mixed)
+hipError_t launch_batch_stage(int variant, int tile, const BatchArgs &b, int block, hipStream_t s);
+hipError_t launch_batch_gradient(int variant, int tile, const BatchArgs &b, int block, hipStream_t s);
+// out[c][0..4) = max over the residual partials [ranges[c].x, ranges[c].y) (one workgroup per case)
+hipError_t launch_batch_resid(const double *part, const int2 *ranges, double *const *out, int ncases, hipStream_t s);
+int tile_threads(int P, int tile);  // workgroup size of a tile
+int tile_elems(int P, int tile);    // elements per tile
+
+// Per-variant launchers (one object file each, from hg_batch_inst.hip).
+#define HG_DECLARE_BATCH_VARIANT(V)                                                                          \
+    hipError_t batch_stage_v##V(int tile, const BatchArgs &b, int block, hipStream_t s);          \
+    hipError_t batch_gradient_v##V(int tile, const BatchArgs &b, int block, hipStream_t s);
+HG_DECLARE_BATCH_VARIANT(0)
+HG_DECLARE_BATCH_VARIANT(1)
+HG_DECLARE_BATCH_VARIANT(2)
+HG_DECLARE_BATCH_VARIANT(3)
+HG_DECLARE_BATCH_VARIANT(4)
+HG_DECLARE_BATCH_VARIANT(5)
+#undef HG_DECLARE_BATCH_VARIANT
+
 // Launchers (return hipError_t of the launch).
 hipError_t launch_prolong(int P, bool mixed, const StageArgs &a, hipStream_t s);
 // tile: 0 = narrow workgroup (the widest element tile that fits one wavefront),
@@ -79,6 +112,8 @@ bool supported_order(int N);
 // Per-order launchers, explicitly instantiated once per P (one object file per order).
 template <int P> hipError_t stage_p(int tile, bool mixed, const StageArgs &a, hipStream_t s);
 template <int P> int stage_blocks_p(int tile, int K);
+template <int P> int tile_threads_p(int tile);
+template <int P> int tile_elems_p(int tile);
 template <int P> hipError_t gradient_p(int tile, bool mixed, const StageArgs &a, hipStream_t s);
 template <int P> hipError_t prolong_p(bool mixed, const StageArgs &a, hipStream_t s);
 template <int P> hipError_t dt_elem_p(const double *Q, int K, double gamma, double gm1, double *umax, double *amax,

@@ -76,6 +76,24 @@ __global__ void k_cons_final(const double *partial, int nb, double *out) {
     }
 }
 
+// Residual of each case of a batch: max over its tiles' partials (NaN propagates).
+__global__ __launch_bounds__(256) void k_batch_resid(const double *part, const int2 *ranges, double *const *out) {
+    __shared__ double s[256][4];
+    const int t = threadIdx.x, c = blockIdx.x;
+    const int2 r = ranges[c];
+    double m[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = r.x + t; b < r.y; b += 256)
+        for (int q = 0; q < 4; ++q) m[q] = nan_max(m[q], part[(int64_t)b * 4 + q]);
+    for (int q = 0; q < 4; ++q) s[t][q] = m[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int q = 0; q < 4; ++q) s[t][q] = nan_max(s[t][q], s[t + w][q]);
+        __syncthreads();
+    }
+    if (t < 4) out[c][t] = s[0][t];
+}
+
 }  // namespace
 
 bool supported_order(int N) { return N >= 1 && N <= 10; }
@@ -116,6 +134,42 @@ hipError_t launch_gradient(int P, int tile, bool mixed, const StageArgs &a, hipS
 hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream_t st) {
     hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, part, nb, out);
     return hipGetLastError();
+}
+
+int tile_threads(int P, int tile) {
+#define HG_CALL(PV) tile_threads_p<PV>(tile)
+    switch (P) {
+    case 2: return HG_CALL(2);
+    case 3: return HG_CALL(3);
+    case 4: return HG_CALL(4);
+    case 5: return HG_CALL(5);
+    case 6: return HG_CALL(6);
+    case 7: return HG_CALL(7);
+    case 8: return HG_CALL(8);
+    case 9: return HG_CALL(9);
+    case 10: return HG_CALL(10);
+    case 11: return HG_CALL(11);
+    default: return 0;
+    }
+#undef HG_CALL
+}
+
+int tile_elems(int P, int tile) {
+#define HG_CALL(PV) tile_elems_p<PV>(tile)
+    switch (P) {
+    case 2: return HG_CALL(2);
+    case 3: return HG_CALL(3);
+    case 4: return HG_CALL(4);
+    case 5: return HG_CALL(5);
+    case 6: return HG_CALL(6);
+    case 7: return HG_CALL(7);
+    case 8: return HG_CALL(8);
+    case 9: return HG_CALL(9);
+    case 10: return HG_CALL(10);
+    case 11: return HG_CALL(11);
+    default: return 0;
+    }
+#undef HG_CALL
 }
 
 int stage_blocks(int P, int tile, int K) {
@@ -179,6 +233,37 @@ hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const dou
     hipError_t e = cons_partial(P, Q, jac, w, K, partial, nbk, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, st, partial, cons_blocks(P, K), out);
+    return hipGetLastError();
+}
+
+}  // namespace hg
+
+namespace hg {
+
+hipError_t launch_batch_stage(int variant, int tile, const BatchArgs &b, int block, hipStream_t st) {
+    switch (variant) {
+    case 0: return batch_stage_v0(tile, b, block, st);
+    case 1: return batch_stage_v1(tile, b, block, st);
+    case 2: return batch_stage_v2(tile, b, block, st);
+    case 3: return batch_stage_v3(tile, b, block, st);
+    case 4: return batch_stage_v4(tile, b, block, st);
+    case 5: return batch_stage_v5(tile, b, block, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_gradient(int variant, int tile, const BatchArgs &b, int block, hipStream_t st) {
+    switch (variant) {
+    case 2: return batch_gradient_v2(tile, b, block, st);
+    case 3: return batch_gradient_v3(tile, b, block, st);
+    case 4: return batch_gradient_v4(tile, b, block, st);
+    case 5: return batch_gradient_v5(tile, b, block, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_resid(const double *part, const int2 *ranges, double *const *out, int ncases, hipStream_t st) {
+    hipLaunchKernelGGL(k_batch_resid, dim3(ncases), dim3(256), 0, st, part, ranges, out);
     return hipGetLastError();
 }
 

@@ -1,0 +1,77 @@
+// Internal definition of the library handle (shared by the C ABI and the batched form).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hg_device.h"
+#include "hg_mesh.h"
+#include "horses2d_gpu.h"
+
+namespace hg {
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+}  // namespace hg
+
+struct hg_handle {
+    hg::Mesh mesh;
+    int K = 0, N = 0, P = 0;
+    int64_t nodes = 0, ndof = 0;
+    bool mixed = false;
+    hg_options opt{};
+    hg_physics phys{};
+    hg::PhysParams ph{};
+    hipStream_t stream = nullptr;
+    std::vector<hg::DevBuf> bufs;
+    // device arrays
+    double *Q = nullptr, *G = nullptr, *QDot = nullptr;
+    void *tr[2] = {nullptr, nullptr};
+    void *dQ = nullptr, *dtr = nullptr;
+    void *metric = nullptr, *fmet = nullptr;
+    int4 *face = nullptr;
+    int *forder = nullptr;
+    double *egeo = nullptr, *einvh = nullptr, *zone = nullptr;
+    double *hatD = nullptr, *D = nullptr, *lb = nullptr, *lbw = nullptr, *dlbw = nullptr, *w = nullptr;
+    double *jac64 = nullptr;
+    double *dx = nullptr, *umax = nullptr, *amax = nullptr, *dtstate = nullptr, *dtout = nullptr;
+    double *cons_partial = nullptr, *cons_out = nullptr;
+    int cons_nb = 0;
+    double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
+    int cur = 0;
+    int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    int64_t prof_n[2] = {0, 0};
+    double prof_ms[2] = {0.0, 0.0};
+
+    template <typename T>
+    hipError_t alloc(T **p, size_t count) {
+        void *raw = nullptr;
+        hipError_t e = hipMalloc(&raw, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return e;
+        bufs.push_back({raw, count * sizeof(T)});
+        *p = (T *)raw;
+        return hipSuccess;
+    }
+    hipError_t alloc_raw(void **p, size_t bytes) {
+        hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+        if (e != hipSuccess) return e;
+        bufs.push_back({*p, bytes});
+        return hipSuccess;
+    }
+};
+
+namespace hg {
+// StageArgs with every per-handle pointer and the physics filled in (per-stage fields zero).
+StageArgs handle_args(hg_handle *h);
+// Records the message returned by hg_last_error and returns 1.
+int handle_error(const std::string &m);
+}  // namespace hg

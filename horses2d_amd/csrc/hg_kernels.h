@@ -66,6 +66,7 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_prolong(StageArgs a) {
         if (e < a.K) {
             double tq[4];
             face_trace<P, 4, 4>(sQ + s * T::PP * 4, sLb, f, k, tq);
+            tq[3] = slot_energy<TR>(a.ph, tq[3]);
             TR *o = (TR *)a.tr_out + ((int64_t)(e * 4 + f) * P + k) * 4;
 #pragma unroll
             for (int c = 0; c < 4; ++c) o[c] = (TR)tq[c];
@@ -78,17 +79,26 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_prolong(StageArgs a) {
 // (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622, DGViscousMethods.f90:538-917).
 // IP: interior gradient (QuadElement_QuadProcedures.incf:42-88).
 // Writes dQ at the nodes and the gradient traces of the four faces.
-template <int P, int EPB, bool BR1, typename R, typename TR>
-__global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
+template <int P, int EPB, bool BR1, typename R>
+struct GradSmem {
     using T = Tile<P, EPB>;
-    constexpr int PP = T::PP, NN = T::NN, NF = T::NF, TPB = T::TPB;
-    __shared__ double sQ[NN * 4];
-    __shared__ R sJa[BR1 ? NN * 4 : 1];
-    __shared__ R sFu[BR1 ? NF * 8 : 1];
-    __shared__ double sDQ[NN * 8];
-    __shared__ double sLb[2 * P];
-    __shared__ R sHat[PP], sLbw[2 * P], sD[PP];
-    const int t = threadIdx.x, e0 = blockIdx.x * EPB;
+    double sQ[T::NN * 4];
+    double sDQ[T::NN * 8];
+    double sLb[2 * P];
+    R sJa[BR1 ? T::NN * 4 : 1];
+    R sFu[BR1 ? T::NF * 8 : 1];
+    R sHat[T::PP], sLbw[2 * P], sD[T::PP];
+};
+
+// One element tile [e0, e0+EPB) of the gradient; any workgroup size >= Tile<P, EPB>::TPB.
+template <int P, int EPB, bool BR1, typename R, typename TR>
+__device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, GradSmem<P, EPB, BR1, R> &S) {
+    using T = Tile<P, EPB>;
+    constexpr int PP = T::PP, NN = T::NN, NF = T::NF;
+    const int TPB = blockDim.x;
+    double *sQ = S.sQ, *sDQ = S.sDQ, *sLb = S.sLb;
+    R *sJa = S.sJa, *sFu = S.sFu, *sHat = S.sHat, *sLbw = S.sLbw, *sD = S.sD;
+    const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
     if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; }
     for (int x = t; x < PP; x += TPB) { sHat[x] = (R)a.hatD[x]; sD[x] = (R)a.D[x]; }
@@ -134,6 +144,7 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
         const int kind = fr.x & 15;
         double tq[4];
         face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
+        tq[3] = slot_energy<TR>(a.ph, tq[3]);
         R qs[4] = {(R)(TR)tq[0], (R)(TR)tq[1], (R)(TR)tq[2], (R)(TR)tq[3]};
         R us[4];
         if (kind <= FK_PERIODIC) {
@@ -156,6 +167,7 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
     if (node_active) {
         const int i = n % P, j = n / P;
         const double *q = sQ + s * PP * 4;
+        auto qslot = [&](double v, int c) -> R { return (R)(c == 3 ? slot_energy<R>(a.ph, v) : v); };
         R dq[8];
         if (BR1) {
             const R *Ja = sJa + s * PP * 4;
@@ -165,9 +177,9 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
                 for (int c = 0; c < 4; ++c) {
                     R acc = R(0);
 #pragma unroll
-                    for (int l = 0; l < P; ++l) acc += (R)q[(l + P * j) * 4 + c] * Ja[(l + P * j) * 4 + d * 2 + 0] * sHat[i * P + l];
+                    for (int l = 0; l < P; ++l) acc += qslot(q[(l + P * j) * 4 + c], c) * Ja[(l + P * j) * 4 + d * 2 + 0] * sHat[i * P + l];
 #pragma unroll
-                    for (int l = 0; l < P; ++l) acc += (R)q[(i + P * l) * 4 + c] * Ja[(i + P * l) * 4 + d * 2 + 1] * sHat[j * P + l];
+                    for (int l = 0; l < P; ++l) acc += qslot(q[(i + P * l) * 4 + c], c) * Ja[(i + P * l) * 4 + d * 2 + 1] * sHat[j * P + l];
                     dq[d * 4 + c] = -acc;
                 }
             const int ord = a.forder[e];
@@ -186,9 +198,9 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
             for (int c = 0; c < 4; ++c) {
                 R dxi = R(0), deta = R(0);
 #pragma unroll
-                for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * (R)q[(l + P * j) * 4 + c];
+                for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * qslot(q[(l + P * j) * 4 + c], c);
 #pragma unroll
-                for (int l = 0; l < P; ++l) deta += sD[j * P + l] * (R)q[(i + P * l) * 4 + c];
+                for (int l = 0; l < P; ++l) deta += sD[j * P + l] * qslot(q[(i + P * l) * 4 + c], c);
                 dq[c] = (dxi * j11 + deta * j12) / jac;
                 dq[4 + c] = (dxi * j21 + deta * j22) / jac;
             }
@@ -210,22 +222,41 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
     }
 }
 
+template <int P, int EPB, bool BR1, typename R, typename TR>
+__global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
+    __shared__ GradSmem<P, EPB, BR1, R> S;
+    gradient_body<P, EPB, BR1, R, TR>(a, blockIdx.x * EPB, S);
+}
+
 // ------------------------------------------------------------------ the stage kernel
-template <int P, int EPB, bool NS, bool IP, typename R, typename TR>
-__global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
+template <int P, int EPB, bool IP, typename R>
+struct StageSmem {
     using T = Tile<P, EPB>;
-    constexpr int PP = T::PP, NN = T::NN, NF = T::NF, TPB = T::TPB;
-    __shared__ double sQ[NN * 4];
-    __shared__ R sCF[NN * 4];
-    __shared__ R sCG[NN * 4];
-    __shared__ R sF[NF * 4];
-    __shared__ R sU[IP ? NF * 4 : 1];
-    __shared__ R sV[IP ? NF * 4 : 1];
-    __shared__ R sU2[IP ? NF * 4 : 1];
-    __shared__ R sV2[IP ? NF * 4 : 1];
-    __shared__ double sLb[2 * P];
-    __shared__ R sHat[PP], sLbw[2 * P], sDlbw[2 * P];
-    const int t = threadIdx.x, e0 = blockIdx.x * EPB;
+    double sQ[T::NN * 4];
+    double sLb[2 * P];
+    double sRes[4][4];
+    R sCF[T::NN * 4];
+    R sCG[T::NN * 4];
+    R sF[T::NF * 4];
+    R sU[IP ? T::NF * 4 : 1];
+    R sV[IP ? T::NF * 4 : 1];
+    R sU2[IP ? T::NF * 4 : 1];
+    R sV2[IP ? T::NF * 4 : 1];
+    R sHat[T::PP], sLbw[2 * P], sDlbw[2 * P];
+};
+
+// One element tile [e0, e0+EPB) of a stage; any workgroup size >= Tile<P, EPB>::TPB (at most
+// 256).  rslot is the residual partial this tile writes; cq, cg are the stage's RK factors.
+template <int P, int EPB, bool NS, bool IP, typename R, typename TR>
+__device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, const int64_t rslot, const double cq,
+                                           const double cg, StageSmem<P, EPB, IP, R> &S) {
+    using T = Tile<P, EPB>;
+    constexpr int PP = T::PP, NN = T::NN, NF = T::NF;
+    const int TPB = blockDim.x;
+    double *sQ = S.sQ, *sLb = S.sLb;
+    R *sCF = S.sCF, *sCG = S.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
+    R *sHat = S.sHat, *sLbw = S.sLbw, *sDlbw = S.sDlbw;
+    const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
     const bool ipgrad = IP && a.ph.ip_gradflux;
     if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; sDlbw[t] = (R)a.dlbw[t]; }
@@ -297,7 +328,8 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
     if (node_active) {
         R q[4], f[4], gg[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) q[c] = (R)sQ[t * 4 + c];
+        for (int c = 0; c < 3; ++c) q[c] = (R)sQ[t * 4 + c];
+        q[3] = (R)slot_energy<R>(a.ph, sQ[t * 4 + 3]);
         inviscid_flux(ph, q, f, gg);
         if (NS) {
             R fv[4], gv[4];
@@ -317,6 +349,7 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
         const int kind = fr.x & 15;
         double tq[4];
         face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
+        tq[3] = slot_energy<TR>(a.ph, tq[3]);
         R qs[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) qs[c] = (R)(TR)tq[c];   // the same rounding the neighbor reads
@@ -479,17 +512,17 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
             double gv[4];
             if (a.first_stage) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) gv[c] = a.cq * qd[c];
+                for (int c = 0; c < 4; ++c) gv[c] = cq * qd[c];
             } else {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) gv[c] = fma(a.am, gold[c], a.cq * qd[c]);
+                for (int c = 0; c < 4; ++c) gv[c] = fma(a.am, gold[c], cq * qd[c]);
             }
             double2 *go = (double2 *)(a.G + g * 4);
             go[0] = make_double2(gv[0], gv[1]);
             go[1] = make_double2(gv[2], gv[3]);
             double qn[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) qn[c] = fma(a.cg, gv[c], sQ[t * 4 + c]);
+            for (int c = 0; c < 4; ++c) qn[c] = fma(cg, gv[c], sQ[t * 4 + c]);
             double2 *qo2 = (double2 *)(a.Q + g * 4);
             qo2[0] = make_double2(qn[0], qn[1]);
             qo2[1] = make_double2(qn[2], qn[3]);
@@ -500,7 +533,7 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
     if (a.do_resid) {
         // max|QDot| per equation: wavefront shuffle, then across the workgroup's wavefronts in
         // LDS; one partial per workgroup (reduced by k_resid_final).  NaN wins every compare.
-        __shared__ double sRes[TPB / 64][4];
+        double(*sRes)[4] = S.sRes;
         double m[4];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -515,9 +548,8 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
         __syncthreads();
         if (t < 4) {
             double v = sRes[0][t];
-#pragma unroll
             for (int w = 1; w < TPB / 64; ++w) v = nan_max(v, sRes[w][t]);
-            a.resid_part[(int64_t)blockIdx.x * 4 + t] = v;
+            a.resid_part[rslot * 4 + t] = v;
         }
     }
     if (a.mode == 0) {
@@ -525,11 +557,18 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
         if (face_active) {
             double tq[4];
             face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
+            tq[3] = slot_energy<TR>(a.ph, tq[3]);
             TR *o = (TR *)a.tr_out + ((int64_t)(fe * 4 + ff) * P + fk) * 4;
 #pragma unroll
             for (int c = 0; c < 4; ++c) o[c] = (TR)tq[c];
         }
     }
+}
+
+template <int P, int EPB, bool NS, bool IP, typename R, typename TR>
+__global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
+    __shared__ StageSmem<P, EPB, IP, R> S;
+    stage_body<P, EPB, NS, IP, R, TR>(a, blockIdx.x * EPB, blockIdx.x, a.cq, a.cg, S);
 }
 
 // ------------------------------------------------------------------ time step and sums
@@ -620,6 +659,16 @@ hipError_t stage_p(int tile, bool mixed, const StageArgs &a, hipStream_t st) {
 }
 
 template <int P>
+int tile_threads_p(int tile) {
+    return tile == 0 ? Tile<P, Tiles<P>::small>::TPB : Tile<P, Tiles<P>::large>::TPB;
+}
+
+template <int P>
+int tile_elems_p(int tile) {
+    return tile == 0 ? Tiles<P>::small : Tiles<P>::large;
+}
+
+template <int P>
 int stage_blocks_p(int tile, int K) {
     return tile == 0 ? tile_blocks<P, Tiles<P>::small>(K) : tile_blocks<P, Tiles<P>::large>(K);
 }
@@ -663,6 +712,8 @@ hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, i
     template hipError_t stage_p<PV>(int, bool, const StageArgs &, hipStream_t);                               \
     template hipError_t gradient_p<PV>(int, bool, const StageArgs &, hipStream_t);                            \
     template int stage_blocks_p<PV>(int, int);                                                                \
+    template int tile_threads_p<PV>(int);                                                                     \
+    template int tile_elems_p<PV>(int);                                                                       \
     template hipError_t prolong_p<PV>(bool, const StageArgs &, hipStream_t);                                  \
     template hipError_t dt_elem_p<PV>(const double *, int, double, double, double *, double *, hipStream_t);  \
     template int cons_blocks_p<PV>(int);                                                                      \

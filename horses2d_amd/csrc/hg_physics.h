@@ -10,13 +10,48 @@
 
 namespace hg {
 
+// In the FP32 form (R = float) the energy slot of every state handed to these functions holds
+// rho*e - Eref, with Eref = pref/(gamma-1) for a reference pressure pref of the handle, so that
+// gm1*(rho*e - KE) is the pressure perturbation p - pref: the momentum fluxes carry
+// rho*u*u + (p - pref), whose FP32 rounding is relative to the perturbation instead of to p.
+// The constant part pref*n has zero discrete divergence (free-stream preservation), and the
+// energy flux, sound speed, enthalpy and temperature add the constants back.  The FP64 form
+// (R = double) uses the reference's expressions unchanged.
 template <typename R>
 struct Phys {
-    R gamma, gm1, cp, cv, lambda, invSGM, mu, cpPr, sigma0, ip_eps;
+    static constexpr bool shift = sizeof(R) == 4;
+    R gamma, gm1, cp, cv, lambda, invSGM, mu, cpPr, sigma0, ip_eps, pref, Eref, Href;
+    double Eref_d;
     __device__ explicit Phys(const PhysParams &p)
         : gamma((R)p.gamma), gm1((R)p.gm1), cp((R)p.cp), cv((R)p.cv), lambda((R)p.lambda),
-          invSGM((R)p.invSGM), mu((R)p.mu), cpPr((R)p.cpPr), sigma0((R)p.sigma0), ip_eps((R)p.ip_eps) {}
+          invSGM((R)p.invSGM), mu((R)p.mu), cpPr((R)p.cpPr), sigma0((R)p.sigma0), ip_eps((R)p.ip_eps),
+          pref((R)p.pref), Eref((R)p.Eref), Href((R)(p.pref + p.Eref)), Eref_d(p.Eref) {}
 };
+
+// thermodynamic pressure from the pressure carried by the flux
+template <typename R>
+__device__ inline R p_full(const Phys<R> &ph, R pm) {
+    if constexpr (Phys<R>::shift) return pm + ph.pref;
+    else return pm;
+}
+// total energy from the energy slot
+template <typename R>
+__device__ inline R e_full(const Phys<R> &ph, R e) {
+    if constexpr (Phys<R>::shift) return e + ph.Eref;
+    else return e;
+}
+// rho*e + p from the energy slot and the flux pressure
+template <typename R>
+__device__ inline R e_plus_p(const Phys<R> &ph, R e, R pm) {
+    if constexpr (Phys<R>::shift) return (e + pm) + ph.Href;
+    else return e + pm;
+}
+// FP64 trace or node state -> the energy slot of the compute type
+template <typename R>
+__device__ inline double slot_energy(const PhysParams &p, double e) {
+    if constexpr (sizeof(R) == 4) return e - p.Eref;
+    else return e;
+}
 
 __device__ inline double dsqrt(double x) { return sqrt(x); }
 __device__ inline float dsqrt(float x) { return sqrtf(x); }
@@ -32,8 +67,9 @@ __device__ inline void inviscid_flux(const Phys<R> &ph, const R *u, R *f, R *g) 
     R invRho = R(1) / u[0];
     R vx = u[1] * invRho, vy = u[2] * invRho;
     R p = ph.gm1 * (u[3] - R(0.5) * u[1] * vx - R(0.5) * u[2] * vy);
-    f[0] = u[1]; f[1] = u[1] * vx + p; f[2] = u[1] * vy; f[3] = (u[3] + p) * vx;
-    g[0] = u[2]; g[1] = f[2]; g[2] = u[2] * vy + p; g[3] = (u[3] + p) * vy;
+    const R h = e_plus_p(ph, u[3], p);
+    f[0] = u[1]; f[1] = u[1] * vx + p; f[2] = u[1] * vy; f[3] = h * vx;
+    g[0] = u[2]; g[1] = f[2]; g[2] = u[2] * vy + p; g[3] = h * vy;
 #pragma unroll
     for (int k = 0; k < 4; ++k) { f[k] *= ph.invSGM; g[k] *= ph.invSGM; }
 }
@@ -44,11 +80,11 @@ __device__ void roe_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny
     R rhoL = qL[0], invRhoL = R(1) / rhoL, sqrtRhoL = dsqrt(rhoL);
     R uL = (qL[1] * nx + qL[2] * ny) * invRhoL, vL = (-qL[1] * ny + qL[2] * nx) * invRhoL;
     R pL = ph.gm1 * (qL[3] - R(0.5) * rhoL * (uL * uL + vL * vL));
-    R HL = ph.cp * pL * invRhoL + R(0.5) * (uL * uL + vL * vL);
+    R HL = ph.cp * p_full(ph, pL) * invRhoL + R(0.5) * (uL * uL + vL * vL);
     R rhoR = qR[0], invRhoR = R(1) / rhoR, sqrtRhoR = dsqrt(rhoR);
     R uR = (qR[1] * nx + qR[2] * ny) * invRhoR, vR = (-qR[1] * ny + qR[2] * nx) * invRhoR;
     R pR = ph.gm1 * (qR[3] - R(0.5) * rhoR * (uR * uR + vR * vR));
-    R HR = ph.cp * pR * invRhoR + R(0.5) * (uR * uR + vR * vR);
+    R HR = ph.cp * p_full(ph, pR) * invRhoR + R(0.5) * (uR * uR + vR * vR);
     R invrho = R(1) / (sqrtRhoL + sqrtRhoR);
     R u = (sqrtRhoL * uL + sqrtRhoR * uR) * invrho;
     R v = (sqrtRhoL * vL + sqrtRhoR * vR) * invrho;
@@ -86,13 +122,13 @@ __device__ void hll_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny
     R rhoL = qL[0], invRhoL = R(1) / rhoL;
     R uL = (qL[1] * nx + qL[2] * ny) * invRhoL, vL = (-qL[1] * ny + qL[2] * nx) * invRhoL;
     R pL = ph.gm1 * (qL[3] - R(0.5) * rhoL * (uL * uL + vL * vL));
-    R HL = ph.cp * pL * invRhoL + R(0.5) * (uL * uL + vL * vL);
-    R aL = dsqrt(ph.gamma * pL * invRhoL), rhoeL = qL[3];
+    R HL = ph.cp * p_full(ph, pL) * invRhoL + R(0.5) * (uL * uL + vL * vL);
+    R aL = dsqrt(ph.gamma * p_full(ph, pL) * invRhoL), rhoeL = qL[3];
     R rhoR = qR[0], invRhoR = R(1) / rhoR;
     R uR = (qR[1] * nx + qR[2] * ny) * invRhoR, vR = (-qR[1] * ny + qR[2] * nx) * invRhoR;
     R pR = ph.gm1 * (qR[3] - R(0.5) * rhoR * (uR * uR + vR * vR));
-    R HR = ph.cp * pR * invRhoR + R(0.5) * (uR * uR + vR * vR);
-    R aR = dsqrt(ph.gamma * pR * invRhoR), rhoeR = qR[3];
+    R HR = ph.cp * p_full(ph, pR) * invRhoR + R(0.5) * (uR * uR + vR * vR);
+    R aR = dsqrt(ph.gamma * p_full(ph, pR) * invRhoR), rhoeR = qR[3];
     R sL = dsqrt(rhoL), sR = dsqrt(rhoR), invrho = R(1) / (sL + sR);
     R u = (sL * uL + sR * uR) * invrho, v = (sL * vL + sR * vR) * invrho, H = (sL * HL + sR * HR) * invrho;
     R a = dsqrt(ph.gm1 * (H - R(0.5) * (u * u + v * v)));
@@ -138,9 +174,9 @@ __device__ void exact_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R 
     R qnR[4] = {qR[0], qR[1] * nx + qR[2] * ny, -qR[1] * ny + qR[2] * nx, qR[3]};
     R wL[6], wR[6];
     wL[0] = qnL[0]; wL[1] = qnL[1] / qnL[0]; wL[2] = qnL[2] / qnL[0];
-    wL[3] = gm1 * (qnL[3] - R(0.5) * (qnL[1] * wL[1] + qnL[2] * wL[2])); wL[4] = R(0); wL[5] = dsqrt(g * wL[3] / wL[0]);
+    wL[3] = p_full(ph, gm1 * (qnL[3] - R(0.5) * (qnL[1] * wL[1] + qnL[2] * wL[2]))); wL[4] = R(0); wL[5] = dsqrt(g * wL[3] / wL[0]);
     wR[0] = qnR[0]; wR[1] = qnR[1] / qnR[0]; wR[2] = qnR[2] / qnR[0];
-    wR[3] = gm1 * (qnR[3] - R(0.5) * (qnR[1] * wR[1] + qnR[2] * wR[2])); wR[4] = R(0); wR[5] = dsqrt(g * wR[3] / wR[0]);
+    wR[3] = p_full(ph, gm1 * (qnR[3] - R(0.5) * (qnR[1] * wR[1] + qnR[2] * wR[2]))); wR[4] = R(0); wR[5] = dsqrt(g * wR[3] / wR[0]);
     R FL, dFL, FR, dFR;
     R ps = dpow((wL[5] + wR[5] - R(0.5) * gm1 * (wR[1] - wL[1])) /
                     (wL[5] / dpow(wL[3], R(0.5) / cp) + wR[5] / dpow(wR[3], R(0.5) / cp)),
@@ -203,6 +239,9 @@ __device__ void exact_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R 
             Fs[3] = us * (cp * pstar + R(0.5) * rs * (us * us + wR[2] * wR[2]));
         }
     }
+    if constexpr (Phys<R>::shift) {
+        if (Fs[0] != R(0) || Fs[1] != R(0) || Fs[3] != R(0)) Fs[1] -= ph.pref;   // a sampled state was taken
+    }
     R Ft = Fs[1] * nx - Fs[2] * ny;
     Fs[2] = Fs[1] * ny + Fs[2] * nx;
     F[0] = Fs[0] * ph.invSGM; F[1] = Ft * ph.invSGM; F[2] = Fs[2] * ph.invSGM; F[3] = Fs[3] * ph.invSGM;
@@ -217,12 +256,13 @@ __device__ void lf_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny,
     R uL = qL[1] * iL, vL = qL[2] * iL, uR = qR[1] * iR, vR = qR[2] * iR;
     R pL = ph.gm1 * (qL[3] - R(0.5) * qL[1] * uL - R(0.5) * qL[2] * vL);
     R pR = ph.gm1 * (qR[3] - R(0.5) * qR[1] * uR - R(0.5) * qR[2] * vR);
-    fL[0] = qL[1]; fL[1] = qL[1] * uL + pL; fL[2] = qL[1] * vL; fL[3] = (qL[3] + pL) * uL;
-    gL[0] = qL[2]; gL[1] = fL[2]; gL[2] = qL[2] * vL + pL; gL[3] = (qL[3] + pL) * vL;
-    fR[0] = qR[1]; fR[1] = qR[1] * uR + pR; fR[2] = qR[1] * vR; fR[3] = (qR[3] + pR) * uR;
-    gR[0] = qR[2]; gR[1] = fR[2]; gR[2] = qR[2] * vR + pR; gR[3] = (qR[3] + pR) * vR;
-    R lL = dabs(uL * nx + vL * ny) + dsqrt(ph.gamma * pL * iL);
-    R lR = dabs(uR * nx + vR * ny) + dsqrt(ph.gamma * pR * iR);
+    const R hL = e_plus_p(ph, qL[3], pL), hR = e_plus_p(ph, qR[3], pR);
+    fL[0] = qL[1]; fL[1] = qL[1] * uL + pL; fL[2] = qL[1] * vL; fL[3] = hL * uL;
+    gL[0] = qL[2]; gL[1] = fL[2]; gL[2] = qL[2] * vL + pL; gL[3] = hL * vL;
+    fR[0] = qR[1]; fR[1] = qR[1] * uR + pR; fR[2] = qR[1] * vR; fR[3] = hR * uR;
+    gR[0] = qR[2]; gR[1] = fR[2]; gR[2] = qR[2] * vR + pR; gR[3] = hR * vR;
+    R lL = dabs(uL * nx + vL * ny) + dsqrt(ph.gamma * p_full(ph, pL) * iL);
+    R lR = dabs(uR * nx + vR * ny) + dsqrt(ph.gamma * p_full(ph, pR) * iR);
     R lam = lL > lR ? lL : lR;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -247,8 +287,9 @@ __device__ inline void viscous_flux(const Phys<R> &ph, const R *q, const R *dq, 
     R invRho = R(1) / q[0], u = q[1] * invRho, v = q[2] * invRho, uDR = u * invRho, vDR = v * invRho;
     const R *dx = dq, *dy = dq + 4;
     R divV = -uDR * dx[0] + invRho * dx[1] - vDR * dy[0] + invRho * dy[2];
-    R dxT = ph.gm1 * invRho * (-q[3] * invRho * dx[0] + dx[3] + u * (u * dx[0] - dx[1]) + v * (v * dx[0] - dx[2]));
-    R dyT = ph.gm1 * invRho * (-q[3] * invRho * dy[0] + dy[3] + u * (u * dy[0] - dy[1]) + v * (v * dy[0] - dy[2]));
+    const R E = e_full(ph, q[3]);
+    R dxT = ph.gm1 * invRho * (-E * invRho * dx[0] + dx[3] + u * (u * dx[0] - dx[1]) + v * (v * dx[0] - dx[2]));
+    R dyT = ph.gm1 * invRho * (-E * invRho * dy[0] + dy[3] + u * (u * dy[0] - dy[1]) + v * (v * dy[0] - dy[2]));
     f[0] = R(0);
     f[1] = R(2) * (-uDR * dx[0] + invRho * dx[1]) + ph.lambda * divV;
     f[2] = -vDR * dx[0] - uDR * dy[0] + invRho * dy[1] + invRho * dx[2];
@@ -283,13 +324,13 @@ __device__ inline void ghost_state(int kind, int flags, const double *zd, const 
                                    R *uB, R *uSB, int &neumann) {
     neumann = 0;
     if (kind == FK_FARFIELD) {
-        R q[4] = {(R)zd[0], (R)zd[1], (R)zd[2], (R)zd[3]};
+        R q[4] = {(R)zd[0], (R)zd[1], (R)zd[2], (R)(Phys<R>::shift ? zd[3] - ph.Eref_d : zd[3])};
         R rhoI = q[0], uI = q[1] / rhoI, vI = q[2] / rhoI;
         R pI = ph.gm1 * (q[3] - R(0.5) * q[1] * uI - R(0.5) * q[2] * vI);
         R rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
         R vnL = uL * nx + vL * ny;
         R pL = ph.gm1 * (Qt[3] - R(0.5) * Qt[1] * uL - R(0.5) * Qt[2] * vL);
-        R aL = dsqrt(ph.gamma * pL / rhoL);
+        R aL = dsqrt(ph.gamma * p_full(ph, pL) / rhoL);
         R ML = dsqrt(uL * uL + vL * vL) / aL;
         if (vnL < R(0)) {
             if (ML > R(1)) {
@@ -321,9 +362,10 @@ __device__ inline void ghost_state(int kind, int flags, const double *zd, const 
         uB[1] = R(2) * uB[0] * vwx - Qt[1];
         uB[2] = R(2) * uB[0] * vwy - Qt[2];
         R Ek = R(0.5) * (uB[1] * uB[1] + uB[2] * uB[2]) / uB[0];
-        uB[3] = iso ? ph.cv * uB[0] * R(1) * Tw + Ek : Qt[3];
+        uB[3] = iso ? (ph.cv * uB[0] * R(1) * Tw + Ek) - (Phys<R>::shift ? ph.Eref : R(0)) : Qt[3];
         uSB[0] = Qt[0]; uSB[1] = uSB[0] * vwx; uSB[2] = uSB[0] * vwy;
-        uSB[3] = iso ? ph.cv * uSB[0] * R(1) * Tw + R(0.5) * (uSB[1] * uSB[1] + uSB[2] * uSB[2]) / uSB[0]
+        uSB[3] = iso ? (ph.cv * uSB[0] * R(1) * Tw + R(0.5) * (uSB[1] * uSB[1] + uSB[2] * uSB[2]) / uSB[0]) -
+                           (Phys<R>::shift ? ph.Eref : R(0))
                      : Qt[3] - Ek + R(0.5) * uSB[0] * (vwx * vwx + vwy * vwy);
     }
 }

@@ -49,7 +49,7 @@ class _Options(ctypes.Structure):
 
 
 class _BatchOptions(ctypes.Structure):
-    _fields_ = [("tile_elems_override", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+    _fields_ = [("tile", ctypes.c_int32), ("order", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
 
 
 # every symbol declared in include/horses2d_gpu.h
@@ -59,7 +59,7 @@ EXPORTS = [
     "hg_upload_state", "hg_download_state", "hg_download_qdot", "hg_time_derivative", "hg_rk_step",
     "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_synchronize",
     "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_batch_create", "hg_batch_rk_step",
-    "hg_batch_synchronize", "hg_batch_destroy",
+    "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_destroy",
 ]
 
 _lib = None
@@ -101,6 +101,7 @@ def load_library(path: str = LIB_PATH):
         "hg_batch_create": ([vp, i32, vp, vp], ctypes.c_int),
         "hg_batch_rk_step": ([vp, vp, vp], ctypes.c_int),
         "hg_batch_synchronize": ([vp], ctypes.c_int),
+        "hg_batch_schedule": ([vp, vp, vp], ctypes.c_int),
         "hg_batch_destroy": ([vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -315,3 +316,42 @@ class DGSEM:
                 dt = self.estimate_dt(dt_max, self.case.cfl)
         self.dt = dt
         return np.array(history)
+
+
+class Batch:
+    """Many DGSEM handles advanced together (hg_batch_*): one launch per RK stage over the element
+    tiles of all cases, the step replayed as a HIP graph; each case keeps its own dt."""
+
+    def __init__(self, members, tile: int = -1, order: int = -1, use_graph: bool = True):
+        self._L = load_library()
+        self.members = list(members)
+        arr = (ctypes.c_void_p * len(self.members))(*[m.handle for m in self.members])
+        opt = _BatchOptions(tile, order, 1 if use_graph else 0)
+        self._b = ctypes.c_void_p()
+        _check(self._L.hg_batch_create(arr, len(self.members), ctypes.byref(opt), ctypes.byref(self._b)),
+               "hg_batch_create")
+
+    def rk_step(self, t, dt):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(t, np.float64), (len(self.members),)))
+        dt = np.ascontiguousarray(np.asarray(dt, np.float64))
+        assert dt.shape == (len(self.members),)
+        _check(self._L.hg_batch_rk_step(self._b, _ptr(t), _ptr(dt)), "hg_batch_rk_step")
+
+    def synchronize(self):
+        _check(self._L.hg_batch_synchronize(self._b), "hg_batch_synchronize")
+
+    def schedule(self):
+        t = ctypes.c_int32(); o = ctypes.c_int32()
+        _check(self._L.hg_batch_schedule(self._b, ctypes.byref(t), ctypes.byref(o)), "hg_batch_schedule")
+        return int(t.value), int(o.value)
+
+    def close(self):
+        if getattr(self, "_b", None) is not None and self._b.value:
+            self._L.hg_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -193,19 +193,28 @@ int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile);
 int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile);
 
 /* ------------------------------------------------------------------------ */
-/* Batched form: many independent cases advanced together, one persistent
- * launch per RK stage over a (case, element-tile) work queue, captured into a
- * HIP graph.  No reference counterpart: the reference runs one case per
- * process; each case keeps its own dt (TimeIntegrator_t%dt). */
+/* Batched form: many independent cases (own mesh, polynomial order, Riemann solver and
+ * dt) advanced together, one launch per RK stage (plus one per gradient pass for
+ * Navier-Stokes) over a work list of element tiles of all cases, the whole step captured
+ * in a HIP graph.  The workgroup tile and the order of the work list are chosen by a
+ * timing autotuner at the first step.  All cases of a batch share the equations
+ * (Euler / Navier-Stokes BR1 / Navier-Stokes IP), the precision and the time scheme.
+ * No reference counterpart: the reference runs one case per process
+ * (TimeIntegrator_t%dt per case, DGTimeIntegrator.f90:159-255). */
 typedef struct {
-    int32_t tile_elems_override; /* 0: autotuned tile size per polynomial order */
-    int32_t use_graph;           /* 1: capture the three stages in a hipGraph */
+    int32_t tile;      /* -1: autotune, 0: narrow, 1: wide workgroup tile */
+    int32_t order;     /* -1: autotune, 0: case-major, 1: costliest tiles first, 2: cases interleaved */
+    int32_t use_graph; /* 1: replay the captured step as a HIP graph, 0: launch kernel by kernel */
 } hg_batch_options;
 
+/* The batch keeps pointers to the handles (not copies); they must outlive it.  Between
+ * hg_batch_rk_step and hg_batch_synchronize the cases must not be used directly. */
 int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_options *opt, hg_batch **out);
-/* One RK step of every case; t[c], dt[c] per case. */
+/* One RK step of every case: t[c], dt[c] per case (hg_rk_step semantics per case). */
 int hg_batch_rk_step(hg_batch *b, const double *t, const double *dt);
 int hg_batch_synchronize(hg_batch *b);
+/* The autotuned (or fixed) tile and order; -1 before the first step. */
+int hg_batch_schedule(hg_batch *b, int32_t *tile, int32_t *order);
 int hg_batch_destroy(hg_batch *b);
 
 #ifdef __cplusplus

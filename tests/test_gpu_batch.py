@@ -1,0 +1,94 @@
+"""Batched form: many cases of different mesh size and polynomial order advanced by one launch
+per RK stage over all their element tiles (HIP-graph replay) must give, bit for bit, the state
+and residuals of advancing each case with its own handle."""
+import numpy as np
+import pytest
+
+from horses2d_amd import cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def make(cases, precision):
+    from horses2d_amd import gpu
+    out = []
+    for c in cases:
+        d = gpu.DGSEM(c, precision)
+        d.upload(c.initial_state(d.coords()))
+        out.append(d)
+    return out
+
+
+def run_pair(cases, precision, steps, **bopt):
+    from horses2d_amd import gpu
+    single = make(cases, precision)
+    batched = make(cases, precision)
+    dts = np.array([d.estimate_dt(c.dt_max, c.cfl) for d, c in zip(single, cases)])
+    for d, c in zip(batched, cases):
+        d.estimate_dt(c.dt_max, c.cfl)
+    for it in range(steps):
+        for d, dt in zip(single, dts):
+            d.rk_step(it * dt, dt)
+    b = gpu.Batch(batched, **bopt)
+    for it in range(steps):
+        b.rk_step(it * dts, dts)
+    b.synchronize()
+    return single, batched, b
+
+
+def assert_same(single, batched):
+    for s, m in zip(single, batched):
+        np.testing.assert_array_equal(m.state(), s.state())
+        np.testing.assert_array_equal(m.residuals(), s.residuals())
+
+
+def small_ensemble(count=6):
+    return C.ensemble_cases(count, seed_base=77, nmin=3, nmax=9, Nmin=1, Nmax=9)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_euler_ensemble_fp64_bitwise(use_graph):
+    cases = small_ensemble()
+    single, batched, b = run_pair(cases, "fp64", 4, use_graph=use_graph)
+    assert_same(single, batched)
+    tile, order = b.schedule()
+    assert tile in (0, 1) and order in (0, 1, 2)
+
+
+@pytest.mark.parametrize("tile,order", [(0, 0), (1, 1), (0, 2), (1, 0)])
+def test_every_schedule_gives_the_same_bits(tile, order):
+    cases = small_ensemble(4)
+    single, batched, b = run_pair(cases, "mixed", 3, tile=tile, order=order)
+    assert_same(single, batched)
+    assert b.schedule() == (tile, order)
+
+
+def test_navier_stokes_batch_and_case_parity():
+    """NS (BR1) cases, one of which already took a step before the batch was formed (its trace
+    buffers are at the other parity); mixed Riemann solvers across cases."""
+    from horses2d_amd import gpu
+    cases = [C.ns_tgv_case(4, 3), C.ns_tgv_case(5, 5, riemann="roe"), C.ns_cylinder_case(3, 12, N=4)]
+    single = make(cases, "fp64")
+    batched = make(cases, "fp64")
+    dts = np.array([d.estimate_dt(c.dt_max, c.cfl) for d, c in zip(single, cases)])
+    single[1].rk_step(0.0, dts[1])
+    batched[1].rk_step(0.0, dts[1])
+    for it in range(3):
+        for d, dt in zip(single, dts):
+            d.rk_step(it * dt, dt)
+    b = gpu.Batch(batched)
+    for it in range(3):
+        b.rk_step(it * dts, dts)
+    b.synchronize()
+    assert_same(single, batched)
+    # the handles stay usable on their own after the batch
+    for s, m, dt in zip(single, batched, dts):
+        s.rk_step(0.0, dt)
+        m.rk_step(0.0, dt)
+    assert_same(single, batched)
+
+
+def test_ip_batch_mixed():
+    cases = [C.ns_kh_case(4, 4), C.ns_kh_case(3, 6)]
+    single, batched, _ = run_pair(cases, "mixed", 3)
+    assert_same(single, batched)

@@ -1,0 +1,77 @@
+"""Mixed-precision form (FP64 state, RK accumulation and reductions; FP32 fluxes, Riemann solves,
+contractions, metric and gradients) against the FP64 form on the same inputs and time steps.
+
+The FP64 form matches the CPU oracle within 1e-12 (test_gpu_parity.py), so it stands in for the
+oracle here.  Errors are relative L2 norms over nodes per conserved quantity: mass, momentum
+(the vector (rho u, rho v), so that a small transverse component is measured against the flow's
+momentum) and energy.
+"""
+import numpy as np
+import pytest
+
+from horses2d_amd import cases as C
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    """[mass, momentum, energy] relative L2 errors of a against b."""
+    a = a.reshape(-1, 4); b = b.reshape(-1, 4)
+    d2 = np.sum((a - b) ** 2, axis=0); n2 = np.sum(b ** 2, axis=0)
+    return np.sqrt(np.array([d2[0] / n2[0], (d2[1] + d2[2]) / (n2[1] + n2[2]), d2[3] / n2[3]]))
+
+
+def run(case, precision, steps):
+    from horses2d_amd import gpu
+    dg = gpu.DGSEM(case, precision)
+    dg.upload(case.initial_state(dg.coords()))
+    ref = gpu.DGSEM(case, "fp64")
+    ref.upload(case.initial_state(ref.coords()))
+    dt = ref.estimate_dt(case.dt_max, case.cfl)
+    for it in range(steps):
+        dg.rk_step(it * dt, dt)
+    return dg
+
+
+CASES = {
+    "euler_roe": (lambda: C.euler_vortex_case(8, 4), 3e-7),
+    "euler_hll": (lambda: C.euler_vortex_case(6, 5, riemann="hll"), 3e-7),
+    "euler_exact": (lambda: C.euler_vortex_case(6, 3, riemann="exact"), 3e-7),
+    "euler_lf": (lambda: C.euler_vortex_case(6, 6, riemann="lf"), 3e-7),
+    "ns_br1_lf_tgv": (lambda: C.ns_tgv_case(6, 6, Re=1600.0), 3e-7),
+    "ns_ip_roe_kh": (lambda: C.ns_kh_case(6, 7, Re=1.0e4), 3e-7),
+    "ns_br1_cylinder": (lambda: C.ns_cylinder_case(4, 16, N=3, Re=200.0), 3e-7),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mixed_form_tracks_the_fp64_form(name):
+    make, bound = CASES[name]
+    case = make()
+    steps = 20
+    a = run(case, "mixed", steps).state()
+    b = run(case, "fp64", steps).state()
+    err = rel_l2(a, b)
+    print(name, "rel L2 after", steps, "steps:", err)
+    assert np.all(err <= bound), (name, err)
+
+
+def test_mixed_form_free_stream_and_conservation():
+    """A uniform state stays uniform, and on a periodic mesh the FP64 conservation sums hold."""
+    from horses2d_amd import gpu
+    case = C.ns_tgv_case(6, 5)
+    case.ic = C.uniform(C.freestream_state(0.1, 0.3))
+    dg = gpu.DGSEM(case, "mixed")
+    dg.upload(case.initial_state(dg.coords()))
+    qd = dg.time_derivative(0.0)
+    assert np.max(np.abs(qd)) < 1e-4          # FP32 rounding of O(1) fluxes times the N^2/h lift
+    case = C.ns_tgv_case(6, 5)
+    dg = gpu.DGSEM(case, "mixed")
+    dg.upload(case.initial_state(dg.coords()))
+    s0 = dg.conservation_sums()
+    dt = dg.estimate_dt(case.dt_max, case.cfl)
+    for it in range(10):
+        dg.rk_step(it * dt, dt)
+    s1 = dg.conservation_sums()
+    scale = np.abs(s0) + 1.0
+    assert np.all(np.abs(s1 - s0) <= 1e-9 * scale), (s0, s1)
