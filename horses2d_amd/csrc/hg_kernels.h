@@ -82,12 +82,14 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_prolong(StageArgs a) {
 template <int P, int EPB, bool BR1, typename R>
 struct GradSmem {
     using T = Tile<P, EPB>;
-    double sQ[T::NN * 4];
-    double sDQ[T::NN * 8];
-    double sLb[2 * P];
-    R sJa[BR1 ? T::NN * 4 : 1];
-    R sFu[BR1 ? T::NF * 8 : 1];
-    R sHat[T::PP], sLbw[2 * P], sD[T::PP];
+    alignas(16) double sQ[T::NN * 4];
+    alignas(16) double sDQ[T::NN * 8];
+    alignas(16) double sLb[2 * P];
+    alignas(16) R sJa[BR1 ? T::NN * 4 : 1];
+    alignas(16) R sFu[BR1 ? T::NF * 8 : 1];
+    alignas(16) R sHat[T::PP];
+    alignas(16) R sLbw[2 * P];
+    alignas(16) R sD[T::PP];
 };
 
 // One element tile [e0, e0+EPB) of the gradient; any workgroup size >= Tile<P, EPB>::TPB.
@@ -232,17 +234,19 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
 template <int P, int EPB, bool IP, typename R>
 struct StageSmem {
     using T = Tile<P, EPB>;
-    double sQ[T::NN * 4];
-    double sLb[2 * P];
-    double sRes[4][4];
-    R sCF[T::NN * 4];
-    R sCG[T::NN * 4];
-    R sF[T::NF * 4];
-    R sU[IP ? T::NF * 4 : 1];
-    R sV[IP ? T::NF * 4 : 1];
-    R sU2[IP ? T::NF * 4 : 1];
-    R sV2[IP ? T::NF * 4 : 1];
-    R sHat[T::PP], sLbw[2 * P], sDlbw[2 * P];
+    alignas(16) double sQ[T::NN * 4];
+    alignas(16) double sLb[2 * P];
+    alignas(16) double sRes[4][4];
+    alignas(16) R sCF[T::NN * 4];
+    alignas(16) R sCG[T::NN * 4];
+    alignas(16) R sF[T::NF * 4];
+    alignas(16) R sU[IP ? T::NF * 4 : 1];
+    alignas(16) R sV[IP ? T::NF * 4 : 1];
+    alignas(16) R sU2[IP ? T::NF * 4 : 1];
+    alignas(16) R sV2[IP ? T::NF * 4 : 1];
+    alignas(16) R sHat[T::PP];
+    alignas(16) R sLbw[2 * P];
+    alignas(16) R sDlbw[2 * P];
 };
 
 // One element tile [e0, e0+EPB) of a stage; any workgroup size >= Tile<P, EPB>::TPB (at most
