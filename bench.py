@@ -7,8 +7,9 @@ flux, RK3, seeded +-1e-3 velocity noise -- synthetic input of the stated shape.
 A "step" is one RK3 time step (three stages) of the whole mesh.  The FP64 form and the
 mixed form (FP64 state/RK, FP32 fluxes and contractions) are both timed over the same
 steps from the same initial state; the mixed form's relative L2 error against the FP64
-form after those steps is reported per field.  ``value`` is the mixed form's rate when that
-error is <= 1e-6 and the FP64 rate otherwise.
+form after those steps is reported per field (nodal and quadrature-weighted) and per conserved
+quantity (mass, momentum vector, energy).  ``value`` is the mixed form's rate when the
+conserved-quantity errors are all <= 1e-6 and the FP64 rate otherwise.
 
 Prints one JSON line.  With ``--gpus`` above 1 it exits: this build runs on one GPU.
 """
@@ -85,12 +86,29 @@ def time_form(case, precision, steps, warmup, Q0, dt):
 
 
 def rel_l2(a, b, w=None):
+    """Per-field relative L2 error of a against b (nodal, or weighted by w per node)."""
     a = a.reshape(-1, 4); b = b.reshape(-1, 4)
     if w is None:
         w = np.ones(a.shape[0])
     num = np.sqrt(np.sum(w[:, None] * (a - b) ** 2, axis=0))
     den = np.sqrt(np.sum(w[:, None] * b ** 2, axis=0))
     return num / den
+
+
+def rel_l2_conserved(a, b):
+    """Relative L2 errors of mass, momentum (the vector (rho u, rho v)) and energy."""
+    a = a.reshape(-1, 4); b = b.reshape(-1, 4)
+    d2 = np.sum((a - b) ** 2, axis=0); n2 = np.sum(b ** 2, axis=0)
+    return np.sqrt(np.array([d2[0] / n2[0], (d2[1] + d2[2]) / (n2[1] + n2[2]), d2[3] / n2[3]]))
+
+
+def quadrature_weights(case):
+    """w_i w_j J per node (the discrete L2 inner product of the DGSEM)."""
+    from horses2d_amd import gpu
+    _, _, jac = gpu.build_geometry(case)
+    xg, wg = np.polynomial.legendre.leggauss(int(case.N[0]) + 1)
+    w = 0.5 * wg
+    return np.tile(np.outer(w, w).T.reshape(-1), case.mesh.K) * jac
 
 
 def cpu_baseline(case, steps: int):
@@ -109,6 +127,73 @@ def cpu_baseline(case, steps: int):
     return case.ndof * stages * steps / wall, wall
 
 
+def ensemble_bench(steps: int, warmup: int, count: int = 64):
+    """Batched ensemble: ``count`` isentropic-vortex Euler cases (sides 32..128, N 3..8, seeded
+    vortices, own dt each).  Times the plain FP64 form (one handle and one stream per case),
+    the batched FP64 form and the batched mixed form over the same steps from the same states."""
+    from horses2d_amd import cases as C, gpu
+    cases = C.ensemble_cases(count)
+    ndof = sum(c.ndof for c in cases)
+    stages = 3
+
+    def build(precision):
+        hs = [gpu.DGSEM(c, precision) for c in cases]
+        q0 = [h.state() for h in hs]
+        return hs, q0
+
+    def reset(hs, q0):
+        for h, q in zip(hs, q0):
+            h.upload(q)
+            h.reset_dt_state()
+
+    hs, q0 = build("fp64")
+    dts = np.array([h.estimate_dt(c.dt_max, c.cfl) for h, c in zip(hs, cases)])
+    out = {"workload": f"euler_vortex_ensemble_{count}", "cases": count, "ndof": int(ndof),
+           "N": [int(c.N[0]) for c in cases], "sides": [int(round(np.sqrt(c.mesh.K))) for c in cases]}
+
+    def timed(step_fn, sync_fn):
+        for _ in range(warmup):
+            step_fn()
+        sync_fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step_fn()
+        sync_fn()
+        return time.perf_counter() - t0
+
+    # plain FP64 form: every case on its own handle and stream
+    reset(hs, q0)
+    wall = timed(lambda: [h.rk_step(0.0, dt) for h, dt in zip(hs, dts)], lambda: [h.synchronize() for h in hs])
+    out["fp64_per_case"] = {"dof_stage_per_s": ndof * stages * steps / wall, "ms_per_step": 1e3 * wall / steps}
+    # batched FP64 (warm-up steps advance the state, so reset and time again from q0 for the error)
+    reset(hs, q0)
+    b = gpu.Batch(hs)
+    wall = timed(lambda: b.rk_step(0.0, dts), b.synchronize)
+    out["batched_fp64"] = {"dof_stage_per_s": ndof * stages * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                           "schedule": list(b.schedule())}
+    ref = [h.state() for h in hs]
+    b.close()
+    for h in hs:
+        h.close()
+    hm, q0m = build("mixed")
+    for h, q in zip(hm, q0):
+        h.upload(q)
+    bm = gpu.Batch(hm)
+    wall = timed(lambda: bm.rk_step(0.0, dts), bm.synchronize)
+    errs = np.array([rel_l2_conserved(h.state(), r) for h, r in zip(hm, ref)])
+    out["batched_mixed"] = {"dof_stage_per_s": ndof * stages * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                            "schedule": list(bm.schedule()),
+                            "max_rel_l2_mass_momentum_energy_vs_fp64": [float(v) for v in errs.max(axis=0)],
+                            "after_steps": steps + warmup}
+    bm.close()
+    for h in hm:
+        h.close()
+    base = out["fp64_per_case"]["dof_stage_per_s"]
+    out["batched_fp64"]["speedup_vs_fp64_per_case"] = out["batched_fp64"]["dof_stage_per_s"] / base
+    out["batched_mixed"]["speedup_vs_fp64_per_case"] = out["batched_mixed"]["dof_stage_per_s"] / base
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,8 +201,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="tgv", choices=["tgv", "vortex", "kh", "cylinder"])
     ap.add_argument("--forms", default="fp64,mixed")
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble section")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it", file=sys.stderr)
@@ -157,15 +244,20 @@ def main():
     err = None
     if "fp64" in results and "mixed" in results:
         from horses2d_amd.cases import SCHEME  # noqa: F401
-        e_nodal = rel_l2(results["mixed"]["Q"], results["fp64"]["Q"])
-        linf = np.max(np.abs(results["mixed"]["Q"].reshape(-1, 4) - results["fp64"]["Q"].reshape(-1, 4)), axis=0) / \
-            np.max(np.abs(results["fp64"]["Q"].reshape(-1, 4)), axis=0)
-        err = {"rel_l2_nodal": [float(v) for v in e_nodal], "rel_linf": [float(v) for v in linf],
+        Qm, Qd = results["mixed"]["Q"], results["fp64"]["Q"]
+        e_nodal = rel_l2(Qm, Qd)
+        e_quad = rel_l2(Qm, Qd, quadrature_weights(case))
+        e_cons = rel_l2_conserved(Qm, Qd)
+        linf = np.max(np.abs(Qm.reshape(-1, 4) - Qd.reshape(-1, 4)), axis=0) / np.max(np.abs(Qd.reshape(-1, 4)), axis=0)
+        err = {"after_steps": args.steps,
+               "rel_l2_mass_momentum_energy": [float(v) for v in e_cons],
+               "rel_l2_nodal": [float(v) for v in e_nodal], "rel_l2_quadrature": [float(v) for v in e_quad],
+               "rel_linf": [float(v) for v in linf],
                "reference": "FP64 GPU form (matches the CPU oracle within 1e-12)"}
         out_forms["mixed"]["error_vs_fp64"] = err
         out_forms["mixed"]["speedup_vs_fp64"] = out_forms["mixed"]["dof_stage_per_s"] / out_forms["fp64"]["dof_stage_per_s"]
 
-    if "mixed" in out_forms and err is not None and max(err["rel_l2_nodal"]) <= 1e-6:
+    if "mixed" in out_forms and err is not None and max(err["rel_l2_mass_momentum_energy"]) <= 1e-6:
         best = "mixed"
     else:
         best = "fp64" if "fp64" in out_forms else forms[0]
@@ -197,6 +289,8 @@ def main():
                      "avg_launch_ms": bf["stage_kernel_avg_ms"]},
         "forms": out_forms,
     }
+    if not args.no_ensemble:
+        line["ensemble"] = ensemble_bench(args.ensemble_steps, 2)
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1))))
         os.environ["OMP_NUM_THREADS"] = str(threads)

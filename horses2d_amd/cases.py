@@ -311,5 +311,9 @@ def ensemble_cases(count: int = 64, seed_base: int = 1234, nmin: int = 32, nmax:
         xc, yc = (float(v) for v in rng.uniform(-2.0, 2.0, size=2))
         c = euler_vortex_case(n, N, steps, beta=beta, xc=xc, yc=yc)
         c.name = f"ensemble_{s}_{n}x{n}_N{N}"
+        # no CFL stated for the ensemble: the reference default (0.4 stays only where N <= 3;
+        # at N = 4 on the finer ensemble meshes CFL 0.4 amplifies rounding-level perturbations
+        # ~1e4-fold within 20 steps before they decay)
+        c.cfl = stable_cfl(N)
         out.append(c)
     return out
