@@ -204,12 +204,22 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble section")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run 5 device-to-device copies of 512 MiB (calibrates HBM byte counters)")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe:
+        import torch
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")     # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     from horses2d_amd import gpu
     case, phys = make_case(args.config)
     stages = {"rk3": 3, "rk5": 5, "euler": 1}[case.scheme]
