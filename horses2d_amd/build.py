@@ -20,15 +20,21 @@ SOURCES = ["hg_mesh.cpp", "hg_api.cpp", "hg_device.hip", "hg_batch.hip"]
 ORDERS = range(2, 12)   # points per direction P = N+1: one object per order from hg_inst.hip
 ARCH = os.environ.get("HG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+# -ffp-contract=on: multiply-adds are fused only within one source expression, decided before
+# optimisation, so every kernel that inlines the same device function (tile and solver
+# variants, the batched kernels) rounds it identically.
+COMMON = ["-O3", "-std=c++17", "-ffp-contract=on", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def _units():
     units = [(s, os.path.splitext(s)[0] + ".o", []) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    units += [("hg_inst.hip", f"hg_inst_p{p}.o", [f"-DHG_P={p}"]) for p in ORDERS]
-    # batched kernels: 2 * physics (Euler, NS BR1, NS IP) + mixed
-    units += [("hg_batch_inst.hip", f"hg_batch_v{v}.o", [f"-DHG_BV={v}"]) for v in range(6)]
+    # per order and equations (0 Euler, 1 NS BR1, 2 NS IP)
+    units += [("hg_inst.hip", f"hg_inst_p{p}_{ph}.o", [f"-DHG_P={p}", f"-DHG_PHYS={ph}"])
+              for p in ORDERS for ph in range(3)]
+    # batched kernels: variant 2 * physics + mixed, Riemann solver + 1 (0: chosen at run time)
+    units += [("hg_batch_inst.hip", f"hg_batch_v{v}_rs{r}.o", [f"-DHG_BV={v}", f"-DHG_RS1={r}"])
+              for v in range(6) for r in range(5)]
     return units
 
 

@@ -64,8 +64,6 @@ hg::StageArgs base_args(hg_handle *h) {
     a.Q = h->Q;
     a.G = h->G;
     a.QDot = h->QDot;
-    a.dQ = h->dQ;
-    a.dQ_out = h->dQ;
     a.dtr = h->dtr;
     a.dtr_out = h->dtr;
     a.metric = h->metric;
@@ -236,6 +234,11 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     else if (opt->ip_method == HG_IP_NIPG) { ph.ip_eps = 1.0; ph.ip_gradflux = 1; }
     else { ph.ip_eps = 0.0; ph.ip_gradflux = 0; }
     if (ph.viscous != 2) ph.ip_gradflux = 0;
+    // the stage kernel compiled for the interior solver, unless a zone asks for another one
+    ph.rs_kernel = opt->riemann;
+    for (const auto &Z : m.zones)
+        if (Z.riemann >= 0 && Z.riemann != opt->riemann) ph.rs_kernel = -1;
+    if (ph.rs_kernel < 0) h->tile[0] = 1;   // the run-time-solver form exists for the wide tile only
 
     HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // ---- host-side tables
@@ -309,7 +312,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
         for (int n = 0; n < PP; ++n) {
             size_t g = (size_t)e * PP + n;
             for (int c = 0; c < 4; ++c) metric[c * h->nodes + g] = el.Ja[c * PP + n];
-            metric[4 * h->nodes + g] = el.jac[n];
+            metric[4 * h->nodes + g] = 1.0 / el.jac[n];   // the kernels multiply by 1/jac
             jac[g] = el.jac[n];
         }
         double best = 1e300;
@@ -333,6 +336,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
                         v[3] += el.Ja[3 * PP + n] * lbv[l];  // Ja(IY,IY) -> Jeta_y
                         v[4] += el.jac[n] * lbv[l];
                     }
+                    v[4] = 1.0 / v[4];
                     for (int c = 0; c < 5; ++c) fmet[(((size_t)e * 4 + f) * P + k) * 5 + c] = v[c];
                 }
             }
@@ -352,7 +356,6 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(h->alloc_raw(&h->tr[1], tsz));
     if (ph.ns) {
         const size_t rs = h->mixed ? sizeof(float) : sizeof(double);
-        HG_CHECK(h->alloc_raw(&h->dQ, (size_t)h->nodes * 8 * rs));
         HG_CHECK(h->alloc_raw(&h->dtr, (size_t)K * 4 * P * 8 * rs));
     }
     HG_CHECK(upload_real(h.get(), &h->metric, metric, h->mixed));
@@ -587,6 +590,7 @@ int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile) {
     if (!h) return set_err("hg_set_tiles: null handle");
     if (stage_tile < -1 || stage_tile > 1 || gradient_tile < -1 || gradient_tile > 1)
         return set_err("hg_set_tiles: tile must be -1 (autotune), 0 (narrow) or 1 (wide)");
+    if (h->ph.rs_kernel < 0 && stage_tile != 1) stage_tile = 1;   // only the wide tile has the run-time solver form
     h->tile[0] = stage_tile;
     h->tile[1] = gradient_tile;
     return 0;

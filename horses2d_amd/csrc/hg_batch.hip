@@ -60,6 +60,7 @@ struct hg_batch {
     hg_batch_options opt{};
     Scheme sch{};
     int variant = 0, nslots = 0;
+    int rs = -1;                           // Riemann solver compiled into the stage kernel (-1: per face)
     hipStream_t stream = nullptr;
     std::vector<void *> bufs;
     hg::StageArgs *slots = nullptr;        // [case][nslots]
@@ -112,7 +113,7 @@ hipError_t enqueue_step(hg_batch *b, int parity) {
         hipError_t e;
         if (ns && (e = hg::launch_batch_gradient(b->variant, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess)
             return e;
-        if ((e = hg::launch_batch_stage(b->variant, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess) return e;
+        if ((e = hg::launch_batch_stage(b->variant, b->rs, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess) return e;
     }
     return hg::launch_batch_resid(b->resid_part, b->ranges[b->tile], b->resid_out, (int)b->cases.size(), b->stream);
 }
@@ -135,7 +136,7 @@ int autotune(hg_batch *b) {
             auto run = [&]() -> hipError_t {
                 hipError_t e = hipSuccess;
                 if (ns) e = hg::launch_batch_gradient(b->variant, t, a, b->block[t], b->stream);
-                if (e == hipSuccess) e = hg::launch_batch_stage(b->variant, t, a, b->block[t], b->stream);
+                if (e == hipSuccess) e = hg::launch_batch_stage(b->variant, b->rs, t, a, b->block[t], b->stream);
                 return e;
             };
             HB_CHECK(run());
@@ -197,6 +198,9 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     b->sch = scheme_of(h0->opt.scheme);
     b->variant = 2 * (h0->ph.ns ? (h0->ph.viscous == 2 ? 2 : 1) : 0) + (h0->mixed ? 1 : 0);
     b->nslots = 2 * b->sch.nst + 1;
+    b->rs = h0->ph.rs_kernel;
+    for (int c = 1; c < ncases; ++c)
+        if (cases[c]->ph.rs_kernel != b->rs) b->rs = -1;
     HB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto *h : b->cases) HB_CHECK(hipStreamSynchronize(h->stream));
 

@@ -18,18 +18,18 @@ struct TileOf {
     static constexpr int EPB = TILE == 0 ? Tiles<P>::small : Tiles<P>::large;
 };
 
-template <int TILE, bool IP, typename R>
+template <int TILE, bool NS, bool IP, typename R>
 union BatchStageSmem {
-    StageSmem<2, TileOf<2, TILE>::EPB, IP, R> p2;
-    StageSmem<3, TileOf<3, TILE>::EPB, IP, R> p3;
-    StageSmem<4, TileOf<4, TILE>::EPB, IP, R> p4;
-    StageSmem<5, TileOf<5, TILE>::EPB, IP, R> p5;
-    StageSmem<6, TileOf<6, TILE>::EPB, IP, R> p6;
-    StageSmem<7, TileOf<7, TILE>::EPB, IP, R> p7;
-    StageSmem<8, TileOf<8, TILE>::EPB, IP, R> p8;
-    StageSmem<9, TileOf<9, TILE>::EPB, IP, R> p9;
-    StageSmem<10, TileOf<10, TILE>::EPB, IP, R> p10;
-    StageSmem<11, TileOf<11, TILE>::EPB, IP, R> p11;
+    StageSmem<2, TileOf<2, TILE>::EPB, NS, IP, R> p2;
+    StageSmem<3, TileOf<3, TILE>::EPB, NS, IP, R> p3;
+    StageSmem<4, TileOf<4, TILE>::EPB, NS, IP, R> p4;
+    StageSmem<5, TileOf<5, TILE>::EPB, NS, IP, R> p5;
+    StageSmem<6, TileOf<6, TILE>::EPB, NS, IP, R> p6;
+    StageSmem<7, TileOf<7, TILE>::EPB, NS, IP, R> p7;
+    StageSmem<8, TileOf<8, TILE>::EPB, NS, IP, R> p8;
+    StageSmem<9, TileOf<9, TILE>::EPB, NS, IP, R> p9;
+    StageSmem<10, TileOf<10, TILE>::EPB, NS, IP, R> p10;
+    StageSmem<11, TileOf<11, TILE>::EPB, NS, IP, R> p11;
 };
 
 template <int TILE, bool BR1, typename R>
@@ -58,11 +58,11 @@ __device__ inline int4 uniform_item(const int4 *__restrict__ items, const int *_
                      __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
 }
 
-template <int TILE, bool NS, bool IP, typename R, typename TR>
+template <int TILE, bool NS, bool IP, int RS, typename R, typename TR>
 __global__ __launch_bounds__(256) void k_batch_stage(const StageArgs *__restrict__ slots, const int4 *__restrict__ items,
                                                      const int *__restrict__ sched, const double *__restrict__ dt,
                                                      BatchArgs b) {
-    __shared__ BatchStageSmem<TILE, IP, R> S;
+    __shared__ BatchStageSmem<TILE, NS, IP, R> S;
     const int k = blockIdx.x;
     if (k < b.nitems) {
         const int4 it = uniform_item(items, sched, k);
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(256) void k_batch_stage(const StageArgs *__restrict
             if (b.dt_cq) cq *= d;
             if (b.dt_cg) cg *= d;
         }
-#define HG_CALL(PV, M) stage_body<PV, TileOf<PV, TILE>::EPB, NS, IP, R, TR>(a, it.y, it.w, cq, cg, S.M)
+#define HG_CALL(PV, M) stage_body<PV, TileOf<PV, TILE>::EPB, NS, IP, RS, R, TR>(a, it.y, it.w, cq, cg, S.M)
         switch (it.z) {
             HG_BATCH_CASE(2, HG_CALL)
             HG_BATCH_CASE(3, HG_CALL)
@@ -121,13 +121,13 @@ __global__ __launch_bounds__(256) void k_batch_gradient(const StageArgs *__restr
 #undef HG_BATCH_CASE
 
 // Launchers of one physics/precision variant (instantiated by hg_batch_inst.hip); grid = nitems.
-template <bool NS, bool IP, typename R, typename TR>
+template <bool NS, bool IP, int RS, typename R, typename TR>
 hipError_t batch_stage_variant(int tile, const BatchArgs &b, int block, hipStream_t st) {
     if (tile == 0)
-        hipLaunchKernelGGL((k_batch_stage<0, NS, IP, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
+        hipLaunchKernelGGL((k_batch_stage<0, NS, IP, RS, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
                            b.sched, b.dt, b);
     else
-        hipLaunchKernelGGL((k_batch_stage<1, NS, IP, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
+        hipLaunchKernelGGL((k_batch_stage<1, NS, IP, RS, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
                            b.sched, b.dt, b);
     return hipGetLastError();
 }

@@ -25,6 +25,7 @@ struct PhysParams {
     // Reference pressure of the FP32 form (pref, Eref = pref/(gamma-1)); zero in the FP64 form.
     double pref, Eref;
     int riemann, viscous, ip_gradflux, ns;
+    int rs_kernel;  // Riemann solver compiled into the stage kernel (0..3), -1: chosen per face at run time
 };
 
 // Everything a stage or gradient kernel reads or writes.  Uniform P per handle.
@@ -36,8 +37,6 @@ struct StageArgs {
     double *QDot;               // derivative output (derivative-only mode) or nullptr
     const void *tr_in;          // Q traces [(e*4+f)*P + k][4] (FP64 or FP32)
     void *tr_out;               // next-stage Q traces
-    const void *dQ;             // gradient [node][2][4] (NS)
-    void *dQ_out;               // gradient output (gradient kernels)
     const void *dtr;            // gradient traces [(e*4+f)*P + k][2][4]
     void *dtr_out;
     const void *metric;         // 5 planes of nodes: Ja11, Ja12, Ja21, Ja22, jac
@@ -68,16 +67,23 @@ struct BatchArgs {
 };
 
 // variant = 2 * physics (0 Euler, 1 Navier-Stokes BR1, 2 Navier-Stokes IP) + (1 if mixed)
-hipError_t launch_batch_stage(int variant, int tile, const BatchArgs &b, int block, hipStream_t s);
+hipError_t launch_batch_stage(int variant, int rs, int tile, const BatchArgs &b, int block, hipStream_t s);
 hipError_t launch_batch_gradient(int variant, int tile, const BatchArgs &b, int block, hipStream_t s);
 // out[c][0..4) = max over the residual partials [ranges[c].x, ranges[c].y) (one workgroup per case)
 hipError_t launch_batch_resid(const double *part, const int2 *ranges, double *const *out, int ncases, hipStream_t s);
 int tile_threads(int P, int tile);  // workgroup size of a tile
 int tile_elems(int P, int tile);    // elements per tile
 
-// Per-variant launchers (one object file each, from hg_batch_inst.hip).
-#define HG_DECLARE_BATCH_VARIANT(V)                                                                          \
-    hipError_t batch_stage_v##V(int tile, const BatchArgs &b, int block, hipStream_t s);          \
+// Per-variant launchers (one object file per variant and Riemann solver, from
+// hg_batch_inst.hip): batch_stage_v<V>_rs<RS + 1>, batch_gradient_v<V>.
+#define HG_DECLARE_BATCH_STAGE(V, RS1) \
+    hipError_t batch_stage_v##V##_rs##RS1(int tile, const BatchArgs &b, int block, hipStream_t s);
+#define HG_DECLARE_BATCH_VARIANT(V)                                                            \
+    HG_DECLARE_BATCH_STAGE(V, 0)                                                               \
+    HG_DECLARE_BATCH_STAGE(V, 1)                                                               \
+    HG_DECLARE_BATCH_STAGE(V, 2)                                                               \
+    HG_DECLARE_BATCH_STAGE(V, 3)                                                               \
+    HG_DECLARE_BATCH_STAGE(V, 4)                                                               \
     hipError_t batch_gradient_v##V(int tile, const BatchArgs &b, int block, hipStream_t s);
 HG_DECLARE_BATCH_VARIANT(0)
 HG_DECLARE_BATCH_VARIANT(1)
@@ -86,6 +92,7 @@ HG_DECLARE_BATCH_VARIANT(3)
 HG_DECLARE_BATCH_VARIANT(4)
 HG_DECLARE_BATCH_VARIANT(5)
 #undef HG_DECLARE_BATCH_VARIANT
+#undef HG_DECLARE_BATCH_STAGE
 
 // Launchers (return hipError_t of the launch).
 hipError_t launch_prolong(int P, bool mixed, const StageArgs &a, hipStream_t s);
@@ -110,7 +117,7 @@ hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const dou
 bool supported_order(int N);
 
 // Per-order launchers, explicitly instantiated once per P (one object file per order).
-template <int P> hipError_t stage_p(int tile, bool mixed, const StageArgs &a, hipStream_t s);
+template <int P, int PHYS> hipError_t stage_phys_p(int tile, bool mixed, const StageArgs &a, hipStream_t s);
 template <int P> int stage_blocks_p(int tile, int K);
 template <int P> int tile_threads_p(int tile);
 template <int P> int tile_elems_p(int tile);

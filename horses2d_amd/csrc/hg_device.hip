@@ -119,10 +119,17 @@ hipError_t launch_prolong(int P, bool mixed, const StageArgs &a, hipStream_t st)
 #undef HG_CALL
 }
 
-hipError_t launch_stage(int P, int tile, bool mixed, const StageArgs &a, hipStream_t st) {
-#define HG_CALL(PV) stage_p<PV>(tile, mixed, a, st)
+template <int PHYS>
+static hipError_t launch_stage_phys(int P, int tile, bool mixed, const StageArgs &a, hipStream_t st) {
+#define HG_CALL(PV) stage_phys_p<PV, PHYS>(tile, mixed, a, st)
     HG_DISPATCH_P(P, HG_CALL)
 #undef HG_CALL
+}
+
+hipError_t launch_stage(int P, int tile, bool mixed, const StageArgs &a, hipStream_t st) {
+    if (!a.ph.ns) return launch_stage_phys<0>(P, tile, mixed, a, st);
+    if (a.ph.viscous == 2) return launch_stage_phys<2>(P, tile, mixed, a, st);
+    return launch_stage_phys<1>(P, tile, mixed, a, st);
 }
 
 hipError_t launch_gradient(int P, int tile, bool mixed, const StageArgs &a, hipStream_t st) {
@@ -240,16 +247,26 @@ hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const dou
 
 namespace hg {
 
-hipError_t launch_batch_stage(int variant, int tile, const BatchArgs &b, int block, hipStream_t st) {
+hipError_t launch_batch_stage(int variant, int rs, int tile, const BatchArgs &b, int block, hipStream_t st) {
+#define HG_RS(V)                                                    \
+    switch (rs) {                                                   \
+    case -1: return batch_stage_v##V##_rs0(tile, b, block, st);     \
+    case 0: return batch_stage_v##V##_rs1(tile, b, block, st);      \
+    case 1: return batch_stage_v##V##_rs2(tile, b, block, st);      \
+    case 2: return batch_stage_v##V##_rs3(tile, b, block, st);      \
+    case 3: return batch_stage_v##V##_rs4(tile, b, block, st);      \
+    default: return hipErrorInvalidValue;                           \
+    }
     switch (variant) {
-    case 0: return batch_stage_v0(tile, b, block, st);
-    case 1: return batch_stage_v1(tile, b, block, st);
-    case 2: return batch_stage_v2(tile, b, block, st);
-    case 3: return batch_stage_v3(tile, b, block, st);
-    case 4: return batch_stage_v4(tile, b, block, st);
-    case 5: return batch_stage_v5(tile, b, block, st);
+    case 0: HG_RS(0)
+    case 1: HG_RS(1)
+    case 2: HG_RS(2)
+    case 3: HG_RS(3)
+    case 4: HG_RS(4)
+    case 5: HG_RS(5)
     default: return hipErrorInvalidValue;
     }
+#undef HG_RS
 }
 
 hipError_t launch_batch_gradient(int variant, int tile, const BatchArgs &b, int block, hipStream_t st) {

@@ -32,7 +32,7 @@ struct hg_handle {
     // device arrays
     double *Q = nullptr, *G = nullptr, *QDot = nullptr;
     void *tr[2] = {nullptr, nullptr};
-    void *dQ = nullptr, *dtr = nullptr;
+    void *dtr = nullptr;
     void *metric = nullptr, *fmet = nullptr;
     int4 *face = nullptr;
     int *forder = nullptr;

@@ -3,8 +3,11 @@
 // Data layout in HBM (uniform polynomial order P = N+1 per handle):
 //   Q, G      : FP64, reference layout  Q[(e*P*P + i + P*j)*4 + eq]
 //   traces    : [(e*4 + f)*P + k][4]  element-local face orientation, FP64 or FP32
-//   gradient  : dQ[(e*P*P + n)*8 + d*4 + eq], FP64 or FP32
-//   metric    : five planes of e*P*P+n: Ja11, Ja12, Ja21, Ja22, jac (FP64 or FP32)
+//   gradient traces: [(e*4 + f)*P + k][d*4 + eq], FP64 or FP32 (the gradient at the nodes is
+//                    recomputed inside each stage and never stored)
+//   metric    : five planes of e*P*P+n: Ja11, Ja12, Ja21, Ja22, 1/jac (FP64 or FP32); the
+//               divisions by the Jacobian (DGSpatialDiscretizationMethods.f90:226-233) are
+//               multiplications by its reciprocal (a last-bit difference, not bit-exact)
 //
 // Tiling: a workgroup owns EPB whole elements (one lane per node, one lane per face node);
 // two tile shapes per order are compiled (the widest that fits one wavefront, and the widest
@@ -74,17 +77,142 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_prolong(StageArgs a) {
     }
 }
 
-// ------------------------------------------------------------------ gradient kernel
-// BR1: dQ = ( -Vector_StdVolumeGreen(Q) + sum_faces Vector_StdFace(uStar n dS) ) / jac
-// (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622, DGViscousMethods.f90:538-917).
-// IP: interior gradient (QuadElement_QuadProcedures.incf:42-88).
-// Writes dQ at the nodes and the gradient traces of the four faces.
+// ------------------------------------------------------------------ gradient helpers
+// Both the trace kernel (k_gradient) and the stage kernel evaluate the element gradient with
+// these functions, so the gradient a stage recomputes for its own element is the one whose
+// face traces its neighbours read.
+
+// The state slot the compute type sees (energy shifted in the FP32 form).
+template <typename R>
+__device__ __forceinline__ R qslot(const PhysParams &pp, double v, int c) {
+    return (R)(c == 3 ? slot_energy<R>(pp, v) : v);
+}
+
+// BR1 solution flux at a face node: uStar n dS in both directions (DGViscous_BR1.f90:313-328;
+// boundary faces use the ghost state's uSB, DGViscousMethods.f90:786-917).
+template <typename R>
+__device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> &ph, int flags, int y, const R *qs,
+                                              const R *qo, R nx, R ny, R dS, R *out) {
+    const int kind = flags & 15;
+    R us[4];
+    if (kind <= FK_PERIODIC) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) us[c] = R(0.5) * (qs[c] + qo[c]);
+        if (kind == FK_INT_R) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) us[c] = -us[c];
+        }
+    } else {
+        R uB[4];
+        int neu;
+        ghost_state<R>(kind, flags, a.zone + 8 * y, ph, qs, nx, ny, uB, us, neu);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) { out[c] = us[c] * dS * nx; out[4 + c] = us[c] * dS * ny; }
+}
+
+// BR1 gradient at node (i, j): ( -Vector_StdVolumeGreen(Q) + sum_faces Vector_StdFace(uStar) ) / jac
+// (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622), faces lifted in edge-id order.
+// q: element state [n*4], Ja: element metric [n*4], Fu: element face fluxes [(f*P + k)*8].
+template <int P, typename R>
+__device__ __forceinline__ void br1_node_gradient(const PhysParams &pp, const double *q, const R *Ja, const R *sHat,
+                                                  const R *Fu, const R *sLbw, int ord, int i, int j, R ijac, R *dq) {
+    // Every accumulator receives its xi terms (l = 0..P-1) then its eta terms; each state value
+    // is read once and used for both directions, which keeps the live set small.
+    R acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = R(0);
+#pragma unroll
+    for (int l = 0; l < P; ++l) {
+        const int m = l + P * j;
+        const R h = sHat[i * P + l];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const R v = qslot<R>(pp, q[m * 4 + c], c);
+            acc[c] += v * Ja[m * 4 + 0] * h;
+            acc[4 + c] += v * Ja[m * 4 + 2] * h;
+        }
+    }
+#pragma unroll
+    for (int l = 0; l < P; ++l) {
+        const int m = i + P * l;
+        const R h = sHat[j * P + l];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const R v = qslot<R>(pp, q[m * 4 + c], c);
+            acc[c] += v * Ja[m * 4 + 1] * h;
+            acc[4 + c] += v * Ja[m * 4 + 3] * h;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) dq[c] = -acc[c];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) {
+        const int f = (ord >> (2 * o)) & 3;
+        const R *F = Fu + (f * P + ((f == 0 || f == 2) ? i : j)) * 8;
+        const R lw = (f == 0) ? sLbw[j] : (f == 1) ? sLbw[P + i] : (f == 2) ? sLbw[P + j] : sLbw[i];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) dq[c] += F[c] * lw;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) dq[c] = dq[c] * ijac;
+}
+
+// Interior (broken) gradient of the IP method (QuadElement_QuadProcedures.incf:42-88).
+template <int P, typename R>
+__device__ __forceinline__ void ip_node_gradient(const PhysParams &pp, const double *q, const R *sD, int i, int j, R j11,
+                                                 R j12, R j21, R j22, R ijac, R *dq) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        R dxi = R(0), deta = R(0);
+#pragma unroll
+        for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * qslot<R>(pp, q[(l + P * j) * 4 + c], c);
+#pragma unroll
+        for (int l = 0; l < P; ++l) deta += sD[j * P + l] * qslot<R>(pp, q[(i + P * l) * 4 + c], c);
+        dq[c] = (dxi * j11 + deta * j12) * ijac;
+        dq[4 + c] = (dxi * j21 + deta * j22) * ijac;
+    }
+}
+
+// Face-node geometry and the neighbour's Q trace (periodic and interior faces).
+template <int P, typename R, typename TR>
+__device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, int fk, int4 &fr, R &nx, R &ny, R &dS,
+                                          R *qo, int64_t &nslot) {
+    fr = a.face[fe * 4 + ff];
+    const bool curved = fr.x & FF_CURVED, selfrev = fr.x & FF_SELFREV, rev = fr.x & FF_REV;
+    const int gi = curved ? (selfrev ? P - 1 - fk : fk) : 0;
+    const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
+    nx = (R)geo[0]; ny = (R)geo[1]; dS = (R)geo[2];
+    nslot = (int64_t)fr.y * P + (rev ? P - 1 - fk : fk);
+    if ((fr.x & 15) <= FK_PERIODIC) {
+        const TR *nbt = (const TR *)a.tr_in + nslot * 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
+    }
+}
+
+// Own Q trace of a face node from the LDS tile, in the compute type, rounded like the stored
+// traces the neighbour reads.
+template <int P, typename R, typename TR>
+__device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe, const double *sLb, int ff, int fk,
+                                          R *qs) {
+    double tq[4];
+    face_trace<P, 4, 4>(sQe, sLb, ff, fk, tq);
+    tq[3] = slot_energy<TR>(a.ph, tq[3]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) qs[c] = (R)(TR)tq[c];
+}
+
+// ------------------------------------------------------------------ gradient trace kernel
+// Evaluates the element gradient (BR1 or IP) and writes the gradient traces of the element's
+// four faces, which the neighbours' stage kernels read.  The gradient at the nodes is not
+// stored: each stage kernel recomputes it for its own element.
 template <int P, int EPB, bool BR1, typename R>
 struct GradSmem {
     using T = Tile<P, EPB>;
     alignas(16) double sQ[T::NN * 4];
-    alignas(16) double sDQ[T::NN * 8];
     alignas(16) double sLb[2 * P];
+    alignas(16) R sDQ[T::NN * 8];
     alignas(16) R sJa[BR1 ? T::NN * 4 : 1];
     alignas(16) R sFu[BR1 ? T::NF * 8 : 1];
     alignas(16) R sHat[T::PP];
@@ -98,8 +226,8 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     using T = Tile<P, EPB>;
     constexpr int PP = T::PP, NN = T::NN, NF = T::NF;
     const int TPB = blockDim.x;
-    double *sQ = S.sQ, *sDQ = S.sDQ, *sLb = S.sLb;
-    R *sJa = S.sJa, *sFu = S.sFu, *sHat = S.sHat, *sLbw = S.sLbw, *sD = S.sD;
+    double *sQ = S.sQ, *sLb = S.sLb;
+    R *sDQ = S.sDQ, *sJa = S.sJa, *sFu = S.sFu, *sHat = S.sHat, *sLbw = S.sLbw, *sD = S.sD;
     const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
     if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; }
@@ -114,13 +242,15 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     // ---- hoisted global loads
     const int s = t / PP, n = t % PP, e = e0 + s;
     const bool node_active = (t < NN) && (e < a.K);
-    R jac = R(1), j11 = R(0), j12 = R(0), j21 = R(0), j22 = R(0);
+    R ijac = R(1), j11 = R(0), j12 = R(0), j21 = R(0), j22 = R(0);
+    int ord = 0;
     if (node_active) {
         const int64_t g = nb0 + s * PP + n;
         j11 = ld<R>(a.metric, 0 * a.nodes + g); j12 = ld<R>(a.metric, 1 * a.nodes + g);
         j21 = ld<R>(a.metric, 2 * a.nodes + g); j22 = ld<R>(a.metric, 3 * a.nodes + g);
-        jac = ld<R>(a.metric, 4 * a.nodes + g);
+        ijac = ld<R>(a.metric, 4 * a.nodes + g);
         if (BR1) {
+            ord = a.forder[e];
             sJa[t * 4 + 0] = j11; sJa[t * 4 + 1] = j12; sJa[t * 4 + 2] = j21; sJa[t * 4 + 3] = j22;
         }
     }
@@ -128,88 +258,22 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     const bool face_active = BR1 && (t < NF) && (fe < a.K);
     int4 fr = make_int4(0, 0, 0, 0);
     R nx = R(0), ny = R(0), dS = R(0), qo[4] = {R(0), R(0), R(0), R(0)};
-    if (face_active) {
-        fr = a.face[fe * 4 + ff];
-        const bool curved = fr.x & FF_CURVED, selfrev = fr.x & FF_SELFREV, rev = fr.x & FF_REV;
-        const int gi = curved ? (selfrev ? P - 1 - fk : fk) : 0;
-        const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
-        nx = (R)geo[0]; ny = (R)geo[1]; dS = (R)geo[2];
-        if ((fr.x & 15) <= FK_PERIODIC) {
-            const TR *nbt = (const TR *)a.tr_in + ((int64_t)fr.y * P + (rev ? P - 1 - fk : fk)) * 4;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
-        }
-    }
+    int64_t nslot = 0;
+    if (face_active) load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
     __syncthreads();
-    // ---- face phase (BR1): solution Riemann flux uStar dS n
     if (face_active) {
-        const int kind = fr.x & 15;
-        double tq[4];
-        face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
-        tq[3] = slot_energy<TR>(a.ph, tq[3]);
-        R qs[4] = {(R)(TR)tq[0], (R)(TR)tq[1], (R)(TR)tq[2], (R)(TR)tq[3]};
-        R us[4];
-        if (kind <= FK_PERIODIC) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) us[c] = R(0.5) * (qs[c] + qo[c]);
-            if (kind == FK_INT_R) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) us[c] = -us[c];
-            }
-        } else {
-            R uB[4];
-            int neu;
-            ghost_state<R>(kind, fr.x, a.zone + 8 * fr.y, ph, qs, nx, ny, uB, us, neu);
-        }
-        R *o = sFu + t * 8;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) { o[c] = us[c] * dS * nx; o[4 + c] = us[c] * dS * ny; }
+        R qs[4];
+        own_trace<P, R, TR>(a, sQ + fs * PP * 4, sLb, ff, fk, qs);
+        br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 8);
     }
     if (BR1) __syncthreads();
     if (node_active) {
         const int i = n % P, j = n / P;
-        const double *q = sQ + s * PP * 4;
-        auto qslot = [&](double v, int c) -> R { return (R)(c == 3 ? slot_energy<R>(a.ph, v) : v); };
         R dq[8];
-        if (BR1) {
-            const R *Ja = sJa + s * PP * 4;
+        if (BR1) br1_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sJa + s * PP * 4, sHat, sFu + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        else ip_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
 #pragma unroll
-            for (int d = 0; d < 2; ++d)
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    R acc = R(0);
-#pragma unroll
-                    for (int l = 0; l < P; ++l) acc += qslot(q[(l + P * j) * 4 + c], c) * Ja[(l + P * j) * 4 + d * 2 + 0] * sHat[i * P + l];
-#pragma unroll
-                    for (int l = 0; l < P; ++l) acc += qslot(q[(i + P * l) * 4 + c], c) * Ja[(i + P * l) * 4 + d * 2 + 1] * sHat[j * P + l];
-                    dq[d * 4 + c] = -acc;
-                }
-            const int ord = a.forder[e];
-#pragma unroll
-            for (int o = 0; o < 4; ++o) {
-                const int f = (ord >> (2 * o)) & 3;
-                const R *Fu = sFu + ((s * 4 + f) * P + ((f == 0 || f == 2) ? i : j)) * 8;
-                const R lw = (f == 0) ? sLbw[j] : (f == 1) ? sLbw[P + i] : (f == 2) ? sLbw[P + j] : sLbw[i];
-#pragma unroll
-                for (int c = 0; c < 8; ++c) dq[c] += Fu[c] * lw;
-            }
-#pragma unroll
-            for (int c = 0; c < 8; ++c) dq[c] = dq[c] / jac;
-        } else {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                R dxi = R(0), deta = R(0);
-#pragma unroll
-                for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * qslot(q[(l + P * j) * 4 + c], c);
-#pragma unroll
-                for (int l = 0; l < P; ++l) deta += sD[j * P + l] * qslot(q[(i + P * l) * 4 + c], c);
-                dq[c] = (dxi * j11 + deta * j12) / jac;
-                dq[4 + c] = (dxi * j21 + deta * j22) / jac;
-            }
-        }
-        R *go = (R *)a.dQ_out + (nb0 + s * PP + n) * 8;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) { go[c] = dq[c]; sDQ[t * 8 + c] = (double)dq[c]; }
+        for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
     }
     __syncthreads();
     if (t < NF) {
@@ -231,40 +295,48 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
 }
 
 // ------------------------------------------------------------------ the stage kernel
-template <int P, int EPB, bool IP, typename R>
+template <int P, int EPB, bool NS, bool IP, typename R>
 struct StageSmem {
     using T = Tile<P, EPB>;
     alignas(16) double sQ[T::NN * 4];
     alignas(16) double sLb[2 * P];
     alignas(16) double sRes[4][4];
-    alignas(16) R sCF[T::NN * 4];
+    alignas(16) R sDQ[NS ? T::NN * 8 : 1];
+    alignas(16) R sCF[T::NN * 4];              // BR1: the metric Ja during the gradient phase
     alignas(16) R sCG[T::NN * 4];
-    alignas(16) R sF[T::NF * 4];
+    alignas(16) R sF[NS ? T::NF * 8 : T::NF * 4];  // BR1: uStar fluxes during the gradient phase
     alignas(16) R sU[IP ? T::NF * 4 : 1];
     alignas(16) R sV[IP ? T::NF * 4 : 1];
     alignas(16) R sU2[IP ? T::NF * 4 : 1];
     alignas(16) R sV2[IP ? T::NF * 4 : 1];
     alignas(16) R sHat[T::PP];
+    alignas(16) R sD[IP ? T::PP : 1];
     alignas(16) R sLbw[2 * P];
     alignas(16) R sDlbw[2 * P];
 };
 
 // One element tile [e0, e0+EPB) of a stage; any workgroup size >= Tile<P, EPB>::TPB (at most
 // 256).  rslot is the residual partial this tile writes; cq, cg are the stage's RK factors.
-template <int P, int EPB, bool NS, bool IP, typename R, typename TR>
+// Navier-Stokes: the element's own gradient (nodes and face traces) is recomputed here from
+// the LDS tile and the neighbours' Q traces; only the neighbours' gradient traces are read.
+template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, const int64_t rslot, const double cq,
-                                           const double cg, StageSmem<P, EPB, IP, R> &S) {
+                                           const double cg, StageSmem<P, EPB, NS, IP, R> &S) {
     using T = Tile<P, EPB>;
     constexpr int PP = T::PP, NN = T::NN, NF = T::NF;
+    constexpr bool BR1 = NS && !IP;
     const int TPB = blockDim.x;
     double *sQ = S.sQ, *sLb = S.sLb;
-    R *sCF = S.sCF, *sCG = S.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
-    R *sHat = S.sHat, *sLbw = S.sLbw, *sDlbw = S.sDlbw;
+    R *sDQ = S.sDQ, *sCF = S.sCF, *sCG = S.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
+    R *sHat = S.sHat, *sD = S.sD, *sLbw = S.sLbw, *sDlbw = S.sDlbw;
     const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
     const bool ipgrad = IP && a.ph.ip_gradflux;
     if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; sDlbw[t] = (R)a.dlbw[t]; }
-    for (int x = t; x < PP; x += TPB) sHat[x] = (R)a.hatD[x];
+    for (int x = t; x < PP; x += TPB) {
+        sHat[x] = (R)a.hatD[x];
+        if (IP) sD[x] = (R)a.D[x];
+    }
     const int ne = min(EPB, a.K - e0);
     const int64_t nb0 = (int64_t)e0 * PP;
     {
@@ -277,22 +349,15 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     const int s = t / PP, n = t % PP, e = e0 + s;
     const bool node_active = (t < NN) && (e < a.K);
     const int64_t g = nb0 + s * PP + n;
-    R j11 = R(0), j12 = R(0), j21 = R(0), j22 = R(0), jac = R(1);
-    R dqn[NS ? 8 : 1];
-    double gold[4] = {0.0, 0.0, 0.0, 0.0};
+    R j11 = R(0), j12 = R(0), j21 = R(0), j22 = R(0), ijac = R(1);
+    int ord = 0;
     if (node_active) {
         j11 = ld<R>(a.metric, 0 * a.nodes + g); j12 = ld<R>(a.metric, 1 * a.nodes + g);
         j21 = ld<R>(a.metric, 2 * a.nodes + g); j22 = ld<R>(a.metric, 3 * a.nodes + g);
-        jac = ld<R>(a.metric, 4 * a.nodes + g);
-        if (NS) {
-            const TR *dqp = (const TR *)a.dQ + g * 8;
-#pragma unroll
-            for (int c = 0; c < (NS ? 8 : 1); ++c) dqn[c] = (R)dqp[c];
-        }
-        if (a.mode == 0 && !a.first_stage) {
-            const double2 *gp = (const double2 *)(a.G + g * 4);
-            double2 g01 = gp[0], g23 = gp[1];
-            gold[0] = g01.x; gold[1] = g01.y; gold[2] = g23.x; gold[3] = g23.y;
+        ijac = ld<R>(a.metric, 4 * a.nodes + g);
+        ord = a.forder[e];
+        if (BR1) {
+            sCF[t * 4 + 0] = j11; sCF[t * 4 + 1] = j12; sCF[t * 4 + 2] = j21; sCF[t * 4 + 3] = j22;
         }
     }
     // ---- hoisted global loads: face lane
@@ -300,33 +365,37 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     const bool face_active = (t < NF) && (fe < a.K);
     int4 fr = make_int4(0, 0, 0, 0);
     R nx = R(0), ny = R(0), dS = R(0), qo[4] = {R(0), R(0), R(0), R(0)};
-    R ds[NS ? 8 : 1], dn[NS ? 8 : 1];
-    R invh = R(0);
-    if (face_active) {
-        fr = a.face[fe * 4 + ff];
-        const bool curved = fr.x & FF_CURVED, selfrev = fr.x & FF_SELFREV, rev = fr.x & FF_REV;
-        const int gi = curved ? (selfrev ? P - 1 - fk : fk) : 0;
-        const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
-        nx = (R)geo[0]; ny = (R)geo[1]; dS = (R)geo[2];
-        const bool inner = (fr.x & 15) <= FK_PERIODIC;
-        const int64_t nslot = (int64_t)fr.y * P + (rev ? P - 1 - fk : fk);
-        if (inner) {
-            const TR *nbt = (const TR *)a.tr_in + nslot * 4;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
-        }
-        if (NS) {
-            const int64_t oslot = (int64_t)(fe * 4 + ff) * P + fk;
-#pragma unroll
-            for (int c = 0; c < (NS ? 8 : 1); ++c) ds[c] = (R)((const TR *)a.dtr)[oslot * 8 + c];
-            if (inner) {
-#pragma unroll
-                for (int c = 0; c < (NS ? 8 : 1); ++c) dn[c] = (R)((const TR *)a.dtr)[nslot * 8 + c];
-            }
-            if (IP) invh = (R)a.einvh[fr.w];
-        }
-    }
+    int64_t nslot = 0;
+    if (face_active) load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
     __syncthreads();
+
+    R qs[4] = {R(0), R(0), R(0), R(0)};
+    if (face_active) {
+        own_trace<P, R, TR>(a, sQ + fs * PP * 4, sLb, ff, fk, qs);
+        if (BR1) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 8);
+    }
+    if (BR1) __syncthreads();
+
+    // ---- own gradient (Navier-Stokes), kept in LDS
+    if (NS && node_active) {
+        const int i = n % P, j = n / P;
+        R dq[8];
+        if (BR1) br1_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        else ip_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
+    }
+    // neighbour gradient traces, issued before the barrier and consumed in the face phase
+    R dn[NS ? 8 : 1];
+    R invh = R(0);
+    if (NS && face_active) {
+        if ((fr.x & 15) <= FK_PERIODIC) {
+#pragma unroll
+            for (int c = 0; c < (NS ? 8 : 1); ++c) dn[c] = (R)((const TR *)a.dtr)[nslot * 8 + c];
+        }
+        if (IP) invh = (R)a.einvh[fr.w];
+    }
+    if (NS) __syncthreads();
 
     // ---- node phase: contravariant fluxes (DGSpatial_QDotVolumeLoop, DGWeakIntegrals.f90:111-171)
     if (node_active) {
@@ -336,8 +405,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         q[3] = (R)slot_energy<R>(a.ph, sQ[t * 4 + 3]);
         inviscid_flux(ph, q, f, gg);
         if (NS) {
-            R fv[4], gv[4];
-            viscous_flux(ph, q, dqn, fv, gv);
+            R dq[NS ? 8 : 1], fv[4], gv[4];
+#pragma unroll
+            for (int c = 0; c < (NS ? 8 : 1); ++c) dq[c] = sDQ[t * 8 + c];
+            viscous_flux(ph, q, dq, fv, gv);
 #pragma unroll
             for (int c = 0; c < 4; ++c) { f[c] = (f[c] - ph.mu * fv[c]) - R(0); gg[c] = (gg[c] - ph.mu * gv[c]) - R(0); }
         }
@@ -351,12 +422,13 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     // ---- face phase: Riemann fluxes of the element's own four faces
     if (face_active) {
         const int kind = fr.x & 15;
-        double tq[4];
-        face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
-        tq[3] = slot_energy<TR>(a.ph, tq[3]);
-        R qs[4];
+        R ds[NS ? 8 : 1];
+        if (NS) {
+            double tg[8];
+            face_trace<P, 8, 8>(sDQ + fs * PP * 8, sLb, ff, fk, tg);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) qs[c] = (R)(TR)tq[c];   // the same rounding the neighbor reads
+            for (int c = 0; c < (NS ? 8 : 1); ++c) ds[c] = (R)(TR)tg[c];
+        }
         int rs = a.ph.riemann;
         R uSB[4] = {R(0), R(0), R(0), R(0)};
         int neumann = 0;
@@ -366,8 +438,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             if (zr) rs = zr - 1;
         }
         R Fi[4];
-        if (kind == FK_INT_R) riemann<R>(rs, ph, qo, qs, nx, ny, Fi);
-        else riemann<R>(rs, ph, qs, qo, nx, ny, Fi);
+        if (kind == FK_INT_R) riemann<RS, R>(rs, ph, qo, qs, nx, ny, Fi);
+        else riemann<RS, R>(rs, ph, qs, qo, nx, ny, Fi);
         R F[4];
         if (!NS) {
 #pragma unroll
@@ -443,8 +515,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
                 const R *fm = (const R *)a.fmet + ((int64_t)(fe * 4 + ff) * P + fk) * 5;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    sU[t * 4 + c] = (G8[c] * fm[0] + G8[4 + c] * fm[1]) / fm[4];
-                    sV[t * 4 + c] = (G8[c] * fm[2] + G8[4 + c] * fm[3]) / fm[4];
+                    sU[t * 4 + c] = (G8[c] * fm[0] + G8[4 + c] * fm[1]) * fm[4];   // fm[4] = 1/jac at the face node
+                    sV[t * 4 + c] = (G8[c] * fm[2] + G8[4 + c] * fm[3]) * fm[4];
                 }
             }
         }
@@ -478,17 +550,22 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
 
     // ---- node phase 2: volume contraction, ordered face lifts, /jac, RK update
     double qd[4] = {0.0, 0.0, 0.0, 0.0};
+    double gold[4] = {0.0, 0.0, 0.0, 0.0};
+    if (node_active && a.mode == 0 && !a.first_stage) {
+        const double2 *gp = (const double2 *)(a.G + g * 4);
+        const double2 g01 = gp[0], g23 = gp[1];
+        gold[0] = g01.x; gold[1] = g01.y; gold[2] = g23.x; gold[3] = g23.y;
+    }
     if (node_active) {
         const int i = n % P, j = n / P;
         R acc[4] = {R(0), R(0), R(0), R(0)};
         const R *cF = sCF + s * PP * 4, *cG = sCG + s * PP * 4;
-#pragma unroll
+#pragma unroll 1
         for (int l = 0; l < P; ++l) {
             const R hi = sHat[i * P + l], hj = sHat[j * P + l];
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[c] = acc[c] + hi * cF[(l + P * j) * 4 + c] + hj * cG[(i + P * l) * 4 + c];
         }
-        const int ord = a.forder[e];
 #pragma unroll
         for (int o = 0; o < 4; ++o) {
             const int f = (ord >> (2 * o)) & 3;
@@ -507,7 +584,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             }
         }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) qd[c] = (double)(acc[c] / jac);
+        for (int c = 0; c < 4; ++c) qd[c] = (double)(acc[c] * ijac);   // QDot / jac
         if (a.mode == 1) {
             double2 *o = (double2 *)(a.QDot + g * 4);
             o[0] = make_double2(qd[0], qd[1]);
@@ -569,10 +646,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     }
 }
 
-template <int P, int EPB, bool NS, bool IP, typename R, typename TR>
+template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
 __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
-    __shared__ StageSmem<P, EPB, IP, R> S;
-    stage_body<P, EPB, NS, IP, R, TR>(a, blockIdx.x * EPB, blockIdx.x, a.cq, a.cg, S);
+    __shared__ StageSmem<P, EPB, NS, IP, R> S;
+    stage_body<P, EPB, NS, IP, RS, R, TR>(a, blockIdx.x * EPB, blockIdx.x, a.cq, a.cg, S);
 }
 
 // ------------------------------------------------------------------ time step and sums
@@ -625,21 +702,26 @@ __global__ __launch_bounds__(256) void k_cons_partial(const double *Q, const dou
 template <int P, int EPB>
 inline int tile_blocks(int K) { return (K + EPB - 1) / EPB; }
 
-template <int P, int EPB>
-hipError_t stage_tile(bool mixed, const StageArgs &a, hipStream_t st) {
+// PHYS: 0 Euler, 1 Navier-Stokes BR1, 2 Navier-Stokes IP.
+template <int P, int EPB, int PHYS, int RS>
+hipError_t stage_launch(bool mixed, const StageArgs &a, hipStream_t st) {
     constexpr int TPB = Tile<P, EPB>::TPB;
+    constexpr bool NS = PHYS >= 1, IP = PHYS == 2;
     const int nb = tile_blocks<P, EPB>(a.K);
-    const bool ns = a.ph.ns != 0, ip = a.ph.viscous == 2;
-    if (mixed) {
-        if (!ns) hipLaunchKernelGGL((k_stage<P, EPB, false, false, float, float>), dim3(nb), dim3(TPB), 0, st, a);
-        else if (ip) hipLaunchKernelGGL((k_stage<P, EPB, true, true, float, float>), dim3(nb), dim3(TPB), 0, st, a);
-        else hipLaunchKernelGGL((k_stage<P, EPB, true, false, float, float>), dim3(nb), dim3(TPB), 0, st, a);
-    } else {
-        if (!ns) hipLaunchKernelGGL((k_stage<P, EPB, false, false, double, double>), dim3(nb), dim3(TPB), 0, st, a);
-        else if (ip) hipLaunchKernelGGL((k_stage<P, EPB, true, true, double, double>), dim3(nb), dim3(TPB), 0, st, a);
-        else hipLaunchKernelGGL((k_stage<P, EPB, true, false, double, double>), dim3(nb), dim3(TPB), 0, st, a);
-    }
+    if (mixed) hipLaunchKernelGGL((k_stage<P, EPB, NS, IP, RS, float, float>), dim3(nb), dim3(TPB), 0, st, a);
+    else hipLaunchKernelGGL((k_stage<P, EPB, NS, IP, RS, double, double>), dim3(nb), dim3(TPB), 0, st, a);
     return hipGetLastError();
+}
+
+template <int P, int EPB, int PHYS>
+hipError_t stage_tile(bool mixed, const StageArgs &a, hipStream_t st) {
+    switch (a.ph.rs_kernel) {
+    case 0: return stage_launch<P, EPB, PHYS, 0>(mixed, a, st);
+    case 1: return stage_launch<P, EPB, PHYS, 1>(mixed, a, st);
+    case 2: return stage_launch<P, EPB, PHYS, 2>(mixed, a, st);
+    case 3: return stage_launch<P, EPB, PHYS, 3>(mixed, a, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int P, int EPB>
@@ -657,9 +739,12 @@ hipError_t gradient_tile(bool mixed, const StageArgs &a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int P>
-hipError_t stage_p(int tile, bool mixed, const StageArgs &a, hipStream_t st) {
-    return tile == 0 ? stage_tile<P, Tiles<P>::small>(mixed, a, st) : stage_tile<P, Tiles<P>::large>(mixed, a, st);
+// The run-time-solver form (rs_kernel = -1) is compiled for the wide tile only.
+template <int P, int PHYS>
+hipError_t stage_phys_p(int tile, bool mixed, const StageArgs &a, hipStream_t st) {
+    if (a.ph.rs_kernel < 0) return stage_launch<P, Tiles<P>::large, PHYS, -1>(mixed, a, st);
+    return tile == 0 ? stage_tile<P, Tiles<P>::small, PHYS>(mixed, a, st)
+                     : stage_tile<P, Tiles<P>::large, PHYS>(mixed, a, st);
 }
 
 template <int P>
@@ -712,12 +797,14 @@ hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, i
     return hipGetLastError();
 }
 
+#define HG_INSTANTIATE_STAGE(PV, PHYS) \
+    template hipError_t stage_phys_p<PV, PHYS>(int, bool, const StageArgs &, hipStream_t);
+
 #define HG_INSTANTIATE_ORDER(PV)                                                                              \
-    template hipError_t stage_p<PV>(int, bool, const StageArgs &, hipStream_t);                               \
-    template hipError_t gradient_p<PV>(int, bool, const StageArgs &, hipStream_t);                            \
     template int stage_blocks_p<PV>(int, int);                                                                \
     template int tile_threads_p<PV>(int);                                                                     \
     template int tile_elems_p<PV>(int);                                                                       \
+    template hipError_t gradient_p<PV>(int, bool, const StageArgs &, hipStream_t);                            \
     template hipError_t prolong_p<PV>(bool, const StageArgs &, hipStream_t);                                  \
     template hipError_t dt_elem_p<PV>(const double *, int, double, double, double *, double *, hipStream_t);  \
     template int cons_blocks_p<PV>(int);                                                                      \

@@ -271,13 +271,23 @@ __device__ void lf_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny,
     }
 }
 
-template <typename R>
+// Riemann solver RS fixed at compile time (0 Roe, 1 HLL, 2 exact, 3 Lax-Friedrichs), or
+// RS = -1: chosen at run time by id.  A kernel that contains every solver allocates registers
+// for the largest of them (the exact solver), so each handle runs the instantiation of its
+// own solver and the run-time form is kept for meshes whose zones override the solver.
+template <int RS, typename R>
 __device__ inline void riemann(int id, const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
-    switch (id) {
-    case 1: hll_flux(ph, qL, qR, nx, ny, F); break;
-    case 2: exact_flux(ph, qL, qR, nx, ny, F); break;
-    case 3: lf_flux(ph, qL, qR, nx, ny, F); break;
-    default: roe_flux(ph, qL, qR, nx, ny, F); break;
+    if constexpr (RS == 0) roe_flux(ph, qL, qR, nx, ny, F);
+    else if constexpr (RS == 1) hll_flux(ph, qL, qR, nx, ny, F);
+    else if constexpr (RS == 2) exact_flux(ph, qL, qR, nx, ny, F);
+    else if constexpr (RS == 3) lf_flux(ph, qL, qR, nx, ny, F);
+    else {
+        switch (id) {
+        case 1: hll_flux(ph, qL, qR, nx, ny, F); break;
+        case 2: exact_flux(ph, qL, qR, nx, ny, F); break;
+        case 3: lf_flux(ph, qL, qR, nx, ny, F); break;
+        default: roe_flux(ph, qL, qR, nx, ny, F); break;
+        }
     }
 }
 
@@ -374,17 +384,18 @@ __device__ inline void ghost_state(int kind, int flags, const double *zd, const 
 // Trace of a face from the staged element state (DGSpatialDiscretizationMethods.f90:1520-1574).
 // Always accumulated in FP64 with explicit fma in l order so that every kernel that forms
 // a trace (prolong, stage epilogue, stage recomputation) produces the same bits.
-template <int P, int STRIDE, int NV>
-__device__ inline void face_trace(const double *sX /* element block, [n*STRIDE + c] */, const double *sLb, int f, int k,
+template <int P, int STRIDE, int NV, typename T>
+__device__ inline void face_trace(const T *sX /* element block, [n*STRIDE + c] */, const double *sLb, int f, int k,
                                   double *out) {
     const double *lb = (f == 0 || f == 3) ? sLb : sLb + P;
 #pragma unroll
     for (int c = 0; c < NV; ++c) out[c] = 0.0;
-#pragma unroll
+    // a rolled loop keeps one node's values live (unrolled, all P nodes are loaded up front)
+#pragma unroll 1
     for (int l = 0; l < P; ++l) {
         int n = (f == 0 || f == 2) ? (k + P * l) : (l + P * k);
 #pragma unroll
-        for (int c = 0; c < NV; ++c) out[c] = fma(sX[n * STRIDE + c], lb[l], out[c]);
+        for (int c = 0; c < NV; ++c) out[c] = fma((double)sX[n * STRIDE + c], lb[l], out[c]);
     }
 }
 
