@@ -152,7 +152,41 @@ int autotune(hg_handle *h) {
     return 0;
 }
 
-int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode) {
+// Williamson coefficients of the handle's scheme: G = am G + cq QDot, Q += cg G, with cq and/or
+// cg multiplied by dt (DGTimeIntegrator.f90:263-365).
+struct StepCoefs {
+    int nst;
+    double am[5], cq[5], cg[5];
+    int dt_cq, dt_cg;
+};
+
+StepCoefs step_coefs(int scheme) {
+    StepCoefs r{};
+    if (scheme == HG_SCHEME_RK3) {
+        // TimeIntegrator_WilliamsonRK3 (DGTimeIntegrator.f90:282-317)
+        const double am[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
+        const double gm[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
+        r.nst = 3;
+        for (int s = 0; s < 3; ++s) { r.am[s] = am[s]; r.cq[s] = 1.0; r.cg[s] = gm[s]; }
+        r.dt_cq = 0; r.dt_cg = 1;
+    } else if (scheme == HG_SCHEME_RK5) {
+        // TimeIntegrator_WilliamsonRK5 (DGTimeIntegrator.f90:319-365): G = am G + dt QDot, Q += gm G
+        const double am[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
+        const double gm[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
+        r.nst = 5;
+        for (int s = 0; s < 5; ++s) { r.am[s] = am[s]; r.cq[s] = 1.0; r.cg[s] = gm[s]; }
+        r.dt_cq = 1; r.dt_cg = 0;
+    } else {
+        // TimeIntegrator_ExplicitEuler (DGTimeIntegrator.f90:263-280)
+        r.nst = 1;
+        r.am[0] = 0.0; r.cq[0] = 1.0; r.cg[0] = 1.0;
+        r.dt_cq = 0; r.dt_cg = 1;
+    }
+    return r;
+}
+
+int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode,
+              const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0) {
     if (autotune(h)) return 1;
     if (h->ph.ns) {
         hg::StageArgs g = base_args(h);
@@ -168,10 +202,45 @@ int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool la
     a.first_stage = first ? 1 : 0;
     a.mode = mode;
     a.do_resid = last ? 1 : 0;
+    a.dt_dev = dt_dev;
+    a.dt_cq = dt_cq;
+    a.dt_cg = dt_cg;
     HG_CHECK(timed(h, 0, [&] { return hg::launch_stage(h->P, h->tile[0], h->mixed, a, h->stream); }));
     if (last)
         HG_CHECK(hg::launch_resid_final(h->resid_part, hg::stage_blocks(h->P, h->tile[0], h->K), h->resid, h->stream));
     if (mode == 0) h->cur ^= 1;
+    return 0;
+}
+
+void drop_graphs(hg_handle *h) {
+    for (int p = 0; p < 2; ++p) {
+        if (h->gexec[p]) hipGraphExecDestroy(h->gexec[p]);
+        if (h->graph[p]) hipGraphDestroy(h->graph[p]);
+        h->gexec[p] = nullptr;
+        h->graph[p] = nullptr;
+    }
+}
+
+// Captures one RK step for each trace parity (the kernels read dt from h->dt_dev).
+int build_graphs(hg_handle *h, const StepCoefs &k) {
+    const int cur0 = h->cur;
+    for (int p = 0; p < 2; ++p) {
+        h->cur = p;
+        HG_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int bad = 0;
+        for (int s = 0; s < k.nst && !bad; ++s)
+            bad = run_stage(h, k.am[s], k.cq[s], k.cg[s], s == 0, s == k.nst - 1, 0, h->dt_dev, k.dt_cq, k.dt_cg);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        h->cur = cur0;
+        if (bad) {
+            if (g) hipGraphDestroy(g);
+            return 1;
+        }
+        HG_CHECK(e);
+        h->graph[p] = g;
+        HG_CHECK(hipGraphInstantiate(&h->gexec[p], g, nullptr, nullptr, 0));
+    }
     return 0;
 }
 
@@ -381,6 +450,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     h->cons_nb = (K + std::max(1, 256 / PP) - 1) / std::max(1, 256 / PP);
     HG_CHECK(h->alloc(&h->cons_partial, 4 * (size_t)h->cons_nb));
     HG_CHECK(h->alloc(&h->cons_out, 4));
+    HG_CHECK(h->alloc(&h->dt_dev, 1));
     HG_CHECK(h->alloc(&h->resid, 4));
     HG_CHECK(h->alloc(&h->resid_part, 4 * (size_t)K));   // at least one element per workgroup
     HG_CHECK(hipMemset(h->resid, 0, 4 * sizeof(double)));
@@ -395,6 +465,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
 int hg_destroy(hg_handle *h) {
     if (!h) return 0;
     if (h->stream) hipStreamSynchronize(h->stream);
+    drop_graphs(h);
     for (auto &b : h->bufs) hipFree(b.p);
     for (auto &e : h->ev_pool) hipEventDestroy(e);
     for (auto &p : h->ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
@@ -494,6 +565,7 @@ int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
         h->ph.pref = std::isfinite(pref) ? pref : 0.0;
         h->ph.Eref = h->ph.pref / h->ph.gm1;
     }
+    drop_graphs(h);   // the captured kernels carry the physics parameters by value
     HG_CHECK(hipMemcpyAsync(h->Q, Q, ndof * sizeof(double), hipMemcpyHostToDevice, h->stream));
     hg::StageArgs a = base_args(h);
     h->cur = 0;
@@ -526,24 +598,36 @@ int hg_time_derivative(hg_handle *h, double t) {
 }
 
 int hg_rk_step(hg_handle *h, double t, double dt) {
-    (void)t;
+    (void)t;  // no time-dependent boundary data on the GPU path
     if (!h) return set_err("hg_rk_step: null handle");
-    if (h->opt.scheme == HG_SCHEME_RK3) {
-        // TimeIntegrator_WilliamsonRK3 (DGTimeIntegrator.f90:282-317)
-        const double am[3] = {0.0, -5.0 / 9.0, -153.0 / 128.0};
-        const double gm[3] = {1.0 / 3.0, 15.0 / 16.0, 8.0 / 15.0};
-        for (int s = 0; s < 3; ++s)
-            if (run_stage(h, am[s], 1.0, gm[s] * dt, s == 0, s == 2, 0)) return 1;
-    } else if (h->opt.scheme == HG_SCHEME_RK5) {
-        // TimeIntegrator_WilliamsonRK5 (DGTimeIntegrator.f90:319-365): G = am G + dt QDot, Q += gm G
-        const double am[5] = {0.0, -0.4178904745, -1.192151694643, -1.697784692471, -1.514183444257};
-        const double gm[5] = {0.1496590219993, 0.3792103129999, 0.8229550293869, 0.6994504559488, 0.1530572479681};
-        for (int s = 0; s < 5; ++s)
-            if (run_stage(h, am[s], dt, gm[s], s == 0, s == 4, 0)) return 1;
-    } else {
-        // TimeIntegrator_ExplicitEuler (DGTimeIntegrator.f90:263-280)
-        if (run_stage(h, 0.0, 1.0, dt, true, true, 0)) return 1;
+    const StepCoefs k = step_coefs(h->opt.scheme);
+    if (h->use_graph && !h->prof) {
+        // replay the captured step (per-kernel event timing uses the direct path below)
+        if (autotune(h)) return 1;
+        if (!h->dt_valid || dt != h->dt_last) {
+            h->dt_last = dt;
+            h->dt_valid = true;
+            HG_CHECK(hipMemcpyAsync(h->dt_dev, &h->dt_last, sizeof(double), hipMemcpyHostToDevice, h->stream));
+        }
+        if (!h->gexec[0] && build_graphs(h, k)) {
+            drop_graphs(h);
+            return 1;
+        }
+        HG_CHECK(hipGraphLaunch(h->gexec[h->cur], h->stream));
+        if (k.nst & 1) h->cur ^= 1;
+        return 0;
     }
+    for (int s = 0; s < k.nst; ++s) {
+        const double cq = k.dt_cq ? k.cq[s] * dt : k.cq[s];
+        const double cg = k.dt_cg ? k.cg[s] * dt : k.cg[s];
+        if (run_stage(h, k.am[s], cq, cg, s == 0, s == k.nst - 1, 0)) return 1;
+    }
+    return 0;
+}
+
+int hg_set_graph(hg_handle *h, int32_t on) {
+    if (!h) return set_err("hg_set_graph: null handle");
+    h->use_graph = on != 0;
     return 0;
 }
 
@@ -593,6 +677,7 @@ int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile) {
     if (h->ph.rs_kernel < 0 && stage_tile != 1) stage_tile = 1;   // only the wide tile has the run-time solver form
     h->tile[0] = stage_tile;
     h->tile[1] = gradient_tile;
+    drop_graphs(h);
     return 0;
 }
 

@@ -48,6 +48,8 @@ struct StageArgs {
     const void *fmet;           // IP: per face node (Jxi_x, Jxi_y, Jeta_x, Jeta_y, jac)
     const double *hatD, *D, *lb, *lbw, *dlbw;  // spectral operators (P*P, P*P, 2P, 2P, 2P)
     double am, cq, cg;          // G = am*G + cq*QDot ; Q = Q + cg*G
+    const double *dt_dev;       // if set, cq (dt_cq) and/or cg (dt_cg) are multiplied by *dt_dev
+    int dt_cq, dt_cg;
     int first_stage;            // G is not read (am = 0)
     int mode;                   // 0: RK update + next traces, 1: derivative only
     double *resid_part;         // per workgroup max|QDot| per equation (4 doubles per workgroup)

@@ -45,6 +45,13 @@ struct hg_handle {
     double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
     int cur = 0;
     int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet
+    // one RK step captured as a HIP graph per trace parity; the step size is read from dt_dev
+    bool use_graph = true;
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    double *dt_dev = nullptr;
+    double dt_last = 0.0;
+    bool dt_valid = false;
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;

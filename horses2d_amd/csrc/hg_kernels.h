@@ -649,7 +649,13 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
 template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
 __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
     __shared__ StageSmem<P, EPB, NS, IP, R> S;
-    stage_body<P, EPB, NS, IP, RS, R, TR>(a, blockIdx.x * EPB, blockIdx.x, a.cq, a.cg, S);
+    double cq = a.cq, cg = a.cg;
+    if (a.dt_dev) {   // graph replay: the step size lives in device memory
+        const double d = *a.dt_dev;
+        if (a.dt_cq) cq *= d;
+        if (a.dt_cg) cg *= d;
+    }
+    stage_body<P, EPB, NS, IP, RS, R, TR>(a, blockIdx.x * EPB, blockIdx.x, cq, cg, S);
 }
 
 // ------------------------------------------------------------------ time step and sums

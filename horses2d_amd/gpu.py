@@ -58,7 +58,7 @@ EXPORTS = [
     "hg_build_connectivity", "hg_build_geometry",
     "hg_upload_state", "hg_download_state", "hg_download_qdot", "hg_time_derivative", "hg_rk_step",
     "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_synchronize",
-    "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_batch_create", "hg_batch_rk_step",
+    "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_set_graph", "hg_batch_create", "hg_batch_rk_step",
     "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_destroy",
 ]
 
@@ -98,6 +98,7 @@ def load_library(path: str = LIB_PATH):
         "hg_profile_read": ([vp, i32, vp, vp], ctypes.c_int),
         "hg_set_tiles": ([vp, i32, i32], ctypes.c_int),
         "hg_get_tiles": ([vp, vp, vp], ctypes.c_int),
+        "hg_set_graph": ([vp, i32], ctypes.c_int),
         "hg_batch_create": ([vp, i32, vp, vp], ctypes.c_int),
         "hg_batch_rk_step": ([vp, vp, vp], ctypes.c_int),
         "hg_batch_synchronize": ([vp], ctypes.c_int),
@@ -288,6 +289,10 @@ class DGSEM:
     def set_tiles(self, stage: int = -1, gradient: int = -1):
         """Workgroup tile per kernel: -1 autotune at the next evaluation, 0 narrow, 1 wide."""
         _check(self._L.hg_set_tiles(self._h, stage, gradient), "hg_set_tiles")
+
+    def set_graph(self, on: bool = True):
+        """Replay each RK step as a captured HIP graph (default) or launch kernel by kernel."""
+        _check(self._L.hg_set_graph(self._h, 1 if on else 0), "hg_set_graph")
 
     def tiles(self):
         s = ctypes.c_int32(); g = ctypes.c_int32()

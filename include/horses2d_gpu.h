@@ -190,6 +190,11 @@ int hg_profile_read(hg_handle *h, int32_t which, int64_t *launches, double *ms);
  * one wavefront), 1 = wide (up to four wavefronts).  Results are bit-identical for
  * every tile.  No reference counterpart (performance control only). */
 int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile);
+
+/* hg_rk_step replays one captured HIP graph per step (on = 1, the default) or launches the
+ * kernels one by one (on = 0; also used while hg_profile_enable is on).  Results are
+ * bit-identical.  No reference counterpart (performance control only). */
+int hg_set_graph(hg_handle *h, int32_t on);
 int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile);
 
 /* ------------------------------------------------------------------------ */
