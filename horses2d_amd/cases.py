@@ -290,13 +290,29 @@ def ns_kh_case(n: int = 256, N: int = 7, steps: int = 200, Re: float = 1.0e4, Ma
                 ic=shear_layer(Mach, seed=seed))
 
 
+def viscous_dt_cap(mesh: HiOMesh, N: int, Re: float, c: float = 0.5) -> float:
+    """Upper bound on dt for the explicit viscous terms, c h_min^2 Re / (N+1)^4 with h_min the
+    shortest element side.  The reference's estimate (DGTimeIntegrator.f90:462-511) has no
+    viscous term; its case-file time step (Setup%dt) caps dt, and this value is used for it.
+    On the 512 x 512 cylinder grid (h_min = 3.6e-3) RK3 is stable at 5e-7 and diverges at 2e-6
+    from the impulsive start; the cap gives 3.2e-7."""
+    p = mesh.points
+    q = mesh.quads
+    h = np.inf
+    for c0 in range(4):
+        d = p[q[:, c0]] - p[q[:, (c0 + 1) % 4]]
+        h = min(h, float(np.min(np.hypot(d[:, 0], d[:, 1]))))
+    return c * h * h * Re / float(N + 1) ** 4
+
+
 def ns_cylinder_case(nr: int = 512, nt: int = 512, N: int = 7, steps: int = 2000, Re: float = 200.0,
                      Mach: float = 0.2) -> Case:
     """Flow past a cylinder (radius 0.5) on an O-grid to R=20; BR1, Roe, far field + adiabatic wall."""
     qinf = freestream_state(Mach)
     mesh = annulus(nr, nt, 0.5, 20.0, qinf)
     return Case(f"ns_cylinder_{nr}x{nt}_N{N}", mesh, np.full(mesh.K, N, np.int32), Mach, Re, 0.72, True,
-                riemann="roe", viscous="br1", cfl=stable_cfl(N), steps=steps, ic=uniform(qinf))
+                riemann="roe", viscous="br1", cfl=stable_cfl(N), dt_max=viscous_dt_cap(mesh, N, Re), steps=steps,
+                ic=uniform(qinf))
 
 
 def ensemble_cases(count: int = 64, seed_base: int = 1234, nmin: int = 32, nmax: int = 128,
