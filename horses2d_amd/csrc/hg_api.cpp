@@ -120,7 +120,7 @@ int autotune(hg_handle *h) {
         hipError_t e = launch();
         if (e != hipSuccess) return e;
         hipEventRecord(ev[0], h->stream);
-        for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < 6; ++r)
             if ((e = launch()) != hipSuccess) return e;
         hipEventRecord(ev[1], h->stream);
         if ((e = hipEventSynchronize(ev[1])) != hipSuccess) return e;
