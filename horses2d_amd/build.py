@@ -38,9 +38,19 @@ def _units():
     return units
 
 
-def _compile(unit) -> str:
+def _newest_header() -> float:
+    hdrs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    hdrs += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    return max(os.path.getmtime(h) for h in hdrs)
+
+
+def _compile(unit, incremental: bool = False) -> str:
     src, objname, defs = unit
     obj = os.path.join(BUILD, objname)
+    if incremental and os.path.exists(obj):
+        t = os.path.getmtime(obj)
+        if t > os.path.getmtime(os.path.join(CSRC, src)) and t > _newest_header():
+            return obj
     cmd = [HIPCC] + COMMON + DEVICE + defs + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -75,12 +85,14 @@ def build_fortran(verbose: bool = True) -> str:
     return DRIVER
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, incremental: bool = False) -> str:
+    """Compiles every unit (incremental=True skips objects newer than their source and every
+    header) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     units = _units()
     jobs = int(os.environ.get("HG_BUILD_JOBS", str(min(8, os.cpu_count() or 1))))
     with concurrent.futures.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(_compile, units))
+        objs = list(ex.map(lambda u: _compile(u, incremental), units))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lamdhip64"]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -95,5 +107,5 @@ def build(verbose: bool = True) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)
