@@ -113,12 +113,14 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
 
 // BR1 gradient at node (i, j): ( -Vector_StdVolumeGreen(Q) + sum_faces Vector_StdFace(uStar) ) / jac
 // (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622), faces lifted in edge-id order.
-// q: element state [n*4], Ja: element metric [n*4], Fu: element face fluxes [(f*P + k)*8].
+// q: element state in the compute type [n*4] (energy slot shifted in the FP32 form),
+// Ja: element metric [n*4], Fu: element face fluxes [(f*P + k)*8].
 template <int P, typename R>
-__device__ __forceinline__ void br1_node_gradient(const PhysParams &pp, const double *q, const R *Ja, const R *sHat,
-                                                  const R *Fu, const R *sLbw, int ord, int i, int j, R ijac, R *dq) {
+__device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const R *sHat, const R *Fu, const R *sLbw,
+                                                  int ord, int i, int j, R ijac, R *dq) {
     // Every accumulator receives its xi terms (l = 0..P-1) then its eta terms; each state value
-    // is read once and used for both directions, which keeps the live set small.
+    // is read once and used for both directions, and the metric-weighted derivative entry
+    // Ja * hatD is formed once per node for all four variables.
     R acc[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[c] = R(0);
@@ -126,22 +128,24 @@ __device__ __forceinline__ void br1_node_gradient(const PhysParams &pp, const do
     for (int l = 0; l < P; ++l) {
         const int m = l + P * j;
         const R h = sHat[i * P + l];
+        const R w0 = Ja[m * 4 + 0] * h, w2 = Ja[m * 4 + 2] * h;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const R v = qslot<R>(pp, q[m * 4 + c], c);
-            acc[c] += v * Ja[m * 4 + 0] * h;
-            acc[4 + c] += v * Ja[m * 4 + 2] * h;
+            const R v = q[m * 4 + c];
+            acc[c] += v * w0;
+            acc[4 + c] += v * w2;
         }
     }
 #pragma unroll
     for (int l = 0; l < P; ++l) {
         const int m = i + P * l;
         const R h = sHat[j * P + l];
+        const R w1 = Ja[m * 4 + 1] * h, w3 = Ja[m * 4 + 3] * h;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const R v = qslot<R>(pp, q[m * 4 + c], c);
-            acc[c] += v * Ja[m * 4 + 1] * h;
-            acc[4 + c] += v * Ja[m * 4 + 3] * h;
+            const R v = q[m * 4 + c];
+            acc[c] += v * w1;
+            acc[4 + c] += v * w3;
         }
     }
 #pragma unroll
@@ -158,19 +162,36 @@ __device__ __forceinline__ void br1_node_gradient(const PhysParams &pp, const do
     for (int c = 0; c < 8; ++c) dq[c] = dq[c] * ijac;
 }
 
-// Interior (broken) gradient of the IP method (QuadElement_QuadProcedures.incf:42-88).
+// Interior (broken) gradient of the IP method (QuadElement_QuadProcedures.incf:42-88); q as above.
 template <int P, typename R>
-__device__ __forceinline__ void ip_node_gradient(const PhysParams &pp, const double *q, const R *sD, int i, int j, R j11,
-                                                 R j12, R j21, R j22, R ijac, R *dq) {
+__device__ __forceinline__ void ip_node_gradient(const R *q, const R *sD, int i, int j, R j11, R j12, R j21, R j22,
+                                                 R ijac, R *dq) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         R dxi = R(0), deta = R(0);
 #pragma unroll
-        for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * qslot<R>(pp, q[(l + P * j) * 4 + c], c);
+        for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * q[(l + P * j) * 4 + c];
 #pragma unroll
-        for (int l = 0; l < P; ++l) deta += sD[j * P + l] * qslot<R>(pp, q[(i + P * l) * 4 + c], c);
+        for (int l = 0; l < P; ++l) deta += sD[j * P + l] * q[(i + P * l) * 4 + c];
         dq[c] = (dxi * j11 + deta * j12) * ijac;
         dq[4 + c] = (dxi * j21 + deta * j22) * ijac;
+    }
+}
+
+// Stages an element tile's state from HBM into LDS: FP64 always, plus the compute-type copy
+// (energy slot shifted) that the gradient contractions read when the compute type is FP32.
+template <typename R, bool COPY, bool KEEP64 = true>
+__device__ __forceinline__ void load_tile(const StageArgs &a, const int64_t nb0, const int ne, const int PP, double *sQ,
+                                          R *sQf, const int t, const int TPB) {
+    const double2 *src = (const double2 *)(a.Q + nb0 * 4);
+    double2 *dst = (double2 *)sQ;
+    for (int x = t; x < ne * PP * 2; x += TPB) {
+        const double2 v = src[x];
+        if constexpr (KEEP64) dst[x] = v;
+        if constexpr (COPY) {
+            // pairs (0,1) and (2,3) of a node: the odd pair holds the energy
+            ((float2 *)sQf)[x] = make_float2((float)v.x, (float)((x & 1) ? slot_energy<R>(a.ph, v.y) : v.y));
+        }
     }
 }
 
@@ -189,6 +210,15 @@ __device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, in
 #pragma unroll
         for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
     }
+}
+
+// Own Q trace of a face node as the previous stage (or the prolongation) stored it: the values
+// the neighbour reads, in the compute type.
+template <int P, typename R, typename TR>
+__device__ __forceinline__ void stored_trace(const StageArgs &a, int fe, int ff, int fk, R *qs) {
+    const TR *o = (const TR *)a.tr_in + ((int64_t)(fe * 4 + ff) * P + fk) * 4;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) qs[c] = (R)o[c];
 }
 
 // Own Q trace of a face node from the LDS tile, in the compute type, rounded like the stored
@@ -210,8 +240,9 @@ __device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe,
 template <int P, int EPB, bool BR1, typename R>
 struct GradSmem {
     using T = Tile<P, EPB>;
-    alignas(16) double sQ[T::NN * 4];
+    alignas(16) double sQ[sizeof(R) == 4 ? 2 : T::NN * 4];   // FP64 form: the state
     alignas(16) double sLb[2 * P];
+    alignas(16) R sQf[sizeof(R) == 4 ? T::NN * 4 : 1];        // FP32 form: the compute-type state
     alignas(16) R sDQ[T::NN * 8];
     alignas(16) R sJa[BR1 ? T::NN * 4 : 1];
     alignas(16) R sFu[BR1 ? T::NF * 8 : 1];
@@ -234,11 +265,10 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     for (int x = t; x < PP; x += TPB) { sHat[x] = (R)a.hatD[x]; sD[x] = (R)a.D[x]; }
     const int ne = min(EPB, a.K - e0);
     const int64_t nb0 = (int64_t)e0 * PP;
-    {
-        const double2 *src = (const double2 *)(a.Q + nb0 * 4);
-        double2 *dst = (double2 *)sQ;
-        for (int x = t; x < ne * PP * 2; x += TPB) dst[x] = src[x];
-    }
+    constexpr bool COPY = sizeof(R) == 4;
+    load_tile<R, COPY, !COPY>(a, nb0, ne, PP, sQ, S.sQf, t, TPB);
+    const R *sQr;
+    if constexpr (COPY) sQr = S.sQf; else sQr = (const R *)sQ;
     // ---- hoisted global loads
     const int s = t / PP, n = t % PP, e = e0 + s;
     const bool node_active = (t < NN) && (e < a.K);
@@ -259,19 +289,19 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     int4 fr = make_int4(0, 0, 0, 0);
     R nx = R(0), ny = R(0), dS = R(0), qo[4] = {R(0), R(0), R(0), R(0)};
     int64_t nslot = 0;
-    if (face_active) load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
-    __syncthreads();
+    R qs[4] = {R(0), R(0), R(0), R(0)};
     if (face_active) {
-        R qs[4];
-        own_trace<P, R, TR>(a, sQ + fs * PP * 4, sLb, ff, fk, qs);
-        br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 8);
+        load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
+        stored_trace<P, R, TR>(a, fe, ff, fk, qs);
     }
+    __syncthreads();
+    if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 8);
     if (BR1) __syncthreads();
     if (node_active) {
         const int i = n % P, j = n / P;
         R dq[8];
-        if (BR1) br1_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sJa + s * PP * 4, sHat, sFu + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
-        else ip_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
+        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sJa + s * PP * 4, sHat, sFu + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
 #pragma unroll
         for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
     }
@@ -301,9 +331,8 @@ struct StageSmem {
     alignas(16) double sQ[T::NN * 4];
     alignas(16) double sLb[2 * P];
     alignas(16) double sRes[4][4];
-    alignas(16) R sDQ[NS ? T::NN * 8 : 1];
     alignas(16) R sCF[T::NN * 4];              // BR1: the metric Ja during the gradient phase
-    alignas(16) R sCG[T::NN * 4];
+    alignas(16) R sCG[T::NN * 4];              // NS, FP32: the compute-type state until the node phase
     alignas(16) R sF[NS ? T::NF * 8 : T::NF * 4];  // BR1: uStar fluxes during the gradient phase
     alignas(16) R sU[IP ? T::NF * 4 : 1];
     alignas(16) R sV[IP ? T::NF * 4 : 1];
@@ -317,8 +346,10 @@ struct StageSmem {
 
 // One element tile [e0, e0+EPB) of a stage; any workgroup size >= Tile<P, EPB>::TPB (at most
 // 256).  rslot is the residual partial this tile writes; cq, cg are the stage's RK factors.
-// Navier-Stokes: the element's own gradient (nodes and face traces) is recomputed here from
-// the LDS tile and the neighbours' Q traces; only the neighbours' gradient traces are read.
+// Navier-Stokes: the element's nodal gradient is recomputed here from the LDS tile and the Q
+// traces (it stays in registers); the gradient traces of both sides of a face are read from
+// what the gradient kernel stored, which is the same function of the same data.  The own Q
+// traces are read from the stored traces as well.
 template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, const int64_t rslot, const double cq,
                                            const double cg, StageSmem<P, EPB, NS, IP, R> &S) {
@@ -327,7 +358,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     constexpr bool BR1 = NS && !IP;
     const int TPB = blockDim.x;
     double *sQ = S.sQ, *sLb = S.sLb;
-    R *sDQ = S.sDQ, *sCF = S.sCF, *sCG = S.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
+    R *sCF = S.sCF, *sCG = S.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
     R *sHat = S.sHat, *sD = S.sD, *sLbw = S.sLbw, *sDlbw = S.sDlbw;
     const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
@@ -339,11 +370,12 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     }
     const int ne = min(EPB, a.K - e0);
     const int64_t nb0 = (int64_t)e0 * PP;
-    {
-        const double2 *src = (const double2 *)(a.Q + nb0 * 4);
-        double2 *dst = (double2 *)sQ;
-        for (int x = t; x < ne * PP * 2; x += TPB) dst[x] = src[x];
-    }
+    // NS in FP32: a compute-type copy of the state for the gradient contractions, in sCG (each
+    // node lane reads its own entry once more before the node phase overwrites it)
+    constexpr bool COPY = NS && sizeof(R) == 4;
+    load_tile<R, COPY>(a, nb0, ne, PP, sQ, sCG, t, TPB);
+    const R *sQr;
+    if constexpr (COPY) sQr = sCG; else sQr = (const R *)sQ;
 
     // ---- hoisted global loads: node lane
     const int s = t / PP, n = t % PP, e = e0 + s;
@@ -366,26 +398,26 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     int4 fr = make_int4(0, 0, 0, 0);
     R nx = R(0), ny = R(0), dS = R(0), qo[4] = {R(0), R(0), R(0), R(0)};
     int64_t nslot = 0;
-    if (face_active) load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
-    __syncthreads();
-
     R qs[4] = {R(0), R(0), R(0), R(0)};
     if (face_active) {
-        own_trace<P, R, TR>(a, sQ + fs * PP * 4, sLb, ff, fk, qs);
-        if (BR1) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 8);
+        load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
+        stored_trace<P, R, TR>(a, fe, ff, fk, qs);
     }
-    if (BR1) __syncthreads();
+    __syncthreads();
 
-    // ---- own gradient (Navier-Stokes), kept in LDS
+    if (BR1) {
+        if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 8);
+        __syncthreads();
+    }
+
+    // ---- own nodal gradient (Navier-Stokes), kept in registers for the node phase
+    R dq[NS ? 8 : 1];
     if (NS && node_active) {
         const int i = n % P, j = n / P;
-        R dq[8];
-        if (BR1) br1_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
-        else ip_node_gradient<P, R>(a.ph, sQ + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
-#pragma unroll
-        for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
+        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
     }
-    // neighbour gradient traces, issued before the barrier and consumed in the face phase
+    // the neighbour's gradient trace, issued before the barrier and consumed in the face phase
     R dn[NS ? 8 : 1];
     R invh = R(0);
     if (NS && face_active) {
@@ -395,19 +427,23 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         }
         if (IP) invh = (R)a.einvh[fr.w];
     }
-    if (NS) __syncthreads();
+    // the node phase overwrites sCF (the metric, BR1) and sCG (the FP32 state copy)
+    if (BR1 || COPY) __syncthreads();
 
     // ---- node phase: contravariant fluxes (DGSpatial_QDotVolumeLoop, DGWeakIntegrals.f90:111-171)
     if (node_active) {
         R q[4], f[4], gg[4];
+        if constexpr (COPY) {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) q[c] = (R)sQ[t * 4 + c];
-        q[3] = (R)slot_energy<R>(a.ph, sQ[t * 4 + 3]);
+            for (int c = 0; c < 4; ++c) q[c] = sQr[t * 4 + c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) q[c] = (R)sQ[t * 4 + c];
+            q[3] = (R)slot_energy<R>(a.ph, sQ[t * 4 + 3]);
+        }
         inviscid_flux(ph, q, f, gg);
         if (NS) {
-            R dq[NS ? 8 : 1], fv[4], gv[4];
-#pragma unroll
-            for (int c = 0; c < (NS ? 8 : 1); ++c) dq[c] = sDQ[t * 8 + c];
+            R fv[4], gv[4];
             viscous_flux(ph, q, dq, fv, gv);
 #pragma unroll
             for (int c = 0; c < 4; ++c) { f[c] = (f[c] - ph.mu * fv[c]) - R(0); gg[c] = (gg[c] - ph.mu * gv[c]) - R(0); }
@@ -422,12 +458,11 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     // ---- face phase: Riemann fluxes of the element's own four faces
     if (face_active) {
         const int kind = fr.x & 15;
-        R ds[NS ? 8 : 1];
+        R ds[NS ? 8 : 1];   // own gradient trace
         if (NS) {
-            double tg[8];
-            face_trace<P, 8, 8>(sDQ + fs * PP * 8, sLb, ff, fk, tg);
+            const TR *dt = (const TR *)a.dtr + ((int64_t)(fe * 4 + ff) * P + fk) * 8;
 #pragma unroll
-            for (int c = 0; c < (NS ? 8 : 1); ++c) ds[c] = (R)(TR)tg[c];
+            for (int c = 0; c < (NS ? 8 : 1); ++c) ds[c] = (R)dt[c];
         }
         int rs = a.ph.riemann;
         R uSB[4] = {R(0), R(0), R(0), R(0)};

@@ -388,12 +388,16 @@ template <int P, int STRIDE, int NV, typename T>
 __device__ inline void face_trace(const T *sX /* element block, [n*STRIDE + c] */, const double *sLb, int f, int k,
                                   double *out) {
     const double *lb = (f == 0 || f == 3) ? sLb : sLb + P;
+    // nodes (k, l) of a horizontal face or (l, k) of a vertical one: a base and a stride, so
+    // the loop body has no face-dependent branch
+    const bool horiz = (f == 0 || f == 2);
+    const int base = horiz ? k : P * k, step = horiz ? P : 1;
 #pragma unroll
     for (int c = 0; c < NV; ++c) out[c] = 0.0;
     // a rolled loop keeps one node's values live (unrolled, all P nodes are loaded up front)
 #pragma unroll 1
     for (int l = 0; l < P; ++l) {
-        int n = (f == 0 || f == 2) ? (k + P * l) : (l + P * k);
+        const int n = base + step * l;
 #pragma unroll
         for (int c = 0; c < NV; ++c) out[c] = fma((double)sX[n * STRIDE + c], lb[l], out[c]);
     }
