@@ -89,7 +89,8 @@ __device__ __forceinline__ R qslot(const PhysParams &pp, double v, int c) {
 }
 
 // BR1 solution flux at a face node: uStar n dS in both directions (DGViscous_BR1.f90:313-328;
-// boundary faces use the ghost state's uSB, DGViscousMethods.f90:786-917).
+// boundary faces use the ghost state's uSB, DGViscousMethods.f90:786-917), stored as
+// out[0..3] = uStar dS and out[4..5] = (nx, ny); the lift forms (uStar dS) nx and (uStar dS) ny.
 template <typename R>
 __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> &ph, int flags, int y, const R *qs,
                                               const R *qo, R nx, R ny, R dS, R *out) {
@@ -108,13 +109,14 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
         ghost_state<R>(kind, flags, a.zone + 8 * y, ph, qs, nx, ny, uB, us, neu);
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) { out[c] = us[c] * dS * nx; out[4 + c] = us[c] * dS * ny; }
+    for (int c = 0; c < 4; ++c) out[c] = us[c] * dS;
+    out[4] = nx; out[5] = ny;
 }
 
 // BR1 gradient at node (i, j): ( -Vector_StdVolumeGreen(Q) + sum_faces Vector_StdFace(uStar) ) / jac
 // (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622), faces lifted in edge-id order.
 // q: element state in the compute type [n*4] (energy slot shifted in the FP32 form),
-// Ja: element metric [n*4], Fu: element face fluxes [(f*P + k)*8].
+// Ja: element metric [n*4], Fu: element uStar fluxes [(f*P + k)*6] (see br1_face_flux).
 template <int P, typename R>
 __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const R *sHat, const R *Fu, const R *sLbw,
                                                   int ord, int i, int j, R ijac, R *dq) {
@@ -153,10 +155,15 @@ __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const
 #pragma unroll
     for (int o = 0; o < 4; ++o) {
         const int f = (ord >> (2 * o)) & 3;
-        const R *F = Fu + (f * P + ((f == 0 || f == 2) ? i : j)) * 8;
+        const R *F = Fu + (f * P + ((f == 0 || f == 2) ? i : j)) * 6;
         const R lw = (f == 0) ? sLbw[j] : (f == 1) ? sLbw[P + i] : (f == 2) ? sLbw[P + j] : sLbw[i];
+        const R fnx = F[4], fny = F[5];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) dq[c] += F[c] * lw;
+        for (int c = 0; c < 4; ++c) {
+            const R ux = F[c] * fnx, uy = F[c] * fny;
+            dq[c] += ux * lw;
+            dq[4 + c] += uy * lw;
+        }
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) dq[c] = dq[c] * ijac;
@@ -240,15 +247,22 @@ __device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe,
 template <int P, int EPB, bool BR1, typename R>
 struct GradSmem {
     using T = Tile<P, EPB>;
-    alignas(16) double sQ[sizeof(R) == 4 ? 2 : T::NN * 4];   // FP64 form: the state
+    struct Inputs {
+        alignas(16) double sQ[sizeof(R) == 4 ? 2 : T::NN * 4];   // FP64 form: the state
+        alignas(16) R sQf[sizeof(R) == 4 ? T::NN * 4 : 1];        // FP32 form: the compute-type state
+        alignas(16) R sJa[BR1 ? T::NN * 4 : 1];
+    };
+    // the state and the metric are dead once every node's gradient is formed: the gradient
+    // tile takes their place
+    union {
+        Inputs in;
+        alignas(16) R sDQ[T::NN * 8];
+    } u;
     alignas(16) double sLb[2 * P];
-    alignas(16) R sQf[sizeof(R) == 4 ? T::NN * 4 : 1];        // FP32 form: the compute-type state
-    alignas(16) R sDQ[T::NN * 8];
-    alignas(16) R sJa[BR1 ? T::NN * 4 : 1];
-    alignas(16) R sFu[BR1 ? T::NF * 8 : 1];
-    alignas(16) R sHat[T::PP];
+    alignas(16) R sFu[BR1 ? T::NF * 6 : 1];
+    alignas(16) R sHat[BR1 ? T::PP : 1];
     alignas(16) R sLbw[2 * P];
-    alignas(16) R sD[T::PP];
+    alignas(16) R sD[BR1 ? 1 : T::PP];
 };
 
 // One element tile [e0, e0+EPB) of the gradient; any workgroup size >= Tile<P, EPB>::TPB.
@@ -257,18 +271,21 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     using T = Tile<P, EPB>;
     constexpr int PP = T::PP, NN = T::NN, NF = T::NF;
     const int TPB = blockDim.x;
-    double *sQ = S.sQ, *sLb = S.sLb;
-    R *sDQ = S.sDQ, *sJa = S.sJa, *sFu = S.sFu, *sHat = S.sHat, *sLbw = S.sLbw, *sD = S.sD;
+    double *sQ = S.u.in.sQ, *sLb = S.sLb;
+    R *sDQ = S.u.sDQ, *sJa = S.u.in.sJa, *sFu = S.sFu, *sHat = S.sHat, *sLbw = S.sLbw, *sD = S.sD;
     const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
     if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; }
-    for (int x = t; x < PP; x += TPB) { sHat[x] = (R)a.hatD[x]; sD[x] = (R)a.D[x]; }
+    for (int x = t; x < PP; x += TPB) {
+        if (BR1) sHat[x] = (R)a.hatD[x];
+        else sD[x] = (R)a.D[x];
+    }
     const int ne = min(EPB, a.K - e0);
     const int64_t nb0 = (int64_t)e0 * PP;
     constexpr bool COPY = sizeof(R) == 4;
-    load_tile<R, COPY, !COPY>(a, nb0, ne, PP, sQ, S.sQf, t, TPB);
+    load_tile<R, COPY, !COPY>(a, nb0, ne, PP, sQ, S.u.in.sQf, t, TPB);
     const R *sQr;
-    if constexpr (COPY) sQr = S.sQf; else sQr = (const R *)sQ;
+    if constexpr (COPY) sQr = S.u.in.sQf; else sQr = (const R *)sQ;
     // ---- hoisted global loads
     const int s = t / PP, n = t % PP, e = e0 + s;
     const bool node_active = (t < NN) && (e < a.K);
@@ -295,13 +312,16 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         stored_trace<P, R, TR>(a, fe, ff, fk, qs);
     }
     __syncthreads();
-    if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 8);
+    if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 6);
     if (BR1) __syncthreads();
+    R dq[8];
     if (node_active) {
         const int i = n % P, j = n / P;
-        R dq[8];
-        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sJa + s * PP * 4, sHat, sFu + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sJa + s * PP * 4, sHat, sFu + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
         else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
+    }
+    __syncthreads();   // every lane is done with the state and the metric
+    if (node_active) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
     }
@@ -333,7 +353,7 @@ struct StageSmem {
     alignas(16) double sRes[4][4];
     alignas(16) R sCF[T::NN * 4];              // BR1: the metric Ja during the gradient phase
     alignas(16) R sCG[T::NN * 4];              // NS, FP32: the compute-type state until the node phase
-    alignas(16) R sF[NS ? T::NF * 8 : T::NF * 4];  // BR1: uStar fluxes during the gradient phase
+    alignas(16) R sF[NS ? T::NF * 6 : T::NF * 4];  // BR1: uStar fluxes during the gradient phase
     alignas(16) R sU[IP ? T::NF * 4 : 1];
     alignas(16) R sV[IP ? T::NF * 4 : 1];
     alignas(16) R sU2[IP ? T::NF * 4 : 1];
@@ -406,7 +426,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     __syncthreads();
 
     if (BR1) {
-        if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 8);
+        if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 6);
         __syncthreads();
     }
 
@@ -414,7 +434,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     R dq[NS ? 8 : 1];
     if (NS && node_active) {
         const int i = n % P, j = n / P;
-        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
         else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
     }
     // the neighbour's gradient trace, issued before the barrier and consumed in the face phase
