@@ -20,6 +20,10 @@ constexpr int FF_SELFREV = 1 << 6;   // edge geometry index runs backwards (RIGH
 constexpr int FF_ISOTHERMAL = 1 << 7;
 constexpr int FF_RS_SHIFT = 8;       // (zone Riemann solver + 1), 0 = interior method
 
+// Viscous face-flux traces: per element face node, mu times this side's viscous flux from its
+// own Q and gradient traces, (f1, f2, f3, g2, g3) and one pad (f0 = g0 = 0, g1 = f2).
+constexpr int VT = 6;
+
 struct PhysParams {
     double gamma, gm1, cp, cv, lambda, invSGM, mu, cpPr, sigma0, ip_eps, mach_sqrtg;
     // Reference pressure of the FP32 form (pref, Eref = pref/(gamma-1)); zero in the FP64 form.
@@ -37,7 +41,7 @@ struct StageArgs {
     double *QDot;               // derivative output (derivative-only mode) or nullptr
     const void *tr_in;          // Q traces [(e*4+f)*P + k][4] (FP64 or FP32)
     void *tr_out;               // next-stage Q traces
-    const void *dtr;            // gradient traces [(e*4+f)*P + k][2][4]
+    const void *dtr;            // viscous face-flux traces [(e*4+f)*P + k][VT] (FP64 or FP32)
     void *dtr_out;
     const void *metric;         // 5 planes of nodes: Ja11, Ja12, Ja21, Ja22, jac
     const int4 *face;           // per element face record

@@ -3,8 +3,9 @@
 // Data layout in HBM (uniform polynomial order P = N+1 per handle):
 //   Q, G      : FP64, reference layout  Q[(e*P*P + i + P*j)*4 + eq]
 //   traces    : [(e*4 + f)*P + k][4]  element-local face orientation, FP64 or FP32
-//   gradient traces: [(e*4 + f)*P + k][d*4 + eq], FP64 or FP32 (the gradient at the nodes is
-//                    recomputed inside each stage and never stored)
+//   viscous flux traces: [(e*4 + f)*P + k][VT], FP64 or FP32: mu times each side's viscous
+//               flux at its face nodes (the gradient itself is never stored: the nodal
+//               gradient is recomputed inside each stage)
 //   metric    : five planes of e*P*P+n: Ja11, Ja12, Ja21, Ja22, 1/jac (FP64 or FP32); the
 //               divisions by the Jacobian (DGSpatialDiscretizationMethods.f90:226-233) are
 //               multiplications by its reciprocal (a last-bit difference, not bit-exact)
@@ -219,6 +220,13 @@ __device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, in
     }
 }
 
+// mu (f, g) of one side of a face node from its viscous flux trace (layout of VT).
+template <typename R>
+__device__ __forceinline__ void viscous_trace_flux(const R *v, R *f, R *g) {
+    f[0] = R(0); f[1] = v[0]; f[2] = v[1]; f[3] = v[2];
+    g[0] = R(0); g[1] = v[1]; g[2] = v[3]; g[3] = v[4];
+}
+
 // Own Q trace of a face node as the previous stage (or the prolongation) stored it: the values
 // the neighbour reads, in the compute type.
 template <int P, typename R, typename TR>
@@ -241,9 +249,10 @@ __device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe,
 }
 
 // ------------------------------------------------------------------ gradient trace kernel
-// Evaluates the element gradient (BR1 or IP) and writes the gradient traces of the element's
-// four faces, which the neighbours' stage kernels read.  The gradient at the nodes is not
-// stored: each stage kernel recomputes it for its own element.
+// Evaluates the element gradient (BR1 or IP), its traces on the element's four faces and, from
+// them and the Q traces, this side's viscous flux at every face node, which both elements of
+// the face read in their stage kernels.  The gradient at the nodes is not stored: each stage
+// kernel recomputes it for its own element.
 template <int P, int EPB, bool BR1, typename R>
 struct GradSmem {
     using T = Tile<P, EPB>;
@@ -302,17 +311,18 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         }
     }
     const int fs = t / (4 * P), ff = (t / P) % 4, fk = t % P, fe = e0 + fs;
-    const bool face_active = BR1 && (t < NF) && (fe < a.K);
+    const bool face_active = (t < NF) && (fe < a.K);
     int4 fr = make_int4(0, 0, 0, 0);
     R nx = R(0), ny = R(0), dS = R(0), qo[4] = {R(0), R(0), R(0), R(0)};
     int64_t nslot = 0;
     R qs[4] = {R(0), R(0), R(0), R(0)};
     if (face_active) {
-        load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
+        if (BR1) load_face<P, R, TR>(a, fe, ff, fk, fr, nx, ny, dS, qo, nslot);
+        else fr = a.face[fe * 4 + ff];
         stored_trace<P, R, TR>(a, fe, ff, fk, qs);
     }
     __syncthreads();
-    if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 6);
+    if (BR1 && face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 6);
     if (BR1) __syncthreads();
     R dq[8];
     if (node_active) {
@@ -326,15 +336,19 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
     }
     __syncthreads();
-    if (t < NF) {
-        const int s2 = t / (4 * P), f = (t / P) % 4, k = t % P, e2 = e0 + s2;
-        if (e2 < a.K) {
-            double tg[8];
-            face_trace<P, 8, 8>(sDQ + s2 * PP * 8, sLb, f, k, tg);
-            TR *o = (TR *)a.dtr_out + ((int64_t)(e2 * 4 + f) * P + k) * 8;
+    if (face_active) {
+        // this side's viscous flux at the face node from its Q and gradient traces
+        // (ViscousFluxes.f90:39-80, adiabatic walls :279-318), times mu
+        double tg[8];
+        face_trace<P, 8, 8>(sDQ + fs * PP * 8, sLb, ff, fk, tg);
+        R ds[8], fv[4], gv[4];
 #pragma unroll
-            for (int c = 0; c < 8; ++c) o[c] = (TR)tg[c];
-        }
+        for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
+        if ((fr.x & 15) == FK_WALL && !(fr.x & FF_ISOTHERMAL)) adiabatic_flux(ph, qs, ds, fv, gv);
+        else viscous_flux(ph, qs, ds, fv, gv);
+        TR *o = (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT;
+        o[0] = (TR)(ph.mu * fv[1]); o[1] = (TR)(ph.mu * fv[2]); o[2] = (TR)(ph.mu * fv[3]);
+        o[3] = (TR)(ph.mu * gv[2]); o[4] = (TR)(ph.mu * gv[3]); o[5] = TR(0);
     }
 }
 
@@ -367,9 +381,8 @@ struct StageSmem {
 // One element tile [e0, e0+EPB) of a stage; any workgroup size >= Tile<P, EPB>::TPB (at most
 // 256).  rslot is the residual partial this tile writes; cq, cg are the stage's RK factors.
 // Navier-Stokes: the element's nodal gradient is recomputed here from the LDS tile and the Q
-// traces (it stays in registers); the gradient traces of both sides of a face are read from
-// what the gradient kernel stored, which is the same function of the same data.  The own Q
-// traces are read from the stored traces as well.
+// traces (it stays in registers); the viscous fluxes of both sides of a face are read from
+// what the gradient kernel stored.  The own Q traces are read from the stored traces as well.
 template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, const int64_t rslot, const double cq,
                                            const double cg, StageSmem<P, EPB, NS, IP, R> &S) {
@@ -437,13 +450,13 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
         else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
     }
-    // the neighbour's gradient trace, issued before the barrier and consumed in the face phase
-    R dn[NS ? 8 : 1];
+    // the neighbour's viscous flux trace, issued before the barrier and consumed in the face phase
+    R vn[NS ? VT : 1];
     R invh = R(0);
     if (NS && face_active) {
         if ((fr.x & 15) <= FK_PERIODIC) {
 #pragma unroll
-            for (int c = 0; c < (NS ? 8 : 1); ++c) dn[c] = (R)((const TR *)a.dtr)[nslot * 8 + c];
+            for (int c = 0; c < (NS ? VT : 1); ++c) vn[c] = (R)((const TR *)a.dtr)[nslot * VT + c];
         }
         if (IP) invh = (R)a.einvh[fr.w];
     }
@@ -478,11 +491,11 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     // ---- face phase: Riemann fluxes of the element's own four faces
     if (face_active) {
         const int kind = fr.x & 15;
-        R ds[NS ? 8 : 1];   // own gradient trace
+        R vs[NS ? VT : 1];   // own viscous flux trace
         if (NS) {
-            const TR *dt = (const TR *)a.dtr + ((int64_t)(fe * 4 + ff) * P + fk) * 8;
+            const TR *vt = (const TR *)a.dtr + ((int64_t)(fe * 4 + ff) * P + fk) * VT;
 #pragma unroll
-            for (int c = 0; c < (NS ? 8 : 1); ++c) ds[c] = (R)dt[c];
+            for (int c = 0; c < (NS ? VT : 1); ++c) vs[c] = (R)vt[c];
         }
         int rs = a.ph.riemann;
         R uSB[4] = {R(0), R(0), R(0), R(0)};
@@ -505,15 +518,15 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             const R pen = IP ? ph.sigma0 * ph.mu * (R)(P - 1) * (R)(P - 1) * invh : R(0);
             if (kind <= FK_PERIODIC) {
                 const R *uL = (kind == FK_INT_R) ? qo : qs, *uR = (kind == FK_INT_R) ? qs : qo;
-                const R *dL = (kind == FK_INT_R) ? dn : ds, *dR = (kind == FK_INT_R) ? ds : dn;
+                const R *vL = (kind == FK_INT_R) ? vn : vs, *vR = (kind == FK_INT_R) ? vs : vn;
                 R fL[4], gL[4], fR[4], gR[4];
-                viscous_flux(ph, uL, dL, fL, gL);
-                viscous_flux(ph, uR, dR, fR, gR);
+                viscous_trace_flux(vL, fL, gL);
+                viscous_trace_flux(vR, fR, gR);
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     R acc = R(0);
-                    acc = acc + R(0.5) * (ph.mu * fL[c] + ph.mu * fR[c]) * nx;
-                    acc = acc + R(0.5) * (ph.mu * gL[c] + ph.mu * gR[c]) * ny;
+                    acc = acc + R(0.5) * (fL[c] + fR[c]) * nx;
+                    acc = acc + R(0.5) * (gL[c] + gR[c]) * ny;
                     Fv[c] = acc;
                 }
                 if (IP) {
@@ -534,13 +547,12 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             } else if (!neumann) {
                 const bool adiab = kind == FK_WALL && !(fr.x & FF_ISOTHERMAL);
                 R fv4[4], gv4[4];
-                if (adiab) adiabatic_flux(ph, qs, ds, fv4, gv4);
-                else viscous_flux(ph, qs, ds, fv4, gv4);
+                viscous_trace_flux(vs, fv4, gv4);   // adiabatic or not, as the gradient kernel chose
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     R acc = R(0);
-                    acc = acc + ph.mu * fv4[c] * nx;
-                    acc = acc + ph.mu * gv4[c] * ny;
+                    acc = acc + fv4[c] * nx;
+                    acc = acc + gv4[c] * ny;
                     Fv[c] = acc;
                 }
                 if (IP) {
