@@ -425,7 +425,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(h->alloc_raw(&h->tr[1], tsz));
     if (ph.ns) {
         const size_t rs = h->mixed ? sizeof(float) : sizeof(double);
-        HG_CHECK(h->alloc_raw(&h->dtr, (size_t)K * 4 * P * VT * rs));
+        HG_CHECK(h->alloc_raw(&h->dtr, (size_t)K * 4 * P * hg::VT * rs));
     }
     HG_CHECK(upload_real(h.get(), &h->metric, metric, h->mixed));
     if (!fmet.empty()) HG_CHECK(upload_real(h.get(), &h->fmet, fmet, h->mixed));

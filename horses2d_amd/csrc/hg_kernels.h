@@ -118,7 +118,7 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
 // (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622), faces lifted in edge-id order.
 // q: element state in the compute type [n*4] (energy slot shifted in the FP32 form),
 // Ja: element metric [n*4], Fu: element uStar fluxes [(f*P + k)*6] (see br1_face_flux).
-template <int P, typename R>
+template <int P, typename R, int QS = 4, int JS = 4>
 __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const R *sHat, const R *Fu, const R *sLbw,
                                                   int ord, int i, int j, R ijac, R *dq) {
     // Every accumulator receives its xi terms (l = 0..P-1) then its eta terms; each state value
@@ -131,10 +131,10 @@ __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const
     for (int l = 0; l < P; ++l) {
         const int m = l + P * j;
         const R h = sHat[i * P + l];
-        const R w0 = Ja[m * 4 + 0] * h, w2 = Ja[m * 4 + 2] * h;
+        const R w0 = Ja[m * JS + 0] * h, w2 = Ja[m * JS + 2] * h;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const R v = q[m * 4 + c];
+            const R v = q[m * QS + c];
             acc[c] += v * w0;
             acc[4 + c] += v * w2;
         }
@@ -143,10 +143,10 @@ __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const
     for (int l = 0; l < P; ++l) {
         const int m = i + P * l;
         const R h = sHat[j * P + l];
-        const R w1 = Ja[m * 4 + 1] * h, w3 = Ja[m * 4 + 3] * h;
+        const R w1 = Ja[m * JS + 1] * h, w3 = Ja[m * JS + 3] * h;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const R v = q[m * 4 + c];
+            const R v = q[m * QS + c];
             acc[c] += v * w1;
             acc[4 + c] += v * w3;
         }
@@ -171,16 +171,16 @@ __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const
 }
 
 // Interior (broken) gradient of the IP method (QuadElement_QuadProcedures.incf:42-88); q as above.
-template <int P, typename R>
+template <int P, typename R, int QS = 4>
 __device__ __forceinline__ void ip_node_gradient(const R *q, const R *sD, int i, int j, R j11, R j12, R j21, R j22,
                                                  R ijac, R *dq) {
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         R dxi = R(0), deta = R(0);
 #pragma unroll
-        for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * q[(l + P * j) * 4 + c];
+        for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * q[(l + P * j) * QS + c];
 #pragma unroll
-        for (int l = 0; l < P; ++l) deta += sD[j * P + l] * q[(i + P * l) * 4 + c];
+        for (int l = 0; l < P; ++l) deta += sD[j * P + l] * q[(i + P * l) * QS + c];
         dq[c] = (dxi * j11 + deta * j12) * ijac;
         dq[4 + c] = (dxi * j21 + deta * j22) * ijac;
     }
@@ -188,14 +188,15 @@ __device__ __forceinline__ void ip_node_gradient(const R *q, const R *sD, int i,
 
 // Stages an element tile's state from HBM into LDS: FP64 always, plus the compute-type copy
 // (energy slot shifted) that the gradient contractions read when the compute type is FP32.
-template <typename R, bool COPY, bool KEEP64 = true>
+// The FP64 tile has a node stride of QS doubles (4, or 6 in the gradient kernel).
+template <typename R, bool COPY, bool KEEP64 = true, int QS = 4>
 __device__ __forceinline__ void load_tile(const StageArgs &a, const int64_t nb0, const int ne, const int PP, double *sQ,
                                           R *sQf, const int t, const int TPB) {
     const double2 *src = (const double2 *)(a.Q + nb0 * 4);
     double2 *dst = (double2 *)sQ;
     for (int x = t; x < ne * PP * 2; x += TPB) {
         const double2 v = src[x];
-        if constexpr (KEEP64) dst[x] = v;
+        if constexpr (KEEP64) dst[(x >> 1) * (QS / 2) + (x & 1)] = v;
         if constexpr (COPY) {
             // pairs (0,1) and (2,3) of a node: the odd pair holds the energy
             ((float2 *)sQf)[x] = make_float2((float)v.x, (float)((x & 1) ? slot_energy<R>(a.ph, v.y) : v.y));
@@ -253,19 +254,31 @@ __device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe,
 // them and the Q traces, this side's viscous flux at every face node, which both elements of
 // the face read in their stage kernels.  The gradient at the nodes is not stored: each stage
 // kernel recomputes it for its own element.
+// Node strides (in elements of the tile's type) of the gradient kernel's LDS tiles, chosen so
+// that the P nodes of a row or a column of an element fall in distinct LDS banks for 16-byte
+// reads: FP64 state and metric 48 B, FP64 gradient 80 B; the FP32 state copy and metric
+// (16 B per node) need no padding, the FP32 gradient is padded to 48 B.
+template <typename R>
+struct GradStrides {
+    static constexpr int QS = sizeof(R) == 8 ? 6 : 4;    // state (FP64 tile or FP32 copy)
+    static constexpr int JS = sizeof(R) == 8 ? 6 : 4;    // metric
+    static constexpr int DS = sizeof(R) == 8 ? 10 : 12;  // gradient
+};
+
 template <int P, int EPB, bool BR1, typename R>
 struct GradSmem {
     using T = Tile<P, EPB>;
+    using ST = GradStrides<R>;
     struct Inputs {
-        alignas(16) double sQ[sizeof(R) == 4 ? 2 : T::NN * 4];   // FP64 form: the state
-        alignas(16) R sQf[sizeof(R) == 4 ? T::NN * 4 : 1];        // FP32 form: the compute-type state
-        alignas(16) R sJa[BR1 ? T::NN * 4 : 1];
+        alignas(16) double sQ[sizeof(R) == 4 ? 2 : T::NN * ST::QS];   // FP64 form: the state
+        alignas(16) R sQf[sizeof(R) == 4 ? T::NN * ST::QS : 1];        // FP32 form: the compute-type state
+        alignas(16) R sJa[BR1 ? T::NN * ST::JS : 1];
     };
     // the state and the metric are dead once every node's gradient is formed: the gradient
     // tile takes their place
     union {
         Inputs in;
-        alignas(16) R sDQ[T::NN * 8];
+        alignas(16) R sDQ[T::NN * ST::DS];
     } u;
     alignas(16) double sLb[2 * P];
     alignas(16) R sFu[BR1 ? T::NF * 6 : 1];
@@ -292,7 +305,9 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     const int ne = min(EPB, a.K - e0);
     const int64_t nb0 = (int64_t)e0 * PP;
     constexpr bool COPY = sizeof(R) == 4;
-    load_tile<R, COPY, !COPY>(a, nb0, ne, PP, sQ, S.u.in.sQf, t, TPB);
+    using ST = GradStrides<R>;
+    constexpr int QS = ST::QS, JS = ST::JS, DS = ST::DS;
+    load_tile<R, COPY, !COPY, QS>(a, nb0, ne, PP, sQ, S.u.in.sQf, t, TPB);
     const R *sQr;
     if constexpr (COPY) sQr = S.u.in.sQf; else sQr = (const R *)sQ;
     // ---- hoisted global loads
@@ -307,7 +322,7 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         ijac = ld<R>(a.metric, 4 * a.nodes + g);
         if (BR1) {
             ord = a.forder[e];
-            sJa[t * 4 + 0] = j11; sJa[t * 4 + 1] = j12; sJa[t * 4 + 2] = j21; sJa[t * 4 + 3] = j22;
+            sJa[t * JS + 0] = j11; sJa[t * JS + 1] = j12; sJa[t * JS + 2] = j21; sJa[t * JS + 3] = j22;
         }
     }
     const int fs = t / (4 * P), ff = (t / P) % 4, fk = t % P, fe = e0 + fs;
@@ -327,20 +342,20 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     R dq[8];
     if (node_active) {
         const int i = n % P, j = n / P;
-        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sJa + s * PP * 4, sHat, sFu + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
-        else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
+        if (BR1) br1_node_gradient<P, R, QS, JS>(sQr + s * PP * QS, sJa + s * PP * JS, sHat, sFu + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
+        else ip_node_gradient<P, R, QS>(sQr + s * PP * QS, sD, i, j, j11, j12, j21, j22, ijac, dq);
     }
     __syncthreads();   // every lane is done with the state and the metric
     if (node_active) {
 #pragma unroll
-        for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
+        for (int c = 0; c < 8; ++c) sDQ[t * DS + c] = dq[c];
     }
     __syncthreads();
     if (face_active) {
         // this side's viscous flux at the face node from its Q and gradient traces
         // (ViscousFluxes.f90:39-80, adiabatic walls :279-318), times mu
         double tg[8];
-        face_trace<P, 8, 8>(sDQ + fs * PP * 8, sLb, ff, fk, tg);
+        face_trace<P, DS, 8>(sDQ + fs * PP * DS, sLb, ff, fk, tg);
         R ds[8], fv[4], gv[4];
 #pragma unroll
         for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
