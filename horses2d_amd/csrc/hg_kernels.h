@@ -283,7 +283,7 @@ struct GradSmem {
     alignas(16) double sLb[2 * P];
     alignas(16) R sFu[BR1 ? T::NF * 6 : 1];
     alignas(16) R sHat[BR1 ? T::PP : 1];
-    alignas(16) R sLbw[2 * P];
+    alignas(16) R sLbw[4 * P];   // lbw, then lb in the compute type
     alignas(16) R sD[BR1 ? 1 : T::PP];
 };
 
@@ -297,7 +297,7 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     R *sDQ = S.u.sDQ, *sJa = S.u.in.sJa, *sFu = S.sFu, *sHat = S.sHat, *sLbw = S.sLbw, *sD = S.sD;
     const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
-    if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; }
+    if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; sLbw[2 * P + t] = (R)a.lb[t]; }
     for (int x = t; x < PP; x += TPB) {
         if (BR1) sHat[x] = (R)a.hatD[x];
         else sD[x] = (R)a.D[x];
@@ -354,11 +354,15 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     if (face_active) {
         // this side's viscous flux at the face node from its Q and gradient traces
         // (ViscousFluxes.f90:39-80, adiabatic walls :279-318), times mu
-        double tg[8];
-        face_trace<P, DS, 8>(sDQ + fs * PP * DS, sLb, ff, fk, tg);
         R ds[8], fv[4], gv[4];
+        if constexpr (sizeof(R) == 4) {   // FP32 form: accumulated in FP32
+            face_trace_r<P, DS, 8>(sDQ + fs * PP * DS, sLbw + 2 * P, ff, fk, ds);
+        } else {
+            double tg[8];
+            face_trace<P, DS, 8>(sDQ + fs * PP * DS, sLb, ff, fk, tg);
 #pragma unroll
-        for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
+            for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
+        }
         if ((fr.x & 15) == FK_WALL && !(fr.x & FF_ISOTHERMAL)) adiabatic_flux(ph, qs, ds, fv, gv);
         else viscous_flux(ph, qs, ds, fv, gv);
         TR *o = (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT;

@@ -403,4 +403,22 @@ __device__ inline void face_trace(const T *sX /* element block, [n*STRIDE + c] *
     }
 }
 
+// face_trace accumulated in the tile's own type.  The FP32 form uses it for the gradient
+// traces, which only the kernel that forms them ever evaluates (the viscous face fluxes are
+// stored), so no other kernel needs to reproduce these bits.
+template <int P, int STRIDE, int NV, typename T>
+__device__ inline void face_trace_r(const T *sX, const T *sLb, int f, int k, T *out) {
+    const T *lb = (f == 0 || f == 3) ? sLb : sLb + P;
+    const bool horiz = (f == 0 || f == 2);
+    const int base = horiz ? k : P * k, step = horiz ? P : 1;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) out[c] = T(0);
+#pragma unroll 1
+    for (int l = 0; l < P; ++l) {
+        const int n = base + step * l;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) out[c] = fma(sX[n * STRIDE + c], lb[l], out[c]);
+    }
+}
+
 }  // namespace hg
