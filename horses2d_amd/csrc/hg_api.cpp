@@ -67,6 +67,7 @@ hg::StageArgs base_args(hg_handle *h) {
     a.dtr = h->dtr;
     a.dtr_out = h->dtr;
     a.metric = h->metric;
+    a.metric_elem = h->metric_elem;
     a.face = h->face;
     a.forder = h->forder;
     a.egeo = h->egeo;
@@ -427,7 +428,35 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
         const size_t rs = h->mixed ? sizeof(float) : sizeof(double);
         HG_CHECK(h->alloc_raw(&h->dtr, (size_t)K * 4 * P * hg::VT * rs));
     }
-    HG_CHECK(upload_real(h.get(), &h->metric, metric, h->mixed));
+    // The FP32 form stores one metric record per element when every element's metric is
+    // constant to FP32 resolution (straight-sided parallelograms): each node's value differs
+    // from the element's first node by less than half an FP32 ulp of the element's largest
+    // Ja component (1/jac: of its own magnitude), i.e. by less than the FP32 rounding the form
+    // applies to the metric anyway.
+    std::vector<double> metric_e;
+    if (h->mixed) {
+        const size_t nn = (size_t)h->nodes;
+        bool constant = true;
+        for (int e = 0; e < K && constant; ++e) {
+            const size_t g0 = (size_t)e * PP;
+            double scale = 0.0;
+            for (int c = 0; c < 4; ++c)
+                for (int n = 0; n < PP; ++n) scale = std::fmax(scale, std::fabs(metric[c * nn + g0 + n]));
+            for (int c = 0; c < 5 && constant; ++c) {
+                const double ref = metric[c * nn + g0];
+                const double tol = std::ldexp(c < 4 ? scale : std::fabs(ref), -25);
+                for (int n = 1; n < PP; ++n)
+                    if (std::fabs(metric[c * nn + g0 + n] - ref) > tol) { constant = false; break; }
+            }
+        }
+        if (constant) {
+            metric_e.resize(5 * (size_t)K);
+            for (int c = 0; c < 5; ++c)
+                for (int e = 0; e < K; ++e) metric_e[(size_t)c * K + e] = metric[c * nn + (size_t)e * PP];
+            h->metric_elem = 1;
+        }
+    }
+    HG_CHECK(upload_real(h.get(), &h->metric, h->metric_elem ? metric_e : metric, h->mixed));
     if (!fmet.empty()) HG_CHECK(upload_real(h.get(), &h->fmet, fmet, h->mixed));
     HG_CHECK(upload(h.get(), &h->face, face));
     HG_CHECK(upload(h.get(), &h->forder, forder));

@@ -43,7 +43,8 @@ struct StageArgs {
     void *tr_out;               // next-stage Q traces
     const void *dtr;            // viscous face-flux traces [(e*4+f)*P + k][VT] (FP64 or FP32)
     void *dtr_out;
-    const void *metric;         // 5 planes of nodes: Ja11, Ja12, Ja21, Ja22, jac
+    const void *metric;         // 5 planes: Ja11, Ja12, Ja21, Ja22, 1/jac, per node or per element
+    int metric_elem;            // 1: one metric record per element (element-constant metric)
     const int4 *face;           // per element face record
     const int *forder;          // per element: faces sorted by global edge id (2 bits each)
     const double *egeo;         // edge geometry triplets (nx, ny, dS)

@@ -34,6 +34,7 @@ struct hg_handle {
     void *tr[2] = {nullptr, nullptr};
     void *dtr = nullptr;
     void *metric = nullptr, *fmet = nullptr;
+    int metric_elem = 0;   // metric stored per element (FP32 form, element-constant metric)
     int4 *face = nullptr;
     int *forder = nullptr;
     double *egeo = nullptr, *einvh = nullptr, *zone = nullptr;

@@ -6,7 +6,8 @@
 //   viscous flux traces: [(e*4 + f)*P + k][VT], FP64 or FP32: mu times each side's viscous
 //               flux at its face nodes (the gradient itself is never stored: the nodal
 //               gradient is recomputed inside each stage)
-//   metric    : five planes of e*P*P+n: Ja11, Ja12, Ja21, Ja22, 1/jac (FP64 or FP32); the
+//   metric    : five planes of e*P*P+n: Ja11, Ja12, Ja21, Ja22, 1/jac (FP64 or FP32; the FP32
+//               form stores one record per element when the metric is element-constant); the
 //               divisions by the Jacobian (DGSpatialDiscretizationMethods.f90:226-233) are
 //               multiplications by its reciprocal (a last-bit difference, not bit-exact)
 //
@@ -204,6 +205,17 @@ __device__ __forceinline__ void load_tile(const StageArgs &a, const int64_t nb0,
     }
 }
 
+// Metric of node g (element e): five planes per node, or per element when the handle stores an
+// element-constant metric.
+template <typename R>
+__device__ __forceinline__ void load_metric(const StageArgs &a, int e, int64_t g, R &j11, R &j12, R &j21, R &j22,
+                                            R &ijac) {
+    const int64_t i = a.metric_elem ? (int64_t)e : g, st = a.metric_elem ? (int64_t)a.K : a.nodes;
+    j11 = ld<R>(a.metric, i); j12 = ld<R>(a.metric, st + i);
+    j21 = ld<R>(a.metric, 2 * st + i); j22 = ld<R>(a.metric, 3 * st + i);
+    ijac = ld<R>(a.metric, 4 * st + i);
+}
+
 // Face-node geometry and the neighbour's Q trace (periodic and interior faces).
 template <int P, typename R, typename TR>
 __device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, int fk, int4 &fr, R &nx, R &ny, R &dS,
@@ -317,9 +329,7 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     int ord = 0;
     if (node_active) {
         const int64_t g = nb0 + s * PP + n;
-        j11 = ld<R>(a.metric, 0 * a.nodes + g); j12 = ld<R>(a.metric, 1 * a.nodes + g);
-        j21 = ld<R>(a.metric, 2 * a.nodes + g); j22 = ld<R>(a.metric, 3 * a.nodes + g);
-        ijac = ld<R>(a.metric, 4 * a.nodes + g);
+        load_metric<R>(a, e, g, j11, j12, j21, j22, ijac);
         if (BR1) {
             ord = a.forder[e];
             sJa[t * JS + 0] = j11; sJa[t * JS + 1] = j12; sJa[t * JS + 2] = j21; sJa[t * JS + 3] = j22;
@@ -436,9 +446,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     R j11 = R(0), j12 = R(0), j21 = R(0), j22 = R(0), ijac = R(1);
     int ord = 0;
     if (node_active) {
-        j11 = ld<R>(a.metric, 0 * a.nodes + g); j12 = ld<R>(a.metric, 1 * a.nodes + g);
-        j21 = ld<R>(a.metric, 2 * a.nodes + g); j22 = ld<R>(a.metric, 3 * a.nodes + g);
-        ijac = ld<R>(a.metric, 4 * a.nodes + g);
+        load_metric<R>(a, e, g, j11, j12, j21, j22, ijac);
         ord = a.forder[e];
         if (BR1) {
             sCF[t * 4 + 0] = j11; sCF[t * 4 + 1] = j12; sCF[t * 4 + 2] = j21; sCF[t * 4 + 3] = j22;
