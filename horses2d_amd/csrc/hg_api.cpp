@@ -64,8 +64,8 @@ hg::StageArgs base_args(hg_handle *h) {
     a.Q = h->Q;
     a.G = h->G;
     a.QDot = h->QDot;
-    a.dtr = h->dtr;
-    a.dtr_out = h->dtr;
+    a.dtr = h->dtr[h->ip_fused ? h->cur : 0];
+    a.dtr_out = h->dtr[h->ip_fused ? h->cur : 0];
     a.metric = h->metric;
     a.metric_elem = h->metric_elem;
     a.face = h->face;
@@ -189,14 +189,18 @@ StepCoefs step_coefs(int scheme) {
 int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode,
               const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0) {
     if (autotune(h)) return 1;
-    if (h->ph.ns) {
+    // BR1: the viscous face-flux traces of the current state, every stage; interior penalty:
+    // only when the previous stage's epilogue did not write them (after an upload)
+    if (h->ph.ns && (!h->ip_fused || !h->vtr_valid)) {
         hg::StageArgs g = base_args(h);
         g.tr_in = h->tr[h->cur];
         HG_CHECK(timed(h, 1, [&] { return hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream); }));
+        if (h->ip_fused) h->vtr_valid = true;
     }
     hg::StageArgs a = base_args(h);
     a.tr_in = h->tr[h->cur];
     a.tr_out = h->tr[h->cur ^ 1];
+    a.dtr_out = h->dtr[h->ip_fused ? (h->cur ^ 1) : 0];
     a.am = am;
     a.cq = cq;
     a.cg = cg;
@@ -249,6 +253,19 @@ int build_graphs(hg_handle *h, const StepCoefs &k) {
 
 namespace hg {
 StageArgs handle_args(hg_handle *h) { return base_args(h); }
+
+void *vtr_current(hg_handle *h) { return h->dtr[h->ip_fused ? h->cur : 0]; }
+void *vtr_next(hg_handle *h) { return h->dtr[h->ip_fused ? (h->cur ^ 1) : 0]; }
+
+int ensure_vtr(hg_handle *h) {
+    if (!h->ip_fused || h->vtr_valid) return 0;
+    if (autotune(h)) return 1;
+    hg::StageArgs g = base_args(h);
+    g.tr_in = h->tr[h->cur];
+    HG_CHECK(hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream));
+    h->vtr_valid = true;
+    return 0;
+}
 int handle_error(const std::string &m) { return set_err(m); }
 }  // namespace hg
 
@@ -426,7 +443,9 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(h->alloc_raw(&h->tr[1], tsz));
     if (ph.ns) {
         const size_t rs = h->mixed ? sizeof(float) : sizeof(double);
-        HG_CHECK(h->alloc_raw(&h->dtr, (size_t)K * 4 * P * hg::VT * rs));
+        h->ip_fused = ph.viscous == HG_VISCOUS_IP;
+        for (int b = 0; b < (h->ip_fused ? 2 : 1); ++b)
+            HG_CHECK(h->alloc_raw(&h->dtr[b], (size_t)K * 4 * P * hg::VT * rs));
     }
     // The FP32 form stores one metric record per element when every element's metric is
     // constant to FP32 resolution (straight-sided parallelograms): each node's value differs
@@ -600,6 +619,7 @@ int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     h->cur = 0;
     a.tr_out = h->tr[0];
     HG_CHECK(hg::launch_prolong(h->P, h->mixed, a, h->stream));
+    h->vtr_valid = false;
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -633,6 +653,7 @@ int hg_rk_step(hg_handle *h, double t, double dt) {
     if (h->use_graph && !h->prof) {
         // replay the captured step (per-kernel event timing uses the direct path below)
         if (autotune(h)) return 1;
+        if (hg::ensure_vtr(h)) return 1;   // the captured step holds no gradient launch for IP
         if (!h->dt_valid || dt != h->dt_last) {
             h->dt_last = dt;
             h->dt_valid = true;

@@ -104,9 +104,10 @@ hg::BatchArgs args_of(const hg_batch *b, int tile, int order, int slot) {
     return a;
 }
 
-// One RK step of all cases for the given trace parity (launches only).
+// One RK step of all cases for the given trace parity (launches only).  Interior penalty: no
+// gradient launch, each stage's epilogue forms the next stage's viscous face-flux traces.
 hipError_t enqueue_step(hg_batch *b, int parity) {
-    const bool ns = b->cases[0]->ph.ns != 0;
+    const bool ns = b->cases[0]->ph.ns != 0 && !b->cases[0]->ip_fused;
     for (int s = 0; s < b->sch.nst; ++s) {
         const int slot = parity * b->sch.nst + s;
         hg::BatchArgs a = args_of(b, b->tile, b->order, slot);
@@ -202,6 +203,8 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     for (int c = 1; c < ncases; ++c)
         if (cases[c]->ph.rs_kernel != b->rs) b->rs = -1;
     HB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    for (auto *h : b->cases)
+        if (hg::ensure_vtr(h)) return 1;
     for (auto *h : b->cases) HB_CHECK(hipStreamSynchronize(h->stream));
 
     // work lists per tile: (case, first element, P, residual slot); residual ranges per case
@@ -266,6 +269,8 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
                 const int in = h->cur ^ p ^ (s & 1);
                 a.tr_in = h->tr[in];
                 a.tr_out = h->tr[in ^ 1];
+                a.dtr = h->dtr[h->ip_fused ? in : 0];
+                a.dtr_out = h->dtr[h->ip_fused ? (in ^ 1) : 0];
                 a.am = b->sch.am[s];
                 a.cq = b->sch.cq[s];
                 a.cg = b->sch.cg[s];
@@ -275,7 +280,7 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
                 a.resid_part = b->resid_part;
                 slots[(size_t)c * b->nslots + p * b->sch.nst + s] = a;
             }
-        hg::StageArgs d = hg::handle_args(h);
+        hg::StageArgs d = hg::handle_args(h);   // dtr, dtr_out: the current traces
         d.tr_in = h->tr[h->cur];
         d.tr_out = h->tr[h->cur ^ 1];
         d.cq = 1.0;

@@ -32,7 +32,12 @@ struct hg_handle {
     // device arrays
     double *Q = nullptr, *G = nullptr, *QDot = nullptr;
     void *tr[2] = {nullptr, nullptr};
-    void *dtr = nullptr;
+    // viscous face-flux traces: one buffer (BR1: k_gradient writes it before each stage), or a
+    // ping-pong pair indexed like tr (interior penalty: each stage's epilogue writes the next
+    // stage's; k_gradient runs only when the current one is stale, after an upload)
+    void *dtr[2] = {nullptr, nullptr};
+    bool ip_fused = false;
+    bool vtr_valid = false;
     void *metric = nullptr, *fmet = nullptr;
     int metric_elem = 0;   // metric stored per element (FP32 form, element-constant metric)
     int4 *face = nullptr;
@@ -80,6 +85,11 @@ struct hg_handle {
 namespace hg {
 // StageArgs with every per-handle pointer and the physics filled in (per-stage fields zero).
 StageArgs handle_args(hg_handle *h);
+// Viscous face-flux traces of the current state: the buffer, and (interior penalty) making
+// sure it is up to date by running k_gradient once if the state was uploaded since.
+void *vtr_current(hg_handle *h);
+void *vtr_next(hg_handle *h);
+int ensure_vtr(hg_handle *h);
 // Records the message returned by hg_last_error and returns 1.
 int handle_error(const std::string &m);
 }  // namespace hg

@@ -175,15 +175,22 @@ __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const
 template <int P, typename R, int QS = 4>
 __device__ __forceinline__ void ip_node_gradient(const R *q, const R *sD, int i, int j, R j11, R j12, R j21, R j22,
                                                  R ijac, R *dq) {
+    // l outermost (each accumulator still sums l = 0..P-1 in order): one node's values live
+    R dxi[4] = {R(0), R(0), R(0), R(0)}, deta[4] = {R(0), R(0), R(0), R(0)};
+#pragma unroll 1
+    for (int l = 0; l < P; ++l) {
+        const R di = sD[i * P + l], dj = sD[j * P + l];
+        const R *qx = q + (l + P * j) * QS, *qy = q + (i + P * l) * QS;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            dxi[c] += di * qx[c];
+            deta[c] += dj * qy[c];
+        }
+    }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        R dxi = R(0), deta = R(0);
-#pragma unroll
-        for (int l = 0; l < P; ++l) dxi += sD[i * P + l] * q[(l + P * j) * QS + c];
-#pragma unroll
-        for (int l = 0; l < P; ++l) deta += sD[j * P + l] * q[(i + P * l) * QS + c];
-        dq[c] = (dxi * j11 + deta * j12) * ijac;
-        dq[4 + c] = (dxi * j21 + deta * j22) * ijac;
+        dq[c] = (dxi[c] * j11 + deta[c] * j12) * ijac;
+        dq[4 + c] = (dxi[c] * j21 + deta[c] * j22) * ijac;
     }
 }
 
@@ -259,6 +266,29 @@ __device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe,
     tq[3] = slot_energy<TR>(a.ph, tq[3]);
 #pragma unroll
     for (int c = 0; c < 4; ++c) qs[c] = (R)(TR)tq[c];
+}
+
+// This side's viscous flux at a face node from its Q trace qs and the element's nodal gradient
+// tile (node stride DS), times mu, stored in the VT layout (ViscousFluxes.f90:39-80, adiabatic
+// walls :279-318).  The gradient trace is accumulated in FP64 in the FP64 form and in FP32 in
+// the FP32 form (lbR: lb in the compute type).
+template <int P, int DS, typename R, typename TR>
+__device__ __forceinline__ void viscous_face_trace(const StageArgs &a, const Phys<R> &ph, int flags, const R *qs,
+                                                   const R *sDQe, const double *sLb, const R *lbR, int ff, int fk,
+                                                   TR *o) {
+    R ds[8], fv[4], gv[4];
+    if constexpr (sizeof(R) == 4) {
+        face_trace_r<P, DS, 8>(sDQe, lbR, ff, fk, ds);
+    } else {
+        double tg[8];
+        face_trace<P, DS, 8>(sDQe, sLb, ff, fk, tg);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
+    }
+    if ((flags & 15) == FK_WALL && !(flags & FF_ISOTHERMAL)) adiabatic_flux(ph, qs, ds, fv, gv);
+    else viscous_flux(ph, qs, ds, fv, gv);
+    o[0] = (TR)(ph.mu * fv[1]); o[1] = (TR)(ph.mu * fv[2]); o[2] = (TR)(ph.mu * fv[3]);
+    o[3] = (TR)(ph.mu * gv[2]); o[4] = (TR)(ph.mu * gv[3]); o[5] = TR(0);
 }
 
 // ------------------------------------------------------------------ gradient trace kernel
@@ -361,24 +391,9 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         for (int c = 0; c < 8; ++c) sDQ[t * DS + c] = dq[c];
     }
     __syncthreads();
-    if (face_active) {
-        // this side's viscous flux at the face node from its Q and gradient traces
-        // (ViscousFluxes.f90:39-80, adiabatic walls :279-318), times mu
-        R ds[8], fv[4], gv[4];
-        if constexpr (sizeof(R) == 4) {   // FP32 form: accumulated in FP32
-            face_trace_r<P, DS, 8>(sDQ + fs * PP * DS, sLbw + 2 * P, ff, fk, ds);
-        } else {
-            double tg[8];
-            face_trace<P, DS, 8>(sDQ + fs * PP * DS, sLb, ff, fk, tg);
-#pragma unroll
-            for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
-        }
-        if ((fr.x & 15) == FK_WALL && !(fr.x & FF_ISOTHERMAL)) adiabatic_flux(ph, qs, ds, fv, gv);
-        else viscous_flux(ph, qs, ds, fv, gv);
-        TR *o = (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT;
-        o[0] = (TR)(ph.mu * fv[1]); o[1] = (TR)(ph.mu * fv[2]); o[2] = (TR)(ph.mu * fv[3]);
-        o[3] = (TR)(ph.mu * gv[2]); o[4] = (TR)(ph.mu * gv[3]); o[5] = TR(0);
-    }
+    if (face_active)
+        viscous_face_trace<P, DS, R, TR>(a, ph, fr.x, qs, sDQ + fs * PP * DS, sLb, sLbw + 2 * P, ff, fk,
+                                         (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT);
 }
 
 template <int P, int EPB, bool BR1, typename R, typename TR>
@@ -394,8 +409,15 @@ struct StageSmem {
     alignas(16) double sQ[T::NN * 4];
     alignas(16) double sLb[2 * P];
     alignas(16) double sRes[4][4];
-    alignas(16) R sCF[T::NN * 4];              // BR1: the metric Ja during the gradient phase
-    alignas(16) R sCG[T::NN * 4];              // NS, FP32: the compute-type state until the node phase
+    struct Fluxes {
+        alignas(16) R sCF[T::NN * 4];          // BR1: the metric Ja during the gradient phase
+        alignas(16) R sCG[T::NN * 4];          // NS, FP32: the compute-type state until the node phase
+    };
+    // IP: the epilogue's gradient tile takes the place of the contravariant fluxes
+    union {
+        Fluxes f;
+        alignas(16) R sDQ[IP ? T::NN * 8 : 1];
+    } u;
     alignas(16) R sF[NS ? T::NF * 6 : T::NF * 4];  // BR1: uStar fluxes during the gradient phase
     alignas(16) R sU[IP ? T::NF * 4 : 1];
     alignas(16) R sV[IP ? T::NF * 4 : 1];
@@ -405,6 +427,7 @@ struct StageSmem {
     alignas(16) R sD[IP ? T::PP : 1];
     alignas(16) R sLbw[2 * P];
     alignas(16) R sDlbw[2 * P];
+    alignas(16) R sLbR[IP ? 2 * P : 1];   // lb in the compute type (IP epilogue, FP32 traces)
 };
 
 // One element tile [e0, e0+EPB) of a stage; any workgroup size >= Tile<P, EPB>::TPB (at most
@@ -412,6 +435,9 @@ struct StageSmem {
 // Navier-Stokes: the element's nodal gradient is recomputed here from the LDS tile and the Q
 // traces (it stays in registers); the viscous fluxes of both sides of a face are read from
 // what the gradient kernel stored.  The own Q traces are read from the stored traces as well.
+// Interior penalty: the gradient is element-local, so the RK update's epilogue also forms the
+// next stage's viscous face-flux traces from the updated tile (the work of k_gradient, with
+// the same device functions and the same bits), and no gradient kernel runs between stages.
 template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, const int64_t rslot, const double cq,
                                            const double cg, StageSmem<P, EPB, NS, IP, R> &S) {
@@ -420,12 +446,15 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     constexpr bool BR1 = NS && !IP;
     const int TPB = blockDim.x;
     double *sQ = S.sQ, *sLb = S.sLb;
-    R *sCF = S.sCF, *sCG = S.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
+    R *sCF = S.u.f.sCF, *sCG = S.u.f.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
     R *sHat = S.sHat, *sD = S.sD, *sLbw = S.sLbw, *sDlbw = S.sDlbw;
     const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
     const bool ipgrad = IP && a.ph.ip_gradflux;
-    if (t < 2 * P) { sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; sDlbw[t] = (R)a.dlbw[t]; }
+    if (t < 2 * P) {
+        sLb[t] = a.lb[t]; sLbw[t] = (R)a.lbw[t]; sDlbw[t] = (R)a.dlbw[t];
+        if (IP) S.sLbR[t] = (R)a.lb[t];
+    }
     for (int x = t; x < PP; x += TPB) {
         sHat[x] = (R)a.hatD[x];
         if (IP) sD[x] = (R)a.D[x];
@@ -729,6 +758,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     }
     if (a.mode == 0) {
         __syncthreads();
+        R qn[4];   // the next stage's own Q trace in the compute type (IP epilogue)
         if (face_active) {
             double tq[4];
             face_trace<P, 4, 4>(sQ + fs * PP * 4, sLb, ff, fk, tq);
@@ -736,6 +766,42 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             TR *o = (TR *)a.tr_out + ((int64_t)(fe * 4 + ff) * P + fk) * 4;
 #pragma unroll
             for (int c = 0; c < 4; ++c) o[c] = (TR)tq[c];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) qn[c] = (R)(TR)tq[c];
+        }
+        if constexpr (IP) {
+            // the updated state in the compute type (FP32: a fresh copy, the node phase
+            // overwrote the first one), the broken gradient, its traces and this side's
+            // viscous flux: what k_gradient evaluates for the updated state
+            const R *sQn;
+            if constexpr (COPY) {
+                if (node_active) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) sCG[t * 4 + c] = qslot<R>(a.ph, sQ[t * 4 + c], c);
+                }
+                __syncthreads();
+                sQn = sCG;
+            } else {
+                sQn = (const R *)sQ;
+            }
+            R dq[8];
+            if (node_active) {
+                // the metric again from memory: keeping it live from the node phase costs the
+                // FP64 kernel a wavefront per SIMD
+                R m11, m12, m21, m22, mij;
+                load_metric<R>(a, e, g, m11, m12, m21, m22, mij);
+                ip_node_gradient<P, R>(sQn + s * PP * 4, sD, n % P, n / P, m11, m12, m21, m22, mij, dq);
+            }
+            __syncthreads();   // the state copy and the fluxes' space are free
+            R *sDQ = S.u.sDQ;
+            if (node_active) {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
+            }
+            __syncthreads();
+            if (face_active)
+                viscous_face_trace<P, 8, R, TR>(a, ph, fr.x, qn, sDQ + fs * PP * 8, sLb, S.sLbR, ff, fk,
+                                                (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT);
         }
     }
 }
