@@ -119,16 +119,19 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
 // (DGViscous_BR1.f90:48-111, DGWeakIntegrals.f90:503-622), faces lifted in edge-id order.
 // q: element state in the compute type [n*4] (energy slot shifted in the FP32 form),
 // Ja: element metric [n*4], Fu: element uStar fluxes [(f*P + k)*6] (see br1_face_flux).
-template <int P, typename R, int QS = 4, int JS = 4>
+// ROLLED keeps the two l loops rolled (one node's values live; the stage kernel, where the
+// registers bound occupancy); the sums are the same either way.
+template <int P, typename R, int QS = 4, int JS = 4, bool ROLLED = false>
 __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const R *sHat, const R *Fu, const R *sLbw,
                                                   int ord, int i, int j, R ijac, R *dq) {
     // Every accumulator receives its xi terms (l = 0..P-1) then its eta terms; each state value
     // is read once and used for both directions, and the metric-weighted derivative entry
     // Ja * hatD is formed once per node for all four variables.
+    constexpr int UNROLL = ROLLED ? 1 : P;
     R acc[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[c] = R(0);
-#pragma unroll
+#pragma unroll UNROLL
     for (int l = 0; l < P; ++l) {
         const int m = l + P * j;
         const R h = sHat[i * P + l];
@@ -140,7 +143,7 @@ __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const
             acc[4 + c] += v * w2;
         }
     }
-#pragma unroll
+#pragma unroll UNROLL
     for (int l = 0; l < P; ++l) {
         const int m = i + P * l;
         const R h = sHat[j * P + l];
@@ -503,7 +506,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     R dq[NS ? 8 : 1];
     if (NS && node_active) {
         const int i = n % P, j = n / P;
-        if (BR1) br1_node_gradient<P, R>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
+        if (BR1) br1_node_gradient<P, R, 4, 4, true>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
         else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
     }
     // the neighbour's viscous flux trace, issued before the barrier and consumed in the face phase
