@@ -45,13 +45,18 @@ struct Tile {
     static constexpr int TPB = (((NN > NF ? NN : NF) + 63) / 64) * 64;
 };
 
-// Tile choices: 0 = widest tile within one wavefront, 1 = widest within four wavefronts.
+// Tile choices: 0 = widest tile within one wavefront, 1 = widest within four wavefronts whose
+// Euler FP64 LDS tile (state, two contravariant fluxes, face fluxes: 96 P^2 + 128 P bytes per
+// element) stays within 31 KB, so that five workgroups fit a CU's LDS (this caps P = 4 at 15
+// elements instead of 16; the batched kernels' LDS is the largest tile over all orders).
 template <int P>
 struct Tiles {
     static constexpr int a64 = 64 / (P * P), b64 = 64 / (4 * P);
     static constexpr int a256 = 256 / (P * P), b256 = 256 / (4 * P);
+    static constexpr int lds_cap = (31 * 1024) / (96 * P * P + 128 * P);
     static constexpr int small = (a64 < b64 ? a64 : b64) > 0 ? (a64 < b64 ? a64 : b64) : 1;
-    static constexpr int large = (a256 < b256 ? a256 : b256) > 0 ? (a256 < b256 ? a256 : b256) : 1;
+    static constexpr int wide = (a256 < b256 ? a256 : b256) > 0 ? (a256 < b256 ? a256 : b256) : 1;
+    static constexpr int large = (wide < lds_cap ? wide : lds_cap) > 0 ? (wide < lds_cap ? wide : lds_cap) : 1;
 };
 
 // ------------------------------------------------------------------ prolongation
