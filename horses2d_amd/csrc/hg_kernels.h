@@ -426,11 +426,11 @@ struct StageSmem {
         Fluxes f;
         alignas(16) R sDQ[IP ? T::NN * 8 : 1];
     } u;
-    alignas(16) R sF[NS ? T::NF * 6 : T::NF * 4];  // BR1: uStar fluxes during the gradient phase
+    alignas(16) R sF[NS && !IP ? T::NF * 6 : T::NF * 4];  // BR1: uStar fluxes during the gradient phase
+    // IP gradient-face terms: the face phase's contravariant fluxes, then (same arrays) the
+    // U / V lift coefficients
     alignas(16) R sU[IP ? T::NF * 4 : 1];
     alignas(16) R sV[IP ? T::NF * 4 : 1];
-    alignas(16) R sU2[IP ? T::NF * 4 : 1];
-    alignas(16) R sV2[IP ? T::NF * 4 : 1];
     alignas(16) R sHat[T::PP];
     alignas(16) R sD[IP ? T::PP : 1];
     alignas(16) R sLbw[2 * P];
@@ -454,7 +454,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     constexpr bool BR1 = NS && !IP;
     const int TPB = blockDim.x;
     double *sQ = S.sQ, *sLb = S.sLb;
-    R *sCF = S.u.f.sCF, *sCG = S.u.f.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV, *sU2 = S.sU2, *sV2 = S.sV2;
+    R *sCF = S.u.f.sCF, *sCG = S.u.f.sCG, *sF = S.sF, *sU = S.sU, *sV = S.sV;
     R *sHat = S.sHat, *sD = S.sD, *sLbw = S.sLbw, *sDlbw = S.sDlbw;
     const int t = threadIdx.x;
     const Phys<R> ph(a.ph);
@@ -663,6 +663,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     // ---- IP gradient face terms: U (lbw coefficient) and V (dlbw coefficient) per face node;
     // horizontal faces: U = hatD . Fc, V = Gc ; vertical faces: U = hatD . Gc, V = Fc
     if (ipgrad) {
+        R u2[4], v2[4];
         if (face_active) {
             const R *Fc = sU + (fs * 4 + ff) * P * 4, *Gc = sV + (fs * 4 + ff) * P * 4;
             const bool horiz = (ff == 0 || ff == 2);
@@ -672,9 +673,14 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
                 R acc = R(0);
 #pragma unroll
                 for (int l = 0; l < P; ++l) acc += sHat[fk * P + l] * src[l * 4 + c];
-                sU2[t * 4 + c] = acc;
-                sV2[t * 4 + c] = horiz ? Gc[fk * 4 + c] : Fc[fk * 4 + c];
+                u2[c] = acc;
+                v2[c] = horiz ? Gc[fk * 4 + c] : Fc[fk * 4 + c];
             }
+        }
+        __syncthreads();   // every face has read its fluxes: U and V take their place
+        if (face_active) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { sU[t * 4 + c] = u2[c]; sV[t * 4 + c] = v2[c]; }
         }
         __syncthreads();
     }
@@ -709,7 +715,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             for (int c = 0; c < 4; ++c) acc[c] = acc[c] - Ff[c] * lw;
             if (ipgrad) {
                 const R dlw = sDlbw[side + oth];
-                const R *Uf = sU2 + ((s * 4 + f) * P + idx) * 4, *Vf = sV2 + ((s * 4 + f) * P + idx) * 4;
+                const R *Uf = sU + ((s * 4 + f) * P + idx) * 4, *Vf = sV + ((s * 4 + f) * P + idx) * 4;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) acc[c] = acc[c] - (Uf[c] * lw + Vf[c] * dlw);
             }
