@@ -109,8 +109,10 @@ hipError_t timed(hg_handle *h, int which, F &&launch) {
 }
 
 // Chooses, for each kernel whose tile is not fixed yet, the faster of the two workgroup
-// tiles by timing derivative-only evaluations of the current state.  Both tiles produce
-// bit-identical results; only QDot and the gradient scratch arrays are written.
+// tiles by timing it on the current state.  The stage kernel is timed as it runs in a step (RK
+// update, next traces) on a scratch copy of Q with cg = 0, G in the QDot buffer, and the next
+// stage's trace buffers as outputs (the first real stage overwrites them).  Both tiles
+// produce bit-identical results.
 int autotune(hg_handle *h) {
     const bool need_g = h->ph.ns && h->tile[1] < 0, need_s = h->tile[0] < 0;
     if (!need_g && !need_s) return 0;
@@ -136,16 +138,30 @@ int autotune(hg_handle *h) {
             err = time_of([&] { return hg::launch_gradient(h->P, k, h->mixed, g, h->stream); }, &t[k]);
         if (err == hipSuccess) h->tile[1] = t[0] <= t[1] ? 0 : 1;
     }
+    double *Qs = nullptr;
+    if (need_s && err == hipSuccess) err = hipMalloc((void **)&Qs, (size_t)h->ndof * sizeof(double));
+    if (need_s && err == hipSuccess)
+        err = hipMemcpyAsync(Qs, h->Q, (size_t)h->ndof * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
     if (need_s && err == hipSuccess) {
         hg::StageArgs a = base_args(h);
+        a.Q = Qs;
+        a.G = h->QDot;
         a.tr_in = h->tr[h->cur];
+        a.tr_out = h->tr[h->cur ^ 1];
+        a.dtr_out = h->dtr[h->ip_fused ? (h->cur ^ 1) : 0];
+        a.am = 0.0;
         a.cq = 1.0;
+        a.cg = 0.0;
         a.first_stage = 1;
-        a.mode = 1;
+        a.mode = 0;
         float t[2] = {0.f, 0.f};
         for (int k = 0; k < 2 && err == hipSuccess; ++k)
             err = time_of([&] { return hg::launch_stage(h->P, k, h->mixed, a, h->stream); }, &t[k]);
         if (err == hipSuccess) h->tile[0] = t[0] <= t[1] ? 0 : 1;
+    }
+    if (Qs) {
+        hipStreamSynchronize(h->stream);
+        hipFree(Qs);
     }
     hipEventDestroy(ev[0]);
     hipEventDestroy(ev[1]);
@@ -602,6 +618,7 @@ int hg_build_geometry(const hg_mesh_desc *mesh, int64_t nnodes, double *X, doubl
 int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     if (!h || !Q) return set_err("hg_upload_state: null argument");
     if (ndof != h->ndof) return set_err("hg_upload_state: ndof mismatch");
+    const double pref_old = h->ph.pref;
     if (h->mixed) {
         // Reference pressure of the FP32 form: the mean nodal pressure of the uploaded state.
         double sum = 0.0;
@@ -613,7 +630,9 @@ int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
         h->ph.pref = std::isfinite(pref) ? pref : 0.0;
         h->ph.Eref = h->ph.pref / h->ph.gm1;
     }
-    drop_graphs(h);   // the captured kernels carry the physics parameters by value
+    // the captured kernels carry the physics parameters (the FP32 form's reference pressure)
+    // by value: recapture only if it changed
+    if (h->ph.pref != pref_old) drop_graphs(h);
     HG_CHECK(hipMemcpyAsync(h->Q, Q, ndof * sizeof(double), hipMemcpyHostToDevice, h->stream));
     hg::StageArgs a = base_args(h);
     h->cur = 0;

@@ -14,20 +14,45 @@ namespace {
 
 __device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
 
-__global__ __launch_bounds__(256) void k_resid_final(const double *part, int nb, double *out) {
-    __shared__ double s[256][4];
+// max|QDot| per equation over the stage kernel's per-workgroup partials (4 doubles each), one
+// workgroup: every lane keeps four partials' loads in flight, then wavefront shuffles and LDS.
+// The maximum does not depend on the order (NaN wins every compare), so the result is exact.
+__global__ __launch_bounds__(1024) void k_resid_final(const double *part, int nb, double *out) {
+    __shared__ double s[16][4];
     const int t = threadIdx.x;
+    const double2 *p2 = (const double2 *)part;
     double m[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = t; b < nb; b += 256)
-        for (int c = 0; c < 4; ++c) m[c] = nan_max(m[c], part[(int64_t)b * 4 + c]);
-    for (int c = 0; c < 4; ++c) s[t][c] = m[c];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w)
-            for (int c = 0; c < 4; ++c) s[t][c] = nan_max(s[t][c], s[t + w][c]);
-        __syncthreads();
+    int b = t;
+    for (; b + 3 * 1024 < nb; b += 4 * 1024) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[2 * u] = p2[(int64_t)(b + u * 1024) * 2];
+            v[2 * u + 1] = p2[(int64_t)(b + u * 1024) * 2 + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            m[0] = nan_max(m[0], v[2 * u].x); m[1] = nan_max(m[1], v[2 * u].y);
+            m[2] = nan_max(m[2], v[2 * u + 1].x); m[3] = nan_max(m[3], v[2 * u + 1].y);
+        }
     }
-    if (t < 4) out[t] = s[0][t];
+    for (; b < nb; b += 1024) {
+        const double2 a = p2[(int64_t)b * 2], c = p2[(int64_t)b * 2 + 1];
+        m[0] = nan_max(m[0], a.x); m[1] = nan_max(m[1], a.y); m[2] = nan_max(m[2], c.x); m[3] = nan_max(m[3], c.y);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        for (int off = 32; off > 0; off >>= 1) m[c] = nan_max(m[c], __shfl_xor(m[c], off, 64));
+    if ((t & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s[t >> 6][c] = m[c];
+    }
+    __syncthreads();
+    if (t < 4) {
+        double v = s[0][t];
+        for (int w = 1; w < 16; ++w) v = nan_max(v, s[w][t]);
+        out[t] = v;
+    }
 }
 
 // TimeIntegrator_EstimateTimeStep (DGTimeIntegrator.f90:462-511), second half: the running
@@ -139,7 +164,7 @@ hipError_t launch_gradient(int P, int tile, bool mixed, const StageArgs &a, hipS
 }
 
 hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, part, nb, out);
+    hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(1024), 0, st, part, nb, out);
     return hipGetLastError();
 }
 
