@@ -166,3 +166,32 @@ def test_graph_replay_matches_direct_launches(name):
         out.append((dg.state(), dg.residuals()))
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["ns_br1_lf_tgv", "ns_ip_roe_kh"])
+def test_upload_between_steps_matches_fresh_handle(name, precision):
+    """hg_upload_state keeps the captured step when the FP32 form's reference pressure is
+    unchanged and recaptures it otherwise; interior penalty re-forms its viscous traces.
+    Either way, stepping after an upload gives the bits of a fresh handle."""
+    from horses2d_amd import gpu
+    case = CASES[name]()
+    fresh = gpu.DGSEM(case, precision)
+    X = fresh.coords()
+    Q0 = case.initial_state(X)
+    Q1 = Q0.copy()
+    Q1[3::4] *= 1.01           # another state: another mean pressure
+    dt = 0.5 * fresh.estimate_dt(case.dt_max, case.cfl)
+    reused = gpu.DGSEM(case, precision)
+    reused.upload(Q0)
+    for it in range(3):
+        reused.rk_step(0.0, dt)
+    for Q in (Q0, Q1):
+        reused.upload(Q)
+        fresh = gpu.DGSEM(case, precision)
+        fresh.upload(Q)
+        for it in range(3):
+            reused.rk_step(0.0, dt)
+            fresh.rk_step(0.0, dt)
+        np.testing.assert_array_equal(reused.state(), fresh.state())
+        np.testing.assert_array_equal(reused.residuals(), fresh.residuals())
