@@ -63,7 +63,7 @@ __global__ __launch_bounds__(256) void k_batch_stage(const StageArgs *__restrict
                                                      const int *__restrict__ sched, const double *__restrict__ dt,
                                                      BatchArgs b) {
     __shared__ BatchStageSmem<TILE, NS, IP, R> S;
-    const int k = blockIdx.x;
+    const int k = xcd_tile(blockIdx.x, gridDim.x);   // position in the schedule
     if (k < b.nitems) {
         const int4 it = uniform_item(items, sched, k);
         const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void k_batch_gradient(const StageArgs *__restr
                                                         const int4 *__restrict__ items,
                                                         const int *__restrict__ sched, BatchArgs b) {
     __shared__ BatchGradSmem<TILE, BR1, R> S;
-    const int k = blockIdx.x;
+    const int k = xcd_tile(blockIdx.x, gridDim.x);   // position in the schedule
     if (k < b.nitems) {
         const int4 it = uniform_item(items, sched, k);
         const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];

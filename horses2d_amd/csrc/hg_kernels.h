@@ -36,6 +36,15 @@ __device__ inline T ld(const void *p, int64_t i) { return ((const T *)p)[i]; }
 
 __device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
 
+// Tile handled by workgroup b of nb.  The dispatcher hands consecutive workgroups to the eight
+// XCDs in turn; this gives each XCD a contiguous range of tiles (a band of the mesh), so that
+// the traces a tile reads from its neighbours were mostly written through the same XCD's L2.
+// A bijection of [0, nb): every tile is processed once, wherever it runs.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int x = b & 7, q = b >> 3, per = nb >> 3, rem = nb & 7;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
 template <int P, int EPB_>
 struct Tile {
     static constexpr int PP = P * P;
@@ -407,7 +416,7 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
 template <int P, int EPB, bool BR1, typename R, typename TR>
 __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
     __shared__ GradSmem<P, EPB, BR1, R> S;
-    gradient_body<P, EPB, BR1, R, TR>(a, blockIdx.x * EPB, S);
+    gradient_body<P, EPB, BR1, R, TR>(a, xcd_tile(blockIdx.x, gridDim.x) * EPB, S);
 }
 
 // ------------------------------------------------------------------ the stage kernel
@@ -829,7 +838,8 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
         if (a.dt_cq) cq *= d;
         if (a.dt_cg) cg *= d;
     }
-    stage_body<P, EPB, NS, IP, RS, R, TR>(a, blockIdx.x * EPB, blockIdx.x, cq, cg, S);
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    stage_body<P, EPB, NS, IP, RS, R, TR>(a, b * EPB, b, cq, cg, S);
 }
 
 // ------------------------------------------------------------------ time step and sums
