@@ -53,8 +53,13 @@ __device__ inline double slot_energy(const PhysParams &p, double e) {
     else return e;
 }
 
+// Square root and reciprocal: IEEE in FP64 (the form that matches the oracle); the hardware's
+// 1-ulp instructions in FP32, whose fluxes carry FP32 rounding anyway (the correctly rounded
+// FP32 sequences cost about five and ten instructions).
 __device__ inline double dsqrt(double x) { return sqrt(x); }
-__device__ inline float dsqrt(float x) { return sqrtf(x); }
+__device__ inline float dsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline double rcp(double x) { return 1.0 / x; }
+__device__ inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline double dpow(double a, double b) { return pow(a, b); }
 __device__ inline float dpow(float a, float b) { return powf(a, b); }
 __device__ inline double dabs(double x) { return fabs(x); }
@@ -64,7 +69,7 @@ __device__ inline float dabs(float x) { return fabsf(x); }
 // InviscidFluxes.f90:104-147
 template <typename R>
 __device__ inline void inviscid_flux(const Phys<R> &ph, const R *u, R *f, R *g) {
-    R invRho = R(1) / u[0];
+    R invRho = rcp(u[0]);
     R vx = u[1] * invRho, vy = u[2] * invRho;
     R p = ph.gm1 * (u[3] - R(0.5) * u[1] * vx - R(0.5) * u[2] * vy);
     const R h = e_plus_p(ph, u[3], p);
@@ -77,15 +82,15 @@ __device__ inline void inviscid_flux(const Phys<R> &ph, const R *u, R *f, R *g) 
 // RiemannSolvers.f90:189-305
 template <typename R>
 __device__ void roe_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
-    R rhoL = qL[0], invRhoL = R(1) / rhoL, sqrtRhoL = dsqrt(rhoL);
+    R rhoL = qL[0], invRhoL = rcp(rhoL), sqrtRhoL = dsqrt(rhoL);
     R uL = (qL[1] * nx + qL[2] * ny) * invRhoL, vL = (-qL[1] * ny + qL[2] * nx) * invRhoL;
     R pL = ph.gm1 * (qL[3] - R(0.5) * rhoL * (uL * uL + vL * vL));
     R HL = ph.cp * p_full(ph, pL) * invRhoL + R(0.5) * (uL * uL + vL * vL);
-    R rhoR = qR[0], invRhoR = R(1) / rhoR, sqrtRhoR = dsqrt(rhoR);
+    R rhoR = qR[0], invRhoR = rcp(rhoR), sqrtRhoR = dsqrt(rhoR);
     R uR = (qR[1] * nx + qR[2] * ny) * invRhoR, vR = (-qR[1] * ny + qR[2] * nx) * invRhoR;
     R pR = ph.gm1 * (qR[3] - R(0.5) * rhoR * (uR * uR + vR * vR));
     R HR = ph.cp * p_full(ph, pR) * invRhoR + R(0.5) * (uR * uR + vR * vR);
-    R invrho = R(1) / (sqrtRhoL + sqrtRhoR);
+    R invrho = rcp(sqrtRhoL + sqrtRhoR);
     R u = (sqrtRhoL * uL + sqrtRhoR * uR) * invrho;
     R v = (sqrtRhoL * vL + sqrtRhoR * vR) * invrho;
     R H = (sqrtRhoL * HL + sqrtRhoR * HR) * invrho;
@@ -119,17 +124,17 @@ __device__ void roe_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny
 // RiemannSolvers.f90:307-399
 template <typename R>
 __device__ void hll_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
-    R rhoL = qL[0], invRhoL = R(1) / rhoL;
+    R rhoL = qL[0], invRhoL = rcp(rhoL);
     R uL = (qL[1] * nx + qL[2] * ny) * invRhoL, vL = (-qL[1] * ny + qL[2] * nx) * invRhoL;
     R pL = ph.gm1 * (qL[3] - R(0.5) * rhoL * (uL * uL + vL * vL));
     R HL = ph.cp * p_full(ph, pL) * invRhoL + R(0.5) * (uL * uL + vL * vL);
     R aL = dsqrt(ph.gamma * p_full(ph, pL) * invRhoL), rhoeL = qL[3];
-    R rhoR = qR[0], invRhoR = R(1) / rhoR;
+    R rhoR = qR[0], invRhoR = rcp(rhoR);
     R uR = (qR[1] * nx + qR[2] * ny) * invRhoR, vR = (-qR[1] * ny + qR[2] * nx) * invRhoR;
     R pR = ph.gm1 * (qR[3] - R(0.5) * rhoR * (uR * uR + vR * vR));
     R HR = ph.cp * p_full(ph, pR) * invRhoR + R(0.5) * (uR * uR + vR * vR);
     R aR = dsqrt(ph.gamma * p_full(ph, pR) * invRhoR), rhoeR = qR[3];
-    R sL = dsqrt(rhoL), sR = dsqrt(rhoR), invrho = R(1) / (sL + sR);
+    R sL = dsqrt(rhoL), sR = dsqrt(rhoR), invrho = rcp(sL + sR);
     R u = (sL * uL + sR * uR) * invrho, v = (sL * vL + sR * vR) * invrho, H = (sL * HL + sR * HR) * invrho;
     R a = dsqrt(ph.gm1 * (H - R(0.5) * (u * u + v * v)));
     R SL = fmin(uL - aL, u - a), SR = fmax(uR + aR, u + a);
@@ -139,7 +144,7 @@ __device__ void hll_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny
     } else if (SR <= R(0)) {
         Fs[0] = rhoR * uR; Fs[1] = Fs[0] * uR + pR; Fs[2] = Fs[0] * vR; Fs[3] = HR * Fs[0];
     } else if ((SL < R(0)) || (SR > R(0))) {
-        R idS = R(1) / (SR - SL);
+        R idS = rcp(SR - SL);
         Fs[0] = (SR * rhoL * uL - SL * rhoR * uR + SL * SR * (rhoR - rhoL)) * idS;
         Fs[1] = (SR * (rhoL * uL * uL + pL) - SL * (rhoR * uR * uR + pR) + SL * SR * (rhoR * uR - rhoL * uL)) * idS;
         Fs[2] = (SR * rhoL * uL * vL - SL * rhoR * uR * vR + SR * SL * (rhoR * vR - rhoL * vL)) * idS;
@@ -252,7 +257,7 @@ __device__ void exact_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R 
 template <typename R>
 __device__ void lf_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
     R fL[4], gL[4], fR[4], gR[4];
-    R iL = R(1) / qL[0], iR = R(1) / qR[0];
+    R iL = rcp(qL[0]), iR = rcp(qR[0]);
     R uL = qL[1] * iL, vL = qL[2] * iL, uR = qR[1] * iR, vR = qR[2] * iR;
     R pL = ph.gm1 * (qL[3] - R(0.5) * qL[1] * uL - R(0.5) * qL[2] * vL);
     R pR = ph.gm1 * (qR[3] - R(0.5) * qR[1] * uR - R(0.5) * qR[2] * vR);
@@ -294,7 +299,7 @@ __device__ inline void riemann(int id, const Phys<R> &ph, const R *qL, const R *
 // ViscousFluxes.f90:39-80; dq[d*4 + eq]
 template <typename R>
 __device__ inline void viscous_flux(const Phys<R> &ph, const R *q, const R *dq, R *f, R *g) {
-    R invRho = R(1) / q[0], u = q[1] * invRho, v = q[2] * invRho, uDR = u * invRho, vDR = v * invRho;
+    R invRho = rcp(q[0]), u = q[1] * invRho, v = q[2] * invRho, uDR = u * invRho, vDR = v * invRho;
     const R *dx = dq, *dy = dq + 4;
     R divV = -uDR * dx[0] + invRho * dx[1] - vDR * dy[0] + invRho * dy[2];
     const R E = e_full(ph, q[3]);
@@ -313,7 +318,7 @@ __device__ inline void viscous_flux(const Phys<R> &ph, const R *q, const R *dq, 
 // ViscousFluxes.f90:279-318 with qB = q (the BR1/IP adiabatic wall uses the interior state)
 template <typename R>
 __device__ inline void adiabatic_flux(const Phys<R> &ph, const R *q, const R *dq, R *f, R *g) {
-    R invRho = R(1) / q[0], u = q[1] * invRho, v = q[2] * invRho, uDR = u * invRho, vDR = v * invRho;
+    R invRho = rcp(q[0]), u = q[1] * invRho, v = q[2] * invRho, uDR = u * invRho, vDR = v * invRho;
     const R *dx = dq, *dy = dq + 4;
     R divV = -uDR * dx[0] + invRho * dx[1] - vDR * dy[0] + invRho * dy[2];
     f[0] = R(0);
