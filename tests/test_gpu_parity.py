@@ -195,3 +195,35 @@ def test_upload_between_steps_matches_fresh_handle(name, precision):
             fresh.rk_step(0.0, dt)
         np.testing.assert_array_equal(reused.state(), fresh.state())
         np.testing.assert_array_equal(reused.residuals(), fresh.residuals())
+
+
+@pytest.mark.parametrize("scheme", ["euler", "rk5"])
+@pytest.mark.parametrize("name", ["ns_br1_lf_tgv", "ns_ip_roe_kh"])
+def test_navier_stokes_other_schemes_match_oracle(name, scheme):
+    """Odd stage counts other than three: the trace buffers (and the interior-penalty viscous
+    traces formed by each stage's epilogue) alternate once per stage."""
+    case = CASES[name]()
+    case.scheme = scheme
+    dg, orc = pair(case)
+    run_both(dg, orc, case, 4)
+    assert rel(dg.state(), orc.state()) <= TOL
+
+
+@pytest.mark.parametrize("name", ["ns_br1_lf_tgv", "ns_ip_roe_kh", "ns_sipg_annulus"])
+def test_time_derivative_after_steps(name):
+    """QDot of a stepped state from the traces the last stage formed (for interior penalty
+    also its viscous traces, from the stage epilogue instead of the gradient kernel) equals,
+    bit for bit, QDot after uploading the same state (fresh prolongation and gradient)."""
+    from horses2d_amd import gpu
+    case = CASES[name]()
+    dg, orc = pair(case)
+    run_both(dg, orc, case, 3)
+    Q = dg.state()
+    qd_g = dg.time_derivative(0.0)
+    fresh = gpu.DGSEM(case, "fp64")
+    fresh.upload(Q)
+    np.testing.assert_array_equal(qd_g, fresh.time_derivative(0.0))
+    # (parity with the oracle: QDot at the initial state and the stepped states themselves are
+    # compared above; QDot of a stepped state is not, because these low-Mach cases' QDot is
+    # 1e-2..1e-3 against fluxes of 1e2, so rounding differences of the states and of fused
+    # versus unfused arithmetic already reach 2e-12 of it)
