@@ -514,6 +514,23 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     h->cons_nb = (K + std::max(1, 256 / PP) - 1) / std::max(1, 256 / PP);
     HG_CHECK(h->alloc(&h->cons_partial, 4 * (size_t)h->cons_nb));
     HG_CHECK(h->alloc(&h->cons_out, 4));
+    {
+        // monitor surfaces: the boundary faces of every zone in ascending edge id
+        std::vector<std::vector<int>> per(m.zones.size());
+        for (size_t k = 0; k < m.ed.size(); ++k) {
+            const hg::Edge &ed = m.ed[k];
+            if (ed.bdry && ed.marker >= 1 && ed.marker <= (int)m.zones.size())
+                per[ed.marker - 1].push_back(ed.quads[0] * 4 + (ed.loc[0] - 1));
+        }
+        std::vector<int> flat;
+        h->bslot_off.assign(1, 0);
+        for (const auto &v : per) {
+            flat.insert(flat.end(), v.begin(), v.end());
+            h->bslot_off.push_back((int)flat.size());
+        }
+        HG_CHECK(upload(h.get(), &h->bslots, flat));
+        HG_CHECK(h->alloc(&h->monitor_out, 12));
+    }
     HG_CHECK(h->alloc(&h->dt_dev, 1));
     HG_CHECK(h->alloc(&h->resid, 4));
     HG_CHECK(h->alloc(&h->resid_part, 4 * (size_t)K));   // at least one element per workgroup
@@ -729,6 +746,48 @@ int hg_conservation_sums(hg_handle *h, double out[4]) {
     if (!h || !out) return set_err("hg_conservation_sums: null argument");
     HG_CHECK(hg::launch_cons_sums(h->P, h->Q, h->jac64, h->w, h->K, h->cons_partial, h->cons_nb, h->cons_out, h->stream));
     HG_CHECK(hipMemcpyAsync(out, h->cons_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_volume_integrals(hg_handle *h, double out[2]) {
+    if (!h || !out) return set_err("hg_volume_integrals: null argument");
+    HG_CHECK(hg::launch_volume_integrals(h->P, h->Q, h->jac64, h->w, h->K, h->ph.gamma, h->ph.gm1, h->cons_partial,
+                                         h->cons_nb, h->monitor_out, h->stream));
+    double v[4];
+    HG_CHECK(hipMemcpyAsync(v, h->monitor_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    out[0] = v[0];
+    out[1] = std::sqrt(v[1]);
+    return 0;
+}
+
+int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
+    if (!h || !out) return set_err("hg_surface_integrals: null argument");
+    const int nz = (int)h->bslot_off.size() - 1;
+    if (marker < 1 || marker > nz) return set_err("hg_surface_integrals: marker out of range");
+    const int first = h->bslot_off[marker - 1], n = h->bslot_off[marker] - first;
+    const void *vtr = nullptr;
+    if (h->ph.ns) {
+        // the viscous face-flux traces of the current state (BR1: those of the last gradient
+        // launch belong to the last stage's input, so the gradient kernel runs once more)
+        if (autotune(h)) return 1;
+        if (h->ip_fused) {
+            if (hg::ensure_vtr(h)) return 1;
+        } else {
+            hg::StageArgs g = base_args(h);
+            g.tr_in = h->tr[h->cur];
+            HG_CHECK(hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream));
+        }
+        vtr = hg::vtr_current(h);
+    }
+    if (n > 0)
+        HG_CHECK(hg::launch_surface_integrals(h->mixed, h->bslots + first, n, h->P, h->tr[h->cur], vtr, h->face,
+                                              h->egeo, h->w, h->ph.gm1, h->mixed ? h->ph.Eref : 0.0,
+                                              h->monitor_out, h->stream));
+    else
+        HG_CHECK(hipMemsetAsync(h->monitor_out, 0, 9 * sizeof(double), h->stream));
+    HG_CHECK(hipMemcpyAsync(out, h->monitor_out, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
 }

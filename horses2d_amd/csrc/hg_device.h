@@ -118,6 +118,15 @@ hipError_t launch_dt_elem(int P, const double *Q, int K, double gamma, double gm
 hipError_t launch_dt_scan(int K, const double *umax, const double *amax, const double *dx, double coef, double N1,
                           double dt_max, double *state /* [umax, amax] */, double *dt_out, hipStream_t s);
 // Conservation sums (deterministic two-pass).
+// Volume monitors [One, dSnorm^2, 0, 0] (QuadMesh_Integrals.incf:22-65) into out[4].
+hipError_t launch_volume_integrals(int P, const double *Q, const double *jac, const double *w, int K, double gamma,
+                                   double gm1, double *partial, int nbk, double *out, hipStream_t s);
+// Surface monitors of one zone (QuadMesh_Integrals.incf:67-346) over its boundary face slots
+// (element*4 + face): out[9] = Surface, pressure, density, pressure force (2), mass-flow, flow,
+// viscous force (2); vtr null for Euler.  Eref: the FP32 form's energy-slot shift.
+hipError_t launch_surface_integrals(bool mixed, const int *slots, int nslots, int P, const void *tr, const void *vtr,
+                                    const int4 *face, const double *egeo, const double *w, double gm1, double Eref,
+                                    double *out, hipStream_t s);
 hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const double *w, int K, double *partial,
                             int nblocks, double *out, hipStream_t s);
 
@@ -133,7 +142,7 @@ template <int P> hipError_t prolong_p(bool mixed, const StageArgs &a, hipStream_
 template <int P> hipError_t dt_elem_p(const double *Q, int K, double gamma, double gm1, double *umax, double *amax,
                                       hipStream_t s);
 template <int P> int cons_blocks_p(int K);
-template <int P> hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, int K,
-                                           double *partial, hipStream_t s);
+template <int P> hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, int K, int mode,
+                                           double gamma, double gm1, double *partial, hipStream_t s);
 
 }  // namespace hg

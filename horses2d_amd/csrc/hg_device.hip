@@ -101,6 +101,58 @@ __global__ void k_cons_final(const double *partial, int nb, double *out) {
     }
 }
 
+// Surface monitors of one zone (QuadMesh_Integrals.incf:67-346): one workgroup; each lane sums a
+// strided share of the zone's face nodes from the stored element-side Q traces (the energy slot
+// shifted back by Eref in the FP32 form) and, for Navier-Stokes, the stored viscous face-flux
+// traces, whose momentum components are the stress tensor (mu (f1, f2, g2) = tau_xx, tau_xy,
+// tau_yy, ViscousFluxes.f90:416-461); then a fixed-order reduction.
+template <typename TR>
+__global__ __launch_bounds__(1024) void k_surface_integrals(const int *slots, int nslots, int P, const TR *tr,
+                                                            const TR *vtr, const int4 *face, const double *egeo,
+                                                            const double *w, double gm1, double Eref, double *out) {
+    __shared__ double s[16][9];
+    const int t = threadIdx.x;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int x = t; x < nslots * P; x += 1024) {
+        const int slot = slots[x / P], k = x % P;
+        const int4 fr = face[slot];
+        const int gi = (fr.x & FF_CURVED) ? ((fr.x & FF_SELFREV) ? P - 1 - k : k) : 0;
+        const int wi = (fr.x & FF_SELFREV) ? P - 1 - k : k;
+        const double *geo = egeo + 3 * ((int64_t)fr.z + gi);
+        const double nx = geo[0], ny = geo[1], dS = geo[2], wq = w[wi];
+        const TR *q = tr + ((int64_t)slot * P + k) * 4;
+        const double r = q[0], ru = q[1], rv = q[2], E = (double)q[3] + Eref;
+        const double p = gm1 * (E - 0.5 * (ru * ru + rv * rv) / r);
+        const double un = ru * nx + rv * ny;
+        acc[0] += wq * dS;
+        acc[1] += p * dS * wq;
+        acc[2] += r * dS * wq;
+        acc[3] += p * wq * nx * dS;
+        acc[4] += p * wq * ny * dS;
+        acc[5] += wq * un * dS;
+        acc[6] += wq * un * dS / r;
+        if (vtr) {
+            const TR *v = vtr + ((int64_t)slot * P + k) * VT;
+            const double txx = v[0], txy = v[1], tyy = v[3];
+            acc[7] += wq * (txx * nx + txy * ny) * dS;
+            acc[8] += wq * (txy * nx + tyy * ny) * dS;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+    if ((t & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) s[t >> 6][c] = acc[c];
+    }
+    __syncthreads();
+    if (t < 9) {
+        double v = s[0][t];
+        for (int wv = 1; wv < 16; ++wv) v += s[wv][t];
+        out[t] = v;
+    }
+}
+
 // Residual of each case of a batch: max over its tiles' partials (NaN propagates).
 __global__ __launch_bounds__(256) void k_batch_resid(const double *part, const int2 *ranges, double *const *out) {
     __shared__ double s[256][4];
@@ -235,9 +287,10 @@ hipError_t launch_dt_scan(int K, const double *umax, const double *amax, const d
     return hipGetLastError();
 }
 
-static hipError_t cons_partial(int P, const double *Q, const double *jac, const double *w, int K, double *partial,
+static hipError_t cons_partial(int P, const double *Q, const double *jac, const double *w, int K, int mode,
+                               double gamma, double gm1, double *partial,
                                int nbk, hipStream_t st) {
-#define HG_CALL(PV) (cons_blocks_p<PV>(K) > nbk ? hipErrorInvalidValue : cons_partial_p<PV>(Q, jac, w, K, partial, st))
+#define HG_CALL(PV) (cons_blocks_p<PV>(K) > nbk ? hipErrorInvalidValue : cons_partial_p<PV>(Q, jac, w, K, mode, gamma, gm1, partial, st))
     HG_DISPATCH_P(P, HG_CALL)
 #undef HG_CALL
 }
@@ -262,9 +315,29 @@ static int cons_blocks(int P, int K) {
 
 hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const double *w, int K, double *partial,
                             int nbk, double *out, hipStream_t st) {
-    hipError_t e = cons_partial(P, Q, jac, w, K, partial, nbk, st);
+    hipError_t e = cons_partial(P, Q, jac, w, K, 0, 0.0, 0.0, partial, nbk, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, st, partial, cons_blocks(P, K), out);
+    return hipGetLastError();
+}
+
+hipError_t launch_volume_integrals(int P, const double *Q, const double *jac, const double *w, int K, double gamma,
+                                   double gm1, double *partial, int nbk, double *out, hipStream_t st) {
+    hipError_t e = cons_partial(P, Q, jac, w, K, 1, gamma, gm1, partial, nbk, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(64), 0, st, partial, cons_blocks(P, K), out);
+    return hipGetLastError();
+}
+
+hipError_t launch_surface_integrals(bool mixed, const int *slots, int nslots, int P, const void *tr, const void *vtr,
+                                    const int4 *face, const double *egeo, const double *w, double gm1, double Eref,
+                                    double *out, hipStream_t st) {
+    if (mixed)
+        hipLaunchKernelGGL((k_surface_integrals<float>), dim3(1), dim3(1024), 0, st, slots, nslots, P,
+                           (const float *)tr, (const float *)vtr, face, egeo, w, gm1, Eref, out);
+    else
+        hipLaunchKernelGGL((k_surface_integrals<double>), dim3(1), dim3(1024), 0, st, slots, nslots, P,
+                           (const double *)tr, (const double *)vtr, face, egeo, w, gm1, Eref, out);
     return hipGetLastError();
 }
 

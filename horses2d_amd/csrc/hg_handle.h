@@ -48,6 +48,11 @@ struct hg_handle {
     double *dx = nullptr, *umax = nullptr, *amax = nullptr, *dtstate = nullptr, *dtout = nullptr;
     double *cons_partial = nullptr, *cons_out = nullptr;
     int cons_nb = 0;
+    // boundary face slots (element*4 + face) of every zone, zones in marker order, each zone's
+    // edges in ascending edge id (the reference's zone edge lists); monitor output
+    int *bslots = nullptr;
+    std::vector<int> bslot_off;   // zone z (marker z+1): [bslot_off[z], bslot_off[z+1])
+    double *monitor_out = nullptr;
     double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
     int cur = 0;
     int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet

@@ -865,9 +865,12 @@ __global__ __launch_bounds__(256) void k_dt_elem(const double *Q, int K, double 
     }
 }
 
+// mode 0: sum_e sum_ij w_i w_j J Q (four conserved variables); mode 1: the volume monitors
+// sum w_i w_j J ("One") and sum w_i w_j J (p / rho^gamma - 1)^2 ("dSnorm" squared),
+// QuadMesh_Integrals.incf:22-65.
 template <int P>
 __global__ __launch_bounds__(256) void k_cons_partial(const double *Q, const double *jac, const double *w, int K,
-                                                      double *partial) {
+                                                      int mode, double gamma, double gm1, double *partial) {
     constexpr int PP = P * P, EPB = 256 / PP > 0 ? 256 / PP : 1;
     __shared__ double sv[256][4];
     const int t = threadIdx.x, s = t / PP, n = t % PP, e = blockIdx.x * EPB + s;
@@ -876,7 +879,15 @@ __global__ __launch_bounds__(256) void k_cons_partial(const double *Q, const dou
         const int i = n % P, j = n / P;
         const int64_t g = (int64_t)e * PP + n;
         const double c = w[i] * w[j] * jac[g];
-        for (int k = 0; k < 4; ++k) v[k] = c * Q[g * 4 + k];
+        if (mode == 0) {
+            for (int k = 0; k < 4; ++k) v[k] = c * Q[g * 4 + k];
+        } else {
+            const double *q = Q + g * 4;
+            const double p = gm1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+            const double d = p / pow(q[0], gamma) - 1.0;
+            v[0] = c;
+            v[1] = c * (d * d);
+        }
     }
     for (int k = 0; k < 4; ++k) sv[t][k] = v[k];
     __syncthreads();
@@ -981,9 +992,10 @@ int cons_blocks_p(int K) {
 }
 
 template <int P>
-hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, int K, double *partial,
-                          hipStream_t st) {
-    hipLaunchKernelGGL((k_cons_partial<P>), dim3(cons_blocks_p<P>(K)), dim3(256), 0, st, Q, jac, w, K, partial);
+hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, int K, int mode, double gamma,
+                          double gm1, double *partial, hipStream_t st) {
+    hipLaunchKernelGGL((k_cons_partial<P>), dim3(cons_blocks_p<P>(K)), dim3(256), 0, st, Q, jac, w, K, mode, gamma,
+                       gm1, partial);
     return hipGetLastError();
 }
 
@@ -998,7 +1010,7 @@ hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, i
     template hipError_t prolong_p<PV>(bool, const StageArgs &, hipStream_t);                                  \
     template hipError_t dt_elem_p<PV>(const double *, int, double, double, double *, double *, hipStream_t);  \
     template int cons_blocks_p<PV>(int);                                                                      \
-    template hipError_t cons_partial_p<PV>(const double *, const double *, const double *, int, double *,     \
-                                           hipStream_t);
+    template hipError_t cons_partial_p<PV>(const double *, const double *, const double *, int, int, double,  \
+                                           double, double *, hipStream_t);
 
 }  // namespace hg

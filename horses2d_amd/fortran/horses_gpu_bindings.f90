@@ -58,7 +58,8 @@ module horses_gpu_bindings
 
    public :: hg_create, hg_destroy, hg_ndof, hg_get_node_coords, hg_upload_state, hg_download_state
    public :: hg_download_qdot, hg_time_derivative, hg_rk_step, hg_residuals, hg_estimate_dt
-   public :: hg_reset_dt_state, hg_conservation_sums, hg_synchronize, hg_last_error
+   public :: hg_reset_dt_state, hg_conservation_sums, hg_volume_integrals, hg_surface_integrals
+   public :: hg_synchronize, hg_last_error
    public :: hgf_check
 
    interface
@@ -143,6 +144,22 @@ module horses_gpu_bindings
          type(c_ptr), value          :: handle
          real(c_double), intent(out) :: sums(4)
       end function hg_conservation_sums
+
+      ! QuadMesh_Integrals.incf:22-65: [One, dSnorm]
+      integer(c_int) function hg_volume_integrals(handle, vals) bind(C, name="hg_volume_integrals")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value          :: handle
+         real(c_double), intent(out) :: vals(2)
+      end function hg_volume_integrals
+
+      ! QuadMesh_Integrals.incf:67-346 over zone `marker`: Surface, pressure, density,
+      ! pressure force (2), mass-flow, flow, viscous force (2)
+      integer(c_int) function hg_surface_integrals(handle, marker, vals) bind(C, name="hg_surface_integrals")
+         import :: c_int, c_ptr, c_double, c_int32_t
+         type(c_ptr), value           :: handle
+         integer(c_int32_t), value    :: marker
+         real(c_double), intent(out)  :: vals(9)
+      end function hg_surface_integrals
 
       integer(c_int) function hg_synchronize(handle) bind(C, name="hg_synchronize")
          import :: c_int, c_ptr

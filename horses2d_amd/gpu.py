@@ -57,7 +57,8 @@ EXPORTS = [
     "hg_create", "hg_destroy", "hg_ndof", "hg_get_node_coords", "hg_get_connectivity", "hg_num_edges",
     "hg_build_connectivity", "hg_build_geometry",
     "hg_upload_state", "hg_download_state", "hg_download_qdot", "hg_time_derivative", "hg_rk_step",
-    "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_synchronize",
+    "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_volume_integrals",
+    "hg_surface_integrals", "hg_synchronize",
     "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_set_graph", "hg_batch_create", "hg_batch_rk_step",
     "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_destroy",
 ]
@@ -92,6 +93,8 @@ def load_library(path: str = LIB_PATH):
         "hg_estimate_dt": ([vp, dbl, dbl, vp], ctypes.c_int),
         "hg_reset_dt_state": ([vp], ctypes.c_int),
         "hg_conservation_sums": ([vp, vp], ctypes.c_int),
+        "hg_volume_integrals": ([vp, vp], ctypes.c_int),
+        "hg_surface_integrals": ([vp, i32, vp], ctypes.c_int),
         "hg_synchronize": ([vp], ctypes.c_int),
         "hg_last_error": ([vp, i32], ctypes.c_int),
         "hg_profile_enable": ([vp, i32], ctypes.c_int),
@@ -273,6 +276,19 @@ class DGSEM:
     def conservation_sums(self) -> np.ndarray:
         r = np.zeros(4)
         _check(self._L.hg_conservation_sums(self._h, _ptr(r)), "hg_conservation_sums")
+        return r
+
+    def volume_integrals(self) -> np.ndarray:
+        """[One (area), dSnorm] of the current state (QuadMesh_Integrals.incf:22-65)."""
+        r = np.zeros(2)
+        _check(self._L.hg_volume_integrals(self._h, _ptr(r)), "hg_volume_integrals")
+        return r
+
+    def surface_integrals(self, marker: int) -> np.ndarray:
+        """[Surface, pressure, density, pressure force x, y, mass-flow, flow, viscous force x, y]
+        over the edges of zone ``marker`` (1-based; QuadMesh_Integrals.incf:67-346)."""
+        r = np.zeros(9)
+        _check(self._L.hg_surface_integrals(self._h, int(marker), _ptr(r)), "hg_surface_integrals")
         return r
 
     def synchronize(self):

@@ -171,6 +171,20 @@ int hg_reset_dt_state(hg_handle *h);
  * a monitor for the mixed-precision form). */
 int hg_conservation_sums(hg_handle *h, double out[4]);
 
+/* Volume monitors of the current state, QuadMesh_Integrals.incf:22-65
+ * (Compute_volumeIntegral): out[0] "One" = sum_e w^T J w (the domain area), out[1]
+ * "dSnorm" = sqrt(sum_e w^T ((p / rho^gamma - 1)^2 J) w).  FP64 in both forms. */
+int hg_volume_integrals(hg_handle *h, double out[2]);
+
+/* Surface monitors of the current state over the edges of boundary zone `marker`
+ * (1-based, the HiOMesh marker), QuadMesh_Integrals.incf:67-346, from the element-side traces
+ * (storage(1)): out[0] "Surface" sum w dS, out[1] "pressure" sum w p dS, out[2] "density"
+ * sum w rho dS, out[3..4] "pressure" force sum w p n dS, out[5] "mass-flow" sum w (rho u.n) dS,
+ * out[6] "flow" sum w (rho u.n)/rho dS, out[7..8] "viscous" force sum w (tau.n) dS (Navier-Stokes
+ * only, tau as ComputeViscousTensor, ViscousFluxes.f90:416-461; zero for Euler).  Replaces the
+ * Monitors.f90 surface-monitor calls into Compute_*SurfaceIntegral. */
+int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]);
+
 /* Blocks until the handle's stream is idle. */
 int hg_synchronize(hg_handle *h);
 

@@ -1691,6 +1691,105 @@ int or_conservation_sums(void *h, double *out) {
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Monitor integrals of the current state (QuadMesh_Integrals.incf:22-346)   */
+/* ------------------------------------------------------------------------ */
+/* VariableConversion.f90:48-55 */
+static double pressure0(const Phys *ph, const double *q) {
+    return ph->gm1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+}
+
+/* Compute_volumeIntegral (:22-65): out[0] "One" = sum_e w^T jac w; out[1] "dSnorm" =
+ * sqrt(sum_e w^T ((p / rho^gamma - 1)^2 jac) w) (BilinearForm_F, MatrixOperations.f90:211-234). */
+int or_volume_integrals(void *h, double *out) {
+    Mesh *m = h;
+    const Phys *ph = &m->ph;
+    double one = 0.0, sn = 0.0;
+    for (int eid = 0; eid < m->K; ++eid) {
+        Elem *e = &m->el[eid];
+        const Spec *s = spec_get(e->N);
+        const int P = e->P;
+        double b1 = 0.0, b2 = 0.0;
+        for (int j = 0; j < P; ++j) {
+            double r1 = 0.0, r2 = 0.0;   /* (A w)(i) summed over i with w(i): X^T (A Y) */
+            for (int i = 0; i < P; ++i) {
+                const double jc = e->jac[i + P * j];
+                const double *q = m->Q + QIDX(e, 0, i, j);
+                const double v = pressure0(ph, q) / pow(q[0], ph->gamma) - 1.0;
+                r1 = r1 + s->w[i] * jc;
+                r2 = r2 + s->w[i] * (v * v * jc);
+            }
+            b1 = b1 + r1 * s->w[j];
+            b2 = b2 + r2 * s->w[j];
+        }
+        one = one + b1;
+        sn = sn + b2;
+    }
+    out[0] = one;
+    out[1] = sqrt(sn);
+    return 0;
+}
+
+/* Surface integrals over the edges of zone `marker` (1-based), in the zone's edge order, with
+ * the element-side traces storage(1) (interpolated from the current state; for Navier-Stokes
+ * also the gradient and its traces, as the residual evaluation forms them):
+ *   out[0] "Surface"  sum w dS                         (Compute_ScalarScalarSurfaceIntegral :83-99)
+ *   out[1] "pressure" sum w p dS                       (:101-126)
+ *   out[2] "density"  sum w rho dS                     (:128-152; the reference indexes
+ *                     storage(1)%Q(0:N, IRHO) of a (1:NCONS, 0:N) array, read here as rho)
+ *   out[3..4] "pressure" sum w p n dS                  (Compute_ScalarVectorSurfaceIntegral :174-204)
+ *   out[5] "mass-flow" sum w (rho u . n) dS            (Compute_VectorVectorSurfaceIntegral :226-252)
+ *   out[6] "flow"      sum w (rho u . n) dS / rho      (:254-281)
+ *   out[7..8] "viscous" sum w (tau . n) dS              (Compute_TensorVectorSurfaceIntegral :305-342,
+ *                     tau of ComputeViscousTensor, ViscousFluxes.f90:416-461; zero for Euler) */
+int or_surface_integrals(void *h, int marker, double *out) {
+    Mesh *m = h;
+    const Phys *ph = &m->ph;
+    if (marker < 1 || marker > m->nmarkers) return fail("or_surface_integrals: bad marker");
+    prolong(m, 0);
+    if (m->ns) {
+        update_bcs(m);
+        if (m->viscous == VISC_BR1) gradient_br1(m);
+        else gradient_ip(m);
+        prolong(m, 1);
+    }
+    for (int k = 0; k < 9; ++k) out[k] = 0.0;
+    const Zone *Z = &m->zones[marker - 1];
+    for (int a = 0; a < Z->nedges; ++a) {
+        const Edge *ed = &m->ed[Z->edges[a]];
+        const Spec *s = spec_get(ed->N);
+        const int P = ed->N + 1;
+        for (int j = 0; j < P; ++j) {
+            const double *q = ed->tq[0] + 4 * j;
+            const double nx = ed->curved ? ed->n[2 * j] : ed->n[0];
+            const double ny = ed->curved ? ed->n[2 * j + 1] : ed->n[1];
+            const double dS = ed->curved ? ed->dS[j] : ed->dS[0];
+            const double w = s->w[j], p = pressure0(ph, q);
+            out[0] = out[0] + w * dS;
+            out[1] = out[1] + p * dS * w;
+            out[2] = out[2] + q[0] * dS * w;
+            out[3] = out[3] + p * w * nx * dS;
+            out[4] = out[4] + p * w * ny * dS;
+            out[5] = out[5] + w * (q[1] * nx + q[2] * ny) * dS;
+            out[6] = out[6] + w * (q[1] * nx + q[2] * ny) * dS / q[0];
+            if (m->ns) {
+                const double *dq = ed->tdq[0];
+                const double dx0 = dq[0 + 4 * j], dx1 = dq[1 + 4 * j], dx2 = dq[2 + 4 * j];
+                const double dy0 = dq[0 + 4 * j + 4 * P], dy1 = dq[1 + 4 * j + 4 * P], dy2 = dq[2 + 4 * j + 4 * P];
+                const double mu = ph->mu, invRho = 1.0 / q[0], u = q[1] * invRho, v = q[2] * invRho;
+                const double uDR = u * invRho, vDR = v * invRho;
+                const double divV = -uDR * dx0 + invRho * dx1 - vDR * dy0 + invRho * dy2;
+                const double txx = 2.0 * mu * (-uDR * dx0 + invRho * dx1) + ph->lambda * mu * divV;
+                const double tyy = 2.0 * mu * (-vDR * dy0 + invRho * dy2) + ph->lambda * mu * divV;
+                const double txy = -mu * vDR * dx0 - mu * uDR * dy0 + mu * invRho * dy1 + mu * invRho * dx2;
+                out[7] = out[7] + w * (txx * nx + txy * ny) * dS;
+                out[8] = out[8] + w * (txy * nx + tyy * ny) * dS;
+            }
+        }
+    }
+    return 0;
+}
+
 /* Point-wise flux exports for tests */
 int or_inviscid_flux(double Mach, int n, const double *q, double *f, double *g) {
     Phys ph; phys_init(&ph, Mach, 0.0, 0.72);

@@ -75,8 +75,9 @@ def lib():
         _lib.or_num_edges.argtypes = [ctypes.c_void_p]
         _lib.or_last_error.restype = ctypes.c_char_p
         for name in ("or_set_state", "or_get_state", "or_get_qdot", "or_get_grad", "or_get_coords",
-                     "or_residuals", "or_conservation_sums"):
+                     "or_residuals", "or_conservation_sums", "or_volume_integrals"):
             getattr(_lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _lib.or_surface_integrals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         _lib.or_get_metrics.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_get_connectivity.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
         _lib.or_get_edge_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
@@ -232,6 +233,18 @@ class Oracle:
 
     def conservation_sums(self):
         r = np.zeros(4); lib().or_conservation_sums(self.h, _ptr(r)); return r
+
+    def volume_integrals(self):
+        """[One, dSnorm] of the current state (QuadMesh_Integrals.incf:22-65)."""
+        r = np.zeros(2); lib().or_volume_integrals(self.h, _ptr(r)); return r
+
+    def surface_integrals(self, marker: int):
+        """[Surface, pressure, density, pressure force x, y, mass-flow, flow, viscous force x, y]
+        over the edges of zone ``marker`` (1-based; QuadMesh_Integrals.incf:67-346)."""
+        r = np.zeros(9)
+        if lib().or_surface_integrals(self.h, int(marker), _ptr(r)) != 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return r
 
     def integrate(self, steps: int, t0: float = 0.0, dt_max: float = 1.0e30, t_end: float = 1.0e30,
                   scheme: str = "rk3"):

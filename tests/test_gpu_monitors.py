@@ -1,0 +1,99 @@
+"""Monitor integrals on the device (QuadMesh_Integrals.incf:22-65 volume, :67-346 surface)
+against the CPU oracle, through the C ABI.
+
+Bar: the FP64 form within 1e-12 of the oracle, relative to the largest entry of the same
+kind (the pressure-, density- and flow-type sums share one scale; the two viscous-force
+entries another, at least mu times the zone's length, since a zone at rest carries only
+rounding noise there); the mixed form within 1e-6 of the FP64 form on the same scales (its
+traces are FP32, its sums FP64).
+"""
+import numpy as np
+import pytest
+
+from horses2d_amd import cases as C
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _annulus(viscous):
+    c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
+    c.viscous = viscous
+    return c
+
+
+CASES = {
+    "euler_vortex": (lambda: C.euler_vortex_case(8, 4), (1, 2, 3, 4)),
+    "ns_br1_annulus": (lambda: _annulus("br1"), (1, 2)),
+    "ns_sipg_annulus": (lambda: _annulus("ip"), (1, 2)),
+    "ns_br1_tgv": (lambda: C.ns_tgv_case(6, 4, Re=1600.0), (1, 3)),
+    "ns_ip_kh": (lambda: C.ns_kh_case(6, 5, Re=1.0e4), (1, 2, 3, 4)),
+}
+
+
+def _close(got, want, tol, what, mu):
+    inv, vis = np.abs(want[:7]).max(), max(np.abs(want[7:]).max(), mu * want[0])
+    np.testing.assert_allclose(got[:7], want[:7], rtol=0, atol=tol * inv, err_msg=what)
+    np.testing.assert_allclose(got[7:], want[7:], rtol=0, atol=tol * max(vis, 1e-300), err_msg=what)
+
+
+@pytest.mark.parametrize("steps", [0, 3])
+@pytest.mark.parametrize("name", list(CASES))
+def test_monitor_integrals_match_oracle(name, steps):
+    from horses2d_amd import gpu
+    from oracle.oracle import Oracle
+    make, markers = CASES[name]
+    case = make()
+    mu = 1.0 / case.Re if case.Re else 0.0
+    dg = gpu.DGSEM(case, "fp64")
+    orc = Oracle(case)
+    Q0 = case.initial_state(dg.coords())
+    dg.upload(Q0)
+    orc.set_state(Q0)
+    dt = dg.estimate_dt(case.dt_max, case.cfl)
+    assert dt == pytest.approx(orc.estimate_dt(case.dt_max, case.cfl), rel=1e-14)
+    for it in range(steps):
+        orc.step(it * dt, dt, case.scheme)
+        dg.rk_step(it * dt, dt)
+    vg, vo = dg.volume_integrals(), orc.volume_integrals()
+    assert vg[0] == pytest.approx(vo[0], rel=TOL)
+    assert vg[1] == pytest.approx(vo[1], rel=1e-10, abs=TOL * vo[0])
+    for mk in markers:
+        _close(dg.surface_integrals(mk), orc.surface_integrals(mk), TOL, f"{name} marker {mk}", mu)
+    if case.navier_stokes:
+        # the monitors leave the step untouched: continuing gives the oracle's state
+        orc.step(steps * dt, dt, case.scheme)
+        dg.rk_step(steps * dt, dt)
+        st_o = orc.state()
+        assert np.max(np.abs(dg.state() - st_o)) <= TOL * np.max(np.abs(st_o))
+
+
+@pytest.mark.parametrize("name", ["ns_br1_annulus", "ns_sipg_annulus", "ns_ip_kh"])
+def test_monitor_integrals_mixed_form(name):
+    from horses2d_amd import gpu
+    make, markers = CASES[name]
+    case = make()
+    mu = 1.0 / case.Re
+    out = []
+    for precision in ("fp64", "mixed"):
+        dg = gpu.DGSEM(case, precision)
+        dg.upload(case.initial_state(dg.coords()))
+        dt = dg.estimate_dt(case.dt_max, case.cfl)
+        for it in range(3):
+            dg.rk_step(it * dt, dt)
+        out.append((dg.volume_integrals(), [dg.surface_integrals(mk) for mk in markers]))
+    (v64, s64), (v32, s32) = out
+    np.testing.assert_allclose(v32, v64, rtol=1e-6, atol=1e-6 * v64[0])
+    for mk, a, b in zip(markers, s32, s64):
+        _close(a, b, 1e-6, f"{name} marker {mk}", mu)
+
+
+def test_monitor_marker_out_of_range_raises():
+    from horses2d_amd import gpu
+    case = C.euler_vortex_case(4, 3)
+    dg = gpu.DGSEM(case, "fp64")
+    dg.upload(case.initial_state(dg.coords()))
+    for bad in (0, 5):
+        with pytest.raises(RuntimeError):
+            dg.surface_integrals(bad)

@@ -392,3 +392,82 @@ def test_viscous_operator_symmetry_and_ip_linearity_in_epsilon():
     ii, Vi = asym("ip", "iipg")
     np.testing.assert_allclose(Vs + Vn, 2 * Vi, rtol=0, atol=1e-10 * np.abs(Vi).max())
     assert ii > 1e-2 and ni > 1e-2 and s > 1e-3, (ii, ni, s)
+
+
+# ------------------------------------------------------------------------------------------
+# Monitors: QuadMesh_Integrals.incf:22-65 (volume), :67-346 (surface)
+# ------------------------------------------------------------------------------------------
+def _cons(rho, mx, my, p):
+    return np.stack([rho, mx, my, p / (GAMMA - 1.0) + 0.5 * (mx * mx + my * my) / rho], axis=-1).reshape(-1)
+
+
+def test_volume_integrals_area_and_entropy_deviation():
+    """One = the domain's area; dSnorm = ||p / rho^gamma - 1||_L2, which is 0 for the isentropic
+    vortex (p = rho^gamma at every node) and delta sqrt(area) for p = (1 + delta) rho^gamma."""
+    case = C.ns_tgv_case(4, 4)
+    o, X = oracle_for(case)
+    ones = np.ones(len(X))
+    o.set_state(_cons(2.0 * ones, 0.0 * ones, 0.0 * ones, 1.1 * 2.0 ** GAMMA * ones))
+    one, dsn = o.volume_integrals()
+    assert one == pytest.approx(4.0 * np.pi ** 2, rel=1e-14)
+    assert dsn == pytest.approx(0.1 * 2.0 * np.pi, rel=1e-13)
+    vortex = C.euler_vortex_case(5, 5)
+    o, _ = oracle_for(vortex)
+    one, dsn = o.volume_integrals()
+    assert one == pytest.approx(100.0, rel=1e-14) and dsn < 1e-13
+    # curved annulus r in [0.5, 20]: the quadratic-arc geometry is within 1e-4 of the disk areas
+    o, _ = oracle_for(C.ns_cylinder_case(4, 16, N=3, Re=200.0))
+    assert o.volume_integrals()[0] == pytest.approx(np.pi * (20.0 ** 2 - 0.5 ** 2), rel=1e-4)
+
+
+def test_surface_integrals_divergence_theorem_on_the_annulus():
+    """Both zones of the annulus close the domain, so the two markers' sums of p n dS and
+    rho u . n dS equal the volume integrals of grad p and div(rho u) (exact here: the linear
+    fields and the quadratic-arc map are interpolated exactly at N = 3, and Gauss quadrature
+    integrates the products exactly).  Density and flow pin the rho factors; a uniform state
+    has no viscous stress."""
+    case = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
+    case.viscous = "ip"
+    o, X = oracle_for(case)
+    x, y = X[:, 0], X[:, 1]
+    c, ax, by = 0.2, 0.3, -0.2
+    o.set_state(_cons(2.0 + 0.0 * x, ax * x, by * y, 1.0 + c * x))
+    one = o.volume_integrals()[0]
+    s1, s2 = o.surface_integrals(1), o.surface_integrals(2)
+    tot = s1 + s2
+    assert s1[0] == pytest.approx(np.pi * 0.5 * 2, rel=1e-4)              # wall perimeter
+    assert s2[0] == pytest.approx(np.pi * 20.0 * 2, rel=1e-4)
+    np.testing.assert_allclose(tot[3:5], [c * one, 0.0], rtol=1e-12, atol=1e-12 * one)
+    assert tot[5] == pytest.approx((ax + by) * one, rel=1e-12)
+    assert tot[6] == pytest.approx(0.5 * (ax + by) * one, rel=1e-12)       # mass flow / rho
+    np.testing.assert_allclose(s1[2], 2.0 * s1[0], rtol=1e-14)
+    # pressure: p = 1 + c x integrates x over a closed centred curve to zero
+    np.testing.assert_allclose(s1[1], s1[0], rtol=1e-12)
+    o.set_state(_cons(1.0 + 0.0 * x, 0.1 + 0.0 * x, 0.0 * x, 1.0 + 0.0 * x))
+    s1 = o.surface_integrals(1)
+    np.testing.assert_allclose(s1[3:9], 0.0, atol=1e-13)
+
+
+def test_surface_integrals_viscous_stress_of_linear_shear_and_strain():
+    """Interior penalty takes the element's own gradient, exact for linear momentum: rho u = a y
+    gives tau_xy = mu a, so the bottom (n = -y) and top (n = +y) zones carry -/+ mu a and the left
+    zone -mu a in y; rho u = a x gives tau_xx = (2 + lambda) mu a and tau_yy = lambda mu a with
+    lambda = -2/3 (ViscousFluxes.f90:416-461)."""
+    case = C.ns_kh_case(4, 4)
+    mu, a = 1.0 / case.Re, 0.3
+    o, X = oracle_for(case)
+    x, y = X[:, 0], X[:, 1]
+    ones = np.ones(len(X))
+    o.set_state(_cons(ones, a * y, 0.0 * ones, ones))
+    want = {1: (-mu * a, 0.0), 2: (mu * a, 0.0), 3: (0.0, -mu * a), 4: (0.0, mu * a)}
+    for mk, f in want.items():
+        s = o.surface_integrals(mk)
+        np.testing.assert_allclose(s[7:9], f, rtol=1e-10, atol=1e-13 * mu)
+        np.testing.assert_allclose(s[0], 1.0, rtol=1e-14)
+    assert o.surface_integrals(3)[5] == pytest.approx(-0.5 * a, rel=1e-13)   # -int_0^1 a y dy
+    o.set_state(_cons(ones, a * x, 0.0 * ones, ones))
+    lam = -2.0 / 3.0
+    np.testing.assert_allclose(o.surface_integrals(4)[7:9], [(2 + lam) * mu * a, 0.0], rtol=1e-10, atol=1e-13 * mu)
+    np.testing.assert_allclose(o.surface_integrals(1)[7:9], [0.0, -lam * mu * a], rtol=1e-10, atol=1e-13 * mu)
+    with pytest.raises(RuntimeError):
+        o.surface_integrals(5)
