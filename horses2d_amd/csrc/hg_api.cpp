@@ -116,6 +116,7 @@ hipError_t timed(hg_handle *h, int which, F &&launch) {
 int autotune(hg_handle *h) {
     const bool need_g = h->ph.ns && h->tile[1] < 0, need_s = h->tile[0] < 0;
     if (!need_g && !need_s) return 0;
+    h->qdot_valid = false;   // the timed stage launches use the QDot buffer as their G
     hipEvent_t ev[2];
     HG_CHECK(hipEventCreate(&ev[0]));
     HG_CHECK(hipEventCreate(&ev[1]));
@@ -548,6 +549,8 @@ int hg_destroy(hg_handle *h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     drop_graphs(h);
     for (auto &b : h->bufs) hipFree(b.p);
+    for (void *p : {(void *)h->probe_ev, (void *)h->probe_l, (void *)h->probe_out})
+        if (p) hipFree(p);
     for (auto &e : h->ev_pool) hipEventDestroy(e);
     for (auto &p : h->ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
     if (h->stream) hipStreamDestroy(h->stream);
@@ -656,6 +659,7 @@ int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     a.tr_out = h->tr[0];
     HG_CHECK(hg::launch_prolong(h->P, h->mixed, a, h->stream));
     h->vtr_valid = false;
+    h->qdot_valid = false;
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -679,12 +683,15 @@ int hg_download_qdot(hg_handle *h, double *QDot, int64_t ndof) {
 int hg_time_derivative(hg_handle *h, double t) {
     (void)t;  // no time-dependent boundary data on the GPU path
     if (!h) return set_err("hg_time_derivative: null handle");
-    return run_stage(h, 0.0, 1.0, 0.0, true, true, 1);
+    if (run_stage(h, 0.0, 1.0, 0.0, true, true, 1)) return 1;
+    h->qdot_valid = true;
+    return 0;
 }
 
 int hg_rk_step(hg_handle *h, double t, double dt) {
     (void)t;  // no time-dependent boundary data on the GPU path
     if (!h) return set_err("hg_rk_step: null handle");
+    h->qdot_valid = false;
     const StepCoefs k = step_coefs(h->opt.scheme);
     if (h->use_graph && !h->prof) {
         // replay the captured step (per-kernel event timing uses the direct path below)
@@ -788,6 +795,62 @@ int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
     else
         HG_CHECK(hipMemsetAsync(h->monitor_out, 0, 9 * sizeof(double), h->stream));
     HG_CHECK(hipMemcpyAsync(out, h->monitor_out, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int hg_probes_set(hg_handle *h, int32_t n, const int32_t *elem, const double *xi, const double *eta,
+                  const int32_t *var) {
+    if (!h) return set_err("hg_probes_set: null handle");
+    if (n < 0) return set_err("hg_probes_set: negative probe count");
+    if (n > 0 && (!elem || !xi || !eta || !var)) return set_err("hg_probes_set: null argument");
+    const int P = h->P;
+    std::vector<int2> ev(n);
+    std::vector<double> l((size_t)n * 2 * P);
+    const hg::Spectral &sp = hg::spectral(h->N);
+    for (int k = 0; k < n; ++k) {
+        if (elem[k] < 0 || elem[k] >= h->K) return set_err("hg_probes_set: element id out of range");
+        if (var[k] < HG_PROBE_RHO || var[k] > HG_PROBE_MACH) return set_err("hg_probes_set: unknown probe variable");
+        if (!std::isfinite(xi[k]) || !std::isfinite(eta[k])) return set_err("hg_probes_set: non-finite coordinate");
+        ev[k] = make_int2(elem[k], var[k]);
+        hg::lagrange_at(sp, xi[k], l.data() + (size_t)k * 2 * P);
+        hg::lagrange_at(sp, eta[k], l.data() + (size_t)k * 2 * P + P);
+    }
+    HG_CHECK(hipStreamSynchronize(h->stream));   // earlier updates may still read the old set
+    h->nprobes = 0;                               // no set until the new one is on the device
+    h->probe_var.clear();
+    if (n > h->probe_cap) {
+        for (void **p : {(void **)&h->probe_ev, (void **)&h->probe_l, (void **)&h->probe_out})
+            if (*p) {
+                HG_CHECK(hipFree(*p));
+                *p = nullptr;
+            }
+        h->probe_cap = 0;
+        HG_CHECK(hipMalloc((void **)&h->probe_ev, sizeof(int2) * n));
+        HG_CHECK(hipMalloc((void **)&h->probe_l, sizeof(double) * 2 * P * n));
+        HG_CHECK(hipMalloc((void **)&h->probe_out, sizeof(double) * n));
+        h->probe_cap = n;
+    }
+    if (n > 0) {
+        HG_CHECK(hipMemcpy(h->probe_ev, ev.data(), sizeof(int2) * n, hipMemcpyHostToDevice));
+        HG_CHECK(hipMemcpy(h->probe_l, l.data(), sizeof(double) * l.size(), hipMemcpyHostToDevice));
+    }
+    h->nprobes = n;
+    h->probe_var.assign(var, var + n);
+    return 0;
+}
+
+int hg_probes_update(hg_handle *h, double *out) {
+    if (!h) return set_err("hg_probes_update: null handle");
+    if (h->nprobes == 0) return 0;
+    if (!out) return set_err("hg_probes_update: null argument");
+    for (int v : h->probe_var)
+        if (v >= HG_PROBE_RHOT && v <= HG_PROBE_RHOET && !h->qdot_valid)
+            return set_err("hg_probes_update: rhot/rhout/rhovt/rhoet probes need hg_time_derivative after the "
+                           "last step or upload");
+    HG_CHECK(hg::launch_probes(h->Q, h->QDot, h->probe_ev, h->probe_l, h->nprobes, h->P, h->ph.gm1, h->ph.gamma,
+                               h->probe_out, h->stream));
+    HG_CHECK(hipMemcpyAsync(out, h->probe_out, sizeof(double) * h->nprobes, hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
 }

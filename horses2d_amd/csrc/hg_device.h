@@ -127,6 +127,10 @@ hipError_t launch_volume_integrals(int P, const double *Q, const double *jac, co
 hipError_t launch_surface_integrals(bool mixed, const int *slots, int nslots, int P, const void *tr, const void *vtr,
                                     const int4 *face, const double *egeo, const double *w, double gm1, double Eref,
                                     double *out, hipStream_t s);
+// Probe values (Probe_Update): n probes, per probe an int2 (element, variable) and 2P
+// interpolant values; variables 4..7 read QDot, the others Q.
+hipError_t launch_probes(const double *Q, const double *QDot, const int2 *ev, const double *l, int n, int P,
+                         double gm1, double gamma, double *out, hipStream_t st);
 hipError_t launch_cons_sums(int P, const double *Q, const double *jac, const double *w, int K, double *partial,
                             int nblocks, double *out, hipStream_t s);
 

@@ -153,6 +153,37 @@ __global__ __launch_bounds__(1024) void k_surface_integrals(const int *slots, in
     }
 }
 
+// Probe_Update (IO/Monitors.f90:739-846), one lane per probe: B = l_xi^T A l_eta with
+// (A l_eta)(i) = sum_j A(i,j) l_eta(j) (BilinearForm, MatrixOperations.f90:185-209) for each
+// conservative variable of Q (or of QDot for rhot .. rhoet), then the tracked variable.
+__global__ __launch_bounds__(64) void k_probes(const double *Q, const double *QDot, const int2 *ev, const double *l,
+                                               int n, int P, double gm1, double gamma, double *out) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= n) return;
+    const int2 x = ev[k];
+    const double *lx = l + (int64_t)k * 2 * P, *ly = lx + P;
+    const double *A = ((x.y >= 4 && x.y <= 7) ? QDot : Q) + (int64_t)x.x * P * P * 4;
+    double v[4];
+    for (int eq = 0; eq < 4; ++eq) {
+        double b = 0.0;
+        for (int i = 0; i < P; ++i) {
+            double r = 0.0;
+            for (int j = 0; j < P; ++j) r += A[(i + P * j) * 4 + eq] * ly[j];
+            b += lx[i] * r;
+        }
+        v[eq] = b;
+    }
+    double o;
+    switch (x.y) {
+    case 8: o = v[1] / v[0]; break;
+    case 9: o = v[2] / v[0]; break;
+    case 10: o = gm1 * (v[3] - 0.5 * (v[1] * v[1] + v[2] * v[2]) / v[0]); break;
+    case 11: o = sqrt(v[1] * v[1] + v[2] * v[2]) / v[0] / sqrt(gamma); break;
+    default: o = v[x.y & 3]; break;
+    }
+    out[k] = o;
+}
+
 // Residual of each case of a batch: max over its tiles' partials (NaN propagates).
 __global__ __launch_bounds__(256) void k_batch_resid(const double *part, const int2 *ranges, double *const *out) {
     __shared__ double s[256][4];
@@ -338,6 +369,12 @@ hipError_t launch_surface_integrals(bool mixed, const int *slots, int nslots, in
     else
         hipLaunchKernelGGL((k_surface_integrals<double>), dim3(1), dim3(1024), 0, st, slots, nslots, P,
                            (const double *)tr, (const double *)vtr, face, egeo, w, gm1, Eref, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_probes(const double *Q, const double *QDot, const int2 *ev, const double *l, int n, int P,
+                         double gm1, double gamma, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_probes, dim3((n + 63) / 64), dim3(64), 0, st, Q, QDot, ev, l, n, P, gm1, gamma, out);
     return hipGetLastError();
 }
 

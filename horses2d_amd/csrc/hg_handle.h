@@ -53,6 +53,13 @@ struct hg_handle {
     int *bslots = nullptr;
     std::vector<int> bslot_off;   // zone z (marker z+1): [bslot_off[z], bslot_off[z+1])
     double *monitor_out = nullptr;
+    // probes: (element, variable) and the interpolants l(xi), l(eta) (2P per probe), owned here
+    // (freed on replacement and in hg_destroy, not through bufs)
+    int nprobes = 0, probe_cap = 0;
+    std::vector<int> probe_var;
+    int2 *probe_ev = nullptr;
+    double *probe_l = nullptr, *probe_out = nullptr;
+    bool qdot_valid = false;   // QDot holds hg_time_derivative of the current state
     double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
     int cur = 0;
     int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet

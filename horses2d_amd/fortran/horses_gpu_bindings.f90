@@ -59,6 +59,7 @@ module horses_gpu_bindings
    public :: hg_create, hg_destroy, hg_ndof, hg_get_node_coords, hg_upload_state, hg_download_state
    public :: hg_download_qdot, hg_time_derivative, hg_rk_step, hg_residuals, hg_estimate_dt
    public :: hg_reset_dt_state, hg_conservation_sums, hg_volume_integrals, hg_surface_integrals
+   public :: hg_probes_set, hg_probes_update
    public :: hg_synchronize, hg_last_error
    public :: hgf_check
 
@@ -160,6 +161,23 @@ module horses_gpu_bindings
          integer(c_int32_t), value    :: marker
          real(c_double), intent(out)  :: vals(9)
       end function hg_surface_integrals
+
+      ! Probe_Initialization (IO/Monitors.f90:647-737): elements 0-based (eID - 1), reference
+      ! coordinates (xi, eta), variables HG_PROBE_* (rho .. Mach in Probe_Update's order, 0-based)
+      integer(c_int) function hg_probes_set(handle, n, elem, xi, eta, var) bind(C, name="hg_probes_set")
+         import :: c_int, c_ptr, c_double, c_int32_t
+         type(c_ptr), value              :: handle
+         integer(c_int32_t), value       :: n
+         integer(c_int32_t), intent(in)  :: elem(*), var(*)
+         real(c_double), intent(in)      :: xi(*), eta(*)
+      end function hg_probes_set
+
+      ! Probe_Update (IO/Monitors.f90:739-846): dimensionless values, one per probe
+      integer(c_int) function hg_probes_update(handle, vals) bind(C, name="hg_probes_update")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value          :: handle
+         real(c_double), intent(out) :: vals(*)
+      end function hg_probes_update
 
       integer(c_int) function hg_synchronize(handle) bind(C, name="hg_synchronize")
          import :: c_int, c_ptr

@@ -58,7 +58,7 @@ EXPORTS = [
     "hg_build_connectivity", "hg_build_geometry",
     "hg_upload_state", "hg_download_state", "hg_download_qdot", "hg_time_derivative", "hg_rk_step",
     "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_volume_integrals",
-    "hg_surface_integrals", "hg_synchronize",
+    "hg_surface_integrals", "hg_probes_set", "hg_probes_update", "hg_synchronize",
     "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_set_graph", "hg_batch_create", "hg_batch_rk_step",
     "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_destroy",
 ]
@@ -95,6 +95,8 @@ def load_library(path: str = LIB_PATH):
         "hg_conservation_sums": ([vp, vp], ctypes.c_int),
         "hg_volume_integrals": ([vp, vp], ctypes.c_int),
         "hg_surface_integrals": ([vp, i32, vp], ctypes.c_int),
+        "hg_probes_set": ([vp, i32, vp, vp, vp, vp], ctypes.c_int),
+        "hg_probes_update": ([vp, vp], ctypes.c_int),
         "hg_synchronize": ([vp], ctypes.c_int),
         "hg_last_error": ([vp, i32], ctypes.c_int),
         "hg_profile_enable": ([vp, i32], ctypes.c_int),
@@ -289,6 +291,27 @@ class DGSEM:
         over the edges of zone ``marker`` (1-based; QuadMesh_Integrals.incf:67-346)."""
         r = np.zeros(9)
         _check(self._L.hg_surface_integrals(self._h, int(marker), _ptr(r)), "hg_surface_integrals")
+        return r
+
+    PROBE_VARIABLES = ("rho", "rhou", "rhov", "rhoe", "rhot", "rhout", "rhovt", "rhoet", "u", "v", "p", "Mach")
+
+    def probes_set(self, elem, xi, eta, variables):
+        """Probes in elements ``elem`` (0-based) at reference coordinates (xi, eta) tracking
+        ``variables`` (names of Probe_Update, IO/Monitors.f90:762-836, or HG_PROBE_* codes)."""
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        x = np.ascontiguousarray(xi, dtype=np.float64)
+        y = np.ascontiguousarray(eta, dtype=np.float64)
+        v = np.ascontiguousarray([self.PROBE_VARIABLES.index(a) if isinstance(a, str) else int(a)
+                                  for a in variables], dtype=np.int32)
+        if not (len(e) == len(x) == len(y) == len(v)):
+            raise ValueError("probes_set: arrays of different lengths")
+        _check(self._L.hg_probes_set(self._h, len(e), _ptr(e), _ptr(x), _ptr(y), _ptr(v)), "hg_probes_set")
+        self._nprobes = len(e)
+
+    def probes_update(self) -> np.ndarray:
+        """Dimensionless probe values of the current state (Probe_Update)."""
+        r = np.zeros(getattr(self, "_nprobes", 0))
+        _check(self._L.hg_probes_update(self._h, _ptr(r) if len(r) else None), "hg_probes_update")
         return r
 
     def synchronize(self):

@@ -185,6 +185,36 @@ int hg_volume_integrals(hg_handle *h, double out[2]);
  * Monitors.f90 surface-monitor calls into Compute_*SurfaceIntegral. */
 int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]);
 
+/* Probe variables, in the order of Probe_Update's cases (IO/Monitors.f90:762-836). */
+#define HG_PROBE_RHO 0
+#define HG_PROBE_RHOU 1
+#define HG_PROBE_RHOV 2
+#define HG_PROBE_RHOE 3
+#define HG_PROBE_RHOT 4
+#define HG_PROBE_RHOUT 5
+#define HG_PROBE_RHOVT 6
+#define HG_PROBE_RHOET 7
+#define HG_PROBE_U 8
+#define HG_PROBE_V 9
+#define HG_PROBE_P 10
+#define HG_PROBE_MACH 11
+
+/* Probes, Probe_Initialization (IO/Monitors.f90:647-737) after findElementWithCoords: n probes,
+ * probe k in element elem[k] (0-based) at reference coordinates (xi[k], eta[k]) (the element's
+ * [0,1]^2), tracking variable var[k] (HG_PROBE_*).  The library evaluates the Lagrange
+ * interpolants l_j(xi), l_j(eta) once (barycentric form, as NodesAndWeightsClass.f90:334-342)
+ * and keeps them on the device.  Replaces any previous set; n = 0 removes it. */
+int hg_probes_set(hg_handle *h, int32_t n, const int32_t *elem, const double *xi, const double *eta,
+                  const int32_t *var);
+
+/* Probe_Update (IO/Monitors.f90:739-846): out[k] = the dimensionless value of probe k,
+ * l_xi^T A l_eta per conservative variable (BilinearForm, MatrixOperations.f90:185-209) and
+ * u, v, p, Mach from those; the caller applies Probe_Update's refValues factor.  A = the current
+ * state for rho .. rhoe, u, v, p, Mach; for rhot .. rhoet, the QDot of the last
+ * hg_time_derivative, which must come after the last step or upload (a Runge-Kutta step keeps
+ * no QDot; the reference's QDot at that point is its last stage's). */
+int hg_probes_update(hg_handle *h, double *out);
+
 /* Blocks until the handle's stream is idle. */
 int hg_synchronize(hg_handle *h);
 

@@ -1790,6 +1790,44 @@ int or_surface_integrals(void *h, int marker, double *out) {
     return 0;
 }
 
+/* Probe_Update (IO/Monitors.f90:739-846) for n probes at reference coordinates (xi, eta) of
+ * elements elem (0-based): the Lagrange interpolants of Probe_Initialization (:715-716,
+ * barycentric as NodesAndWeightsClass.f90:334-342), B = X^T A Y with (A Y)(i) = sum_j A(i,j) Y(j)
+ * (BilinearForm, MatrixOperations.f90:185-209) per conservative variable of Q (QDot for var
+ * 4..7), then var: 0 rho 1 rhou 2 rhov 3 rhoe 4 rhot 5 rhout 6 rhovt 7 rhoet 8 u 9 v 10 p
+ * 11 Mach, dimensionless (without the refValues factors). */
+int or_probes(void *h, int n, const int *elem, const double *xi, const double *eta, const int *var, double *out) {
+    Mesh *m = h;
+    const Phys *ph = &m->ph;
+    for (int k = 0; k < n; ++k) {
+        if (elem[k] < 0 || elem[k] >= m->K || var[k] < 0 || var[k] > 11) return fail("or_probes: bad probe");
+        Elem *e = &m->el[elem[k]];
+        const Spec *s = spec_get(e->N);
+        const int P = e->P;
+        double lx[MAXP], ly[MAXP], v[4];
+        lagrange_barycentric(xi[k], e->N, s->xi, s->wb, lx);
+        lagrange_barycentric(eta[k], e->N, s->xi, s->wb, ly);
+        const double *A = (var[k] >= 4 && var[k] <= 7) ? m->QDot : m->Q;
+        for (int eq = 0; eq < 4; ++eq) {
+            double b = 0.0;
+            for (int i = 0; i < P; ++i) {
+                double r = 0.0;
+                for (int j = 0; j < P; ++j) r = r + A[QIDX(e, eq, i, j)] * ly[j];
+                b = b + lx[i] * r;
+            }
+            v[eq] = b;
+        }
+        switch (var[k]) {
+        case 8: out[k] = v[1] / v[0]; break;
+        case 9: out[k] = v[2] / v[0]; break;
+        case 10: out[k] = ph->gm1 * (v[3] - 0.5 * (v[1] * v[1] + v[2] * v[2]) / v[0]); break;
+        case 11: out[k] = sqrt(v[1] * v[1] + v[2] * v[2]) / v[0] / sqrt(ph->gamma); break;
+        default: out[k] = v[var[k] & 3]; break;
+        }
+    }
+    return 0;
+}
+
 /* Point-wise flux exports for tests */
 int or_inviscid_flux(double Mach, int n, const double *q, double *f, double *g) {
     Phys ph; phys_init(&ph, Mach, 0.0, 0.72);

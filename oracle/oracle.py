@@ -78,6 +78,7 @@ def lib():
                      "or_residuals", "or_conservation_sums", "or_volume_integrals"):
             getattr(_lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_surface_integrals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        _lib.or_probes.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.or_get_metrics.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_get_connectivity.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
         _lib.or_get_edge_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
@@ -243,6 +244,21 @@ class Oracle:
         over the edges of zone ``marker`` (1-based; QuadMesh_Integrals.incf:67-346)."""
         r = np.zeros(9)
         if lib().or_surface_integrals(self.h, int(marker), _ptr(r)) != 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return r
+
+    PROBE_VARIABLES = ("rho", "rhou", "rhov", "rhoe", "rhot", "rhout", "rhovt", "rhoet", "u", "v", "p", "Mach")
+
+    def probes(self, elem, xi, eta, variables):
+        """Probe_Update (IO/Monitors.f90:739-846) at reference coordinates (xi, eta) of elements
+        ``elem`` (0-based); QDot variables read the last time derivative or stage."""
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        x = np.ascontiguousarray(xi, dtype=np.float64)
+        y = np.ascontiguousarray(eta, dtype=np.float64)
+        v = np.ascontiguousarray([self.PROBE_VARIABLES.index(a) if isinstance(a, str) else int(a)
+                                  for a in variables], dtype=np.int32)
+        r = np.zeros(len(e))
+        if lib().or_probes(self.h, len(e), _ptr(e), _ptr(x), _ptr(y), _ptr(v), _ptr(r)) != 0:
             raise RuntimeError(lib().or_last_error().decode())
         return r
 

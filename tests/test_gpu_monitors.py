@@ -97,3 +97,86 @@ def test_monitor_marker_out_of_range_raises():
     for bad in (0, 5):
         with pytest.raises(RuntimeError):
             dg.surface_integrals(bad)
+
+
+VARS = ("rho", "rhou", "rhov", "rhoe", "rhot", "rhout", "rhovt", "rhoet", "u", "v", "p", "Mach")
+
+
+def _probe_set(case, n, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    elem = rng.integers(0, case.mesh.K, size=n)
+    xi, eta = rng.uniform(0.0, 1.0, size=n), rng.uniform(0.0, 1.0, size=n)
+    var = [VARS[k % len(VARS)] for k in range(n)]
+    return elem, xi, eta, var
+
+
+QDOT_VARS = ("rhot", "rhout", "rhovt", "rhoet")
+
+
+def _probe_close(got, want, var, tol, qdot_scale=None):
+    """Per variable, relative to the largest magnitude of that variable among the probes; the
+    QDot variables relative to the max-norm of the whole QDot field (the scale of the QDot
+    parity bar: these low-Mach QDot values are 1e-2 of the fluxes they come from)."""
+    var = np.asarray(var)
+    for v in set(var):
+        m = var == v
+        scale = qdot_scale if v in QDOT_VARS else np.abs(want[m]).max()
+        np.testing.assert_allclose(got[m], want[m], rtol=0, atol=tol * scale, err_msg=v)
+
+
+@pytest.mark.parametrize("name", ["euler_vortex", "ns_br1_annulus", "ns_ip_kh"])
+def test_probes_match_oracle(name):
+    """Probe_Update on the device: every variable, the QDot ones after hg_time_derivative; after
+    steps the state variables (QDot probes then need a new time derivative)."""
+    from horses2d_amd import gpu
+    from oracle.oracle import Oracle
+    case = CASES[name][0]()
+    dg = gpu.DGSEM(case, "fp64")
+    orc = Oracle(case)
+    Q0 = case.initial_state(dg.coords())
+    dg.upload(Q0)
+    orc.set_state(Q0)
+    elem, xi, eta, var = _probe_set(case, 96, 3)
+    dg.probes_set(elem, xi, eta, var)
+    dg.time_derivative(0.0)
+    qd = orc.time_derivative(0.0)
+    _probe_close(dg.probes_update(), orc.probes(elem, xi, eta, var), var, TOL, np.abs(qd).max())
+    dt = dg.estimate_dt(case.dt_max, case.cfl)
+    for it in range(3):
+        orc.step(it * dt, dt, case.scheme)
+        dg.rk_step(it * dt, dt)
+    with pytest.raises(RuntimeError):
+        dg.probes_update()
+    state_vars = [v for v in var if v not in QDOT_VARS]
+    keep = np.array([v in state_vars for v in var])
+    # a smaller set replaces the first
+    dg.probes_set(elem[keep], xi[keep], eta[keep], state_vars)
+    _probe_close(dg.probes_update(), orc.probes(elem[keep], xi[keep], eta[keep], state_vars), state_vars, TOL)
+
+
+def test_probes_mixed_form_and_errors():
+    from horses2d_amd import gpu
+    case = CASES["ns_ip_kh"][0]()
+    elem, xi, eta, var = _probe_set(case, 40, 5)
+    out = []
+    for precision in ("fp64", "mixed"):
+        dg = gpu.DGSEM(case, precision)
+        dg.upload(case.initial_state(dg.coords()))
+        dt = dg.estimate_dt(case.dt_max, case.cfl)
+        for it in range(3):
+            dg.rk_step(it * dt, dt)
+        qd = dg.time_derivative(0.0)
+        dg.probes_set(elem, xi, eta, var)
+        out.append((dg.probes_update(), np.abs(qd).max()))
+    (p64, qs), (p32, _) = out
+    qv = np.array([v in QDOT_VARS for v in var])
+    _probe_close(p32[~qv], p64[~qv], [v for v, q in zip(var, qv) if not q], 1e-6)
+    # the mixed form's QDot carries the FP32 rounding of O(1) fluxes times the N^2/h lift
+    # (about 1e-4 absolute); its accuracy bar is on the stepped state, above
+    _probe_close(p32[qv], p64[qv], [v for v, q in zip(var, qv) if q], 1e-3, qs)
+    with pytest.raises(RuntimeError):
+        dg.probes_set([case.mesh.K], [0.5], [0.5], ["rho"])
+    with pytest.raises(RuntimeError):
+        dg.probes_set([0], [0.5], [0.5], [12])
+    dg.probes_set([], [], [], [])
+    assert dg.probes_update().shape == (0,)

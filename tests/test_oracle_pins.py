@@ -471,3 +471,55 @@ def test_surface_integrals_viscous_stress_of_linear_shear_and_strain():
     np.testing.assert_allclose(o.surface_integrals(1)[7:9], [0.0, -lam * mu * a], rtol=1e-10, atol=1e-13 * mu)
     with pytest.raises(RuntimeError):
         o.surface_integrals(5)
+
+
+def _tensor_interp(vals, xi, eta, N):
+    """Independent tensor interpolation (scipy barycentric) of nodal values at node i + P j."""
+    from scipy.interpolate import BarycentricInterpolator
+    P = N + 1
+    nodes = 0.5 + 0.5 * np.polynomial.legendre.leggauss(P)[0]
+    A = vals.reshape(P, P)                     # [j, i]
+    col = np.array([BarycentricInterpolator(nodes, A[j])(xi) for j in range(P)])
+    return float(BarycentricInterpolator(nodes, col)(eta))
+
+
+def test_probes_interpolate_polynomial_states_exactly():
+    """Probe_Update (IO/Monitors.f90:739-846): on the straight periodic square a state of degree
+    <= N in (x, y) is interpolated exactly, so every probe returns the polynomial (and u, v, p,
+    Mach formed from it) at the physical point of (xi, eta); a probe on a node returns QDot there."""
+    N = 4
+    case = C.ns_tgv_case(4, N)
+    o, X = oracle_for(case)
+    x, y = X[:, 0], X[:, 1]
+    f = [lambda x, y: 3.0 + 0.2 * x + 0.1 * y + 0.01 * x * y * y,
+         lambda x, y: 0.3 * x * x - 0.1 * y ** 3,
+         lambda x, y: 0.05 * x * y - 0.2 * y,
+         lambda x, y: 40.0 + 0.5 * x + 0.2 * y * x * x]
+    o.set_state(np.stack([g(x, y) for g in f], axis=-1).reshape(-1))
+    rng = np.random.Generator(np.random.PCG64(11))
+    n = 24
+    elem = rng.integers(0, case.mesh.K, size=n)
+    xi, eta = rng.uniform(0.0, 1.0, size=n), rng.uniform(0.0, 1.0, size=n)
+    names = ["rho", "rhou", "rhov", "rhoe", "u", "v", "p", "Mach"]
+    P = N + 1
+    for name in names:
+        got = o.probes(elem, xi, eta, [name] * n)
+        for k in range(n):
+            sl = slice(elem[k] * P * P, (elem[k] + 1) * P * P)
+            px = _tensor_interp(x[sl], xi[k], eta[k], N)
+            py = _tensor_interp(y[sl], xi[k], eta[k], N)
+            r, mu, mv, E = (g(px, py) for g in f)
+            want = {"rho": r, "rhou": mu, "rhov": mv, "rhoe": E, "u": mu / r, "v": mv / r,
+                    "p": (GAMMA - 1.0) * (E - 0.5 * (mu * mu + mv * mv) / r),
+                    "Mach": np.sqrt(mu * mu + mv * mv) / r / np.sqrt(GAMMA)}[name]
+            assert got[k] == pytest.approx(want, rel=1e-11, abs=1e-12), (name, k)
+    o.set_state(case.initial_state(X))     # a periodic state: the polynomial one is not
+    qd = o.time_derivative().reshape(-1, 4)
+    assert np.all(np.isfinite(qd))
+    nodes = O.spectral(N)["xi"]            # exactly on a node the interpolant is a unit vector
+    i, j = rng.integers(0, P, size=n), rng.integers(0, P, size=n)
+    for v, name in enumerate(["rhot", "rhout", "rhovt", "rhoet"]):
+        got = o.probes(elem, nodes[i], nodes[j], [name] * n)
+        np.testing.assert_array_equal(got, qd[elem * P * P + i + P * j, v])
+    with pytest.raises(RuntimeError):
+        o.probes([case.mesh.K], [0.5], [0.5], ["rho"])
