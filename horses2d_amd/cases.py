@@ -24,10 +24,16 @@ import numpy as np
 GAMMA = 1.4
 GM1 = 0.4
 
-# Boundary-condition codes of Generic/Defines.h:108-117
+# Boundary-condition codes of Generic/Defines.h:103-111
 BC_PERIODIC = 1
+BC_DIRICHLET = 2
+BC_EULERWALL = 3
 BC_VISCOUSWALL = 4
 BC_FARFIELD = 5
+BC_PRESSUREOUTLET = 6
+BC_PRESSUREINLET = 7
+# pressure outlet "Outflow type" (PressureOutlet.incf:66-72)
+OUTFLOW = {"partially non reflective": 0, "reflective": 1}
 
 RIEMANN = {"roe": 0, "hll": 1, "exact": 2, "lf": 3}
 VISCOUS = {"none": 0, "br1": 1, "ip": 2}
@@ -43,20 +49,23 @@ class Zone:
     direction: int = 0          # periodic: 1 = x-translation, 2 = y-translation
     partner: int = 0            # periodic: connected marker (1-based)
     isothermal: int = 0         # viscous wall
-    q: tuple = (0.0, 0.0, 0.0, 0.0)   # farfield state (dimensionless)
+    q: tuple = (0.0, 0.0, 0.0, 0.0)   # far field, Dirichlet, pressure outlet/inlet state (dimensionless)
     Tw: float = 1.0
     wall_velocity: tuple = (0.0, 0.0)
     riemann: int = -1           # per-zone Riemann solver, -1 = interior method
+    outflow: int = 0            # pressure outlet: OUTFLOW[...]
 
     def ints(self):
         if self.kind == BC_PERIODIC:
             return (self.direction, self.partner)
+        if self.kind == BC_PRESSUREOUTLET:
+            return (self.outflow, 0)
         return (self.isothermal, 0)
 
     def reals(self):
-        if self.kind == BC_FARFIELD:
-            return tuple(float(v) for v in self.q)
-        return (float(self.Tw), float(self.wall_velocity[0]), float(self.wall_velocity[1]), 0.0)
+        if self.kind == BC_VISCOUSWALL:
+            return (float(self.Tw), float(self.wall_velocity[0]), float(self.wall_velocity[1]), 0.0)
+        return tuple(float(v) for v in self.q)
 
 
 @dataclasses.dataclass
@@ -136,6 +145,28 @@ def periodic_rectangle(nx: int, ny: int, x0: float, x1: float, y0: float, y1: fl
     zones = [Zone(BC_PERIODIC, direction=2, partner=2), Zone(BC_PERIODIC, direction=2, partner=1),
              Zone(BC_PERIODIC, direction=1, partner=4), Zone(BC_PERIODIC, direction=1, partner=3)]
     return HiOMesh(points, quads, bedges, bmarker, zones)
+
+
+def channel(nx: int, ny: int, L: float, H: float, zones: List[Zone], skew: float = 0.0) -> HiOMesh:
+    """Quads on [0,L]x[0,H] (interior nodes shifted by ``skew`` times a smooth bump, so the
+    elements are general quadrilaterals) with four zones: markers 1 bottom, 2 right, 3 top,
+    4 left, in that order in ``zones``."""
+    xs = L * np.arange(nx + 1) / nx
+    ys = H * np.arange(ny + 1) / ny
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    bump = np.sin(np.pi * X / L) * np.sin(np.pi * Y / H)
+    X = X + skew * (L / nx) * bump
+    Y = Y + skew * (H / ny) * bump * np.cos(np.pi * X / L)
+    points = np.stack([X.ravel(), Y.ravel()], axis=1)
+    quads = _grid_quads(nx, ny)
+    nid = lambda i, j: i * (ny + 1) + j
+    bottom = [(nid(i, 0), nid(i + 1, 0)) for i in range(nx)]
+    right = [(nid(nx, j), nid(nx, j + 1)) for j in range(ny)]
+    top = [(nid(i + 1, ny), nid(i, ny)) for i in range(nx)]
+    left = [(nid(0, j + 1), nid(0, j)) for j in range(ny)]
+    bedges = np.array(bottom + right + top + left, dtype=np.int32)
+    bmarker = np.array([1] * nx + [2] * ny + [3] * nx + [4] * ny, dtype=np.int32)
+    return HiOMesh(points, quads, bedges, bmarker, list(zones))
 
 
 def annulus(nr: int, nt: int, r0: float, r1: float, q_inf, wall_velocity=(0.0, 0.0),

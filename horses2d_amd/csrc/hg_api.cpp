@@ -366,9 +366,8 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     std::vector<double> zone(8 * std::max<size_t>(m.zones.size(), 1), 0.0);
     for (size_t z = 0; z < m.zones.size(); ++z) {
         const hg::ZoneDesc &Z = m.zones[z];
-        if (Z.type == HG_BC_FARFIELD)
-            for (int c = 0; c < 4; ++c) zone[8 * z + c] = Z.q[c];
-        else { zone[8 * z] = Z.Tw; zone[8 * z + 1] = Z.vw[0]; zone[8 * z + 2] = Z.vw[1]; }
+        if (Z.type == HG_BC_VISCOUSWALL) { zone[8 * z] = Z.Tw; zone[8 * z + 1] = Z.vw[0]; zone[8 * z + 2] = Z.vw[1]; }
+        else for (int c = 0; c < 4; ++c) zone[8 * z + c] = Z.q[c];   // the zone state
     }
     for (int e = 0; e < K; ++e) {
         const hg::Elem &el = m.el[e];
@@ -394,9 +393,17 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
                     y = pe.quads[0] * 4 + (pe.loc[0] - 1);
                     if (ed.inverse) flags |= hg::FF_REV;
                 } else {
-                    flags = Z.type == HG_BC_FARFIELD ? hg::FK_FARFIELD : hg::FK_WALL;
+                    switch (Z.type) {
+                    case HG_BC_FARFIELD: flags = hg::FK_FARFIELD; break;
+                    case HG_BC_VISCOUSWALL: flags = hg::FK_WALL | (Z.isothermal ? hg::FF_ISOTHERMAL : 0); break;
+                    case HG_BC_EULERWALL: flags = hg::FK_EULERWALL; break;
+                    case HG_BC_DIRICHLET: flags = hg::FK_DIRICHLET; break;
+                    case HG_BC_PRESSUREOUTLET:
+                        flags = hg::FK_POUTLET | (Z.isothermal == HG_OUTFLOW_REFLECTIVE ? hg::FF_REFLECTIVE : 0);
+                        break;
+                    default: flags = hg::FK_PINLET; break;
+                    }
                     y = ed.marker - 1;
-                    if (Z.type == HG_BC_VISCOUSWALL && Z.isothermal) flags |= hg::FF_ISOTHERMAL;
                     if (!ph.ns && Z.type == HG_BC_VISCOUSWALL)
                         return set_err("hg_create: viscous walls need the Navier-Stokes form");
                 }

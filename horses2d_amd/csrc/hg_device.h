@@ -13,11 +13,16 @@ enum FaceKind : int {
     FK_PERIODIC = 2,  // periodic boundary edge: partner trace is the outer state
     FK_FARFIELD = 3,  // characteristic far field (Farfield.incf:138-376)
     FK_WALL = 4,      // viscous wall (ViscousWall.incf:158-254)
+    FK_EULERWALL = 5, // Euler wall, prescribed flux (EulerWall.incf:96-159)
+    FK_DIRICHLET = 6, // Dirichlet state (Dirichlet.incf:106-337)
+    FK_POUTLET = 7,   // pressure outlet (PressureOutlet.incf:151-450)
+    FK_PINLET = 8,    // pressure inlet, prescribed flux (PressureInlet.incf:134-410)
 };
 constexpr int FF_CURVED = 1 << 4;    // per-node normals and dS
 constexpr int FF_REV = 1 << 5;       // neighbor trace index runs backwards
 constexpr int FF_SELFREV = 1 << 6;   // edge geometry index runs backwards (RIGHT side of an inverse edge)
-constexpr int FF_ISOTHERMAL = 1 << 7;
+constexpr int FF_ISOTHERMAL = 1 << 7;  // viscous wall: isothermal
+constexpr int FF_REFLECTIVE = 1 << 7;  // pressure outlet: reflective outflow (same bit, other kind)
 constexpr int FF_RS_SHIFT = 8;       // (zone Riemann solver + 1), 0 = interior method
 
 // Viscous face-flux traces: per element face node, mu times this side's viscous flux from its

@@ -122,7 +122,7 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
     } else {
         R uB[4];
         int neu;
-        ghost_state<R>(kind, flags, a.zone + 8 * y, ph, qs, nx, ny, uB, us, neu);
+        ghost_state<R>(kind, flags, a.zone + 8 * y, a.ph, ph, qs, nx, ny, uB, us, neu);
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c] = us[c] * dS;
@@ -302,7 +302,7 @@ __device__ __forceinline__ void viscous_face_trace(const StageArgs &a, const Phy
 #pragma unroll
         for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
     }
-    if ((flags & 15) == FK_WALL && !(flags & FF_ISOTHERMAL)) adiabatic_flux(ph, qs, ds, fv, gv);
+    if (is_adiabatic(flags)) adiabatic_flux(ph, qs, ds, fv, gv);
     else viscous_flux(ph, qs, ds, fv, gv);
     o[0] = (TR)(ph.mu * fv[1]); o[1] = (TR)(ph.mu * fv[2]); o[2] = (TR)(ph.mu * fv[3]);
     o[3] = (TR)(ph.mu * gv[2]); o[4] = (TR)(ph.mu * gv[3]); o[5] = TR(0);
@@ -572,14 +572,21 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         }
         int rs = a.ph.riemann;
         R uSB[4] = {R(0), R(0), R(0), R(0)};
-        int neumann = 0;
+        int neumann = 0, prescribed = 0;
         if (kind > FK_PERIODIC) {
-            ghost_state<R>(kind, fr.x, a.zone + 8 * fr.y, ph, qs, nx, ny, qo, uSB, neumann);
+            prescribed = ghost_state<R>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB, neumann);
             const int zr = (fr.x >> FF_RS_SHIFT) & 15;
             if (zr) rs = zr - 1;
         }
         R Fi[4];
-        if (kind == FK_INT_R) riemann<RS, R>(rs, ph, qo, qs, nx, ny, Fi);
+        if (prescribed) {
+            // WEAK_PRESCRIBED (DGSpatialDiscretizationMethods.f90:1194-1198): the inviscid
+            // normal flux of the boundary's state
+            R fp[4], gp[4];
+            inviscid_flux(ph, qo, fp, gp);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) Fi[c] = fp[c] * nx + gp[c] * ny;
+        } else if (kind == FK_INT_R) riemann<RS, R>(rs, ph, qo, qs, nx, ny, Fi);
         else riemann<RS, R>(rs, ph, qs, qo, nx, ny, Fi);
         R F[4];
         if (!NS) {
@@ -618,7 +625,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
                     for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
                 }
             } else if (!neumann) {
-                const bool adiab = kind == FK_WALL && !(fr.x & FF_ISOTHERMAL);
+                const bool adiab = is_adiabatic(fr.x);
                 R fv4[4], gv4[4];
                 viscous_trace_flux(vs, fv4, gv4);   // adiabatic or not, as the gradient kernel chose
 #pragma unroll

@@ -428,8 +428,11 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
         for (int c = 0; c < 4; ++c) Z.q[c] = in.bc_real[4 * z + c];
         Z.Tw = in.bc_real[4 * z]; Z.vw[0] = in.bc_real[4 * z + 1]; Z.vw[1] = in.bc_real[4 * z + 2];
         Z.riemann = in.bc_riemann ? in.bc_riemann[z] : -1;
-        if (Z.type != HG_BC_PERIODIC && Z.type != HG_BC_FARFIELD && Z.type != HG_BC_VISCOUSWALL)
-            return "unsupported boundary condition type";
+        switch (Z.type) {
+        case HG_BC_PERIODIC: case HG_BC_DIRICHLET: case HG_BC_EULERWALL: case HG_BC_VISCOUSWALL:
+        case HG_BC_FARFIELD: case HG_BC_PRESSUREOUTLET: case HG_BC_PRESSUREINLET: break;
+        default: return "unsupported boundary condition type";
+        }
     }
     for (int k = 0; k < E; ++k) {
         Edge &ed = m.ed[k];

@@ -49,7 +49,7 @@ struct Elem {
 };
 
 struct ZoneDesc {
-    int type = 0, direction = 0, partner = 0, isothermal = 0, riemann = -1;
+    int type = 0, direction = 0, partner = 0, isothermal = 0, riemann = -1;   // isothermal: also the outflow type
     double q[4] = {0, 0, 0, 0};
     double Tw = 1.0, vw[2] = {0, 0};
     std::vector<int> edges;

@@ -12,7 +12,14 @@ module horses_gpu_bindings
    implicit none
    private
 
-   integer(c_int32_t), parameter, public :: HG_BC_PERIODIC = 1, HG_BC_VISCOUSWALL = 4, HG_BC_FARFIELD = 5
+   integer(c_int32_t), parameter, public :: HG_BC_PERIODIC = 1, HG_BC_DIRICHLET = 2, HG_BC_EULERWALL = 3, &
+                                            HG_BC_VISCOUSWALL = 4, HG_BC_FARFIELD = 5, &
+                                            HG_BC_PRESSUREOUTLET = 6, HG_BC_PRESSUREINLET = 7
+   integer(c_int32_t), parameter, public :: HG_OUTFLOW_PARTIALLY_NON_REFLECTIVE = 0, HG_OUTFLOW_REFLECTIVE = 1
+   integer(c_int32_t), parameter, public :: HG_PROBE_RHO = 0, HG_PROBE_RHOU = 1, HG_PROBE_RHOV = 2, &
+                                            HG_PROBE_RHOE = 3, HG_PROBE_RHOT = 4, HG_PROBE_RHOUT = 5, &
+                                            HG_PROBE_RHOVT = 6, HG_PROBE_RHOET = 7, HG_PROBE_U = 8, &
+                                            HG_PROBE_V = 9, HG_PROBE_P = 10, HG_PROBE_MACH = 11
    integer(c_int32_t), parameter, public :: HG_RIEMANN_ROE = 0, HG_RIEMANN_HLL = 1, HG_RIEMANN_EXACT = 2, &
                                             HG_RIEMANN_LF = 3
    integer(c_int32_t), parameter, public :: HG_VISCOUS_NONE = 0, HG_VISCOUS_BR1 = 1, HG_VISCOUS_IP = 2

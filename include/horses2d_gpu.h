@@ -26,10 +26,19 @@ extern "C" {
 typedef struct hg_handle hg_handle;
 typedef struct hg_batch hg_batch;
 
-/* Boundary condition kinds (Generic/Defines.h:108-117 numbering). */
+/* Boundary condition kinds (Generic/Defines.h:103-111 numbering; "Type" of a zone block,
+ * BoundaryConditions/DGBoundaryConditions.f90:244-357). */
 #define HG_BC_PERIODIC 1
+#define HG_BC_DIRICHLET 2
+#define HG_BC_EULERWALL 3
 #define HG_BC_VISCOUSWALL 4
 #define HG_BC_FARFIELD 5
+#define HG_BC_PRESSUREOUTLET 6
+#define HG_BC_PRESSUREINLET 7
+
+/* Pressure outlet "Outflow type" (BoundaryConditions/PressureOutlet.incf:66-72). */
+#define HG_OUTFLOW_PARTIALLY_NON_REFLECTIVE 0
+#define HG_OUTFLOW_REFLECTIVE 1
 
 /* Riemann solvers: Roe, HLL, exact (Physics/RiemannSolvers.f90:33-399) and the
  * Lax-Friedrichs flux, which the reference does not have. */
@@ -79,9 +88,12 @@ typedef struct {
     int32_t nmarkers;
     const int32_t *bc_type;   /* [nmarkers] HG_BC_* */
     const int32_t *bc_int;    /* [nmarkers][2] periodic: (direction 1=x|2=y, partner marker);
-                                 wall: (isothermal 0|1, 0) */
-    const double *bc_real;    /* [nmarkers][4] farfield: q_inf (dimensionless);
-                                 wall: (Tw, vx, vy, 0) */
+                                 viscous wall: (isothermal 0|1, 0);
+                                 pressure outlet: (HG_OUTFLOW_*, 0); others (0, 0) */
+    const double *bc_real;    /* [nmarkers][4] farfield, Dirichlet, pressure outlet, pressure
+                                 inlet: the zone state q (dimensionless, as their _Construct
+                                 forms it; "Use function" states are not supported);
+                                 viscous wall: (Tw, vx, vy, 0); Euler wall: unused */
     const int32_t *bc_riemann;/* [nmarkers] per-zone Riemann solver or -1 (may be NULL) */
 } hg_mesh_desc;
 

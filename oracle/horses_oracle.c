@@ -33,6 +33,8 @@
 #define BC_EULERWALL 3
 #define BC_VISCOUSWALL 4
 #define BC_FARFIELD 5
+#define BC_PRESSUREOUTLET 6
+#define BC_PRESSUREINLET 7
 #define WEAK_RIEMANN 1
 #define WEAK_PRESCRIBED 2
 #define VBC_UNDEFINED 0
@@ -40,6 +42,9 @@
 #define VBC_NEUMANN 2
 #define VBC_PERIODIC 3
 #define VBC_ADIABATIC 4
+/* Pressure outlet "Outflow type" (BoundaryConditions/PressureOutlet.incf:66-72) */
+#define PARTIALLY_NON_REFLECTIVE_OUTFLOW 0
+#define REFLECTIVE_OUTFLOW 1
 
 #define EBOTTOM 1
 #define ERIGHT 2
@@ -536,6 +541,7 @@ typedef struct {
     int inviscidBCType, rs, associated, partner;
     int vbc[MAXP];
     double uB[NCONS * MAXP], uSB[NCONS * MAXP];
+    double FB[NCONS * MAXP];   /* WEAK_PRESCRIBED: F.n per node (times dS when applied) */
     /* traces: storage(LEFT)=tq[0], storage(RIGHT)=tq[1] (boundary: tq[0]) */
     double tq[2][NCONS * MAXP];
     double tdq[2][2 * NCONS * MAXP]; /* [eq + 4*i + 4*P*dim] */
@@ -559,7 +565,8 @@ typedef struct {
     int direction;  /* periodic: 1=x, 2=y */
     int partner;    /* periodic connected marker */
     int wall_iso;   /* viscous wall: 0 adiabatic, 1 isothermal */
-    double q[4];    /* farfield state (dimensionless) */
+    int outflow;    /* pressure outlet: PARTIALLY_NON_REFLECTIVE_OUTFLOW or REFLECTIVE_OUTFLOW */
+    double q[4];    /* far field, Dirichlet, pressure outlet / inlet: the zone state (dimensionless) */
     double Tw, vw[2];
     int rs;         /* zone Riemann solver (-1 = interior method) */
     int associated;
@@ -931,6 +938,7 @@ static Mesh *mesh_build(const MeshIn *in) {
         Z->direction = in->bc_int[2 * (z - 1)];
         Z->partner = in->bc_int[2 * (z - 1) + 1];
         Z->wall_iso = in->bc_int[2 * (z - 1)];
+        Z->outflow = in->bc_int[2 * (z - 1)];
         for (int k = 0; k < 4; ++k) Z->q[k] = in->bc_real[4 * (z - 1) + k];
         Z->Tw = in->bc_real[4 * (z - 1)]; Z->vw[0] = in->bc_real[4 * (z - 1) + 1]; Z->vw[1] = in->bc_real[4 * (z - 1) + 2];
         Z->rs = in->bc_rs ? in->bc_rs[z - 1] : -1;
@@ -939,10 +947,13 @@ static Mesh *mesh_build(const MeshIn *in) {
             if (m->ed[k].bdry && m->ed[k].marker == z) Z->edges[Z->nedges++] = k;
         for (int a = 0; a < Z->nedges; ++a) {
             Edge *ed = &m->ed[Z->edges[a]];
-            ed->inviscidBCType = WEAK_RIEMANN;
+            /* BoundaryConditions_construct, DGBoundaryConditions.f90:244-357 */
+            ed->inviscidBCType = (Z->type == BC_EULERWALL || Z->type == BC_PRESSUREINLET) ? WEAK_PRESCRIBED : WEAK_RIEMANN;
             ed->rs = Z->rs;
             int P = ed->N + 1;
             for (int i = 0; i < P; ++i) ed->vbc[i] = VBC_UNDEFINED;
+            if (Z->type == BC_EULERWALL)   /* EulerWall.incf:44-94 */
+                for (int i = 0; i < P; ++i) ed->vbc[i] = VBC_ADIABATIC;
             if (Z->type == BC_VISCOUSWALL)
                 for (int i = 0; i < P; ++i) ed->vbc[i] = Z->wall_iso ? VBC_DIRICHLET : VBC_ADIABATIC;
             if (Z->type == BC_FARFIELD) /* defined treatment: the reference leaves it unset */
@@ -1072,6 +1083,170 @@ static void update_bcs(Mesh *m) {
                     }
                     for (int k = 0; k < 4; ++k) ed->uSB[4 * i + k] = uB[k];
                     ed->vbc[i] = supersonic_out ? VBC_NEUMANN : VBC_DIRICHLET;
+                }
+            } else if (Z->type == BC_EULERWALL) {
+                /* EulerWall_UpdateSolution (EulerWall.incf:96-159): the wall state q (normal
+                 * momentum removed, energy from p and the tangential momentum) gives the
+                 * prescribed flux FB = sum_d F_d(q) n_d dS; uB mirrors the normal momentum */
+                for (int i = 0; i < P; ++i) {
+                    const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+                    const double dS = ed->curved ? ed->dS[i] : ed->dS[0];
+                    const double *Qt = ed->tq[0] + 4 * i;
+                    double rhovn = Qt[1] * nn[0] + Qt[2] * nn[1];
+                    double rhovt = -Qt[1] * nn[1] + Qt[2] * nn[0];
+                    double p = ph->gm1 * (Qt[3] - 0.5 * (Qt[1] * Qt[1] + Qt[2] * Qt[2]) / Qt[0]);
+                    double q[4] = {Qt[0], Qt[1] - rhovn * nn[0], Qt[2] - rhovn * nn[1],
+                                   ph->cv * p + 0.5 * rhovt * rhovt / Qt[0]};
+                    double f[4], g[4];
+                    inviscid_flux(ph, q, f, g);
+                    for (int k = 0; k < 4; ++k) {
+                        double fb = 0.0;
+                        fb = fb + f[k] * nn[0] * dS;
+                        fb = fb + g[k] * nn[1] * dS;
+                        ed->FB[4 * i + k] = fb;
+                    }
+                    double *uB = ed->uB + 4 * i;
+                    uB[0] = Qt[0];
+                    uB[1] = Qt[1] - 2.0 * rhovn * nn[0];
+                    uB[2] = Qt[2] - 2.0 * rhovn * nn[1];
+                    uB[3] = Qt[3];
+                    for (int k = 0; k < 4; ++k) ed->uSB[4 * i + k] = q[k];
+                }
+            } else if (Z->type == BC_DIRICHLET) {
+                /* DirichletBC_Associate (Dirichlet.incf:78-104): uB = the zone state;
+                 * DirichletBC_UpdateSolution (:106-337): uSB and the viscous type per node */
+                for (int i = 0; i < P; ++i) {
+                    const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+                    const double *Qt = ed->tq[0] + 4 * i;
+                    double *uB = ed->uB + 4 * i, *uSB = ed->uSB + 4 * i;
+                    for (int k = 0; k < 4; ++k) uB[k] = Z->q[k];
+                    double rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
+                    double vnL = uL * nn[0] + vL * nn[1];
+                    double pL = ph->gm1 * (Qt[3] - 0.5 * (Qt[1] * uL + Qt[2] * vL));
+                    double aL = sqrt(ph->gamma * pL / rhoL);
+                    double ML = sqrt(uL * uL + vL * vL) / aL;
+                    if (vnL <= 0.0) {
+                        if (ML > 1.0) {
+                            for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
+                        } else {
+                            double pR = pL;
+                            uSB[0] = uB[0]; uSB[1] = uB[1]; uSB[2] = uB[2];
+                            uSB[3] = ph->cv * pR + 0.5 * (uSB[1] * uSB[1] + uSB[2] * uSB[2]) / uSB[0];
+                        }
+                        ed->vbc[i] = VBC_DIRICHLET;
+                    } else if (ML > 1.0) {
+                        for (int k = 0; k < 4; ++k) uSB[k] = Qt[k];
+                        ed->vbc[i] = VBC_NEUMANN;
+                    } else {
+                        uSB[0] = Qt[0]; uSB[1] = Qt[1]; uSB[2] = Qt[2];
+                        double pB = ph->gm1 * (uB[3] - 0.5 * (uB[1] * uB[1] + uB[2] * uB[2]) / uB[0]);
+                        uSB[3] = ph->cv * pB + 0.5 * (uSB[1] * uL + uSB[2] * vL);
+                        ed->vbc[i] = VBC_DIRICHLET;
+                    }
+                }
+            } else if (Z->type == BC_PRESSUREOUTLET) {
+                /* PressureOutletBC_UpdateSolution (PressureOutlet.incf:151-450) */
+                const double *q = Z->q;
+                double rhoI = q[0], uI = q[1] / rhoI, vI = q[2] / rhoI;
+                double pI = ph->gm1 * (q[3] - 0.5 * q[1] * uI - 0.5 * q[2] * vI);
+                double nrm = sqrt(uI * uI + vI * vI), nIx = uI / nrm, nIy = vI / nrm;
+                for (int i = 0; i < P; ++i) {
+                    const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+                    const double *Qt = ed->tq[0] + 4 * i;
+                    double *uB = ed->uB + 4 * i, *uSB = ed->uSB + 4 * i;
+                    double rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
+                    double vnL = uL * nn[0] + vL * nn[1];
+                    double pL = ph->gm1 * (Qt[3] - 0.5 * Qt[1] * uL - 0.5 * Qt[2] * vL);
+                    double aL = sqrt(ph->gamma * pL / rhoL);
+                    double ML = sqrt(uL * uL + vL * vL) / aL;
+                    double rhoR, uR, vR, pR;
+                    if (ML > 1.0) {   /* supersonic: inflow takes the zone state, outflow the interior */
+                        const double *src = vnL < 0.0 ? q : Qt;
+                        for (int k = 0; k < 4; ++k) { uB[k] = src[k]; uSB[k] = src[k]; }
+                        ed->vbc[i] = vnL < 0.0 ? VBC_DIRICHLET : VBC_NEUMANN;
+                        continue;
+                    }
+                    if (Z->outflow == REFLECTIVE_OUTFLOW) {
+                        double R0 = pL - aL * aL * rhoL;
+                        double Rplus = vnL + 2.0 * ph->cv * aL;
+                        rhoR = -(R0 - pI) / (aL * aL);
+                        double aR = sqrt(ph->gamma * pI / rhoR);
+                        double vnR = Rplus - 2.0 * aR * ph->cv;
+                        uR = uL + (vnR - vnL) * nn[0];
+                        vR = vL + (vnR - vnL) * nn[1];
+                        pR = pI;
+                    } else {
+                        pR = 2.0 * pI - pL;
+                        uR = vnL < 0.0 ? fabs(vnL) * nIx : uL;
+                        vR = vnL < 0.0 ? fabs(vnL) * nIy : vL;
+                        rhoR = rhoL;
+                    }
+                    uB[0] = rhoR; uB[1] = rhoR * uR; uB[2] = rhoR * vR;
+                    uB[3] = ph->cv * pR + 0.5 * (uB[1] * uR + uB[2] * vR);
+                    if (vnL < 0.0) {
+                        for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
+                    } else {
+                        uSB[0] = rhoL; uSB[1] = rhoL * uL; uSB[2] = rhoL * vL;
+                        uSB[3] = ph->cv * pR + 0.5 * rhoL * (uL * uL + vL * vL);
+                    }
+                    ed->vbc[i] = VBC_DIRICHLET;
+                }
+            } else if (Z->type == BC_PRESSUREINLET) {
+                /* PressureInletBC_UpdateSolution (PressureInlet.incf:134-410): prescribed flux
+                 * FB = (F_x(q) n_x + F_y(q) n_y) dS of the state q each branch forms.  The
+                 * curved-edge supersonic outflow reads the trace transposed (:344); the
+                 * straight-edge form (:247) is used for both. */
+                const double *qe = Z->q;
+                double uE = qe[1] / qe[0], vE = qe[2] / qe[0];
+                double pE = ph->gm1 * (qe[3] - 0.5 * (qe[1] * uE + qe[2] * vE));
+                double MExtSq = (uE * uE + vE * vE) / (ph->gamma * pE) * qe[0];
+                double atExtSq = ph->gamma * pE / qe[0] * (1.0 + 0.5 * ph->gm1 * MExtSq);
+                double sExt = pE / pow(qe[0], ph->gamma);
+                double ptExt = pE * pow(1.0 + 0.5 * ph->gm1 * MExtSq, ph->cp);
+                for (int i = 0; i < P; ++i) {
+                    const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+                    const double dS = ed->curved ? ed->dS[i] : ed->dS[0];
+                    const double *Qt = ed->tq[0] + 4 * i;
+                    double *uSB = ed->uSB + 4 * i;
+                    double rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
+                    double vnL = uL * nn[0] + vL * nn[1];
+                    double pL = ph->gm1 * (Qt[3] - 0.5 * (Qt[1] * uL + Qt[2] * vL));
+                    double aL = sqrt(ph->gamma * pL / rhoL);
+                    double ML = sqrt(uL * uL + vL * vL) / aL;
+                    double Rplus = vnL + 2.0 * ph->cv * aL;
+                    double q[4];
+                    if (vnL <= 0.0 && ML > 1.0) {
+                        for (int k = 0; k < 4; ++k) q[k] = qe[k];
+                        for (int k = 0; k < 4; ++k) uSB[k] = q[k];
+                        ed->vbc[i] = VBC_DIRICHLET;
+                    } else if (vnL <= 0.0) {
+                        double a = (ph->gamma + 1.0) / ph->gm1, b = -2 * Rplus;
+                        double c = 0.5 * ph->gm1 * (Rplus * Rplus) - atExtSq;
+                        double aR = (-b + sqrt(b * b - 4.0 * a * c)) / (2.0 * a);
+                        double rhoR = pow(aR * aR / (ph->gamma * sExt), ph->cv);
+                        double pR = aR * aR * rhoR / ph->gamma;
+                        double vnR = Rplus - 2.0 * ph->cv * aR;
+                        double uR = vnR * nn[0], vR = vnR * nn[1];
+                        q[0] = rhoR; q[1] = rhoR * uR; q[2] = rhoR * vR;
+                        q[3] = ph->cv * pR + 0.5 * (q[1] * uR + q[2] * vR);
+                        uSB[0] = rhoR; uSB[1] = rhoR * uR; uSB[2] = rhoR * vR;
+                        uSB[3] = ph->cv * pL + 0.5 * rhoR * (uR * uR + vR * vR);
+                        ed->vbc[i] = VBC_DIRICHLET;
+                    } else if (ML > 1.0) {
+                        for (int k = 0; k < 4; ++k) q[k] = Qt[k];
+                        for (int k = 0; k < 4; ++k) uSB[k] = q[k];
+                        ed->vbc[i] = VBC_NEUMANN;
+                    } else {
+                        q[0] = Qt[0]; q[1] = Qt[1]; q[2] = Qt[2];
+                        q[3] = ph->cv * (2.0 * ptExt - pL) + 0.5 * (q[1] * uL + q[2] * vL);
+                        uSB[0] = q[0]; uSB[1] = q[1]; uSB[2] = q[2];
+                        uSB[3] = ph->cv * ptExt + 0.5 * (q[1] * uL + q[2] * vL);
+                        ed->vbc[i] = VBC_DIRICHLET;
+                    }
+                    double f[4], g[4];
+                    inviscid_flux(ph, q, f, g);
+                    for (int k = 0; k < 4; ++k) ed->FB[4 * i + k] = (f[k] * nn[0] + g[k] * nn[1]) * dS;
+                    for (int k = 0; k < 4; ++k) ed->uB[4 * i + k] = q[k];
                 }
             } else if (Z->type == BC_VISCOUSWALL) {
                 for (int i = 0; i < P; ++i) {
@@ -1429,8 +1604,12 @@ static void time_derivative(Mesh *m) {
                 const double *q = ed->tq[0] + 4 * i;
                 const double *qb = bdry_state(m, ed, i);
                 double Fi[4], Fv[4] = {0, 0, 0, 0}, Gv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-                rs(ph, q, qb, nn, Fi);
-                for (int eq = 0; eq < 4; ++eq) Fi[eq] = Fi[eq] * dS;
+                if (ed->inviscidBCType == WEAK_PRESCRIBED) {   /* Fi = FB (:1194-1198) */
+                    for (int eq = 0; eq < 4; ++eq) Fi[eq] = ed->FB[4 * i + eq];
+                } else {
+                    rs(ph, q, qb, nn, Fi);
+                    for (int eq = 0; eq < 4; ++eq) Fi[eq] = Fi[eq] * dS;
+                }
                 if (m->ns) {
                     double dq[8];
                     for (int d = 0; d < 2; ++d)

@@ -92,3 +92,14 @@ def test_ip_batch_mixed():
     cases = [C.ns_kh_case(4, 4), C.ns_kh_case(3, 6)]
     single, batched, _ = run_pair(cases, "mixed", 3)
     assert_same(single, batched)
+
+
+def test_boundary_condition_kinds_in_a_batch():
+    """Euler walls, pressure inlet/outlet and Dirichlet zones (prescribed and Riemann boundary
+    fluxes) in one batch, NS BR1: bit for bit the single-handle steps."""
+    from test_gpu_parity import _channel, _euler_wall_annulus
+    cases = [_channel(C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1", 1),
+             _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1"),
+             _euler_wall_annulus(True)]
+    single, batched, _ = run_pair(cases, "fp64", 3)
+    assert_same(single, batched)

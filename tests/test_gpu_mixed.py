@@ -33,6 +33,11 @@ def run(case, precision, steps):
     return dg
 
 
+def _parity_case(name):
+    from test_gpu_parity import CASES as P
+    return P[name]()
+
+
 CASES = {
     "euler_roe": (lambda: C.euler_vortex_case(8, 4), 3e-7),
     "euler_hll": (lambda: C.euler_vortex_case(6, 5, riemann="hll"), 3e-7),
@@ -41,6 +46,10 @@ CASES = {
     "ns_br1_lf_tgv": (lambda: C.ns_tgv_case(6, 6, Re=1600.0), 3e-7),
     "ns_ip_roe_kh": (lambda: C.ns_kh_case(6, 7, Re=1.0e4), 3e-7),
     "ns_br1_cylinder": (lambda: C.ns_cylinder_case(4, 16, N=3, Re=200.0), 3e-7),
+    # boundaries formed in FP64 from the FP32 form's traces (measured 1e-8 .. 6e-8)
+    "euler_supersonic_inlet_outlet": (lambda: _parity_case("euler_supersonic_inlet_outlet"), 3e-7),
+    "ns_sipg_pinlet_poutlet": (lambda: _parity_case("ns_sipg_pinlet_poutlet"), 3e-7),
+    "ns_br1_eulerwall_annulus": (lambda: _parity_case("ns_br1_eulerwall_annulus"), 3e-7),
 }
 
 

@@ -62,6 +62,39 @@ def _kh(ip_method):
     return c
 
 
+def _flow_ic(Mach, amp):
+    """A non-uniform flow along x whose velocity reverses (amp > 1) or crosses Mach 1 at some
+    boundary nodes, so every inflow/outflow and sub/supersonic branch of the boundaries runs."""
+    V = Mach * np.sqrt(1.4)
+
+    def ic(x, y):
+        u = V * (1.0 + amp * np.sin(2.1 * y + 0.3) * np.cos(1.3 * x))
+        v = V * amp * 0.5 * np.sin(1.7 * x - 0.4 * y)
+        return C._pack(1.0 + 0.05 * np.cos(x + 2 * y), u, v, 1.0 + 0.05 * np.sin(x - y))
+    return ic
+
+
+def _channel(inlet, outlet, Mach, amp, viscous="none", outflow=0, ip="sipg"):
+    """Skewed 6x3 channel, N = 4: Euler walls below and above, `inlet` on the left and
+    `outlet` on the right, both at the free-stream state of Mach."""
+    q = C.freestream_state(Mach)
+    zones = [C.Zone(C.BC_EULERWALL), C.Zone(outlet, q=tuple(q), outflow=outflow), C.Zone(C.BC_EULERWALL),
+             C.Zone(inlet, q=tuple(q))]
+    mesh = C.channel(6, 3, 3.0, 1.0, zones, skew=0.3)
+    ns = viscous != "none"
+    return C.Case("channel", mesh, np.full(mesh.K, 4, np.int32), Mach, 100.0 if ns else 0.0, 0.72, ns,
+                  riemann="roe", viscous=viscous, ip_method=ip, cfl=C.stable_cfl(4), ic=_flow_ic(Mach, amp))
+
+
+def _euler_wall_annulus(ns):
+    """The curved annulus with an Euler wall on the cylinder (curved prescribed-flux faces)."""
+    c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
+    c.mesh.zones[0] = C.Zone(C.BC_EULERWALL)
+    if not ns:
+        c.navier_stokes, c.Re, c.viscous = False, 0.0, "none"
+    return c
+
+
 CASES = {
     "euler_roe": lambda: C.euler_vortex_case(8, 4),
     "euler_hll": lambda: C.euler_vortex_case(6, 5, riemann="hll"),
@@ -76,6 +109,15 @@ CASES = {
     "ns_sipg_annulus": lambda: _annulus(viscous="ip"),
     "ns_nipg_kh": lambda: _kh("nipg"),
     "ns_iipg_kh": lambda: _kh("iipg"),
+    # Euler walls, pressure inlet and outlet (both outflow types), Dirichlet
+    "euler_pinlet_poutlet": lambda: _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5),
+    "euler_pinlet_reflective_outlet": lambda: _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, outflow=1),
+    "euler_supersonic_inlet_outlet": lambda: _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 1.6, 0.6),
+    "ns_br1_dirichlet_reflective_outlet": lambda: _channel(C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1", 1),
+    "ns_sipg_pinlet_poutlet": lambda: _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "ip"),
+    "ns_br1_supersonic_dirichlet": lambda: _channel(C.BC_DIRICHLET, C.BC_DIRICHLET, 1.6, 0.6, "br1"),
+    "euler_eulerwall_annulus": lambda: _euler_wall_annulus(False),
+    "ns_br1_eulerwall_annulus": lambda: _euler_wall_annulus(True),
 }
 
 

@@ -523,3 +523,66 @@ def test_probes_interpolate_polynomial_states_exactly():
         np.testing.assert_array_equal(got, qd[elem * P * P + i + P * j, v])
     with pytest.raises(RuntimeError):
         o.probes([case.mesh.K], [0.5], [0.5], ["rho"])
+
+
+# ------------------------------------------------------------------------------------------
+# Boundary conditions: BoundaryConditions/{EulerWall,Dirichlet,PressureOutlet,PressureInlet}.incf
+# ------------------------------------------------------------------------------------------
+def _channel_case(inlet, outlet, q, Mach, viscous="none", outflow=0, N=4):
+    zones = [C.Zone(C.BC_EULERWALL), C.Zone(outlet, q=tuple(q), outflow=outflow), C.Zone(C.BC_EULERWALL),
+             C.Zone(inlet, q=tuple(q))]
+    mesh = C.channel(6, 3, 3.0, 1.0, zones, skew=0.3)
+    ns = viscous != "none"
+    return C.Case("channel", mesh, np.full(mesh.K, N, np.int32), Mach, 100.0 if ns else 0.0, 0.72, ns,
+                  riemann="roe", viscous=viscous, ic=C.uniform(q))
+
+
+@pytest.mark.parametrize("Mach", [0.3, 1.6])
+@pytest.mark.parametrize("inlet,outlet,outflow", [(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0),
+                                                  (C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 1),
+                                                  (C.BC_DIRICHLET, C.BC_DIRICHLET, 0)])
+@pytest.mark.parametrize("viscous", ["none", "br1", "ip"])
+def test_inflow_outflow_and_euler_walls_preserve_a_uniform_channel_flow(inlet, outlet, outflow, Mach, viscous):
+    """A uniform flow along the walls of a channel with the inlet and outlet set to the same
+    state is steady: the Euler wall's wall state equals the trace (no normal momentum); the
+    subsonic pressure inlet's quadratic for a_R returns a_L when the total conditions match and
+    the reflective outlet's invariants return the trace; the partially non-reflective outlet
+    gives p_R = 2 p_inf - p_L = p_L; Dirichlet and the supersonic branches take the same state."""
+    q = C.freestream_state(Mach)
+    o, _ = oracle_for(_channel_case(inlet, outlet, q, Mach, viscous, outflow))
+    assert np.max(np.abs(o.time_derivative())) < 1e-12
+
+
+@pytest.mark.parametrize("viscous", ["none", "br1"])
+def test_euler_walls_conserve_mass_and_energy_and_momentum_follows_the_pressure_force(viscous):
+    """A box closed by Euler walls: the wall flux is the pressure flux (0, p n, 0) of the wall
+    state, whose pressure is the trace's.  So the discrete mass (and, without viscosity,
+    energy) is conserved to rounding, and d/dt of the momentum is minus the sum of the
+    surface monitors' pressure forces over the four walls (QuadMesh_Integrals.incf:174-204),
+    times 1/(sqrt(gamma) Mach), the scale of the inviscid flux in the reference's
+    dimensionless time (InviscidFluxes.f90:104-147)."""
+    N = 4
+    zones = [C.Zone(C.BC_EULERWALL) for _ in range(4)]
+    mesh = C.channel(4, 3, 2.0, 1.5, zones, skew=0.3)
+    ns = viscous != "none"
+    case = C.Case("box", mesh, np.full(mesh.K, N, np.int32), 0.3, 100.0 if ns else 0.0, 0.72, ns,
+                  riemann="roe", viscous=viscous)
+    o = O.Oracle(case)
+    X = o.coords()
+    x, y = X[:, 0], X[:, 1]
+    rho = 1.0 + 0.1 * np.sin(2 * x) * np.cos(3 * y)
+    u, v = 0.2 * np.cos(x + y), -0.1 * np.sin(2 * y - x)
+    p = 1.0 + 0.05 * np.cos(x * y)
+    o.set_state(_cons(rho, rho * u, rho * v, p))
+    qd = o.time_derivative().reshape(-1, 4)
+    _, jac = o.metrics()
+    s = O.spectral(N)
+    W = np.tile(np.outer(s["w"], s["w"]).T.reshape(-1), mesh.K) * jac
+    total, scale = W @ qd, W @ np.abs(qd)
+    assert abs(total[0]) <= 1e-12 * scale[0]
+    if not ns:
+        assert abs(total[3]) <= 1e-12 * scale[3]
+        force = sum(o.surface_integrals(mk)[3:5] for mk in (1, 2, 3, 4))
+        np.testing.assert_allclose(total[1:3], -force / (np.sqrt(GAMMA) * case.Mach), rtol=1e-11,
+                                   atol=1e-12 * scale[1:3].max())
+        assert np.all(np.abs(force) > 1e-4)
