@@ -340,8 +340,13 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     if (ph.viscous != 2) ph.ip_gradflux = 0;
     // the stage kernel compiled for the interior solver, unless a zone asks for another one
     ph.rs_kernel = opt->riemann;
-    for (const auto &Z : m.zones)
+    for (const auto &Z : m.zones) {
         if (Z.riemann >= 0 && Z.riemann != opt->riemann) ph.rs_kernel = -1;
+        if (Z.type == HG_BC_EULERWALL || Z.type == HG_BC_DIRICHLET || Z.type == HG_BC_PRESSUREOUTLET ||
+            Z.type == HG_BC_PRESSUREINLET)
+            ph.xbc = 1;
+    }
+    if (ph.xbc) ph.rs_kernel = -1;   // those kinds are compiled into the run-time-solver form
     if (ph.rs_kernel < 0) h->tile[0] = 1;   // the run-time-solver form exists for the wide tile only
 
     HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_batch_gradient(const StageArgs *__restr
     if (k < b.nitems) {
         const int4 it = uniform_item(items, sched, k);
         const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
-#define HG_CALL(PV, M) gradient_body<PV, TileOf<PV, TILE>::EPB, BR1, R, TR>(a, it.y, S.M)
+#define HG_CALL(PV, M) gradient_body<PV, TileOf<PV, TILE>::EPB, BR1, R, TR, true>(a, it.y, S.M)
         switch (it.z) {
             HG_BATCH_CASE(2, HG_CALL)
             HG_BATCH_CASE(3, HG_CALL)

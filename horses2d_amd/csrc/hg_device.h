@@ -35,6 +35,8 @@ struct PhysParams {
     double pref, Eref;
     int riemann, viscous, ip_gradflux, ns;
     int rs_kernel;  // Riemann solver compiled into the stage kernel (0..3), -1: chosen per face at run time
+    int xbc;        // a zone is an Euler wall, Dirichlet, pressure outlet or inlet: the kernels
+                    // compiled with those kinds (stage: the run-time-solver form) are launched
 };
 
 // Everything a stage or gradient kernel reads or writes.  Uniform P per handle.

@@ -107,7 +107,7 @@ __device__ __forceinline__ R qslot(const PhysParams &pp, double v, int c) {
 // BR1 solution flux at a face node: uStar n dS in both directions (DGViscous_BR1.f90:313-328;
 // boundary faces use the ghost state's uSB, DGViscousMethods.f90:786-917), stored as
 // out[0..3] = uStar dS and out[4..5] = (nx, ny); the lift forms (uStar dS) nx and (uStar dS) ny.
-template <typename R>
+template <typename R, bool XB>
 __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> &ph, int flags, int y, const R *qs,
                                               const R *qo, R nx, R ny, R dS, R *out) {
     const int kind = flags & 15;
@@ -122,7 +122,7 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
     } else {
         R uB[4];
         int neu;
-        ghost_state<R>(kind, flags, a.zone + 8 * y, a.ph, ph, qs, nx, ny, uB, us, neu);
+        ghost_state<R, XB>(kind, flags, a.zone + 8 * y, a.ph, ph, qs, nx, ny, uB, us, neu);
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c] = us[c] * dS;
@@ -347,7 +347,8 @@ struct GradSmem {
 };
 
 // One element tile [e0, e0+EPB) of the gradient; any workgroup size >= Tile<P, EPB>::TPB.
-template <int P, int EPB, bool BR1, typename R, typename TR>
+// XB: the Euler-wall, Dirichlet and pressure boundary kinds are compiled in.
+template <int P, int EPB, bool BR1, typename R, typename TR, bool XB = false>
 __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, GradSmem<P, EPB, BR1, R> &S) {
     using T = Tile<P, EPB>;
     constexpr int PP = T::PP, NN = T::NN, NF = T::NF;
@@ -394,7 +395,7 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         stored_trace<P, R, TR>(a, fe, ff, fk, qs);
     }
     __syncthreads();
-    if (BR1 && face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 6);
+    if (BR1 && face_active) br1_face_flux<R, XB>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 6);
     if (BR1) __syncthreads();
     R dq[8];
     if (node_active) {
@@ -413,10 +414,10 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
                                          (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT);
 }
 
-template <int P, int EPB, bool BR1, typename R, typename TR>
+template <int P, int EPB, bool BR1, typename R, typename TR, bool XB = false>
 __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_gradient(StageArgs a) {
     __shared__ GradSmem<P, EPB, BR1, R> S;
-    gradient_body<P, EPB, BR1, R, TR>(a, xcd_tile(blockIdx.x, gridDim.x) * EPB, S);
+    gradient_body<P, EPB, BR1, R, TR, XB>(a, xcd_tile(blockIdx.x, gridDim.x) * EPB, S);
 }
 
 // ------------------------------------------------------------------ the stage kernel
@@ -512,7 +513,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     __syncthreads();
 
     if (BR1) {
-        if (face_active) br1_face_flux<R>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 6);
+        if (face_active) br1_face_flux<R, (RS < 0)>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 6);
         __syncthreads();
     }
 
@@ -574,12 +575,13 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         R uSB[4] = {R(0), R(0), R(0), R(0)};
         int neumann = 0, prescribed = 0;
         if (kind > FK_PERIODIC) {
-            prescribed = ghost_state<R>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB, neumann);
+            prescribed = ghost_state<R, (RS < 0)>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB,
+                                                  neumann);
             const int zr = (fr.x >> FF_RS_SHIFT) & 15;
             if (zr) rs = zr - 1;
         }
         R Fi[4];
-        if (prescribed) {
+        if (RS < 0 && prescribed) {
             // WEAK_PRESCRIBED (DGSpatialDiscretizationMethods.f90:1194-1198): the inviscid
             // normal flux of the boundary's state
             R fp[4], gp[4];
@@ -937,6 +939,11 @@ hipError_t gradient_tile(bool mixed, const StageArgs &a, hipStream_t st) {
     constexpr int TPB = Tile<P, EPB>::TPB;
     const int nb = tile_blocks<P, EPB>(a.K);
     const bool br1 = a.ph.viscous == 1;
+    if (br1 && a.ph.xbc) {   // BR1 lifts the boundary's uSB: the kinds must be compiled in
+        if (mixed) hipLaunchKernelGGL((k_gradient<P, EPB, true, float, float, true>), dim3(nb), dim3(TPB), 0, st, a);
+        else hipLaunchKernelGGL((k_gradient<P, EPB, true, double, double, true>), dim3(nb), dim3(TPB), 0, st, a);
+        return hipGetLastError();
+    }
     if (mixed) {
         if (br1) hipLaunchKernelGGL((k_gradient<P, EPB, true, float, float>), dim3(nb), dim3(TPB), 0, st, a);
         else hipLaunchKernelGGL((k_gradient<P, EPB, false, float, float>), dim3(nb), dim3(TPB), 0, st, a);

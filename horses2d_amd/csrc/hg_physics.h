@@ -456,13 +456,15 @@ __device__ inline int ghost_state_ext(int kind, int flags, const double *zd, con
 // Ghost states of the non-periodic boundaries, computed from the own trace.
 // Farfield.incf:138-376 (uB = uSB; viscous type DIRICHLET, or NEUMANN at supersonic
 // outflow -- the reference leaves it unset) and ViscousWall.incf:158-254 in the compute
-// type; the other kinds through ghost_state_ext.  Returns 1 where uB is the state of a
-// prescribed flux instead of the Riemann solver's outer state.
-template <typename R>
+// type; the other kinds through ghost_state_ext, compiled in only where XB is set (their FP64
+// code would otherwise set the register budget of every FP32 kernel).  Returns 1 where uB is
+// the state of a prescribed flux instead of the Riemann solver's outer state.
+template <typename R, bool XB>
 __device__ inline int ghost_state(int kind, int flags, const double *zd, const PhysParams &pp, const Phys<R> &ph,
                                   const R *Qt, R nx, R ny, R *uB, R *uSB, int &neumann) {
     neumann = 0;
-    if (kind >= FK_EULERWALL) {
+    if constexpr (XB) {
+      if (kind >= FK_EULERWALL) {
         const double es = Phys<R>::shift ? pp.Eref : 0.0;
         const double Qd[4] = {(double)Qt[0], (double)Qt[1], (double)Qt[2], (double)Qt[3] + es};
         double uBd[4], uSd[4];
@@ -472,6 +474,7 @@ __device__ inline int ghost_state(int kind, int flags, const double *zd, const P
         uB[3] = (R)(uBd[3] - es);
         uSB[3] = (R)(uSd[3] - es);
         return pres;
+      }
     }
     if (kind == FK_FARFIELD) {
         R q[4] = {(R)zd[0], (R)zd[1], (R)zd[2], (R)(Phys<R>::shift ? zd[3] - ph.Eref_d : zd[3])};

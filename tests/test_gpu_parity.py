@@ -147,9 +147,13 @@ def test_rk3_steps_match_oracle(name):
 
 @pytest.mark.parametrize("name", list(CASES))
 def test_workgroup_tiles_are_bit_identical(name):
-    """The narrow and wide workgroup tiles evaluate the same expressions per node and face."""
+    """The narrow and wide workgroup tiles evaluate the same expressions per node and face.
+    (Euler-wall, Dirichlet and pressure zones run the stage kernel's run-time-solver form,
+    which has the wide tile only: there the gradient kernel's tiles are compared.)"""
     from horses2d_amd import gpu
     case = CASES[name]()
+    extended = any(z.kind in (C.BC_DIRICHLET, C.BC_EULERWALL, C.BC_PRESSUREOUTLET, C.BC_PRESSUREINLET)
+                   for z in case.mesh.zones)
     runs = []
     for tile in (0, 1):
         dg = gpu.DGSEM(case, "fp64")
@@ -159,7 +163,7 @@ def test_workgroup_tiles_are_bit_identical(name):
         for it in range(3):
             dg.rk_step(it * dt, dt)
         runs.append((dg.state(), dg.residuals()))
-        assert dg.tiles() == (tile, tile)
+        assert dg.tiles() == ((1 if extended else tile), tile)
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
 
