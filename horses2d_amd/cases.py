@@ -39,6 +39,9 @@ RIEMANN = {"roe": 0, "hll": 1, "exact": 2, "lf": 3}
 VISCOUS = {"none": 0, "br1": 1, "ip": 2}
 IP_METHOD = {"sipg": 0, "nipg": 1, "iipg": 2}
 SCHEME = {"euler": 0, "rk3": 1, "rk5": 2}
+# "Artificial dissipation (0/1)" with "type" Physical and "indicator" Jumps-based
+# (Setup.f90:333-349, DGArtificialDissipation.f90:100-137)
+ARTIFICIAL_DISSIPATION = {"none": 0, "physical": 1}
 
 
 @dataclasses.dataclass
@@ -101,6 +104,8 @@ class Case:
     steps: int = 1
     scheme: str = "rk3"
     ic: Optional[Callable] = None        # ic(x, y) -> (..., 4) dimensionless state
+    artificial_dissipation: str = "none"  # ARTIFICIAL_DISSIPATION key
+    ad_intensity: float = 1.0            # "Artificial dissipation intensity" (Ceps)
 
     def initial_state(self, coords: np.ndarray) -> np.ndarray:
         """Evaluate the IC at node coordinates ``coords`` (nnodes, 2) -> flat Q (4*nnodes)."""

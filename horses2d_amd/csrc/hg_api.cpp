@@ -81,6 +81,8 @@ hg::StageArgs base_args(hg_handle *h) {
     a.dlbw = h->dlbw;
     a.resid_part = h->resid_part;
     a.ph = h->ph;
+    a.mu_a = h->ph.ad ? h->mu_a : nullptr;
+    a.ip_epi = h->ip_fused ? 1 : 0;
     return a;
 }
 
@@ -206,6 +208,13 @@ StepCoefs step_coefs(int scheme) {
 int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode,
               const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0) {
     if (autotune(h)) return 1;
+    // artificial viscosity of the stage's state, before anything evaluates a viscous flux
+    // (DGSpatial_QDotVolumeLoop computes it per element first, :255)
+    if (h->ph.ad) {
+        hg::StageArgs v = base_args(h);
+        v.tr_in = h->tr[h->cur];
+        HG_CHECK(hg::launch_art_visc(h->P, h->mixed, v, h->evol, h->w, h->mu_a, h->stream));
+    }
     // BR1: the viscous face-flux traces of the current state, every stage; interior penalty:
     // only when the previous stage's epilogue did not write them (after an upload)
     if (h->ph.ns && (!h->ip_fused || !h->vtr_valid)) {
@@ -346,6 +355,12 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
             Z.type == HG_BC_PRESSUREINLET)
             ph.xbc = 1;
     }
+    if (opt->artificial_dissipation != HG_AD_NONE && opt->artificial_dissipation != HG_AD_PHYSICAL)
+        return set_err("hg_create: unknown artificial dissipation type");
+    // DGArtificialDissipation is a Navier-Stokes module
+    ph.ad = (ph.ns && opt->artificial_dissipation == HG_AD_PHYSICAL) ? 1 : 0;
+    ph.ad_ceps = opt->ad_intensity;
+    if (ph.ad) ph.xbc = 1;
     if (ph.xbc) ph.rs_kernel = -1;   // those kinds are compiled into the run-time-solver form
     if (ph.rs_kernel < 0) h->tile[0] = 1;   // the run-time-solver form exists for the wide tile only
 
@@ -472,7 +487,9 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(h->alloc_raw(&h->tr[1], tsz));
     if (ph.ns) {
         const size_t rs = h->mixed ? sizeof(float) : sizeof(double);
-        h->ip_fused = ph.viscous == HG_VISCOUS_IP;
+        // interior penalty forms the next stage's viscous traces in the stage epilogue, unless
+        // they depend on the next stage's artificial viscosity (all elements' jumps first)
+        h->ip_fused = ph.viscous == HG_VISCOUS_IP && !ph.ad;
         for (int b = 0; b < (h->ip_fused ? 2 : 1); ++b)
             HG_CHECK(h->alloc_raw(&h->dtr[b], (size_t)K * 4 * P * hg::VT * rs));
     }
@@ -518,6 +535,22 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(upload(h.get(), &h->dlbw, dlbw));
     HG_CHECK(upload(h.get(), &h->w, wv));
     HG_CHECK(upload(h.get(), &h->jac64, jac));
+    if (ph.ad) {
+        // element volumes (BilinearForm_F(jac, w, w), QuadMesh_Construct.incf:266)
+        std::vector<double> vol(K);
+        for (int e = 0; e < K; ++e) {
+            double v = 0.0;
+            for (int j = 0; j < P; ++j) {
+                double acc = 0.0;
+                for (int i = 0; i < P; ++i) acc = acc + wv[i] * jac[(size_t)e * PP + i + P * j];
+                v = v + acc * wv[j];
+            }
+            vol[e] = v;
+        }
+        HG_CHECK(upload(h.get(), &h->evol, vol));
+        HG_CHECK(h->alloc(&h->mu_a, K));
+        HG_CHECK(hipMemset(h->mu_a, 0, K * sizeof(double)));
+    }
     HG_CHECK(upload(h.get(), &h->dx, dxe));
     HG_CHECK(h->alloc(&h->umax, K));
     HG_CHECK(h->alloc(&h->amax, K));
@@ -796,6 +829,7 @@ int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
         } else {
             hg::StageArgs g = base_args(h);
             g.tr_in = h->tr[h->cur];
+            if (h->ph.ad) HG_CHECK(hg::launch_art_visc(h->P, h->mixed, g, h->evol, h->w, h->mu_a, h->stream));
             HG_CHECK(hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream));
         }
         vtr = hg::vtr_current(h);
@@ -803,7 +837,7 @@ int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
     if (n > 0)
         HG_CHECK(hg::launch_surface_integrals(h->mixed, h->bslots + first, n, h->P, h->tr[h->cur], vtr, h->face,
                                               h->egeo, h->w, h->ph.gm1, h->mixed ? h->ph.Eref : 0.0,
-                                              h->monitor_out, h->stream));
+                                              h->ph.ad ? h->mu_a : nullptr, h->ph.mu, h->monitor_out, h->stream));
     else
         HG_CHECK(hipMemsetAsync(h->monitor_out, 0, 9 * sizeof(double), h->stream));
     HG_CHECK(hipMemcpyAsync(out, h->monitor_out, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));

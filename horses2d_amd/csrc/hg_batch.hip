@@ -186,6 +186,8 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     for (int c = 0; c < ncases; ++c) {
         hg_handle *h = cases[c];
         if (!h) return hg::handle_error("hg_batch_create: null case handle");
+        if (h->ph.ad)   // the batched step has no per-case indicator launch
+            return hg::handle_error("hg_batch_create: cases with artificial dissipation are stepped on their own");
         if (h->ph.ns != h0->ph.ns || h->ph.viscous != h0->ph.viscous || h->mixed != h0->mixed ||
             h->opt.scheme != h0->opt.scheme)
             return hg::handle_error("hg_batch_create: the cases of a batch must share equations, viscous method, "

@@ -35,8 +35,11 @@ struct PhysParams {
     double pref, Eref;
     int riemann, viscous, ip_gradflux, ns;
     int rs_kernel;  // Riemann solver compiled into the stage kernel (0..3), -1: chosen per face at run time
-    int xbc;        // a zone is an Euler wall, Dirichlet, pressure outlet or inlet: the kernels
-                    // compiled with those kinds (stage: the run-time-solver form) are launched
+    int xbc;        // a zone is an Euler wall, Dirichlet, pressure outlet or inlet, or artificial
+                    // dissipation is on: the kernels compiled with those (stage: the run-time-
+                    // solver form) are launched
+    int ad;         // artificial dissipation: "Physical" type, jumps-based indicator
+    double ad_ceps; // its intensity
 };
 
 // Everything a stage or gradient kernel reads or writes.  Uniform P per handle.
@@ -67,6 +70,8 @@ struct StageArgs {
     double *resid_part;         // per workgroup max|QDot| per equation (4 doubles per workgroup)
     int do_resid;
     PhysParams ph;
+    const double *mu_a;         // artificial viscosity per element (null: none)
+    int ip_epi;                 // interior penalty: the stage epilogue forms the next viscous traces
 };
 
 // The batched form: one launch per RK stage over a work list of element tiles of many cases.
@@ -133,7 +138,10 @@ hipError_t launch_volume_integrals(int P, const double *Q, const double *jac, co
 // viscous force (2); vtr null for Euler.  Eref: the FP32 form's energy-slot shift.
 hipError_t launch_surface_integrals(bool mixed, const int *slots, int nslots, int P, const void *tr, const void *vtr,
                                     const int4 *face, const double *egeo, const double *w, double gm1, double Eref,
-                                    double *out, hipStream_t s);
+                                    const double *mu_a, double mu, double *out, hipStream_t s);
+// Jumps-based artificial viscosity per element from the stage's Q traces (and boundary states).
+hipError_t launch_art_visc(int P, bool mixed, const StageArgs &a, const double *evol, const double *w, double *mu_a,
+                           hipStream_t st);
 // Probe values (Probe_Update): n probes, per probe an int2 (element, variable) and 2P
 // interpolant values; variables 4..7 read QDot, the others Q.
 hipError_t launch_probes(const double *Q, const double *QDot, const int2 *ev, const double *l, int n, int P,
@@ -150,6 +158,8 @@ template <int P> int tile_threads_p(int tile);
 template <int P> int tile_elems_p(int tile);
 template <int P> hipError_t gradient_p(int tile, bool mixed, const StageArgs &a, hipStream_t s);
 template <int P> hipError_t prolong_p(bool mixed, const StageArgs &a, hipStream_t s);
+template <int P>
+hipError_t art_visc_p(bool mixed, const StageArgs &a, const double *evol, const double *w, double *mu_a, hipStream_t s);
 template <int P> hipError_t dt_elem_p(const double *Q, int K, double gamma, double gm1, double *umax, double *amax,
                                       hipStream_t s);
 template <int P> int cons_blocks_p(int K);

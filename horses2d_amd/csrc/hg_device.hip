@@ -109,7 +109,8 @@ __global__ void k_cons_final(const double *partial, int nb, double *out) {
 template <typename TR>
 __global__ __launch_bounds__(1024) void k_surface_integrals(const int *slots, int nslots, int P, const TR *tr,
                                                             const TR *vtr, const int4 *face, const double *egeo,
-                                                            const double *w, double gm1, double Eref, double *out) {
+                                                            const double *w, double gm1, double Eref,
+                                                            const double *mu_a, double mu, double *out) {
     __shared__ double s[16][9];
     const int t = threadIdx.x;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -133,7 +134,9 @@ __global__ __launch_bounds__(1024) void k_surface_integrals(const int *slots, in
         acc[6] += wq * un * dS / r;
         if (vtr) {
             const TR *v = vtr + ((int64_t)slot * P + k) * VT;
-            const double txx = v[0], txy = v[1], tyy = v[3];
+            // the traces carry mu + mu_a of the (boundary) face; the monitor's stress has mu
+            const double sc = mu_a ? mu / (mu + mu_a[slot >> 2]) : 1.0;
+            const double txx = v[0] * sc, txy = v[1] * sc, tyy = v[3] * sc;
             acc[7] += wq * (txx * nx + txy * ny) * dS;
             acc[8] += wq * (txy * nx + tyy * ny) * dS;
         }
@@ -220,6 +223,13 @@ bool supported_order(int N) { return N >= 1 && N <= 10; }
     case 11: return CALL(11);            \
     default: return hipErrorInvalidValue; \
     }
+
+hipError_t launch_art_visc(int P, bool mixed, const StageArgs &a, const double *evol, const double *w, double *mu_a,
+                           hipStream_t st) {
+#define HG_CALL(PV) art_visc_p<PV>(mixed, a, evol, w, mu_a, st)
+    HG_DISPATCH_P(P, HG_CALL)
+#undef HG_CALL
+}
 
 hipError_t launch_prolong(int P, bool mixed, const StageArgs &a, hipStream_t st) {
 #define HG_CALL(PV) prolong_p<PV>(mixed, a, st)
@@ -362,13 +372,13 @@ hipError_t launch_volume_integrals(int P, const double *Q, const double *jac, co
 
 hipError_t launch_surface_integrals(bool mixed, const int *slots, int nslots, int P, const void *tr, const void *vtr,
                                     const int4 *face, const double *egeo, const double *w, double gm1, double Eref,
-                                    double *out, hipStream_t st) {
+                                    const double *mu_a, double mu, double *out, hipStream_t st) {
     if (mixed)
         hipLaunchKernelGGL((k_surface_integrals<float>), dim3(1), dim3(1024), 0, st, slots, nslots, P,
-                           (const float *)tr, (const float *)vtr, face, egeo, w, gm1, Eref, out);
+                           (const float *)tr, (const float *)vtr, face, egeo, w, gm1, Eref, mu_a, mu, out);
     else
         hipLaunchKernelGGL((k_surface_integrals<double>), dim3(1), dim3(1024), 0, st, slots, nslots, P,
-                           (const double *)tr, (const double *)vtr, face, egeo, w, gm1, Eref, out);
+                           (const double *)tr, (const double *)vtr, face, egeo, w, gm1, Eref, mu_a, mu, out);
     return hipGetLastError();
 }
 

@@ -240,6 +240,16 @@ __device__ __forceinline__ void load_metric(const StageArgs &a, int e, int64_t g
     ijac = ld<R>(a.metric, 4 * st + i);
 }
 
+// Viscosity of a face: mu + edge % mu_a, the edge's artificial viscosity being the mean of its
+// elements' (ComputeEdgeViscosity, DGArtificialDissipation.f90:340-363; boundary and periodic
+// faces: the own element's).  Without artificial dissipation, mu.
+template <typename R>
+__device__ __forceinline__ R face_mu(const StageArgs &a, const Phys<R> &ph, int fe, int4 fr) {
+    if (!a.mu_a) return ph.mu;
+    const double ma = (fr.x & 15) <= FK_INT_R ? (a.mu_a[fe] + a.mu_a[fr.y >> 2]) / 2.0 : a.mu_a[fe];
+    return (R)(a.ph.mu + ma);
+}
+
 // Face-node geometry and the neighbour's Q trace (periodic and interior faces).
 template <int P, typename R, typename TR>
 __device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, int fk, int4 &fr, R &nx, R &ny, R &dS,
@@ -292,7 +302,7 @@ __device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe,
 template <int P, int DS, typename R, typename TR>
 __device__ __forceinline__ void viscous_face_trace(const StageArgs &a, const Phys<R> &ph, int flags, const R *qs,
                                                    const R *sDQe, const double *sLb, const R *lbR, int ff, int fk,
-                                                   TR *o) {
+                                                   TR *o, R muf) {
     R ds[8], fv[4], gv[4];
     if constexpr (sizeof(R) == 4) {
         face_trace_r<P, DS, 8>(sDQe, lbR, ff, fk, ds);
@@ -304,8 +314,8 @@ __device__ __forceinline__ void viscous_face_trace(const StageArgs &a, const Phy
     }
     if (is_adiabatic(flags)) adiabatic_flux(ph, qs, ds, fv, gv);
     else viscous_flux(ph, qs, ds, fv, gv);
-    o[0] = (TR)(ph.mu * fv[1]); o[1] = (TR)(ph.mu * fv[2]); o[2] = (TR)(ph.mu * fv[3]);
-    o[3] = (TR)(ph.mu * gv[2]); o[4] = (TR)(ph.mu * gv[3]); o[5] = TR(0);
+    o[0] = (TR)(muf * fv[1]); o[1] = (TR)(muf * fv[2]); o[2] = (TR)(muf * fv[3]);
+    o[3] = (TR)(muf * gv[2]); o[4] = (TR)(muf * gv[3]); o[5] = TR(0);
 }
 
 // ------------------------------------------------------------------ gradient trace kernel
@@ -409,9 +419,11 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         for (int c = 0; c < 8; ++c) sDQ[t * DS + c] = dq[c];
     }
     __syncthreads();
-    if (face_active)
+    if (face_active) {
+        const R muf = XB ? face_mu<R>(a, ph, fe, fr) : ph.mu;
         viscous_face_trace<P, DS, R, TR>(a, ph, fr.x, qs, sDQ + fs * PP * DS, sLb, sLbw + 2 * P, ff, fk,
-                                         (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT);
+                                         (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT, muf);
+    }
 }
 
 template <int P, int EPB, bool BR1, typename R, typename TR, bool XB = false>
@@ -531,6 +543,14 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         if ((fr.x & 15) <= FK_PERIODIC) {
 #pragma unroll
             for (int c = 0; c < (NS ? VT : 1); ++c) vn[c] = (R)((const TR *)a.dtr)[nslot * VT + c];
+            if (RS < 0 && a.mu_a && (fr.x & 15) == FK_PERIODIC) {
+                // a periodic face is a boundary edge of each side, whose viscosity (its own
+                // element's mu_a) both sides' fluxes take; the partner stored its trace with its own
+                const double ms = a.ph.mu + a.mu_a[fe], mp = a.ph.mu + a.mu_a[fr.y >> 2];
+                const R sc = (R)(ms / mp);
+#pragma unroll
+                for (int c = 0; c < (NS ? VT : 1); ++c) vn[c] = vn[c] * sc;
+            }
         }
         if (IP) invh = (R)a.einvh[fr.w];
     }
@@ -552,8 +572,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         if (NS) {
             R fv[4], gv[4];
             viscous_flux(ph, q, dq, fv, gv);
+            // e % mu_a (DGViscous_BR1.f90:307, DGViscous_IP.f90:84)
+            const R mue = (RS < 0 && a.mu_a) ? (R)(a.ph.mu + a.mu_a[e]) : ph.mu;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) { f[c] = (f[c] - ph.mu * fv[c]) - R(0); gg[c] = (gg[c] - ph.mu * gv[c]) - R(0); }
+            for (int c = 0; c < 4; ++c) { f[c] = (f[c] - mue * fv[c]) - R(0); gg[c] = (gg[c] - mue * gv[c]) - R(0); }
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -597,7 +619,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         } else {
             R Fv[4] = {R(0), R(0), R(0), R(0)};
             R G8[8] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-            const R pen = IP ? ph.sigma0 * ph.mu * (R)(P - 1) * (R)(P - 1) * invh : R(0);
+            const R muf = RS < 0 ? face_mu<R>(a, ph, fe, fr) : ph.mu;
+            const R pen = IP ? ph.sigma0 * muf * (R)(P - 1) * (R)(P - 1) * invh : R(0);
             if (kind <= FK_PERIODIC) {
                 const R *uL = (kind == FK_INT_R) ? qo : qs, *uR = (kind == FK_INT_R) ? qs : qo;
                 const R *vL = (kind == FK_INT_R) ? vn : vs, *vR = (kind == FK_INT_R) ? vs : vn;
@@ -622,7 +645,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
                     for (int c = 0; c < 4; ++c) { fgr[c] = (uL[c] - uR[c]) * nx; fgr[4 + c] = (uL[c] - uR[c]) * ny; }
                     const R *own = (kind == FK_INT_R) ? uR : uL;
                     viscous_flux(ph, own, fgr, f4, g4);
-                    const R cf = (kind == FK_INT_R ? -R(0.5) : R(0.5)) * ph.ip_eps * ph.mu;
+                    const R cf = (kind == FK_INT_R ? -R(0.5) : R(0.5)) * ph.ip_eps * muf;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
                 }
@@ -647,7 +670,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
                     for (int c = 0; c < 4; ++c) { fgr[c] = (qs[c] - uSB[c]) * nx; fgr[4 + c] = (qs[c] - uSB[c]) * ny; }
                     if (adiab) adiabatic_flux(ph, qs, fgr, f4, g4);
                     else viscous_flux(ph, qs, fgr, f4, g4);
-                    const R cf = ph.ip_eps * ph.mu;
+                    const R cf = ph.ip_eps * muf;
 #pragma unroll
                     for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
                 }
@@ -802,6 +825,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             for (int c = 0; c < 4; ++c) qn[c] = (R)(TR)tq[c];
         }
         if constexpr (IP) {
+          if (RS >= 0 || a.ip_epi) {   // uniform: the launch's flag
             // the updated state in the compute type (FP32: a fresh copy, the node phase
             // overwrote the first one), the broken gradient, its traces and this side's
             // viscous flux: what k_gradient evaluates for the updated state
@@ -833,7 +857,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             __syncthreads();
             if (face_active)
                 viscous_face_trace<P, 8, R, TR>(a, ph, fr.x, qn, sDQ + fs * PP * 8, sLb, S.sLbR, ff, fk,
-                                                (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT);
+                                                (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT, ph.mu);
+          }
         }
     }
 }
@@ -939,9 +964,16 @@ hipError_t gradient_tile(bool mixed, const StageArgs &a, hipStream_t st) {
     constexpr int TPB = Tile<P, EPB>::TPB;
     const int nb = tile_blocks<P, EPB>(a.K);
     const bool br1 = a.ph.viscous == 1;
-    if (br1 && a.ph.xbc) {   // BR1 lifts the boundary's uSB: the kinds must be compiled in
-        if (mixed) hipLaunchKernelGGL((k_gradient<P, EPB, true, float, float, true>), dim3(nb), dim3(TPB), 0, st, a);
-        else hipLaunchKernelGGL((k_gradient<P, EPB, true, double, double, true>), dim3(nb), dim3(TPB), 0, st, a);
+    // BR1 lifts the boundary's uSB (the boundary kinds must be compiled in); with artificial
+    // dissipation every viscous trace takes the face's mu + mu_a
+    if (a.ph.xbc && (br1 || a.ph.ad)) {
+        if (mixed) {
+            if (br1) hipLaunchKernelGGL((k_gradient<P, EPB, true, float, float, true>), dim3(nb), dim3(TPB), 0, st, a);
+            else hipLaunchKernelGGL((k_gradient<P, EPB, false, float, float, true>), dim3(nb), dim3(TPB), 0, st, a);
+        } else {
+            if (br1) hipLaunchKernelGGL((k_gradient<P, EPB, true, double, double, true>), dim3(nb), dim3(TPB), 0, st, a);
+            else hipLaunchKernelGGL((k_gradient<P, EPB, false, double, double, true>), dim3(nb), dim3(TPB), 0, st, a);
+        }
         return hipGetLastError();
     }
     if (mixed) {
@@ -1013,6 +1045,75 @@ hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, i
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ artificial viscosity
+// Jumps-based artificial viscosity of each element (JumpsBasedElementViscosity,
+// DGArtificialDissipation.f90:277-338), one lane per element, from the stage's Q traces: per
+// face sum_k w_k (2 (qL - qR) / (qL + qR))^2 dS for rho and rho E (Edge_ComputeJumps and its
+// variants, QuadElement_EdgeProcedures.incf:1348-1619: curved interior faces per-node dS, the
+// others dS of the edge's first node; boundary faces the boundary state uSB), normalised by the
+// four faces' area, through the smooth switch, times Ceps sqrt(Volume)/(N+1).  FP64 in both
+// forms (the FP32 form's traces, energy shifted back).
+template <int P, typename TR>
+__global__ __launch_bounds__(256) void k_art_visc(StageArgs a, const double *evol, const double *w, double *mu_a) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= a.K) return;
+    const Phys<double> ph(a.ph);
+    const double es = sizeof(TR) == 4 ? a.ph.Eref : 0.0;
+    double gr = 0.0, ge = 0.0, area = 0.0;
+    for (int f = 0; f < 4; ++f) {
+        const int4 fr = a.face[e * 4 + f];
+        const int kind = fr.x & 15;
+        const bool curved = fr.x & FF_CURVED, selfrev = fr.x & FF_SELFREV, rev = fr.x & FF_REV;
+        double jr = 0.0, je = 0.0, ar = 0.0;
+        for (int k = 0; k < P; ++k) {
+            const TR *o = (const TR *)a.tr_in + ((int64_t)(e * 4 + f) * P + k) * 4;
+            const double qs[4] = {(double)o[0], (double)o[1], (double)o[2], (double)o[3] + es};
+            const int gi = curved ? (selfrev ? P - 1 - k : k) : 0;
+            const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
+            double qo[4], dS = geo[2];
+            ar = ar + w[k] * geo[2];
+            if (kind <= FK_PERIODIC) {
+                const TR *nb = (const TR *)a.tr_in + ((int64_t)fr.y * P + (rev ? P - 1 - k : k)) * 4;
+                qo[0] = (double)nb[0]; qo[1] = (double)nb[1]; qo[2] = (double)nb[2]; qo[3] = (double)nb[3] + es;
+                if (kind == FK_PERIODIC) dS = a.egeo[3 * (int64_t)fr.z + 2];
+            } else {
+                double uB[4];
+                int neu;
+                ghost_state<double, true>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, geo[0], geo[1], uB, qo, neu);
+                dS = a.egeo[3 * (int64_t)fr.z + 2];
+            }
+            const double nr = 2.0 * (qs[0] - qo[0]) / (qs[0] + qo[0]);
+            const double ne = 2.0 * (qs[3] - qo[3]) / (qs[3] + qo[3]);
+            jr = jr + w[k] * nr * nr * dS;
+            je = je + w[k] * ne * ne * dS;
+        }
+        gr = gr + jr;
+        ge = ge + je;
+        area = area + ar;
+    }
+    double gk = (gr > ge ? gr : ge) / area;
+    const double kw = 5.0, s0 = log10(1.0 / pow((double)P, 4.0)) + 1.0;
+    double G;
+    if (!(gk > 1.0e-8)) {
+        G = 0.0;
+    } else {
+        gk = log10(gk);
+        if (gk < s0 - kw) G = 0.0;
+        else if (gk < s0 + kw) G = 0.5 * sin(0.5 * 3.141592653589793 * (gk - s0) / kw) + 0.5;
+        else G = 1.0;
+    }
+    mu_a[e] = a.ph.ad_ceps * (sqrt(evol[e]) / (double)P) * G;
+}
+
+template <int P>
+hipError_t art_visc_p(bool mixed, const StageArgs &a, const double *evol, const double *w, double *mu_a,
+                      hipStream_t st) {
+    const int nb = (a.K + 255) / 256;
+    if (mixed) hipLaunchKernelGGL((k_art_visc<P, float>), dim3(nb), dim3(256), 0, st, a, evol, w, mu_a);
+    else hipLaunchKernelGGL((k_art_visc<P, double>), dim3(nb), dim3(256), 0, st, a, evol, w, mu_a);
+    return hipGetLastError();
+}
+
 #define HG_INSTANTIATE_STAGE(PV, PHYS) \
     template hipError_t stage_phys_p<PV, PHYS>(int, bool, const StageArgs &, hipStream_t);
 
@@ -1022,6 +1123,8 @@ hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, i
     template int tile_elems_p<PV>(int);                                                                       \
     template hipError_t gradient_p<PV>(int, bool, const StageArgs &, hipStream_t);                            \
     template hipError_t prolong_p<PV>(bool, const StageArgs &, hipStream_t);                                  \
+    template hipError_t art_visc_p<PV>(bool, const StageArgs &, const double *, const double *, double *,     \
+                                       hipStream_t);                                                          \
     template hipError_t dt_elem_p<PV>(const double *, int, double, double, double *, double *, hipStream_t);  \
     template int cons_blocks_p<PV>(int);                                                                      \
     template hipError_t cons_partial_p<PV>(const double *, const double *, const double *, int, int, double,  \

@@ -16,6 +16,7 @@ module horses_gpu_bindings
                                             HG_BC_VISCOUSWALL = 4, HG_BC_FARFIELD = 5, &
                                             HG_BC_PRESSUREOUTLET = 6, HG_BC_PRESSUREINLET = 7
    integer(c_int32_t), parameter, public :: HG_OUTFLOW_PARTIALLY_NON_REFLECTIVE = 0, HG_OUTFLOW_REFLECTIVE = 1
+   integer(c_int32_t), parameter, public :: HG_AD_NONE = 0, HG_AD_PHYSICAL = 1
    integer(c_int32_t), parameter, public :: HG_PROBE_RHO = 0, HG_PROBE_RHOU = 1, HG_PROBE_RHOV = 2, &
                                             HG_PROBE_RHOE = 3, HG_PROBE_RHOT = 4, HG_PROBE_RHOUT = 5, &
                                             HG_PROBE_RHOVT = 6, HG_PROBE_RHOET = 7, HG_PROBE_U = 8, &
@@ -61,6 +62,8 @@ module horses_gpu_bindings
       real(c_double)     :: sigma0
       integer(c_int32_t) :: precision
       integer(c_int32_t) :: scheme
+      integer(c_int32_t) :: artificial_dissipation
+      real(c_double)     :: ad_intensity
    end type hg_options
 
    public :: hg_create, hg_destroy, hg_ndof, hg_get_node_coords, hg_upload_state, hg_download_state

@@ -60,7 +60,8 @@ program horses_gpu_driver
    mesh%bc_type = c_loc(bc_type) ; mesh%bc_int = c_loc(bc_int) ; mesh%bc_real = c_loc(bc_real)
    mesh%bc_riemann = c_loc(bc_rs)
    phys = hg_physics(MACH, 0.0d0, 0.72d0)
-   opt = hg_options(0, HG_RIEMANN_ROE, HG_VISCOUS_NONE, HG_IP_SIPG, 1.0d0, HG_PRECISION_FP64, HG_SCHEME_RK3)
+   opt = hg_options(0, HG_RIEMANN_ROE, HG_VISCOUS_NONE, HG_IP_SIPG, 1.0d0, HG_PRECISION_FP64, HG_SCHEME_RK3, &
+                    HG_AD_NONE, 1.0d0)
    call hgf_check(hg_create(mesh, phys, opt, h), "hg_create")
 !
 !  Initial condition on the host (the libproblemfile hook): Shu's isentropic vortex

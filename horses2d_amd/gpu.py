@@ -45,7 +45,8 @@ class _Physics(ctypes.Structure):
 class _Options(ctypes.Structure):
     _fields_ = [("navier_stokes", ctypes.c_int32), ("riemann", ctypes.c_int32), ("viscous", ctypes.c_int32),
                 ("ip_method", ctypes.c_int32), ("sigma0", ctypes.c_double), ("precision", ctypes.c_int32),
-                ("scheme", ctypes.c_int32)]
+                ("scheme", ctypes.c_int32), ("artificial_dissipation", ctypes.c_int32),
+                ("ad_intensity", ctypes.c_double)]
 
 
 class _BatchOptions(ctypes.Structure):
@@ -201,7 +202,8 @@ class DGSEM:
         ph = _Physics(case.Mach, case.Re, case.Pr)
         op = _Options(int(case.navier_stokes), C.RIEMANN[case.riemann],
                       C.VISCOUS[case.viscous] if case.navier_stokes else 0, C.IP_METHOD[case.ip_method],
-                      case.sigma0, PRECISION[precision], C.SCHEME[case.scheme])
+                      case.sigma0, PRECISION[precision], C.SCHEME[case.scheme],
+                      C.ARTIFICIAL_DISSIPATION[case.artificial_dissipation], case.ad_intensity)
         h = ctypes.c_void_p()
         _check(L.hg_create(ctypes.byref(md), ctypes.byref(ph), ctypes.byref(op), ctypes.byref(h)), "hg_create")
         self._h = h

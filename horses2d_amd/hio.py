@@ -271,6 +271,17 @@ def case_from_hiocase(case_path: str, mesh_path: Optional[str] = None, ic=None, 
     scheme = rv("Integration scheme") or "Williamson RK3"
     if scheme not in SCHEME_NAMES:
         raise ValueError(f"Integration scheme '{scheme}' is not supported")
+    # artificial dissipation (Setup.f90:333-349; DGArtificialDissipation.f90:100-137): type
+    # "Physical" (default) with the "Jumps-based" indicator (default); another type name is the
+    # base class, which adds nothing; "Laplacian" and "Residuals-based" are not provided
+    ad_on = _int(rv("Artificial dissipation (0/1)")) == 1
+    ad_type = rv("Artificial dissipation type") or "Physical"
+    ad_ind = rv("Artificial dissipation indicator") or "Jumps-based"
+    ad_ceps = _real(rv("Artificial dissipation intensity"))
+    if ad_on and ad_type == "Laplacian":
+        raise ValueError("the Laplacian artificial dissipation is not supported")
+    if ad_on and ad_type == "Physical" and ad_ind != "Jumps-based":
+        raise ValueError(f"artificial dissipation indicator '{ad_ind}' is not supported")
     cfl = _real(rv("CFL Number"))
     dt = _real(rv("Time step"))
     iters = _int(rv("Number of iterations"))
@@ -287,7 +298,9 @@ def case_from_hiocase(case_path: str, mesh_path: Optional[str] = None, ic=None, 
                   ip_method=ipm if ipm in C.IP_METHOD else "sipg",
                   sigma0=1.0 if sigma0 is None else sigma0, cfl=0.1 if cfl is None else cfl,
                   dt_max=0.01 if dt is None else dt, steps=steps or iters or 1,
-                  scheme=SCHEME_NAMES[scheme], ic=ic)
+                  scheme=SCHEME_NAMES[scheme], ic=ic,
+                  artificial_dissipation="physical" if (ad_on and ad_type == "Physical") else "none",
+                  ad_intensity=1.0 if ad_ceps is None else ad_ceps)
     ic_name = rv("Initial condition") or ""
     if case.ic is None and ic_name == "Uniform":
         case.ic = C.uniform(C.freestream_state(Mach))

@@ -58,6 +58,13 @@ typedef struct hg_batch hg_batch;
 #define HG_PRECISION_FP64 0
 #define HG_PRECISION_MIXED 1 /* FP64 state/RK/reductions, FP32 fluxes and contractions */
 
+/* Artificial dissipation (Approximation/DGArtificialDissipation.f90): none, or the "Physical"
+ * type (the element and edge viscosities mu_a added to mu in every viscous flux) with the
+ * "Jumps-based" indicator.  The "Laplacian" type and the "Residuals-based" indicator (which
+ * the reference leaves without a viscosity function) are not provided. */
+#define HG_AD_NONE 0
+#define HG_AD_PHYSICAL 1
+
 #define HG_SCHEME_EULER 0
 #define HG_SCHEME_RK3 1
 #define HG_SCHEME_RK5 2
@@ -112,6 +119,9 @@ typedef struct {
     double sigma0;         /* "Jumps penalty parameter" */
     int32_t precision;     /* HG_PRECISION_* */
     int32_t scheme;        /* HG_SCHEME_* ("Integration scheme", Setup.f90:354-355) */
+    int32_t artificial_dissipation; /* HG_AD_* ("Artificial dissipation (0/1)", Setup.f90:333-349);
+                                       Navier-Stokes only, as DGArtificialDissipation.f90 */
+    double ad_intensity;   /* "Artificial dissipation intensity" (Ceps) */
 } hg_options;
 
 /* Builds connectivity/geometry on the host, uploads it, allocates device state.

@@ -537,6 +537,7 @@ typedef struct {
     int marker;
     double *X, *dX, *n, *dS; /* X,dX,n: 2*P (straight: dX,n,dS only [0]) */
     double Area, invh;
+    double mu_a;            /* artificial viscosity of the edge (ComputeEdgeViscosity) */
     /* boundary data */
     int inviscidBCType, rs, associated, partner;
     int vbc[MAXP];
@@ -577,6 +578,7 @@ typedef struct {
     Phys ph;
     int ns, riemann, viscous;
     double sigma0, ip_eps; int ip_gradflux;
+    int ad; double ad_ceps; double *mu_e;   /* artificial dissipation: on/off, Ceps, per element */
     int nnodes; double *pts;
     int K; Elem *el;
     int E; Edge *ed;
@@ -779,7 +781,7 @@ static void mesh_free(Mesh *m) {
     for (int i = 0; i < m->E; ++i) { free(m->ed[i].X); free(m->ed[i].dX); free(m->ed[i].n); free(m->ed[i].dS); }
     for (int z = 0; z < m->nmarkers; ++z) free(m->zones[z].edges);
     free(m->el); free(m->ed); free(m->zones); free(m->pts);
-    free(m->Q); free(m->QDot); free(m->dQ); free(m->G);
+    free(m->Q); free(m->QDot); free(m->dQ); free(m->G); free(m->mu_e);
     free(m);
 }
 
@@ -1400,22 +1402,23 @@ static void gradient_ip(Mesh *m) {
 }
 
 /* Viscous Riemann solvers: BR1 (DGViscous_BR1.f90:330-441), IP
- * (DGViscous_IP.f90:88-214).  mu_a = 0 (no artificial dissipation). */
+ * (DGViscous_IP.f90:88-214), with the viscosity mu + edge % mu_a. */
 static void visc_interior(Mesh *m, Edge *ed, const double *uL, const double *uR, const double *dL, const double *dR,
                           const double *nn, double *Fs) {
     const Phys *ph = &m->ph;
+    const double mu = ph->mu + ed->mu_a;
     double fL[4], gL[4], fR[4], gR[4];
     viscous_flux(ph, uL, dL, fL, gL);
     viscous_flux(ph, uR, dR, fR, gR);
     for (int k = 0; k < 4; ++k) {
-        double FL0 = ph->mu * fL[k], FL1 = ph->mu * gL[k], FR0 = ph->mu * fR[k], FR1 = ph->mu * gR[k];
+        double FL0 = mu * fL[k], FL1 = mu * gL[k], FR0 = mu * fR[k], FR1 = mu * gR[k];
         double acc = 0.0;
         acc = acc + 0.5 * (FL0 + FR0) * nn[0];
         acc = acc + 0.5 * (FL1 + FR1) * nn[1];
         Fs[k] = acc;
     }
     if (m->viscous == VISC_IP) {
-        double pen = m->sigma0 * ph->mu * (double)ed->N * (double)ed->N * ed->invh;
+        double pen = m->sigma0 * mu * (double)ed->N * (double)ed->N * ed->invh;
         for (int k = 0; k < 4; ++k) Fs[k] = Fs[k] - pen * (uL[k] - uR[k]);
     }
 }
@@ -1489,17 +1492,75 @@ static void std_face(Mesh *m, Elem *e, int loc, const double *F) {
 }
 
 /* IP gradient Riemann solvers (DGViscous_IP.f90:216-313): G[eq+4i+4P d] */
-static void ip_grad_interior(Mesh *m, const double *uL, const double *uR, const double *nn, double *GL, double *GR) {
+static void ip_grad_interior(Mesh *m, const Edge *ed, const double *uL, const double *uR, const double *nn,
+                             double *GL, double *GR) {
     const Phys *ph = &m->ph;
+    const double mu = ph->mu + ed->mu_a;
     double fg[8], f[4], g[4];
     for (int d = 0; d < 2; ++d)
         for (int k = 0; k < 4; ++k) fg[k + 4 * d] = (uL[k] - uR[k]) * nn[d];
     viscous_flux(ph, uL, fg, f, g);
-    double cL = 0.5 * m->ip_eps * ph->mu;
+    double cL = 0.5 * m->ip_eps * mu;
     for (int k = 0; k < 4; ++k) { GL[k] = cL * f[k]; GL[k + 4] = cL * g[k]; }
     viscous_flux(ph, uR, fg, f, g);
-    double cR = -0.5 * m->ip_eps * ph->mu;
+    double cR = -0.5 * m->ip_eps * mu;
     for (int k = 0; k < 4; ++k) { GR[k] = cR * f[k]; GR[k + 4] = cR * g[k]; }
+}
+
+/* Jumps of variable `var` across an edge (Edge_ComputeJumps and its variants,
+ * QuadElement_EdgeProcedures.incf:1348-1619): nodal 2 (QL - QR) / (QL + QR) squared, weighted
+ * by w dS.  Boundary edges take QR from uSB (periodic: the partner's trace, reversed when
+ * inverse) and, straight or curved, dS(0). */
+static double edge_jumps(Mesh *m, Edge *ed, int var) {
+    const Spec *s = spec_get(ed->N);
+    const int P = ed->N + 1;
+    double acc = 0.0;
+    for (int i = 0; i < P; ++i) {
+        double qL = ed->tq[0][4 * i + var], qR, dS;
+        if (!ed->bdry) {
+            qR = ed->tq[1][4 * (ed->inverse ? P - 1 - i : i) + var];
+            dS = ed->curved ? ed->dS[i] : ed->dS[0];
+        } else {
+            qR = ed->associated ? partner_q(m, ed, i)[var] : ed->uSB[4 * i + var];
+            dS = ed->dS[0];
+        }
+        double nj = 2.0 * (qL - qR) / (qL + qR);
+        acc = acc + s->w[i] * nj * nj * dS;
+    }
+    return acc;
+}
+
+/* Jumps-based artificial viscosity (DGArtificialDissipation.f90:277-363): per element
+ * mu = Ceps sqrt(Volume)/(N+1) G(log10(max(sum J_rho, sum J_rhoE) / sum Area)), G the smooth
+ * switch of half-width k = 5 around s0 = log10((N+1)^-4) + 1; per edge the mean of its
+ * elements' values (one element for boundary edges, periodic ones included). */
+static void artificial_viscosity(Mesh *m) {
+    if (!m->ad) return;
+    for (int eid = 0; eid < m->K; ++eid) {
+        Elem *e = &m->el[eid];
+        double gr = 0.0, ge = 0.0, area = 0.0;
+        for (int f = 0; f < 4; ++f) {
+            Edge *ed = &m->ed[e->edges[f]];
+            gr = gr + edge_jumps(m, ed, 0);
+            ge = ge + edge_jumps(m, ed, 3);
+            area = area + ed->Area;
+        }
+        double gk = (gr > ge ? gr : ge) / area;
+        const double k = 5.0, s0 = log10(1.0 / pow((double)(e->N + 1), 4.0)) + 1.0;
+        double G;
+        if (!(gk > 1.0e-8)) G = 0.0;
+        else {
+            gk = log10(gk);
+            if (gk < s0 - k) G = 0.0;
+            else if (gk < s0 + k) G = 0.5 * sin(0.5 * 3.141592653589793 * (gk - s0) / k) + 0.5;
+            else G = 1.0;
+        }
+        m->mu_e[eid] = m->ad_ceps * (sqrt(e->Volume) / (double)(e->N + 1)) * G;
+    }
+    for (int k = 0; k < m->E; ++k) {
+        Edge *ed = &m->ed[k];
+        ed->mu_a = ed->bdry ? m->mu_e[ed->quads[0]] : (m->mu_e[ed->quads[0]] + m->mu_e[ed->quads[1]]) / 2.0;
+    }
 }
 
 /* DGSpatialDiscretizationMethods.f90:160-235 computeQDot (+ newTimeStep :118-158) */
@@ -1507,6 +1568,7 @@ static void time_derivative(Mesh *m) {
     const Phys *ph = &m->ph;
     prolong(m, 0);
     update_bcs(m);
+    artificial_viscosity(m);
     if (m->ns) {
         if (m->viscous == VISC_BR1) gradient_br1(m);
         else gradient_ip(m);
@@ -1531,7 +1593,8 @@ static void time_derivative(Mesh *m) {
                     for (int d = 0; d < 2; ++d)
                         for (int eq = 0; eq < 4; ++eq) dq[eq + 4 * d] = m->dQ[DQIDX(e, eq, i, j, d)];
                     viscous_flux(ph, q, dq, fv, gv);
-                    for (int eq = 0; eq < 4; ++eq) { f[eq] = (f[eq] - ph->mu * fv[eq]) - 0.0; g[eq] = (g[eq] - ph->mu * gv[eq]) - 0.0; }
+                    const double mu = ph->mu + m->mu_e[eid];   /* e % mu_a, DGViscous_BR1.f90:307 */
+                    for (int eq = 0; eq < 4; ++eq) { f[eq] = (f[eq] - mu * fv[eq]) - 0.0; g[eq] = (g[eq] - mu * gv[eq]) - 0.0; }
                 }
                 for (int eq = 0; eq < 4; ++eq) {
                     cF[eq + 4 * k] = f[eq] * e->Ja[0 * PP + k] + g[eq] * e->Ja[2 * PP + k];
@@ -1575,7 +1638,7 @@ static void time_derivative(Mesh *m) {
                     for (int eq = 0; eq < 4; ++eq) Fs[eq] = ((Fi[eq] - Fv[eq]) - 0.0) * dS;
                     if (gradfl) {
                         double gl[8], gr[8];
-                        ip_grad_interior(m, qL, qR, nn, gl, gr);
+                        ip_grad_interior(m, ed, qL, qR, nn, gl, gr);
                         for (int d = 0; d < 2; ++d)
                             for (int eq = 0; eq < 4; ++eq) {
                                 /* straight: project then *dS(0); curved: *dS(i) then project */
@@ -1623,7 +1686,7 @@ static void time_derivative(Mesh *m) {
                         for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] * dS;
                         if (gradfl) {
                             double gl[8], gr[8];
-                            ip_grad_interior(m, q, qb, nn, gl, gr);
+                            ip_grad_interior(m, ed, q, qb, nn, gl, gr);
                             for (int c = 0; c < 8; ++c) Gv[c] = gl[c] * dS;
                         }
                     } else if (vbc == VBC_ADIABATIC) {
@@ -1632,12 +1695,12 @@ static void time_derivative(Mesh *m) {
                         adiabatic_viscous_flux(ph, q, q, dq, f, g);
                         for (int eq = 0; eq < 4; ++eq) {
                             double acc = 0.0;
-                            acc = acc + ph->mu * f[eq] * nn[0];
-                            acc = acc + ph->mu * g[eq] * nn[1];
+                            acc = acc + (ph->mu + ed->mu_a) * f[eq] * nn[0];
+                            acc = acc + (ph->mu + ed->mu_a) * g[eq] * nn[1];
                             Fv[eq] = acc;
                         }
                         if (m->viscous == VISC_IP) {
-                            double pen = m->sigma0 * ph->mu * (double)ed->N * (double)ed->N * ed->invh;
+                            double pen = m->sigma0 * (ph->mu + ed->mu_a) * (double)ed->N * (double)ed->N * ed->invh;
                             for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] - pen * (q[eq] - usb[eq]);
                         }
                         for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] * dS;
@@ -1646,16 +1709,17 @@ static void time_derivative(Mesh *m) {
                             for (int d = 0; d < 2; ++d)
                                 for (int eq = 0; eq < 4; ++eq) fg[eq + 4 * d] = (q[eq] - usb[eq]) * nn[d];
                             adiabatic_viscous_flux(ph, q, q, fg, f, g);
-                            double c = m->ip_eps * ph->mu;
+                            double c = m->ip_eps * (ph->mu + ed->mu_a);
                             for (int eq = 0; eq < 4; ++eq) { Gv[eq] = c * f[eq] * dS; Gv[eq + 4] = c * g[eq] * dS; }
                         }
                     } else if (vbc == VBC_DIRICHLET) {
                         const double *usb = ed->uSB + 4 * i;
                         double f[4], g[4];
                         viscous_flux(ph, q, dq, f, g);
-                        for (int eq = 0; eq < 4; ++eq) Fv[eq] = ph->mu * f[eq] * nn[0] + ph->mu * g[eq] * nn[1];
+                        for (int eq = 0; eq < 4; ++eq)
+                            Fv[eq] = (ph->mu + ed->mu_a) * f[eq] * nn[0] + (ph->mu + ed->mu_a) * g[eq] * nn[1];
                         if (m->viscous == VISC_IP) {
-                            double pen = m->sigma0 * ph->mu * (double)ed->N * (double)ed->N * ed->invh;
+                            double pen = m->sigma0 * (ph->mu + ed->mu_a) * (double)ed->N * (double)ed->N * ed->invh;
                             for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] - pen * (q[eq] - usb[eq]);
                         }
                         for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] * dS;
@@ -1664,7 +1728,7 @@ static void time_derivative(Mesh *m) {
                             for (int d = 0; d < 2; ++d)
                                 for (int eq = 0; eq < 4; ++eq) fg[eq + 4 * d] = (q[eq] - usb[eq]) * nn[d];
                             viscous_flux(ph, q, fg, f, g);
-                            double c = 1.0 * m->ip_eps * ph->mu;
+                            double c = 1.0 * m->ip_eps * (ph->mu + ed->mu_a);
                             for (int eq = 0; eq < 4; ++eq) { Gv[eq] = c * f[eq] * dS; Gv[eq + 4] = c * g[eq] * dS; }
                         }
                     } /* NEUMANN: Fv = 0, Gv = 0 */
@@ -1695,6 +1759,8 @@ typedef struct {
     double Mach, Re, Pr;
     int ns, riemann, viscous, ip_method; /* ip_method: 0 SIPG, 1 NIPG, 2 IIPG */
     double sigma0;
+    int ad;            /* artificial dissipation: 0 none, 1 "Physical" with the jumps-based indicator */
+    double ad_ceps;    /* "Artificial dissipation intensity" */
 } OrOptions;
 
 void *or_create(const MeshIn *in, const OrOptions *o) {
@@ -1707,6 +1773,10 @@ void *or_create(const MeshIn *in, const OrOptions *o) {
     else if (o->ip_method == 1) { m->ip_eps = 1.0; m->ip_gradflux = 1; }
     else { m->ip_eps = 0.0; m->ip_gradflux = 0; }
     if (!riemann_fn(m->riemann)) { mesh_free(m); fail("unknown Riemann solver"); return NULL; }
+    m->ad = m->ns ? o->ad : 0;   /* the module is Navier-Stokes only */
+    m->ad_ceps = o->ad_ceps;
+    m->mu_e = (double *)calloc(m->K + 1, sizeof(double));
+    for (int k = 0; k < m->E; ++k) m->ed[k].mu_a = 0.0;
     return m;
 }
 void or_destroy(void *h) { mesh_free((Mesh *)h); }

@@ -57,7 +57,8 @@ class _MeshIn(ctypes.Structure):
 class _Options(ctypes.Structure):
     _fields_ = [("Mach", ctypes.c_double), ("Re", ctypes.c_double), ("Pr", ctypes.c_double),
                 ("ns", ctypes.c_int), ("riemann", ctypes.c_int), ("viscous", ctypes.c_int),
-                ("ip_method", ctypes.c_int), ("sigma0", ctypes.c_double)]
+                ("ip_method", ctypes.c_int), ("sigma0", ctypes.c_double),
+                ("ad", ctypes.c_int), ("ad_ceps", ctypes.c_double)]
 
 
 _lib = None
@@ -163,7 +164,8 @@ class Oracle:
         mi.nmarkers = len(zones)
         mi.bc_type = _ptr(bct); mi.bc_int = _ptr(bci); mi.bc_real = _ptr(bcr); mi.bc_rs = _ptr(bcrs)
         op = _Options(case.Mach, case.Re, case.Pr, int(case.navier_stokes), C.RIEMANN[case.riemann],
-                      C.VISCOUS[case.viscous], C.IP_METHOD[case.ip_method], case.sigma0)
+                      C.VISCOUS[case.viscous], C.IP_METHOD[case.ip_method], case.sigma0,
+                      C.ARTIFICIAL_DISSIPATION[case.artificial_dissipation], case.ad_intensity)
         h = lib().or_create(ctypes.byref(mi), ctypes.byref(op))
         if not h:
             raise RuntimeError("oracle: " + lib().or_last_error().decode())

@@ -29,7 +29,14 @@ CASES = {
     "ns_sipg_annulus": (lambda: _annulus("ip"), (1, 2)),
     "ns_br1_tgv": (lambda: C.ns_tgv_case(6, 4, Re=1600.0), (1, 3)),
     "ns_ip_kh": (lambda: C.ns_kh_case(6, 5, Re=1.0e4), (1, 2, 3, 4)),
+    # artificial dissipation: the viscous force keeps the physical viscosity
+    "ns_br1_ad_annulus": (lambda: _parity_case("ns_br1_ad_annulus"), (1, 2)),
 }
+
+
+def _parity_case(name):
+    from test_gpu_parity import CASES as P
+    return P[name]()
 
 
 def _close(got, want, tol, what, mu):

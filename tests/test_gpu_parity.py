@@ -95,6 +95,23 @@ def _euler_wall_annulus(ns):
     return c
 
 
+def _with_density_step(case, x0, ceps=1.0):
+    """Artificial dissipation on (the "Physical" type, jumps-based indicator), and the case's
+    initial density raised by a quarter for x > x0 (velocity and pressure kept): a jump that
+    the element edges do not follow, so the indicator takes many values."""
+    base = case.ic
+
+    def ic(x, y):
+        q = np.asarray(base(x, y), dtype=np.float64).copy()
+        rho, u, v = q[..., 0], q[..., 1] / q[..., 0], q[..., 2] / q[..., 0]
+        p = 0.4 * (q[..., 3] - 0.5 * rho * (u * u + v * v))
+        r = rho * np.where(x > x0, 1.25, 1.0)
+        return C._pack(r, u, v, p)
+    case.ic = ic
+    case.artificial_dissipation, case.ad_intensity = "physical", ceps
+    return case
+
+
 CASES = {
     "euler_roe": lambda: C.euler_vortex_case(8, 4),
     "euler_hll": lambda: C.euler_vortex_case(6, 5, riemann="hll"),
@@ -118,6 +135,12 @@ CASES = {
     "ns_br1_supersonic_dirichlet": lambda: _channel(C.BC_DIRICHLET, C.BC_DIRICHLET, 1.6, 0.6, "br1"),
     "euler_eulerwall_annulus": lambda: _euler_wall_annulus(False),
     "ns_br1_eulerwall_annulus": lambda: _euler_wall_annulus(True),
+    # artificial dissipation (jumps-based, "Physical"): BR1, interior penalty, boundary states, curved faces
+    "ns_br1_ad_tgv": lambda: _with_density_step(C.ns_tgv_case(6, 4, Re=1600.0), 2.9),
+    "ns_sipg_ad_kh": lambda: _with_density_step(C.ns_kh_case(6, 5, Re=1.0e4), 0.47, 2.0),
+    "ns_br1_ad_channel": lambda: _with_density_step(
+        _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1", 1), 1.3),
+    "ns_br1_ad_annulus": lambda: _with_density_step(C.ns_cylinder_case(4, 16, N=3, Re=200.0), 0.2),
 }
 
 
@@ -148,12 +171,13 @@ def test_rk3_steps_match_oracle(name):
 @pytest.mark.parametrize("name", list(CASES))
 def test_workgroup_tiles_are_bit_identical(name):
     """The narrow and wide workgroup tiles evaluate the same expressions per node and face.
-    (Euler-wall, Dirichlet and pressure zones run the stage kernel's run-time-solver form,
-    which has the wide tile only: there the gradient kernel's tiles are compared.)"""
+    (Euler-wall, Dirichlet and pressure zones and artificial dissipation run the stage kernel's
+    run-time-solver form, which has the wide tile only: there the gradient kernel's tiles are
+    compared.)"""
     from horses2d_amd import gpu
     case = CASES[name]()
     extended = any(z.kind in (C.BC_DIRICHLET, C.BC_EULERWALL, C.BC_PRESSUREOUTLET, C.BC_PRESSUREINLET)
-                   for z in case.mesh.zones)
+                   for z in case.mesh.zones) or case.artificial_dissipation != "none"
     runs = []
     for tile in (0, 1):
         dg = gpu.DGSEM(case, "fp64")

@@ -177,6 +177,24 @@ def test_case_from_hiocase_keys_zones_and_nondimensional_states(tmp_path):
     np.testing.assert_allclose(c.initial_state(X).reshape(-1, 4)[0], C.freestream_state(0.2))
 
 
+def test_artificial_dissipation_keys(tmp_path):
+    _, path = _box_files(tmp_path)
+    with open(path) as f:
+        text = f.read()
+    assert hio.case_from_hiocase(path).artificial_dissipation == "none"
+    extra = "\n   Artificial dissipation (0/1) : 1\n   Artificial dissipation intensity : 0.3\n"
+    with open(path, "w") as f:
+        f.write(text.replace("Number of iterations : 77", "Number of iterations : 77" + extra))
+    c = hio.case_from_hiocase(path)
+    assert (c.artificial_dissipation, c.ad_intensity) == ("physical", 0.3)
+    for bad in ("   Artificial dissipation type : Laplacian\n",
+                "   Artificial dissipation indicator : Residuals-based\n"):
+        with open(path, "w") as f:
+            f.write(text.replace("Number of iterations : 77", "Number of iterations : 77" + extra + bad))
+        with pytest.raises(ValueError):
+            hio.case_from_hiocase(path)
+
+
 def test_p_refinement_zones_map_to_orders(tmp_path):
     _, path = _box_files(tmp_path, pref=True)
     c = hio.case_from_hiocase(path)

@@ -586,3 +586,60 @@ def test_euler_walls_conserve_mass_and_energy_and_momentum_follows_the_pressure_
         np.testing.assert_allclose(total[1:3], -force / (np.sqrt(GAMMA) * case.Mach), rtol=1e-11,
                                    atol=1e-12 * scale[1:3].max())
         assert np.all(np.abs(force) > 1e-4)
+
+
+# ------------------------------------------------------------------------------------------
+# Artificial dissipation: DGArtificialDissipation.f90:277-363 (jumps-based, "Physical" type)
+# ------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("viscous", ["br1", "ip"])
+def test_artificial_viscosity_of_a_checkerboard_equals_a_larger_physical_viscosity(viscous):
+    """A density checkerboard, constant per element, with constant energy: every edge has the
+    jump J = (2 (rho1 - rho2) / (rho1 + rho2))^2 Area (and none in rho E), so every element has
+    gk = J_sum / Area_sum = (2 d / s)^2 and the same mu_a = Ceps sqrt(Volume)/(N+1) G(log10 gk),
+    G = 0.5 sin(pi/2 (log10 gk - s0)/5) + 0.5, s0 = log10((N+1)^-4) + 1; each edge's mean is the
+    same value.  QDot then equals the plain Navier-Stokes QDot with 1/Re' = 1/Re + mu_a (every
+    viscous flux, penalty and gradient flux scales with mu + mu_a)."""
+    n, N, Re, ceps = 4, 4, 200.0, 0.7
+    case = C.ns_tgv_case(n, N, Re=Re, viscous=viscous, riemann="roe")
+    case.artificial_dissipation, case.ad_intensity = "physical", ceps
+    o = O.Oracle(case)
+    X = o.coords()
+    P = N + 1
+    h = 2.0 * np.pi / n
+    xc = X.reshape(-1, P * P, 2).mean(axis=1)
+    chk = ((np.floor(xc[:, 0] / h) + np.floor(xc[:, 1] / h)) % 2).astype(int)
+    r1, r2 = 1.0, 1.3
+    rho = np.repeat(np.where(chk == 0, r1, r2), P * P)
+    x, y = X[:, 0], X[:, 1]
+    u, v = 0.1 * np.sin(y), 0.05 * np.cos(x)
+    Q = _cons(rho, rho * u, rho * v, 1.0 + 0.0 * x)
+    Q.reshape(-1, 4)[:, 3] = 2.6                       # constant energy: no rho E jumps
+    o.set_state(Q)
+    qd_ad = o.time_derivative()
+    gk = (2.0 * (r1 - r2) / (r1 + r2)) ** 2
+    s0 = np.log10(1.0 / (N + 1) ** 4) + 1.0
+    G = 0.5 * np.sin(0.5 * np.pi * (np.log10(gk) - s0) / 5.0) + 0.5
+    mu_a = ceps * np.sqrt(h * h) / (N + 1) * G
+    plain = C.ns_tgv_case(n, N, Re=1.0 / (1.0 / Re + mu_a), viscous=viscous, riemann="roe")
+    o2 = O.Oracle(plain)
+    o2.set_state(Q)
+    qd = o2.time_derivative()
+    assert np.max(np.abs(qd_ad - qd)) <= 1e-12 * np.max(np.abs(qd))
+    # and the dissipation is active: without it the QDot differs
+    o3 = O.Oracle(C.ns_tgv_case(n, N, Re=Re, viscous=viscous, riemann="roe"))
+    o3.set_state(Q)
+    assert np.max(np.abs(o3.time_derivative() - qd)) > 1e-6 * np.max(np.abs(qd))
+
+
+def test_artificial_viscosity_vanishes_on_a_smooth_state():
+    """Jumps of a uniform state are zero (gk <= 1e-8 gives G = 0): QDot is the plain one bit
+    for bit."""
+    case = C.ns_tgv_case(4, 4, Re=200.0)
+    case.ic = C.uniform(C.freestream_state(0.1, 0.3))
+    a = O.Oracle(case)
+    a.set_state(case.initial_state(a.coords()))
+    case2 = C.ns_tgv_case(4, 4, Re=200.0)
+    case2.artificial_dissipation = "physical"
+    b = O.Oracle(case2)
+    b.set_state(case.initial_state(b.coords()))
+    np.testing.assert_array_equal(a.time_derivative(), b.time_derivative())
