@@ -11,7 +11,7 @@ form after those steps is reported per field (nodal and quadrature-weighted) and
 quantity (mass, momentum vector, energy).  ``value`` is the mixed form's rate when the
 conserved-quantity errors are all <= 1e-6 and the FP64 rate otherwise.
 
-Prints one JSON line.  With ``--gpus`` above 1 it exits: this build runs on one GPU.
+Prints one JSON line.  With ``--gpus`` above 1 it exits: the library runs on one GPU.
 """
 from __future__ import annotations
 
@@ -224,7 +224,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it", file=sys.stderr)
+        print("bench.py: the library runs on one GPU; multi-GPU runs are not provided", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe:

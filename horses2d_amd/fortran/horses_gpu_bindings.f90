@@ -4,8 +4,8 @@
 !
 !  The solver keeps its HiOCase, HiOMesh, libproblemfile and HORSES2D.NS/.Euler entry
 !  points; a TimeScheme procedure built on hgf_rk_step replaces
-!  TimeIntegrator_WilliamsonRK3 (Approximation/DGTimeIntegrator.f90:282-317).  The
-!  adapter that binds it is shown in INTEGRATION.md.
+!  TimeIntegrator_WilliamsonRK3 (Approximation/DGTimeIntegrator.f90:282-317); the
+!  solver-side adapter is a subroutine of that interface calling hg_rk_step.
 !
 module horses_gpu_bindings
    use iso_c_binding

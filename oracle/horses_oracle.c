@@ -1,10 +1,10 @@
 /*
  * horses_oracle.c -- CPU restatement of HORSES2D's DGSEM residual and explicit
- * time integration, used ONLY as the test oracle (checker) for the HIP path.
+ * time integration, used ONLY as the test oracle for the HIP path.
  *
  * TEST INFRASTRUCTURE: only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg may load this library, and only as the checker / CPU
- * baseline. The product path (horses2d_amd) never links or calls it.
+ * cpu_baseline leg may load this library, and only to check results or as the
+ * CPU baseline. The product path (horses2d_amd) never links or calls it.
  *
  * Every function cites the reference file:line (paths relative to the
  * HORSES2D source tree, Solver/src/...) whose algorithm it restates.  The

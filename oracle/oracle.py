@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (``oracle/horses_oracle.c``).
 
 TEST INFRASTRUCTURE.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the checker
+``cpu_baseline`` leg of ``bench.py`` may import this module, and only to check results
 or as the timed CPU baseline.  The product package ``horses2d_amd`` never imports it.
 
 The oracle restates, in plain C and in the reference's operation order, the HORSES2D
