@@ -171,7 +171,9 @@ __device__ inline void exact_F(const Phys<R> &ph, R p, const R *W, R &F, R &dF) 
 }
 
 // RiemannSolvers.f90:33-187 with ExactRiemann_ComputePStar :405-459 (at most 50 Newton
-// iterations; the star velocity is taken from the last iterate when not converged).
+// iterations; the star velocity is taken from the last iterate when not converged; a guess or
+// iterate that is not a positive number -- an overshoot, where the reference takes the root of
+// a negative pressure -- is raised to 1e-6 min(pL, pR); vacuum generation is sampled exactly).
 template <typename R>
 __device__ void exact_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R ny, R *F) {
     const R g = ph.gamma, gm1 = ph.gm1, cp = ph.cp, cv = ph.cv;
@@ -182,15 +184,47 @@ __device__ void exact_flux(const Phys<R> &ph, const R *qL, const R *qR, R nx, R 
     wL[3] = p_full(ph, gm1 * (qnL[3] - R(0.5) * (qnL[1] * wL[1] + qnL[2] * wL[2]))); wL[4] = R(0); wL[5] = dsqrt(g * wL[3] / wL[0]);
     wR[0] = qnR[0]; wR[1] = qnR[1] / qnR[0]; wR[2] = qnR[2] / qnR[0];
     wR[3] = p_full(ph, gm1 * (qnR[3] - R(0.5) * (qnR[1] * wR[1] + qnR[2] * wR[2]))); wR[4] = R(0); wR[5] = dsqrt(g * wR[3] / wR[0]);
+    if (R(2) * cv * (wL[5] + wR[5]) <= wR[1] - wL[1]) {
+        // Vacuum generation (the pressure positivity condition fails): the reference's
+        // two-rarefaction guess is negative and its iteration has no root; the exact solution is
+        // two rarefactions around a vacuum (Toro, Riemann Solvers and Numerical Methods for
+        // Fluid Dynamics, sec. 4.6), sampled at x/t = 0.
+        R Fs[4] = {R(0), R(0), R(0), R(0)};
+        if (wL[1] - wL[5] >= R(0)) {
+            Fs[0] = qnL[1]; Fs[1] = qnL[1] * wL[1] + wL[3]; Fs[2] = qnL[2] * wL[1];
+            Fs[3] = wL[1] * (cp * wL[3] + R(0.5) * (qnL[1] * wL[1] + qnL[2] * wL[2]));
+        } else if (wL[1] + R(2) * cv * wL[5] > R(0)) {
+            const R uF = R(2) * wL[5] / (g + R(1)) + (g - R(1)) / (g + R(1)) * wL[1];
+            const R rs = wL[0] * dpow(uF / wL[5], R(2) * cv), pF = wL[3] * dpow(rs / wL[0], g);
+            Fs[0] = rs * uF; Fs[1] = rs * uF * uF + pF; Fs[2] = rs * uF * wL[2];
+            Fs[3] = uF * (cp * pF + R(0.5) * rs * (uF * uF + wL[2] * wL[2]));
+        } else if (wR[1] - R(2) * cv * wR[5] < R(0) && wR[1] + wR[5] > R(0)) {
+            const R uF = -R(2) * wR[5] / (g + R(1)) + (g - R(1)) / (g + R(1)) * wR[1];
+            const R rs = wR[0] * dpow(-uF / wR[5], R(2) * cv), pF = wR[3] * dpow(rs / wR[0], g);
+            Fs[0] = rs * uF; Fs[1] = rs * uF * uF + pF; Fs[2] = rs * uF * wR[2];
+            Fs[3] = uF * (cp * pF + R(0.5) * rs * (uF * uF + wR[2] * wR[2]));
+        } else if (wR[1] + wR[5] <= R(0)) {
+            Fs[0] = qnR[1]; Fs[1] = qnR[1] * wR[1] + wR[3]; Fs[2] = qnR[2] * wR[1];
+            Fs[3] = wR[1] * (cp * wR[3] + R(0.5) * (qnR[1] * wR[1] + qnR[2] * wR[2]));
+        }   // else x/t = 0 is in the vacuum: no flux
+        if constexpr (Phys<R>::shift) Fs[1] -= ph.pref;
+        R Ft = Fs[1] * nx - Fs[2] * ny;
+        Fs[2] = Fs[1] * ny + Fs[2] * nx;
+        F[0] = Fs[0] * ph.invSGM; F[1] = Ft * ph.invSGM; F[2] = Fs[2] * ph.invSGM; F[3] = Fs[3] * ph.invSGM;
+        return;
+    }
     R FL, dFL, FR, dFR;
     R ps = dpow((wL[5] + wR[5] - R(0.5) * gm1 * (wR[1] - wL[1])) /
                     (wL[5] / dpow(wL[3], R(0.5) / cp) + wR[5] / dpow(wR[3], R(0.5) / cp)),
                 R(2) * cp);
+    const R pfloor = R(1.0e-6) * (wL[3] < wR[3] ? wL[3] : wR[3]);
+    if (!(ps > R(0))) ps = pfloor;
     for (int it = 1; it <= 50; ++it) {
         exact_F(ph, ps, wL, FL, dFL);
         exact_F(ph, ps, wR, FR, dFR);
         R Fv = FL + FR + wR[1] - wL[1], dF = dFL + dFR, pold = ps;
         ps = pold - Fv / dF;
+        if (!(ps > R(0))) ps = pfloor;
         R cha = R(0.5) * dabs(ps - pold) / (ps + pold);
         if (dabs(cha) < R(1.0e-6)) break;
     }

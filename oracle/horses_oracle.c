@@ -354,20 +354,27 @@ static void exact_F(const Phys *ph, double p, const double *W, double *F, double
     }
 }
 
-/* RiemannSolvers.f90:405-459.  When the Newton iteration does not converge in
- * 50 iterations the reference leaves ustar undefined; here ustar is computed
- * from the last iterate (a defined, documented deviation). */
+/* RiemannSolvers.f90:405-459.  Two defined treatments where the reference has none:
+ * when the Newton iteration does not converge in 50 iterations the reference leaves
+ * ustar undefined, here it comes from the last iterate; and where the two-rarefaction
+ * guess or an iterate is not a positive number (an overshoot), from which the
+ * reference's next step takes the root of a negative pressure, the pressure is raised to
+ * 1e-6 min(pL, pR) instead of going to NaN (vacuum generation is sampled exactly by the
+ * caller).  Where the reference's iterates stay positive, nothing changes. */
 static void exact_pstar(const Phys *ph, const double *WL, const double *WR, double *pstar, double *ustar) {
     double FL, dFL, FR, dFR;
+    const double pfloor = 1.0e-6 * (WL[3] < WR[3] ? WL[3] : WR[3]);
     double ps = pow((WL[5] + WR[5] - 0.5 * ph->gm1 * (WR[1] - WL[1])) /
                         (WL[5] / pow(WL[3], 0.5 / ph->cp) + WR[5] / pow(WR[3], 0.5 / ph->cp)),
                     2.0 * ph->cp);
+    if (!(ps > 0.0)) ps = pfloor;
     for (int it = 1; it <= 50; ++it) {
         exact_F(ph, ps, WL, &FL, &dFL);
         exact_F(ph, ps, WR, &FR, &dFR);
         double Fv = FL + FR + WR[1] - WL[1], dF = dFL + dFR;
         double pold = ps;
         ps = pold - Fv / dF;
+        if (!(ps > 0.0)) ps = pfloor;
         double cha = 0.5 * fabs(ps - pold) / (ps + pold);
         if (fabs(cha) < 1.0e-6) break;
     }
@@ -388,8 +395,37 @@ static void exact_flux(const Phys *ph, const double *qL, const double *qR, const
     wR[0] = qnR[0]; wR[1] = qnR[1] / qnR[0]; wR[2] = qnR[2] / qnR[0];
     wR[3] = gm1 * (qnR[3] - 0.5 * (qnR[1] * wR[1] + qnR[2] * wR[2])); wR[4] = 0; wR[5] = sqrt(g * wR[3] / wR[0]);
     double pstar, ustar, rhostar, uFan, pFan;
-    exact_pstar(ph, wL, wR, &pstar, &ustar);
-    if (ustar >= 0.0) {
+    if (2.0 * cv * (wL[5] + wR[5]) <= wR[1] - wL[1]) {
+        /* Vacuum generation (pressure positivity condition fails): the reference's
+         * two-rarefaction guess is negative and its iteration has no root; the exact solution
+         * is two rarefactions around a vacuum (Toro, Riemann Solvers and Numerical Methods for
+         * Fluid Dynamics, sec. 4.6), sampled at x/t = 0 (vacuum itself: no flux). */
+        if (wL[1] - wL[5] >= 0.0) {
+            Fs[0] = qnL[1]; Fs[1] = qnL[1] * wL[1] + wL[3]; Fs[2] = qnL[2] * wL[1];
+            Fs[3] = wL[1] * (cp * wL[3] + 0.5 * (qnL[1] * wL[1] + qnL[2] * wL[2]));
+        } else if (wL[1] + 2.0 * cv * wL[5] > 0.0) {
+            uFan = 2.0 * wL[5] / (g + 1.0) + (g - 1.0) / (g + 1.0) * wL[1];
+            rhostar = wL[0] * pow(uFan / wL[5], 2.0 * cv);
+            pFan = wL[3] * pow(rhostar / wL[0], g);
+            Fs[0] = rhostar * uFan; Fs[1] = rhostar * uFan * uFan + pFan; Fs[2] = rhostar * uFan * wL[2];
+            Fs[3] = uFan * (cp * pFan + 0.5 * rhostar * (uFan * uFan + wL[2] * wL[2]));
+        } else if (wR[1] - 2.0 * cv * wR[5] < 0.0 && wR[1] + wR[5] > 0.0) {
+            uFan = -2.0 * wR[5] / (g + 1.0) + (g - 1.0) / (g + 1.0) * wR[1];
+            rhostar = wR[0] * pow(-uFan / wR[5], 2.0 * cv);
+            pFan = wR[3] * pow(rhostar / wR[0], g);
+            Fs[0] = rhostar * uFan; Fs[1] = rhostar * uFan * uFan + pFan; Fs[2] = rhostar * uFan * wR[2];
+            Fs[3] = uFan * (cp * pFan + 0.5 * rhostar * (uFan * uFan + wR[2] * wR[2]));
+        } else if (wR[1] + wR[5] <= 0.0) {
+            Fs[0] = qnR[1]; Fs[1] = qnR[1] * wR[1] + wR[3]; Fs[2] = qnR[2] * wR[1];
+            Fs[3] = wR[1] * (cp * wR[3] + 0.5 * (qnR[1] * wR[1] + qnR[2] * wR[2]));
+        }
+        ustar = 1.0; pstar = -1.0;   /* the sampling below is skipped */
+    } else {
+        exact_pstar(ph, wL, wR, &pstar, &ustar);
+    }
+    if (pstar < 0.0) {
+        /* vacuum: sampled above */
+    } else if (ustar >= 0.0) {
         double sh = wL[5] * sqrt((g + 1.0) / (2.0 * g) * pstar / wL[3] + 0.5 / cp);
         if ((pstar <= wL[3]) && (wL[1] >= wL[5])) {
             Fs[0] = qnL[1]; Fs[1] = qnL[1] * wL[1] + wL[3]; Fs[2] = qnL[2] * wL[1];

@@ -149,6 +149,26 @@ def test_exact_solver_textbook(WL, WR, pstar):
     np.testing.assert_allclose(F[0], Fref, rtol=2e-5, atol=2e-6)  # Newton stops at 1e-6 relative
 
 
+def test_exact_solver_vacuum_generation():
+    """Diverging states with 2 (aL + aR)/(gamma - 1) <= uR - uL: the reference's two-rarefaction
+    guess is negative and its Newton iteration has no root.  The solver samples the exact
+    solution, two rarefactions around a vacuum: no flux when x/t = 0 is in the vacuum (the
+    symmetric case and an asymmetric one), the textbook left fan state when x/t = 0 is inside
+    the left rarefaction, and its mirror image for the right one."""
+    def flux(WL, WR):
+        qL = prim_to_cons(*[np.array([c]) for c in (WL[0], WL[1], 0.0, WL[2])])
+        qR = prim_to_cons(*[np.array([c]) for c in (WR[0], WR[1], 0.0, WR[2])])
+        return O.riemann(2, 1.0 / np.sqrt(GAMMA), qL, qR, np.array([[1.0, 0.0]]))[0]
+    np.testing.assert_array_equal(flux((1.0, -4.0, 0.4), (1.0, 4.0, 0.4)), 0.0)
+    np.testing.assert_array_equal(flux((1.0, -6.0, 1.0), (0.5, 5.0, 0.2)), 0.0)
+    WL, WR = (1.0, 0.5, 1.0), (1.0, 13.0, 1.0)
+    Fref, _, _ = toro_exact_interface_flux(WL, WR)
+    F = flux(WL, WR)
+    np.testing.assert_allclose(F, Fref, rtol=1e-13, atol=1e-14)
+    Fm = flux((1.0, -13.0, 1.0), (1.0, -0.5, 1.0))           # mirror image: x -> -x
+    np.testing.assert_allclose(Fm, [-F[0], F[1], 0.0, -F[3]], rtol=1e-13, atol=1e-14)
+
+
 def test_lax_friedrichs_properties():
     """LF is not in the reference (parity unpinned): consistency (above), exact antisymmetry,
     and non-negative entropy-free dissipation of the density jump."""
