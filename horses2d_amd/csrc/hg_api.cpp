@@ -361,8 +361,8 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     ph.ad = (ph.ns && opt->artificial_dissipation == HG_AD_PHYSICAL) ? 1 : 0;
     ph.ad_ceps = opt->ad_intensity;
     if (ph.ad) ph.xbc = 1;
-    if (ph.xbc) ph.rs_kernel = -1;   // those kinds are compiled into the run-time-solver form
-    if (ph.rs_kernel < 0) h->tile[0] = 1;   // the run-time-solver form exists for the wide tile only
+    // the run-time-solver form and the forms with those kinds exist for the wide tile only
+    if (ph.rs_kernel < 0 || ph.xbc) h->tile[0] = 1;
 
     HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // ---- host-side tables
@@ -911,7 +911,8 @@ int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile) {
     if (!h) return set_err("hg_set_tiles: null handle");
     if (stage_tile < -1 || stage_tile > 1 || gradient_tile < -1 || gradient_tile > 1)
         return set_err("hg_set_tiles: tile must be -1 (autotune), 0 (narrow) or 1 (wide)");
-    if (h->ph.rs_kernel < 0 && stage_tile != 1) stage_tile = 1;   // only the wide tile has the run-time solver form
+    // only the wide tile has the run-time solver form and the forms with the newer boundary kinds
+    if ((h->ph.rs_kernel < 0 || h->ph.xbc) && stage_tile != 1) stage_tile = 1;
     h->tile[0] = stage_tile;
     h->tile[1] = gradient_tile;
     drop_graphs(h);

@@ -202,8 +202,10 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     b->variant = 2 * (h0->ph.ns ? (h0->ph.viscous == 2 ? 2 : 1) : 0) + (h0->mixed ? 1 : 0);
     b->nslots = 2 * b->sch.nst + 1;
     b->rs = h0->ph.rs_kernel;
-    for (int c = 1; c < ncases; ++c)
+    for (int c = 0; c < ncases; ++c) {
         if (cases[c]->ph.rs_kernel != b->rs) b->rs = -1;
+        if (cases[c]->ph.xbc) b->rs = -1;   // the batched kernels compile those kinds into that form only
+    }
     HB_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     for (auto *h : b->cases)
         if (hg::ensure_vtr(h)) return 1;

@@ -468,10 +468,13 @@ struct StageSmem {
 // Interior penalty: the gradient is element-local, so the RK update's epilogue also forms the
 // next stage's viscous face-flux traces from the updated tile (the work of k_gradient, with
 // the same device functions and the same bits), and no gradient kernel runs between stages.
-template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
+// XBF: the Euler-wall, Dirichlet and pressure boundary kinds and artificial dissipation are
+// compiled in (always with the run-time solver choice, RS = -1).
+template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR, bool XBF = false>
 __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, const int64_t rslot, const double cq,
                                            const double cg, StageSmem<P, EPB, NS, IP, R> &S) {
     using T = Tile<P, EPB>;
+    constexpr bool XB = RS < 0 || XBF;
     constexpr int PP = T::PP, NN = T::NN, NF = T::NF;
     constexpr bool BR1 = NS && !IP;
     const int TPB = blockDim.x;
@@ -525,7 +528,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     __syncthreads();
 
     if (BR1) {
-        if (face_active) br1_face_flux<R, (RS < 0)>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 6);
+        if (face_active) br1_face_flux<R, XB>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 6);
         __syncthreads();
     }
 
@@ -543,7 +546,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         if ((fr.x & 15) <= FK_PERIODIC) {
 #pragma unroll
             for (int c = 0; c < (NS ? VT : 1); ++c) vn[c] = (R)((const TR *)a.dtr)[nslot * VT + c];
-            if (RS < 0 && a.mu_a && (fr.x & 15) == FK_PERIODIC) {
+            if (XB && a.mu_a && (fr.x & 15) == FK_PERIODIC) {
                 // a periodic face is a boundary edge of each side, whose viscosity (its own
                 // element's mu_a) both sides' fluxes take; the partner stored its trace with its own
                 const double ms = a.ph.mu + a.mu_a[fe], mp = a.ph.mu + a.mu_a[fr.y >> 2];
@@ -573,7 +576,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             R fv[4], gv[4];
             viscous_flux(ph, q, dq, fv, gv);
             // e % mu_a (DGViscous_BR1.f90:307, DGViscous_IP.f90:84)
-            const R mue = (RS < 0 && a.mu_a) ? (R)(a.ph.mu + a.mu_a[e]) : ph.mu;
+            const R mue = (XB && a.mu_a) ? (R)(a.ph.mu + a.mu_a[e]) : ph.mu;
 #pragma unroll
             for (int c = 0; c < 4; ++c) { f[c] = (f[c] - mue * fv[c]) - R(0); gg[c] = (gg[c] - mue * gv[c]) - R(0); }
         }
@@ -597,13 +600,13 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         R uSB[4] = {R(0), R(0), R(0), R(0)};
         int neumann = 0, prescribed = 0;
         if (kind > FK_PERIODIC) {
-            prescribed = ghost_state<R, (RS < 0)>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB,
+            prescribed = ghost_state<R, XB>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB,
                                                   neumann);
             const int zr = (fr.x >> FF_RS_SHIFT) & 15;
             if (zr) rs = zr - 1;
         }
         R Fi[4];
-        if (RS < 0 && prescribed) {
+        if (XB && prescribed) {
             // WEAK_PRESCRIBED (DGSpatialDiscretizationMethods.f90:1194-1198): the inviscid
             // normal flux of the boundary's state
             R fp[4], gp[4];
@@ -619,7 +622,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
         } else {
             R Fv[4] = {R(0), R(0), R(0), R(0)};
             R G8[8] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-            const R muf = RS < 0 ? face_mu<R>(a, ph, fe, fr) : ph.mu;
+            const R muf = XB ? face_mu<R>(a, ph, fe, fr) : ph.mu;
             const R pen = IP ? ph.sigma0 * muf * (R)(P - 1) * (R)(P - 1) * invh : R(0);
             if (kind <= FK_PERIODIC) {
                 const R *uL = (kind == FK_INT_R) ? qo : qs, *uR = (kind == FK_INT_R) ? qs : qo;
@@ -825,7 +828,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             for (int c = 0; c < 4; ++c) qn[c] = (R)(TR)tq[c];
         }
         if constexpr (IP) {
-          if (RS >= 0 || a.ip_epi) {   // uniform: the launch's flag
+          if (!XB || a.ip_epi) {   // uniform: the launch's flag
             // the updated state in the compute type (FP32: a fresh copy, the node phase
             // overwrote the first one), the broken gradient, its traces and this side's
             // viscous flux: what k_gradient evaluates for the updated state
@@ -863,7 +866,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     }
 }
 
-template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR>
+template <int P, int EPB, bool NS, bool IP, int RS, typename R, typename TR, bool XBF = false>
 __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
     __shared__ StageSmem<P, EPB, NS, IP, R> S;
     double cq = a.cq, cg = a.cg;
@@ -873,7 +876,7 @@ __global__ __launch_bounds__((Tile<P, EPB>::TPB)) void k_stage(StageArgs a) {
         if (a.dt_cg) cg *= d;
     }
     const int b = xcd_tile(blockIdx.x, gridDim.x);
-    stage_body<P, EPB, NS, IP, RS, R, TR>(a, b * EPB, b, cq, cg, S);
+    stage_body<P, EPB, NS, IP, RS, R, TR, XBF>(a, b * EPB, b, cq, cg, S);
 }
 
 // ------------------------------------------------------------------ time step and sums
@@ -938,13 +941,13 @@ template <int P, int EPB>
 inline int tile_blocks(int K) { return (K + EPB - 1) / EPB; }
 
 // PHYS: 0 Euler, 1 Navier-Stokes BR1, 2 Navier-Stokes IP.
-template <int P, int EPB, int PHYS, int RS>
+template <int P, int EPB, int PHYS, int RS, bool XBF = false>
 hipError_t stage_launch(bool mixed, const StageArgs &a, hipStream_t st) {
     constexpr int TPB = Tile<P, EPB>::TPB;
     constexpr bool NS = PHYS >= 1, IP = PHYS == 2;
     const int nb = tile_blocks<P, EPB>(a.K);
-    if (mixed) hipLaunchKernelGGL((k_stage<P, EPB, NS, IP, RS, float, float>), dim3(nb), dim3(TPB), 0, st, a);
-    else hipLaunchKernelGGL((k_stage<P, EPB, NS, IP, RS, double, double>), dim3(nb), dim3(TPB), 0, st, a);
+    if (mixed) hipLaunchKernelGGL((k_stage<P, EPB, NS, IP, RS, float, float, XBF>), dim3(nb), dim3(TPB), 0, st, a);
+    else hipLaunchKernelGGL((k_stage<P, EPB, NS, IP, RS, double, double, XBF>), dim3(nb), dim3(TPB), 0, st, a);
     return hipGetLastError();
 }
 
@@ -990,6 +993,16 @@ hipError_t gradient_tile(bool mixed, const StageArgs &a, hipStream_t st) {
 template <int P, int PHYS>
 hipError_t stage_phys_p(int tile, bool mixed, const StageArgs &a, hipStream_t st) {
     if (a.ph.rs_kernel < 0) return stage_launch<P, Tiles<P>::large, PHYS, -1>(mixed, a, st);
+    if (a.ph.xbc) {   // one solver, the boundary kinds and artificial dissipation compiled in (wide tile)
+        constexpr int EPB = Tiles<P>::large;
+        switch (a.ph.rs_kernel) {
+        case 0: return stage_launch<P, EPB, PHYS, 0, true>(mixed, a, st);
+        case 1: return stage_launch<P, EPB, PHYS, 1, true>(mixed, a, st);
+        case 2: return stage_launch<P, EPB, PHYS, 2, true>(mixed, a, st);
+        case 3: return stage_launch<P, EPB, PHYS, 3, true>(mixed, a, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     return tile == 0 ? stage_tile<P, Tiles<P>::small, PHYS>(mixed, a, st)
                      : stage_tile<P, Tiles<P>::large, PHYS>(mixed, a, st);
 }
@@ -1047,7 +1060,7 @@ hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, i
 
 // ------------------------------------------------------------------ artificial viscosity
 // Jumps-based artificial viscosity of each element (JumpsBasedElementViscosity,
-// DGArtificialDissipation.f90:277-338), one lane per element, from the stage's Q traces: per
+// DGArtificialDissipation.f90:277-338), from the stage's Q traces: per
 // face sum_k w_k (2 (qL - qR) / (qL + qR))^2 dS for rho and rho E (Edge_ComputeJumps and its
 // variants, QuadElement_EdgeProcedures.incf:1348-1619: curved interior faces per-node dS, the
 // others dS of the edge's first node; boundary faces the boundary state uSB), normalised by the
@@ -1055,37 +1068,50 @@ hipError_t cons_partial_p(const double *Q, const double *jac, const double *w, i
 // forms (the FP32 form's traces, energy shifted back).
 template <int P, typename TR>
 __global__ __launch_bounds__(256) void k_art_visc(StageArgs a, const double *evol, const double *w, double *mu_a) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= a.K) return;
-    const Phys<double> ph(a.ph);
-    const double es = sizeof(TR) == 4 ? a.ph.Eref : 0.0;
-    double gr = 0.0, ge = 0.0, area = 0.0;
-    for (int f = 0; f < 4; ++f) {
+    // one lane per face node (EPB elements of 4P lanes per workgroup); the first lane of each
+    // element sums its 4P terms face by face in node order, then applies the switch
+    constexpr int FN = 4 * P, EPB = 256 / FN;
+    __shared__ double sj[EPB * FN][3];
+    const int t = threadIdx.x, s = t / FN, l = t % FN, e = blockIdx.x * EPB + s;
+    const bool act = (t < EPB * FN) && (e < a.K);
+    if (act) {
+        const Phys<double> ph(a.ph);
+        const double es = sizeof(TR) == 4 ? a.ph.Eref : 0.0;
+        const int f = l / P, k = l % P;
         const int4 fr = a.face[e * 4 + f];
         const int kind = fr.x & 15;
         const bool curved = fr.x & FF_CURVED, selfrev = fr.x & FF_SELFREV, rev = fr.x & FF_REV;
+        const TR *o = (const TR *)a.tr_in + ((int64_t)(e * 4 + f) * P + k) * 4;
+        const double qs[4] = {(double)o[0], (double)o[1], (double)o[2], (double)o[3] + es};
+        const int gi = curved ? (selfrev ? P - 1 - k : k) : 0;
+        const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
+        double qo[4], dS = geo[2];
+        if (kind <= FK_PERIODIC) {
+            const TR *nb = (const TR *)a.tr_in + ((int64_t)fr.y * P + (rev ? P - 1 - k : k)) * 4;
+            qo[0] = (double)nb[0]; qo[1] = (double)nb[1]; qo[2] = (double)nb[2]; qo[3] = (double)nb[3] + es;
+            if (kind == FK_PERIODIC) dS = a.egeo[3 * (int64_t)fr.z + 2];
+        } else {
+            double uB[4];
+            int neu;
+            ghost_state<double, true>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, geo[0], geo[1], uB, qo, neu);
+            dS = a.egeo[3 * (int64_t)fr.z + 2];
+        }
+        const double nr = 2.0 * (qs[0] - qo[0]) / (qs[0] + qo[0]);
+        const double ne = 2.0 * (qs[3] - qo[3]) / (qs[3] + qo[3]);
+        sj[t][0] = w[k] * nr * nr * dS;
+        sj[t][1] = w[k] * ne * ne * dS;
+        sj[t][2] = w[k] * geo[2];
+    }
+    __syncthreads();
+    if (!act || l != 0) return;
+    double gr = 0.0, ge = 0.0, area = 0.0;
+    for (int f = 0; f < 4; ++f) {
         double jr = 0.0, je = 0.0, ar = 0.0;
         for (int k = 0; k < P; ++k) {
-            const TR *o = (const TR *)a.tr_in + ((int64_t)(e * 4 + f) * P + k) * 4;
-            const double qs[4] = {(double)o[0], (double)o[1], (double)o[2], (double)o[3] + es};
-            const int gi = curved ? (selfrev ? P - 1 - k : k) : 0;
-            const double *geo = a.egeo + 3 * ((int64_t)fr.z + gi);
-            double qo[4], dS = geo[2];
-            ar = ar + w[k] * geo[2];
-            if (kind <= FK_PERIODIC) {
-                const TR *nb = (const TR *)a.tr_in + ((int64_t)fr.y * P + (rev ? P - 1 - k : k)) * 4;
-                qo[0] = (double)nb[0]; qo[1] = (double)nb[1]; qo[2] = (double)nb[2]; qo[3] = (double)nb[3] + es;
-                if (kind == FK_PERIODIC) dS = a.egeo[3 * (int64_t)fr.z + 2];
-            } else {
-                double uB[4];
-                int neu;
-                ghost_state<double, true>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, geo[0], geo[1], uB, qo, neu);
-                dS = a.egeo[3 * (int64_t)fr.z + 2];
-            }
-            const double nr = 2.0 * (qs[0] - qo[0]) / (qs[0] + qo[0]);
-            const double ne = 2.0 * (qs[3] - qo[3]) / (qs[3] + qo[3]);
-            jr = jr + w[k] * nr * nr * dS;
-            je = je + w[k] * ne * ne * dS;
+            const double *v = sj[s * FN + f * P + k];
+            jr = jr + v[0];
+            je = je + v[1];
+            ar = ar + v[2];
         }
         gr = gr + jr;
         ge = ge + je;
@@ -1108,7 +1134,8 @@ __global__ __launch_bounds__(256) void k_art_visc(StageArgs a, const double *evo
 template <int P>
 hipError_t art_visc_p(bool mixed, const StageArgs &a, const double *evol, const double *w, double *mu_a,
                       hipStream_t st) {
-    const int nb = (a.K + 255) / 256;
+    constexpr int EPB = 256 / (4 * P);
+    const int nb = (a.K + EPB - 1) / EPB;
     if (mixed) hipLaunchKernelGGL((k_art_visc<P, float>), dim3(nb), dim3(256), 0, st, a, evol, w, mu_a);
     else hipLaunchKernelGGL((k_art_visc<P, double>), dim3(nb), dim3(256), 0, st, a, evol, w, mu_a);
     return hipGetLastError();
