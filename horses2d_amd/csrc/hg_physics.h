@@ -372,117 +372,120 @@ __device__ inline bool is_adiabatic(int flags) {
     return (kind == FK_WALL && !(flags & FF_ISOTHERMAL)) || kind == FK_EULERWALL;
 }
 
-// Euler wall, Dirichlet, pressure outlet and pressure inlet in FP64 from the full-energy trace
-// Qt (boundary nodes only, so the FP32 form evaluates them in FP64 too).  uB: the Riemann
+// Euler wall, Dirichlet, pressure outlet and pressure inlet from the full-energy trace Qt, in
+// the compute type T (the FP32 form: on the full energy, so its boundary pressures carry FP32
+// rounding of O(1) values, as its fluxes do).  uB: the Riemann
 // solver's outer state or, where the function returns 1 (the reference's WEAK_PRESCRIBED),
 // the state whose inviscid normal flux is the face flux; uSB: the viscous boundary state;
 // neumann: zero viscous face flux at this node.
-__device__ inline int ghost_state_ext(int kind, int flags, const double *zd, const PhysParams &p, const double *Qt,
-                                      double nx, double ny, double *uB, double *uSB, int &neumann) {
-    const double gamma = p.gamma, gm1 = p.gm1, cv = p.cv, cp = p.cp;
-    const double rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
-    const double vnL = uL * nx + vL * ny;
+template <typename T>
+__device__ inline int ghost_state_ext(int kind, int flags, const double *zdd, const PhysParams &p, const T *Qt, T nx,
+                                      T ny, T *uB, T *uSB, int &neumann) {
+    const T gamma = (T)p.gamma, gm1 = (T)p.gm1, cv = (T)p.cv, cp = (T)p.cp;
+    const T zd[4] = {(T)zdd[0], (T)zdd[1], (T)zdd[2], (T)zdd[3]};
+    const T rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
+    const T vnL = uL * nx + vL * ny;
     neumann = 0;
     if (kind == FK_EULERWALL) {
         // EulerWall_UpdateSolution (EulerWall.incf:96-159): wall state without the normal
         // momentum, energy from p and the tangential momentum; viscous type ADIABATIC
-        const double rhovn = Qt[1] * nx + Qt[2] * ny, rhovt = -Qt[1] * ny + Qt[2] * nx;
-        const double pw = gm1 * (Qt[3] - 0.5 * (Qt[1] * Qt[1] + Qt[2] * Qt[2]) / Qt[0]);
+        const T rhovn = Qt[1] * nx + Qt[2] * ny, rhovt = -Qt[1] * ny + Qt[2] * nx;
+        const T pw = gm1 * (Qt[3] - T(0.5) * (Qt[1] * Qt[1] + Qt[2] * Qt[2]) / Qt[0]);
         uB[0] = Qt[0];
         uB[1] = Qt[1] - rhovn * nx;
         uB[2] = Qt[2] - rhovn * ny;
-        uB[3] = cv * pw + 0.5 * rhovt * rhovt / Qt[0];
+        uB[3] = cv * pw + T(0.5) * rhovt * rhovt / Qt[0];
         for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
         return 1;
     }
-    const double pL = gm1 * (Qt[3] - 0.5 * (Qt[1] * uL + Qt[2] * vL));
-    const double aL = sqrt(gamma * pL / rhoL);
-    const double ML = sqrt(uL * uL + vL * vL) / aL;
+    const T pL = gm1 * (Qt[3] - T(0.5) * (Qt[1] * uL + Qt[2] * vL));
+    const T aL = sqrt(gamma * pL / rhoL);
+    const T ML = sqrt(uL * uL + vL * vL) / aL;
     if (kind == FK_DIRICHLET) {
         // Dirichlet.incf:78-104 (uB = the zone state), :106-337 (uSB and the viscous type)
         for (int k = 0; k < 4; ++k) uB[k] = zd[k];
-        if (vnL <= 0.0) {
-            if (ML > 1.0) {
+        if (vnL <= T(0)) {
+            if (ML > T(1)) {
                 for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
             } else {
                 uSB[0] = uB[0]; uSB[1] = uB[1]; uSB[2] = uB[2];
-                uSB[3] = cv * pL + 0.5 * (uSB[1] * uSB[1] + uSB[2] * uSB[2]) / uSB[0];
+                uSB[3] = cv * pL + T(0.5) * (uSB[1] * uSB[1] + uSB[2] * uSB[2]) / uSB[0];
             }
-        } else if (ML > 1.0) {
+        } else if (ML > T(1)) {
             for (int k = 0; k < 4; ++k) uSB[k] = Qt[k];
             neumann = 1;
         } else {
-            const double pB = gm1 * (uB[3] - 0.5 * (uB[1] * uB[1] + uB[2] * uB[2]) / uB[0]);
+            const T pB = gm1 * (uB[3] - T(0.5) * (uB[1] * uB[1] + uB[2] * uB[2]) / uB[0]);
             uSB[0] = Qt[0]; uSB[1] = Qt[1]; uSB[2] = Qt[2];
-            uSB[3] = cv * pB + 0.5 * (uSB[1] * uL + uSB[2] * vL);
+            uSB[3] = cv * pB + T(0.5) * (uSB[1] * uL + uSB[2] * vL);
         }
         return 0;
     }
     if (kind == FK_POUTLET) {
         // PressureOutletBC_UpdateSolution (PressureOutlet.incf:151-450)
-        const double rhoI = zd[0], uI = zd[1] / rhoI, vI = zd[2] / rhoI;
-        const double pI = gm1 * (zd[3] - 0.5 * zd[1] * uI - 0.5 * zd[2] * vI);
-        if (ML > 1.0) {
-            const double *src = vnL < 0.0 ? zd : Qt;
+        const T rhoI = zd[0], uI = zd[1] / rhoI, vI = zd[2] / rhoI;
+        const T pI = gm1 * (zd[3] - T(0.5) * zd[1] * uI - T(0.5) * zd[2] * vI);
+        if (ML > T(1)) {
+            const T *src = vnL < T(0) ? zd : Qt;
             for (int k = 0; k < 4; ++k) { uB[k] = src[k]; uSB[k] = src[k]; }
-            neumann = vnL < 0.0 ? 0 : 1;
+            neumann = vnL < T(0) ? 0 : 1;
             return 0;
         }
-        double rhoR, uR, vR, pR;
+        T rhoR, uR, vR, pR;
         if (flags & FF_REFLECTIVE) {
-            const double R0 = pL - aL * aL * rhoL, Rplus = vnL + 2.0 * cv * aL;
+            const T R0 = pL - aL * aL * rhoL, Rplus = vnL + T(2) * cv * aL;
             rhoR = -(R0 - pI) / (aL * aL);
-            const double aR = sqrt(gamma * pI / rhoR), vnR = Rplus - 2.0 * aR * cv;
+            const T aR = sqrt(gamma * pI / rhoR), vnR = Rplus - T(2) * aR * cv;
             uR = uL + (vnR - vnL) * nx;
             vR = vL + (vnR - vnL) * ny;
             pR = pI;
         } else {
-            const double nrm = sqrt(uI * uI + vI * vI);
-            pR = 2.0 * pI - pL;
-            uR = vnL < 0.0 ? fabs(vnL) * (uI / nrm) : uL;
-            vR = vnL < 0.0 ? fabs(vnL) * (vI / nrm) : vL;
+            const T nrm = sqrt(uI * uI + vI * vI);
+            pR = T(2) * pI - pL;
+            uR = vnL < T(0) ? fabs(vnL) * (uI / nrm) : uL;
+            vR = vnL < T(0) ? fabs(vnL) * (vI / nrm) : vL;
             rhoR = rhoL;
         }
         uB[0] = rhoR; uB[1] = rhoR * uR; uB[2] = rhoR * vR;
-        uB[3] = cv * pR + 0.5 * (uB[1] * uR + uB[2] * vR);
-        if (vnL < 0.0) {
+        uB[3] = cv * pR + T(0.5) * (uB[1] * uR + uB[2] * vR);
+        if (vnL < T(0)) {
             for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
         } else {
             uSB[0] = rhoL; uSB[1] = rhoL * uL; uSB[2] = rhoL * vL;
-            uSB[3] = cv * pR + 0.5 * rhoL * (uL * uL + vL * vL);
+            uSB[3] = cv * pR + T(0.5) * rhoL * (uL * uL + vL * vL);
         }
         return 0;
     }
     // FK_PINLET: PressureInletBC_UpdateSolution (PressureInlet.incf:134-410); uB is the state
     // of the prescribed flux (the curved-edge supersonic outflow reads the trace transposed in
     // the reference, :344; the straight-edge form, :247, is used for both)
-    const double uE = zd[1] / zd[0], vE = zd[2] / zd[0];
-    const double pE = gm1 * (zd[3] - 0.5 * (zd[1] * uE + zd[2] * vE));
-    const double MExtSq = (uE * uE + vE * vE) / (gamma * pE) * zd[0];
-    const double atExtSq = gamma * pE / zd[0] * (1.0 + 0.5 * gm1 * MExtSq);
-    const double sExt = pE / pow(zd[0], gamma);
-    const double ptExt = pE * pow(1.0 + 0.5 * gm1 * MExtSq, cp);
-    const double Rplus = vnL + 2.0 * cv * aL;
-    if (vnL <= 0.0 && ML > 1.0) {
+    const T uE = zd[1] / zd[0], vE = zd[2] / zd[0];
+    const T pE = gm1 * (zd[3] - T(0.5) * (zd[1] * uE + zd[2] * vE));
+    const T MExtSq = (uE * uE + vE * vE) / (gamma * pE) * zd[0];
+    const T atExtSq = gamma * pE / zd[0] * (T(1) + T(0.5) * gm1 * MExtSq);
+    const T sExt = pE / pow(zd[0], gamma);
+    const T ptExt = pE * pow(T(1) + T(0.5) * gm1 * MExtSq, cp);
+    const T Rplus = vnL + T(2) * cv * aL;
+    if (vnL <= T(0) && ML > T(1)) {
         for (int k = 0; k < 4; ++k) { uB[k] = zd[k]; uSB[k] = zd[k]; }
-    } else if (vnL <= 0.0) {
-        const double a = (gamma + 1.0) / gm1, b = -2.0 * Rplus, c = 0.5 * gm1 * (Rplus * Rplus) - atExtSq;
-        const double aR = (-b + sqrt(b * b - 4.0 * a * c)) / (2.0 * a);
-        const double rhoR = pow(aR * aR / (gamma * sExt), cv);
-        const double pR = aR * aR * rhoR / gamma, vnR = Rplus - 2.0 * cv * aR;
-        const double uR = vnR * nx, vR = vnR * ny;
+    } else if (vnL <= T(0)) {
+        const T a = (gamma + T(1)) / gm1, b = -T(2) * Rplus, c = T(0.5) * gm1 * (Rplus * Rplus) - atExtSq;
+        const T aR = (-b + sqrt(b * b - T(4) * a * c)) / (T(2) * a);
+        const T rhoR = pow(aR * aR / (gamma * sExt), cv);
+        const T pR = aR * aR * rhoR / gamma, vnR = Rplus - T(2) * cv * aR;
+        const T uR = vnR * nx, vR = vnR * ny;
         uB[0] = rhoR; uB[1] = rhoR * uR; uB[2] = rhoR * vR;
-        uB[3] = cv * pR + 0.5 * (uB[1] * uR + uB[2] * vR);
+        uB[3] = cv * pR + T(0.5) * (uB[1] * uR + uB[2] * vR);
         uSB[0] = rhoR; uSB[1] = rhoR * uR; uSB[2] = rhoR * vR;
-        uSB[3] = cv * pL + 0.5 * rhoR * (uR * uR + vR * vR);
-    } else if (ML > 1.0) {
+        uSB[3] = cv * pL + T(0.5) * rhoR * (uR * uR + vR * vR);
+    } else if (ML > T(1)) {
         for (int k = 0; k < 4; ++k) { uB[k] = Qt[k]; uSB[k] = Qt[k]; }
         neumann = 1;
     } else {
         uB[0] = Qt[0]; uB[1] = Qt[1]; uB[2] = Qt[2];
-        uB[3] = cv * (2.0 * ptExt - pL) + 0.5 * (uB[1] * uL + uB[2] * vL);
+        uB[3] = cv * (T(2) * ptExt - pL) + T(0.5) * (uB[1] * uL + uB[2] * vL);
         uSB[0] = uB[0]; uSB[1] = uB[1]; uSB[2] = uB[2];
-        uSB[3] = cv * ptExt + 0.5 * (uB[1] * uL + uB[2] * vL);
+        uSB[3] = cv * ptExt + T(0.5) * (uB[1] * uL + uB[2] * vL);
     }
     return 1;
 }
@@ -499,14 +502,14 @@ __device__ inline int ghost_state(int kind, int flags, const double *zd, const P
     neumann = 0;
     if constexpr (XB) {
       if (kind >= FK_EULERWALL) {
-        const double es = Phys<R>::shift ? pp.Eref : 0.0;
-        const double Qd[4] = {(double)Qt[0], (double)Qt[1], (double)Qt[2], (double)Qt[3] + es};
-        double uBd[4], uSd[4];
-        const int pres = ghost_state_ext(kind, flags, zd, pp, Qd, (double)nx, (double)ny, uBd, uSd, neumann);
+        const R es = (R)(Phys<R>::shift ? pp.Eref : 0.0);
+        const R Qd[4] = {Qt[0], Qt[1], Qt[2], Qt[3] + es};
+        R uBd[4], uSd[4];
+        const int pres = ghost_state_ext<R>(kind, flags, zd, pp, Qd, nx, ny, uBd, uSd, neumann);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { uB[k] = (R)uBd[k]; uSB[k] = (R)uSd[k]; }
-        uB[3] = (R)(uBd[3] - es);
-        uSB[3] = (R)(uSd[3] - es);
+        for (int k = 0; k < 3; ++k) { uB[k] = uBd[k]; uSB[k] = uSd[k]; }
+        uB[3] = uBd[3] - es;
+        uSB[3] = uSd[3] - es;
         return pres;
       }
     }
