@@ -32,8 +32,12 @@ BC_VISCOUSWALL = 4
 BC_FARFIELD = 5
 BC_PRESSUREOUTLET = 6
 BC_PRESSUREINLET = 7
+BC_NEWDIRICHLET = 9
 # pressure outlet "Outflow type" (PressureOutlet.incf:66-72)
 OUTFLOW = {"partially non reflective": 0, "reflective": 1}
+# newDirichlet "Mode" and "Outflow pressure" (DGBoundaryConditions.f90:47-54)
+NEWDIRICHLET_MODE = {"specify speed": 1, "specify total pressure": 2}
+NEWDIRICHLET_OUTFLOW = {"static": 1, "total": 2}
 
 RIEMANN = {"roe": 0, "hll": 1, "exact": 2, "lf": 3}
 VISCOUS = {"none": 0, "br1": 1, "ip": 2}
@@ -52,17 +56,21 @@ class Zone:
     direction: int = 0          # periodic: 1 = x-translation, 2 = y-translation
     partner: int = 0            # periodic: connected marker (1-based)
     isothermal: int = 0         # viscous wall
-    q: tuple = (0.0, 0.0, 0.0, 0.0)   # far field, Dirichlet, pressure outlet/inlet state (dimensionless)
+    q: tuple = (0.0, 0.0, 0.0, 0.0)   # far field, (new)Dirichlet, pressure outlet/inlet state (dimensionless)
     Tw: float = 1.0
     wall_velocity: tuple = (0.0, 0.0)
     riemann: int = -1           # per-zone Riemann solver, -1 = interior method
     outflow: int = 0            # pressure outlet: OUTFLOW[...]
+    mode: int = 1               # newDirichlet: NEWDIRICHLET_MODE[...]
+    outflow_pressure: int = 1   # newDirichlet: NEWDIRICHLET_OUTFLOW[...]
 
     def ints(self):
         if self.kind == BC_PERIODIC:
             return (self.direction, self.partner)
         if self.kind == BC_PRESSUREOUTLET:
             return (self.outflow, 0)
+        if self.kind == BC_NEWDIRICHLET:
+            return (self.mode, self.outflow_pressure)
         return (self.isothermal, 0)
 
     def reals(self):

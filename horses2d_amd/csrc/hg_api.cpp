@@ -352,7 +352,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     for (const auto &Z : m.zones) {
         if (Z.riemann >= 0 && Z.riemann != opt->riemann) ph.rs_kernel = -1;
         if (Z.type == HG_BC_EULERWALL || Z.type == HG_BC_DIRICHLET || Z.type == HG_BC_PRESSUREOUTLET ||
-            Z.type == HG_BC_PRESSUREINLET)
+            Z.type == HG_BC_PRESSUREINLET || Z.type == HG_BC_NEWDIRICHLET)
             ph.xbc = 1;
     }
     if (opt->artificial_dissipation != HG_AD_NONE && opt->artificial_dissipation != HG_AD_PHYSICAL)
@@ -420,6 +420,11 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
                     case HG_BC_DIRICHLET: flags = hg::FK_DIRICHLET; break;
                     case HG_BC_PRESSUREOUTLET:
                         flags = hg::FK_POUTLET | (Z.isothermal == HG_OUTFLOW_REFLECTIVE ? hg::FF_REFLECTIVE : 0);
+                        break;
+                    case HG_BC_NEWDIRICHLET:
+                        flags = hg::FK_NEWDIRICHLET |
+                                (Z.nd_mode == HG_NEWDIRICHLET_SPECIFY_TOTAL_PRESSURE ? hg::FF_TOTAL_INFLOW : 0) |
+                                (Z.nd_outflow == HG_NEWDIRICHLET_TOTAL_PRESSURE ? hg::FF_TOTAL_OUTFLOW : 0);
                         break;
                     default: flags = hg::FK_PINLET; break;
                     }

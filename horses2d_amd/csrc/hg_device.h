@@ -17,12 +17,15 @@ enum FaceKind : int {
     FK_DIRICHLET = 6, // Dirichlet state (Dirichlet.incf:106-337)
     FK_POUTLET = 7,   // pressure outlet (PressureOutlet.incf:151-450)
     FK_PINLET = 8,    // pressure inlet, prescribed flux (PressureInlet.incf:134-410)
+    FK_NEWDIRICHLET = 9, // characteristic Dirichlet state (newDirichlet.incf:146-433)
 };
 constexpr int FF_CURVED = 1 << 4;    // per-node normals and dS
 constexpr int FF_REV = 1 << 5;       // neighbor trace index runs backwards
 constexpr int FF_SELFREV = 1 << 6;   // edge geometry index runs backwards (RIGHT side of an inverse edge)
 constexpr int FF_ISOTHERMAL = 1 << 7;  // viscous wall: isothermal
 constexpr int FF_REFLECTIVE = 1 << 7;  // pressure outlet: reflective outflow (same bit, other kind)
+constexpr int FF_TOTAL_INFLOW = 1 << 7; // newDirichlet: subsonic inflow at the total pressure (same bit)
+constexpr int FF_TOTAL_OUTFLOW = 1 << 12; // newDirichlet: subsonic outflow at the total pressure
 constexpr int FF_RS_SHIFT = 8;       // (zone Riemann solver + 1), 0 = interior method
 
 // Viscous face-flux traces: per element face node, mu times this side's viscous flux from its

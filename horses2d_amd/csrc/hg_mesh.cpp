@@ -425,12 +425,13 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
         ZoneDesc &Z = m.zones[z];
         Z.type = in.bc_type[z];
         Z.direction = in.bc_int[2 * z]; Z.partner = in.bc_int[2 * z + 1]; Z.isothermal = in.bc_int[2 * z];
+        Z.nd_mode = in.bc_int[2 * z]; Z.nd_outflow = in.bc_int[2 * z + 1];
         for (int c = 0; c < 4; ++c) Z.q[c] = in.bc_real[4 * z + c];
         Z.Tw = in.bc_real[4 * z]; Z.vw[0] = in.bc_real[4 * z + 1]; Z.vw[1] = in.bc_real[4 * z + 2];
         Z.riemann = in.bc_riemann ? in.bc_riemann[z] : -1;
         switch (Z.type) {
         case HG_BC_PERIODIC: case HG_BC_DIRICHLET: case HG_BC_EULERWALL: case HG_BC_VISCOUSWALL:
-        case HG_BC_FARFIELD: case HG_BC_PRESSUREOUTLET: case HG_BC_PRESSUREINLET: break;
+        case HG_BC_FARFIELD: case HG_BC_PRESSUREOUTLET: case HG_BC_PRESSUREINLET: case HG_BC_NEWDIRICHLET: break;
         default: return "unsupported boundary condition type";
         }
     }

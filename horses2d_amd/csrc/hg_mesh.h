@@ -50,6 +50,7 @@ struct Elem {
 
 struct ZoneDesc {
     int type = 0, direction = 0, partner = 0, isothermal = 0, riemann = -1;   // isothermal: also the outflow type
+    int nd_mode = 0, nd_outflow = 0;   // newDirichlet: HG_NEWDIRICHLET_* mode and outflow pressure
     double q[4] = {0, 0, 0, 0};
     double Tw = 1.0, vw[2] = {0, 0};
     std::vector<int> edges;

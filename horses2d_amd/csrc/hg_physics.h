@@ -456,6 +456,7 @@ __device__ inline int ghost_state_ext(int kind, int flags, const double *zdd, co
         }
         return 0;
     }
+    // FK_PINLET and FK_NEWDIRICHLET share the external total conditions.
     // FK_PINLET: PressureInletBC_UpdateSolution (PressureInlet.incf:134-410); uB is the state
     // of the prescribed flux (the curved-edge supersonic outflow reads the trace transposed in
     // the reference, :344; the straight-edge form, :247, is used for both)
@@ -465,6 +466,36 @@ __device__ inline int ghost_state_ext(int kind, int flags, const double *zdd, co
     const T atExtSq = gamma * pE / zd[0] * (T(1) + T(0.5) * gm1 * MExtSq);
     const T sExt = pE / pow(zd[0], gamma);
     const T ptExt = pE * pow(T(1) + T(0.5) * gm1 * MExtSq, cp);
+    if (kind == FK_NEWDIRICHLET) {
+        // newDirichletBC_UpdateSolution (newDirichlet.incf:146-433): the Riemann solver's outer
+        // state; uSB is the same array (:308-310), NEUMANN only at supersonic outflow
+        if (vnL <= T(0) && ML > T(1)) {
+            for (int k = 0; k < 4; ++k) uB[k] = zd[k];
+        } else if (vnL <= T(0)) {
+            const T pR = pL, rhoR = pow(pR / sExt, T(1) / gamma), aR = sqrt(gamma * pR / rhoR);
+            if (flags & FF_TOTAL_INFLOW) {
+                const T nrm = sqrt(uE * uE + vE * vE);
+                const T MR = pR >= ptExt ? -sqrt(T(2) / gm1 * (pow(pR / ptExt, gm1 / gamma) - T(1)))
+                                         : sqrt(T(2) / gm1 * (pow(ptExt / pR, gm1 / gamma) - T(1)));
+                uB[0] = rhoR;
+                uB[1] = rhoR * aR * MR * (uE / nrm);
+                uB[2] = rhoR * aR * MR * (vE / nrm);
+                uB[3] = cv * pR + T(0.5) * rhoR * aR * aR * MR * MR;
+            } else {
+                uB[0] = zd[0]; uB[1] = zd[1]; uB[2] = zd[2];
+                uB[3] = cv * pR + T(0.5) * (zd[1] * zd[1] + zd[2] * zd[2]) / zd[0];
+            }
+        } else if (ML > T(1)) {
+            for (int k = 0; k < 4; ++k) uB[k] = Qt[k];
+            neumann = 1;
+        } else {
+            const T pO = (flags & FF_TOTAL_OUTFLOW) ? ptExt : pE;
+            uB[0] = Qt[0]; uB[1] = Qt[1]; uB[2] = Qt[2];
+            uB[3] = cv * (T(2) * pO - pL) + T(0.5) * (uB[1] * uL + uB[2] * vL);
+        }
+        for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
+        return 0;
+    }
     const T Rplus = vnL + T(2) * cv * aL;
     if (vnL <= T(0) && ML > T(1)) {
         for (int k = 0; k < 4; ++k) { uB[k] = zd[k]; uSB[k] = zd[k]; }

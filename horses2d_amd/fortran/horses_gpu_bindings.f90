@@ -14,8 +14,12 @@ module horses_gpu_bindings
 
    integer(c_int32_t), parameter, public :: HG_BC_PERIODIC = 1, HG_BC_DIRICHLET = 2, HG_BC_EULERWALL = 3, &
                                             HG_BC_VISCOUSWALL = 4, HG_BC_FARFIELD = 5, &
-                                            HG_BC_PRESSUREOUTLET = 6, HG_BC_PRESSUREINLET = 7
+                                            HG_BC_PRESSUREOUTLET = 6, HG_BC_PRESSUREINLET = 7, &
+                                            HG_BC_NEWDIRICHLET = 9
    integer(c_int32_t), parameter, public :: HG_OUTFLOW_PARTIALLY_NON_REFLECTIVE = 0, HG_OUTFLOW_REFLECTIVE = 1
+   integer(c_int32_t), parameter, public :: HG_NEWDIRICHLET_SPECIFY_SPEED = 1, &
+                                            HG_NEWDIRICHLET_SPECIFY_TOTAL_PRESSURE = 2, &
+                                            HG_NEWDIRICHLET_STATIC_PRESSURE = 1, HG_NEWDIRICHLET_TOTAL_PRESSURE = 2
    integer(c_int32_t), parameter, public :: HG_AD_NONE = 0, HG_AD_PHYSICAL = 1
    integer(c_int32_t), parameter, public :: HG_PROBE_RHO = 0, HG_PROBE_RHOU = 1, HG_PROBE_RHOV = 2, &
                                             HG_PROBE_RHOE = 3, HG_PROBE_RHOT = 4, HG_PROBE_RHOUT = 5, &

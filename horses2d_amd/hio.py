@@ -143,7 +143,8 @@ RIEMANN_NAMES = {"Roe": "roe", "HLL": "hll", "Exact": "exact", "Lax-Friedrichs":
 SCHEME_NAMES = {"Explicit-Euler": "euler", "Williamson RK3": "rk3", "Williamson RK5": "rk5"}
 ZONE_TYPES = {"Periodic": C.BC_PERIODIC, "Dirichlet": C.BC_DIRICHLET, "Euler wall": C.BC_EULERWALL,
               "Viscous wall": C.BC_VISCOUSWALL, "Farfield": C.BC_FARFIELD,
-              "Pressure outlet": C.BC_PRESSUREOUTLET, "Pressure inlet": C.BC_PRESSUREINLET}
+              "Pressure outlet": C.BC_PRESSUREOUTLET, "Pressure inlet": C.BC_PRESSUREINLET,
+              "newDirichlet": C.BC_NEWDIRICHLET}
 
 
 def zone_state(path: str, in_label: str, Mach: float, T_ref: float, p_ref: float) -> Tuple[float, ...]:
@@ -204,12 +205,18 @@ def read_zone(path: str, marker: int, Mach: float, T_ref: float, p_ref: float) -
         vd = read_value_in_region(path, "Velocity direction", lab)
         d = [1.0, 0.0] if not vd else _array(vd)[:2]
         z.wall_velocity = (v * d[0], v * d[1])
-    elif kind in (C.BC_FARFIELD, C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, C.BC_PRESSUREINLET):
+    elif kind in (C.BC_FARFIELD, C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, C.BC_PRESSUREINLET, C.BC_NEWDIRICHLET):
         z.q = zone_state(path, lab, Mach, T_ref, p_ref)
         if kind == C.BC_PRESSUREOUTLET:
             # PressureOutletBC_Construct (PressureOutlet.incf:66-72): default partially non reflective
             of = read_value_in_region(path, "Outflow type", lab)
             z.outflow = C.OUTFLOW["reflective"] if of == "Reflective" else C.OUTFLOW["partially non reflective"]
+        if kind == C.BC_NEWDIRICHLET:
+            # newDirichletBC_Construct (newDirichlet.incf:79-105): defaults Specify speed, Static
+            md = read_value_in_region(path, "Mode", lab)
+            z.mode = C.NEWDIRICHLET_MODE["specify total pressure" if md == "Specify total pressure" else "specify speed"]
+            op = read_value_in_region(path, "Outflow pressure", lab)
+            z.outflow_pressure = C.NEWDIRICHLET_OUTFLOW["total" if op == "Total" else "static"]
     return z
 
 

@@ -35,10 +35,18 @@ typedef struct hg_batch hg_batch;
 #define HG_BC_FARFIELD 5
 #define HG_BC_PRESSUREOUTLET 6
 #define HG_BC_PRESSUREINLET 7
+#define HG_BC_NEWDIRICHLET 9   /* 8 (Riemann) is not supported: its outer state is undefined */
 
 /* Pressure outlet "Outflow type" (BoundaryConditions/PressureOutlet.incf:66-72). */
 #define HG_OUTFLOW_PARTIALLY_NON_REFLECTIVE 0
 #define HG_OUTFLOW_REFLECTIVE 1
+
+/* newDirichlet "Mode" and "Outflow pressure" (BoundaryConditions/DGBoundaryConditions.f90:47-54,
+ * newDirichlet.incf:79-105; any other value selects the reference's default, the first). */
+#define HG_NEWDIRICHLET_SPECIFY_SPEED 1
+#define HG_NEWDIRICHLET_SPECIFY_TOTAL_PRESSURE 2
+#define HG_NEWDIRICHLET_STATIC_PRESSURE 1
+#define HG_NEWDIRICHLET_TOTAL_PRESSURE 2
 
 /* Riemann solvers: Roe, HLL, exact (Physics/RiemannSolvers.f90:33-399) and the
  * Lax-Friedrichs flux, which the reference does not have. */
@@ -96,9 +104,10 @@ typedef struct {
     const int32_t *bc_type;   /* [nmarkers] HG_BC_* */
     const int32_t *bc_int;    /* [nmarkers][2] periodic: (direction 1=x|2=y, partner marker);
                                  viscous wall: (isothermal 0|1, 0);
-                                 pressure outlet: (HG_OUTFLOW_*, 0); others (0, 0) */
+                                 pressure outlet: (HG_OUTFLOW_*, 0);
+                                 newDirichlet: (mode, outflow pressure); others (0, 0) */
     const double *bc_real;    /* [nmarkers][4] farfield, Dirichlet, pressure outlet, pressure
-                                 inlet: the zone state q (dimensionless, as their _Construct
+                                 inlet, newDirichlet: the zone state q (dimensionless, as their _Construct
                                  forms it; "Use function" states are not supported);
                                  viscous wall: (Tw, vx, vy, 0); Euler wall: unused */
     const int32_t *bc_riemann;/* [nmarkers] per-zone Riemann solver or -1 (may be NULL) */

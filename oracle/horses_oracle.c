@@ -35,6 +35,7 @@
 #define BC_FARFIELD 5
 #define BC_PRESSUREOUTLET 6
 #define BC_PRESSUREINLET 7
+#define BC_NEWDIRICHLET 9
 #define WEAK_RIEMANN 1
 #define WEAK_PRESCRIBED 2
 #define VBC_UNDEFINED 0
@@ -45,6 +46,11 @@
 /* Pressure outlet "Outflow type" (BoundaryConditions/PressureOutlet.incf:66-72) */
 #define PARTIALLY_NON_REFLECTIVE_OUTFLOW 0
 #define REFLECTIVE_OUTFLOW 1
+/* newDirichlet "Mode" and "Outflow pressure" (BoundaryConditions/DGBoundaryConditions.f90:47-54) */
+#define SPECIFY_SPEED 1
+#define SPECIFY_TOTAL_PRESSURE 2
+#define STATIC_PRESSURE 1
+#define TOTAL_PRESSURE 2
 
 #define EBOTTOM 1
 #define ERIGHT 2
@@ -603,6 +609,8 @@ typedef struct {
     int partner;    /* periodic connected marker */
     int wall_iso;   /* viscous wall: 0 adiabatic, 1 isothermal */
     int outflow;    /* pressure outlet: PARTIALLY_NON_REFLECTIVE_OUTFLOW or REFLECTIVE_OUTFLOW */
+    int nd_mode;    /* newDirichlet: SPECIFY_SPEED or SPECIFY_TOTAL_PRESSURE */
+    int nd_outflow; /* newDirichlet: STATIC_PRESSURE or TOTAL_PRESSURE */
     double q[4];    /* far field, Dirichlet, pressure outlet / inlet: the zone state (dimensionless) */
     double Tw, vw[2];
     int rs;         /* zone Riemann solver (-1 = interior method) */
@@ -977,6 +985,9 @@ static Mesh *mesh_build(const MeshIn *in) {
         Z->partner = in->bc_int[2 * (z - 1) + 1];
         Z->wall_iso = in->bc_int[2 * (z - 1)];
         Z->outflow = in->bc_int[2 * (z - 1)];
+        /* newDirichletBC_Construct (newDirichlet.incf:79-105): anything else is the default */
+        Z->nd_mode = in->bc_int[2 * (z - 1)] == SPECIFY_TOTAL_PRESSURE ? SPECIFY_TOTAL_PRESSURE : SPECIFY_SPEED;
+        Z->nd_outflow = in->bc_int[2 * (z - 1) + 1] == TOTAL_PRESSURE ? TOTAL_PRESSURE : STATIC_PRESSURE;
         for (int k = 0; k < 4; ++k) Z->q[k] = in->bc_real[4 * (z - 1) + k];
         Z->Tw = in->bc_real[4 * (z - 1)]; Z->vw[0] = in->bc_real[4 * (z - 1) + 1]; Z->vw[1] = in->bc_real[4 * (z - 1) + 2];
         Z->rs = in->bc_rs ? in->bc_rs[z - 1] : -1;
@@ -1228,6 +1239,54 @@ static void update_bcs(Mesh *m) {
                         uSB[3] = ph->cv * pR + 0.5 * rhoL * (uL * uL + vL * vL);
                     }
                     ed->vbc[i] = VBC_DIRICHLET;
+                }
+            } else if (Z->type == BC_NEWDIRICHLET) {
+                /* newDirichletBC_UpdateSolution (newDirichlet.incf:146-433): the outer state of
+                 * the Riemann flux from the characteristics against the zone state; uSB is
+                 * uB itself (:308-310, :429-431), the viscous type Neumann only for
+                 * supersonic outflow.  Straight edges use their one normal (:201). */
+                const double *q = Z->q;
+                double uE = q[1] / q[0], vE = q[2] / q[0];
+                double nrm = sqrt(uE * uE + vE * vE), nIx = uE / nrm, nIy = vE / nrm;
+                double pE = ph->gm1 * (q[3] - 0.5 * (q[1] * uE + q[2] * vE));
+                double MExtSq = (uE * uE + vE * vE) / (ph->gamma * pE) * q[0];
+                double sExt = pE / pow(q[0], ph->gamma);
+                double ptExt = pE * pow(1.0 + 0.5 * ph->gm1 * MExtSq, ph->cp);
+                for (int i = 0; i < P; ++i) {
+                    const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+                    const double *Qt = ed->tq[0] + 4 * i;
+                    double *uB = ed->uB + 4 * i, *uSB = ed->uSB + 4 * i;
+                    double rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
+                    double vnL = uL * nn[0] + vL * nn[1];
+                    double pL = ph->gm1 * (Qt[3] - 0.5 * (Qt[1] * uL + Qt[2] * vL));
+                    double aL = sqrt(ph->gamma * pL / rhoL);
+                    double ML = sqrt(uL * uL + vL * vL) / aL;
+                    ed->vbc[i] = VBC_DIRICHLET;
+                    if (vnL <= 0.0 && ML > 1.0) {
+                        for (int k = 0; k < 4; ++k) uB[k] = q[k];
+                    } else if (vnL <= 0.0) {
+                        double pR = pL, rhoR = pow(pR / sExt, 1.0 / ph->gamma);
+                        double aR = sqrt(ph->gamma * pR / rhoR);
+                        if (Z->nd_mode == SPECIFY_SPEED) {
+                            uB[0] = q[0]; uB[1] = q[1]; uB[2] = q[2];
+                            uB[3] = ph->cv * pR + 0.5 * (uB[1] * uB[1] + uB[2] * uB[2]) / uB[0];
+                        } else {
+                            double MR = pR >= ptExt ? -sqrt(2.0 / ph->gm1 * (pow(pR / ptExt, ph->gm1 / ph->gamma) - 1.0))
+                                                    : sqrt(2.0 / ph->gm1 * (pow(ptExt / pR, ph->gm1 / ph->gamma) - 1.0));
+                            uB[0] = rhoR;
+                            uB[1] = rhoR * aR * MR * nIx;
+                            uB[2] = rhoR * aR * MR * nIy;
+                            uB[3] = ph->cv * pR + 0.5 * rhoR * aR * aR * MR * MR;
+                        }
+                    } else if (ML > 1.0) {
+                        for (int k = 0; k < 4; ++k) uB[k] = Qt[k];
+                        ed->vbc[i] = VBC_NEUMANN;
+                    } else {
+                        double pO = Z->nd_outflow == STATIC_PRESSURE ? pE : ptExt;
+                        uB[0] = Qt[0]; uB[1] = Qt[1]; uB[2] = Qt[2];
+                        uB[3] = ph->cv * (2.0 * pO - pL) + 0.5 * (uB[1] * uL + uB[2] * vL);
+                    }
+                    for (int k = 0; k < 4; ++k) uSB[k] = uB[k];
                 }
             } else if (Z->type == BC_PRESSUREINLET) {
                 /* PressureInletBC_UpdateSolution (PressureInlet.incf:134-410): prescribed flux
@@ -2027,6 +2086,32 @@ int or_volume_integrals(void *h, double *out) {
  *   out[6] "flow"      sum w (rho u . n) dS / rho      (:254-281)
  *   out[7..8] "viscous" sum w (tau . n) dS              (Compute_TensorVectorSurfaceIntegral :305-342,
  *                     tau of ComputeViscousTensor, ViscousFluxes.f90:416-461; zero for Euler) */
+/* The boundary states of zone `marker` from its last update_bcs (the last time derivative
+ * or stage): per edge node, in the zone's edge order, the trace Q (4), the outward normal (2),
+ * uB (4), uSB (4) and the viscous type.  Returns the node count; with NULL arrays only counts. */
+int or_boundary_states(void *h, int marker, double *qt, double *nrm, double *ub, double *usb, int *vbc) {
+    Mesh *m = h;
+    if (marker < 1 || marker > m->nmarkers) return fail("or_boundary_states: bad marker");
+    const Zone *Z = &m->zones[marker - 1];
+    if (Z->type == BC_PERIODIC) return fail("or_boundary_states: periodic zone");
+    int c = 0;
+    for (int a = 0; a < Z->nedges; ++a) {
+        const Edge *ed = &m->ed[Z->edges[a]];
+        for (int i = 0; i <= ed->N; ++i, ++c) {
+            if (!qt) continue;
+            const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+            for (int k = 0; k < 4; ++k) {
+                qt[4 * c + k] = ed->tq[0][4 * i + k];
+                ub[4 * c + k] = ed->uB[4 * i + k];
+                usb[4 * c + k] = ed->uSB ? ed->uSB[4 * i + k] : 0.0;
+            }
+            nrm[2 * c] = nn[0]; nrm[2 * c + 1] = nn[1];
+            vbc[c] = ed->vbc[i];
+        }
+    }
+    return c;
+}
+
 int or_surface_integrals(void *h, int marker, double *out) {
     Mesh *m = h;
     const Phys *ph = &m->ph;

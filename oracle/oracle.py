@@ -79,6 +79,7 @@ def lib():
                      "or_residuals", "or_conservation_sums", "or_volume_integrals"):
             getattr(_lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_surface_integrals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        _lib.or_boundary_states.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.or_probes.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.or_get_metrics.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_get_connectivity.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
@@ -248,6 +249,17 @@ class Oracle:
         if lib().or_surface_integrals(self.h, int(marker), _ptr(r)) != 0:
             raise RuntimeError(lib().or_last_error().decode())
         return r
+
+    def boundary_states(self, marker: int):
+        """(trace, normal, uB, uSB, viscous type) per edge node of zone ``marker`` from the
+        last time derivative or stage; the viscous types are 1 Dirichlet, 2 Neumann, 4 adiabatic."""
+        n = lib().or_boundary_states(self.h, int(marker), None, None, None, None, None)
+        if n < 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        qt, nrm, ub, usb = np.zeros((n, 4)), np.zeros((n, 2)), np.zeros((n, 4)), np.zeros((n, 4))
+        vbc = np.zeros(n, np.int32)
+        lib().or_boundary_states(self.h, int(marker), _ptr(qt), _ptr(nrm), _ptr(ub), _ptr(usb), _ptr(vbc))
+        return qt, nrm, ub, usb, vbc
 
     PROBE_VARIABLES = ("rho", "rhou", "rhov", "rhoe", "rhot", "rhout", "rhovt", "rhoet", "u", "v", "p", "Mach")
 

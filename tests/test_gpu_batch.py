@@ -95,11 +95,12 @@ def test_ip_batch_mixed():
 
 
 def test_boundary_condition_kinds_in_a_batch():
-    """Euler walls, pressure inlet/outlet and Dirichlet zones (prescribed and Riemann boundary
+    """Euler walls, pressure inlet/outlet, Dirichlet and newDirichlet zones (prescribed and Riemann boundary
     fluxes) in one batch, NS BR1: bit for bit the single-handle steps."""
-    from test_gpu_parity import _channel, _euler_wall_annulus
+    from test_gpu_parity import _channel, _euler_wall_annulus, _newdirichlet
     cases = [_channel(C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1", 1),
              _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1"),
-             _euler_wall_annulus(True)]
+             _euler_wall_annulus(True),
+             _newdirichlet(0.3, 1.5, "br1", mode=2, outp=2)]
     single, batched, _ = run_pair(cases, "fp64", 3)
     assert_same(single, batched)

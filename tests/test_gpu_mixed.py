@@ -50,6 +50,8 @@ CASES = {
     "euler_supersonic_inlet_outlet": (lambda: _parity_case("euler_supersonic_inlet_outlet"), 3e-7),
     "ns_sipg_pinlet_poutlet": (lambda: _parity_case("ns_sipg_pinlet_poutlet"), 3e-7),
     "ns_br1_eulerwall_annulus": (lambda: _parity_case("ns_br1_eulerwall_annulus"), 3e-7),
+    "euler_newdirichlet_total_total": (lambda: _parity_case("euler_newdirichlet_total_total"), 3e-7),
+    "ns_sipg_newdirichlet_total_outflow": (lambda: _parity_case("ns_sipg_newdirichlet_total_outflow"), 3e-7),
     # artificial dissipation: the indicator from the FP32 traces (measured 3e-9 .. 6e-8)
     "ns_br1_ad_tgv": (lambda: _parity_case("ns_br1_ad_tgv"), 3e-7),
     "ns_sipg_ad_kh": (lambda: _parity_case("ns_sipg_ad_kh"), 3e-7),

@@ -86,6 +86,14 @@ def _channel(inlet, outlet, Mach, amp, viscous="none", outflow=0, ip="sipg"):
                   riemann="roe", viscous=viscous, ip_method=ip, cfl=C.stable_cfl(4), ic=_flow_ic(Mach, amp))
 
 
+def _newdirichlet(Mach, amp, viscous="none", mode=1, outp=1, ip="sipg"):
+    """The channel with newDirichlet zones at both ends (newDirichlet.incf:146-433)."""
+    case = _channel(C.BC_NEWDIRICHLET, C.BC_NEWDIRICHLET, Mach, amp, viscous, ip=ip)
+    for z in (1, 3):
+        case.mesh.zones[z].mode, case.mesh.zones[z].outflow_pressure = mode, outp
+    return case
+
+
 def _euler_wall_annulus(ns):
     """The curved annulus with an Euler wall on the cylinder (curved prescribed-flux faces)."""
     c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
@@ -133,6 +141,11 @@ CASES = {
     "ns_br1_dirichlet_reflective_outlet": lambda: _channel(C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1", 1),
     "ns_sipg_pinlet_poutlet": lambda: _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "ip"),
     "ns_br1_supersonic_dirichlet": lambda: _channel(C.BC_DIRICHLET, C.BC_DIRICHLET, 1.6, 0.6, "br1"),
+    # newDirichlet: both inflow modes, both outflow pressures, sub- and supersonic
+    "euler_newdirichlet_speed_static": lambda: _newdirichlet(0.3, 1.5),
+    "euler_newdirichlet_total_total": lambda: _newdirichlet(0.3, 1.5, mode=2, outp=2),
+    "ns_br1_newdirichlet_supersonic": lambda: _newdirichlet(1.6, 0.6, "br1", mode=2),
+    "ns_sipg_newdirichlet_total_outflow": lambda: _newdirichlet(0.3, 1.5, "ip", outp=2),
     "euler_eulerwall_annulus": lambda: _euler_wall_annulus(False),
     "ns_br1_eulerwall_annulus": lambda: _euler_wall_annulus(True),
     # artificial dissipation (jumps-based, "Physical"): BR1, interior penalty, boundary states, curved faces
@@ -176,8 +189,8 @@ def test_workgroup_tiles_are_bit_identical(name):
     compared.)"""
     from horses2d_amd import gpu
     case = CASES[name]()
-    extended = any(z.kind in (C.BC_DIRICHLET, C.BC_EULERWALL, C.BC_PRESSUREOUTLET, C.BC_PRESSUREINLET)
-                   for z in case.mesh.zones) or case.artificial_dissipation != "none"
+    extended = any(z.kind in (C.BC_DIRICHLET, C.BC_EULERWALL, C.BC_PRESSUREOUTLET, C.BC_PRESSUREINLET,
+                              C.BC_NEWDIRICHLET) for z in case.mesh.zones) or case.artificial_dissipation != "none"
     runs = []
     for tile in (0, 1):
         dg = gpu.DGSEM(case, "fp64")

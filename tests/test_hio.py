@@ -177,6 +177,28 @@ def test_case_from_hiocase_keys_zones_and_nondimensional_states(tmp_path):
     np.testing.assert_allclose(c.initial_state(X).reshape(-1, 4)[0], C.freestream_state(0.2))
 
 
+def test_newdirichlet_zone_keys(tmp_path):
+    """newDirichletBC_Construct (newDirichlet.incf:22-125): the zone state as the far field
+    forms it; Mode and Outflow pressure take their exact strings, anything else the defaults
+    Specify speed and Static."""
+    _, path = _box_files(tmp_path)
+    with open(path) as f:
+        text = f.read()
+    far = "   Type = Farfield\n"
+    for extra, mode, outp in (("", 1, 1),
+                              ("   Mode = Specify total pressure\n   Outflow pressure = Total\n", 2, 2),
+                              ("   Mode = Specify Speed\n   Outflow pressure = Static\n", 1, 1),
+                              ("   Outflow pressure = Total\n", 1, 2)):
+        with open(path, "w") as f:
+            f.write(text.replace(far, "   Type = newDirichlet\n" + extra))
+        z4 = hio.case_from_hiocase(path).mesh.zones[3]
+        assert (z4.kind, z4.mode, z4.outflow_pressure) == (C.BC_NEWDIRICHLET, mode, outp)
+        assert z4.ints() == (mode, outp)
+        rho, V, a = 2.0 / 1.5, 0.5 * math.sqrt(1.4 * 1.5), math.pi / 6
+        np.testing.assert_allclose(z4.q, [rho, rho * V * math.cos(a), rho * V * math.sin(a),
+                                          2.0 / 0.4 + 0.5 * rho * V * V], rtol=1e-14)
+
+
 def test_artificial_dissipation_keys(tmp_path):
     _, path = _box_files(tmp_path)
     with open(path) as f:
@@ -206,7 +228,7 @@ def test_unsupported_zones_are_refused(tmp_path):
     _, path = _box_files(tmp_path)
     with open(path) as f:
         text = f.read()
-    for bad in ("Type = Riemann", "Type = newDirichlet"):
+    for bad in ("Type = Riemann", "Type = Wall"):
         with open(path, "w") as f:
             f.write(text.replace("Type = Euler wall", bad))
         with pytest.raises(ValueError):

@@ -127,20 +127,20 @@ def test_free_stream_on_reference_meshes():
 
 
 def test_boundary_condition_kinds_accepted_and_refused(gpu_lib):
-    """hg_create's mesh description takes the Euler-wall, Dirichlet and pressure kinds
-    (Generic/Defines.h:103-111 numbering) and refuses the Riemann (8) and newDirichlet (9)
-    kinds, which the library does not provide; the channel's connectivity is the oracle's."""
+    """hg_create's mesh description takes the Euler-wall, Dirichlet, pressure and newDirichlet
+    kinds (Generic/Defines.h:103-112 numbering) and refuses the Riemann kind (8), which the
+    library does not provide; the channel's connectivity is the oracle's."""
     q = tuple(C.freestream_state(0.3))
     def channel(kind):
         zones = [C.Zone(C.BC_EULERWALL), C.Zone(kind, q=q), C.Zone(C.BC_EULERWALL), C.Zone(C.BC_PRESSUREINLET, q=q)]
         mesh = C.channel(5, 3, 2.0, 1.0, zones, skew=0.2)
         return C.Case("channel", mesh, np.full(mesh.K, 3, np.int32), 0.3, ic=C.uniform(q))
-    for kind in (C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, C.BC_FARFIELD):
+    for kind in (C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, C.BC_FARFIELD, C.BC_NEWDIRICHLET):
         case = channel(kind)
         e1, el1, p1 = O.Oracle(case).connectivity()
         e2, el2, p2 = gpu.build_connectivity(case)
         np.testing.assert_array_equal(e1, e2)
         np.testing.assert_array_equal(el1, el2)
-    for kind in (8, 9):
+    for kind in (8, 10):
         with pytest.raises(RuntimeError, match="unsupported boundary condition"):
             gpu.build_connectivity(channel(kind))

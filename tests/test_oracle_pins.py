@@ -663,3 +663,98 @@ def test_artificial_viscosity_vanishes_on_a_smooth_state():
     b = O.Oracle(case2)
     b.set_state(case.initial_state(b.coords()))
     np.testing.assert_array_equal(a.time_derivative(), b.time_derivative())
+
+
+# ------------------------------------------------------------------------------------------
+# newDirichlet (BoundaryConditions/newDirichlet.incf): characteristic outer states
+# ------------------------------------------------------------------------------------------
+def _newdirichlet_channel(Mach, mode, outp, viscous="none"):
+    q = C.freestream_state(Mach)
+    zones = [C.Zone(C.BC_EULERWALL), C.Zone(C.BC_NEWDIRICHLET, q=tuple(q), mode=mode, outflow_pressure=outp),
+             C.Zone(C.BC_EULERWALL), C.Zone(C.BC_NEWDIRICHLET, q=tuple(q), mode=mode, outflow_pressure=outp)]
+    mesh = C.channel(6, 3, 3.0, 1.0, zones, skew=0.3)
+    ns = viscous != "none"
+    return C.Case("channel", mesh, np.full(mesh.K, 4, np.int32), Mach, 100.0 if ns else 0.0, 0.72, ns,
+                  riemann="roe", viscous=viscous, ic=C.uniform(q))
+
+
+@pytest.mark.parametrize("Mach", [0.3, 1.6])
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("viscous", ["none", "br1", "ip"])
+def test_newdirichlet_preserves_a_uniform_channel_flow(mode, Mach, viscous):
+    """At the zone state every branch returns it: subsonic inflow at p_L = p_ext keeps the
+    zone's density and speed (mode speed) or, through the isentrope and the total pressure,
+    rho_R = rho_ext and M_R = M_ext when c_p = gamma/(gamma-1) (mode total pressure); subsonic
+    outflow at the static pressure gives 2 p_ext - p_L = p_L."""
+    o, _ = oracle_for(_newdirichlet_channel(Mach, mode, 1, viscous))
+    assert np.max(np.abs(o.time_derivative())) < 1e-12
+
+
+@pytest.mark.parametrize("mode,outp", [(1, 1), (2, 2), (2, 1), (1, 2)])
+def test_newdirichlet_outer_states_follow_the_characteristics(mode, outp):
+    """On a flow that enters and leaves through both ends, sub- and supersonically, each node's
+    outer state against the relation its branch fixes (newDirichlet.incf:146-433): supersonic
+    inflow the zone state; subsonic inflow the trace pressure and either the zone's density and
+    momentum, or the zone's entropy p/rho^gamma and total pressure with the velocity along the
+    zone's (reversed where p_L exceeds it); supersonic outflow the trace with a Neumann viscous
+    type; subsonic outflow the trace's density and momentum at pressure 2 p_o - p_L, p_o the
+    zone's static or total pressure.  uSB is uB everywhere."""
+    from oracle.oracle import Oracle
+    g, gm1 = 1.4, 0.4
+    Mach = 0.5
+    case = _newdirichlet_channel(Mach, mode, outp, "br1")
+    o = Oracle(case)
+    X = o.coords()
+    a0 = np.sqrt(g)
+    x, y = X[:, 0], X[:, 1]
+    M = 1.7 * np.sin(2 * np.pi * y + 0.4) * np.cos(0.9 * x)
+    rho = 1.0 + 0.1 * np.cos(x + 2 * y)
+    p = 1.0 + 0.3 * np.sin(x - y)
+    u, v = a0 * M, 0.2 * a0 * np.sin(x + y)
+    Q = np.stack([rho, rho * u, rho * v, p / gm1 + 0.5 * rho * (u * u + v * v)], axis=1)
+    o.set_state(Q)
+    o.time_derivative()
+    q = np.asarray(C.freestream_state(Mach))
+    uE, vE = q[1] / q[0], q[2] / q[0]
+    pE = gm1 * (q[3] - 0.5 * q[0] * (uE * uE + vE * vE))
+    ME2 = (uE * uE + vE * vE) * q[0] / (g * pE)
+    ptE = pE * (1 + 0.5 * gm1 * ME2) ** (g / gm1)
+    sE = pE / q[0] ** g
+    pres = lambda w: gm1 * (w[3] - 0.5 * (w[1] ** 2 + w[2] ** 2) / w[0])
+    seen = set()
+    for marker in (2, 4):
+        qt, nrm, ub, usb, vbc = o.boundary_states(marker)
+        np.testing.assert_array_equal(usb, ub)
+        for Qt, n, uB, vb in zip(qt, nrm, ub, vbc):
+            uL, vL = Qt[1] / Qt[0], Qt[2] / Qt[0]
+            vn, pL = uL * n[0] + vL * n[1], pres(Qt)
+            ML = np.hypot(uL, vL) / np.sqrt(g * pL / Qt[0])
+            if vn <= 0 and ML > 1:
+                seen.add("in_sup"); np.testing.assert_allclose(uB, q, rtol=1e-14)
+            elif vn <= 0:
+                seen.add("in_sub")
+                assert pres(uB) == pytest.approx(pL, rel=1e-12)
+                if mode == 1:
+                    np.testing.assert_allclose(uB[:3], q[:3], rtol=1e-14)
+                else:
+                    assert pres(uB) / uB[0] ** g == pytest.approx(sE, rel=1e-12)
+                    V2 = (uB[1] ** 2 + uB[2] ** 2) / uB[0] ** 2
+                    pt = pres(uB) * (1 + 0.5 * gm1 * V2 * uB[0] / (g * pres(uB))) ** (g / gm1)
+                    # the Mach number's sign carries the direction; above the zone's total
+                    # pressure the formula's mirror image gives p_t = p_L^2 / p_t,ext
+                    assert pt == pytest.approx(ptE if pL < ptE else pL * pL / ptE, rel=1e-12)
+                    seen.add("in_sub_fwd" if pL < ptE else "in_sub_rev")
+                    cross = uB[1] * vE - uB[2] * uE
+                    assert abs(cross) <= 1e-12 * np.hypot(uB[1], uB[2])
+                    assert np.sign(uB[1] * uE + uB[2] * vE) == (1 if pL < ptE else -1)
+                assert vb == 1
+            elif ML > 1:
+                seen.add("out_sup"); np.testing.assert_array_equal(uB, Qt); assert vb == 2
+            else:
+                seen.add("out_sub")
+                np.testing.assert_array_equal(uB[:3], Qt[:3])
+                po = pE if outp == 1 else ptE
+                assert pres(uB) == pytest.approx(2 * po - pL, rel=1e-12)
+                assert vb == 1
+    want = {"in_sup", "in_sub", "out_sup", "out_sub"} | ({"in_sub_fwd", "in_sub_rev"} if mode == 2 else set())
+    assert seen == want
