@@ -475,8 +475,13 @@ __device__ inline int ghost_state_ext(int kind, int flags, const double *zdd, co
             const T pR = pL, rhoR = pow(pR / sExt, T(1) / gamma), aR = sqrt(gamma * pR / rhoR);
             if (flags & FF_TOTAL_INFLOW) {
                 const T nrm = sqrt(uE * uE + vE * vE);
-                const T MR = pR >= ptExt ? -sqrt(T(2) / gm1 * (pow(pR / ptExt, gm1 / gamma) - T(1)))
-                                         : sqrt(T(2) / gm1 * (pow(ptExt / pR, gm1 / gamma) - T(1)));
+                // M_R^2 = 2/(gamma-1) ((p_hi/p_lo)^((gamma-1)/gamma) - 1) as expm1(k log1p(d)),
+                // d = (p_hi - p_lo)/p_lo: the same value without the cancellation of x^k - 1
+                // near x = 1 (at Mach 0.3, x^k - 1 = 0.018 would lose 6 bits of FP32)
+                const bool rev = pR >= ptExt;
+                const T d = rev ? (pR - ptExt) / ptExt : (ptExt - pR) / pR;
+                const T m2 = T(2) / gm1 * expm1(gm1 / gamma * log1p(d));
+                const T MR = rev ? -sqrt(m2) : sqrt(m2);
                 uB[0] = rhoR;
                 uB[1] = rhoR * aR * MR * (uE / nrm);
                 uB[2] = rhoR * aR * MR * (vE / nrm);

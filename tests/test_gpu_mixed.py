@@ -50,7 +50,10 @@ CASES = {
     "euler_supersonic_inlet_outlet": (lambda: _parity_case("euler_supersonic_inlet_outlet"), 3e-7),
     "ns_sipg_pinlet_poutlet": (lambda: _parity_case("ns_sipg_pinlet_poutlet"), 3e-7),
     "ns_br1_eulerwall_annulus": (lambda: _parity_case("ns_br1_eulerwall_annulus"), 3e-7),
-    "euler_newdirichlet_total_total": (lambda: _parity_case("euler_newdirichlet_total_total"), 3e-7),
+    # newDirichlet's total-pressure inflow takes its Mach number from p_t/p_L - 1, 0.064 at
+    # Mach 0.3: the FP32 rounding of the trace pressure enters the inflow velocity amplified
+    # about eightfold (measured 5.5e-7 in momentum with the Mach number from x^k - 1)
+    "euler_newdirichlet_total_total": (lambda: _parity_case("euler_newdirichlet_total_total"), 1e-6),
     "ns_sipg_newdirichlet_total_outflow": (lambda: _parity_case("ns_sipg_newdirichlet_total_outflow"), 3e-7),
     # artificial dissipation: the indicator from the FP32 traces (measured 3e-9 .. 6e-8)
     "ns_br1_ad_tgv": (lambda: _parity_case("ns_br1_ad_tgv"), 3e-7),
