@@ -94,6 +94,15 @@ def _newdirichlet(Mach, amp, viscous="none", mode=1, outp=1, ip="sipg"):
     return case
 
 
+def _newdirichlet_annulus(riemann):
+    """The curved annulus with its outer far field replaced by a newDirichlet zone at the same
+    state (curved edges: per-node normals, newDirichlet.incf:313-433), with its own solver."""
+    c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
+    z = c.mesh.zones[1]
+    c.mesh.zones[1] = C.Zone(C.BC_NEWDIRICHLET, q=z.q, mode=2, outflow_pressure=1, riemann=C.RIEMANN[riemann])
+    return c
+
+
 def _euler_wall_annulus(ns):
     """The curved annulus with an Euler wall on the cylinder (curved prescribed-flux faces)."""
     c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
@@ -146,6 +155,7 @@ CASES = {
     "euler_newdirichlet_total_total": lambda: _newdirichlet(0.3, 1.5, mode=2, outp=2),
     "ns_br1_newdirichlet_supersonic": lambda: _newdirichlet(1.6, 0.6, "br1", mode=2),
     "ns_sipg_newdirichlet_total_outflow": lambda: _newdirichlet(0.3, 1.5, "ip", outp=2),
+    "ns_br1_newdirichlet_annulus_hll_zone": lambda: _newdirichlet_annulus("hll"),
     "euler_eulerwall_annulus": lambda: _euler_wall_annulus(False),
     "ns_br1_eulerwall_annulus": lambda: _euler_wall_annulus(True),
     # artificial dissipation (jumps-based, "Physical"): BR1, interior penalty, boundary states, curved faces
