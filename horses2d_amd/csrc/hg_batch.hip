@@ -64,6 +64,7 @@ struct hg_batch {
     hipStream_t stream = nullptr;
     std::vector<void *> bufs;
     hg::StageArgs *slots = nullptr;        // [case][nslots]
+    std::vector<hg::StageArgs> hslots;     // host copy: the indicator launches take their args
     int4 *items[2] = {nullptr, nullptr};   // per tile
     int nitems[2] = {0, 0};
     int block[2] = {0, 0};
@@ -106,12 +107,26 @@ hg::BatchArgs args_of(const hg_batch *b, int tile, int order, int slot) {
 
 // One RK step of all cases for the given trace parity (launches only).  Interior penalty: no
 // gradient launch, each stage's epilogue forms the next stage's viscous face-flux traces.
+// The artificial-viscosity indicator of every case from its slot's current traces: one launch
+// per case, before the gradient pass that reads mu_a (DGSpatialDiscretizationMethods.f90:255).
+hipError_t art_visc_all(hg_batch *b, int slot) {
+    for (size_t c = 0; c < b->cases.size(); ++c) {
+        hg_handle *h = b->cases[c];
+        if (!h->ph.ad) continue;
+        const hg::StageArgs &v = b->hslots[c * b->nslots + slot];
+        hipError_t e = hg::launch_art_visc(h->P, h->mixed, v, h->evol, h->w, h->mu_a, b->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t enqueue_step(hg_batch *b, int parity) {
     const bool ns = b->cases[0]->ph.ns != 0 && !b->cases[0]->ip_fused;
     for (int s = 0; s < b->sch.nst; ++s) {
         const int slot = parity * b->sch.nst + s;
         hg::BatchArgs a = args_of(b, b->tile, b->order, slot);
         hipError_t e;
+        if ((e = art_visc_all(b, slot)) != hipSuccess) return e;
         if (ns && (e = hg::launch_batch_gradient(b->variant, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess)
             return e;
         if ((e = hg::launch_batch_stage(b->variant, b->rs, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess) return e;
@@ -135,8 +150,8 @@ int autotune(hg_batch *b) {
             if (b->opt.order >= 0 && o != b->opt.order) continue;
             hg::BatchArgs a = args_of(b, t, o, dslot);
             auto run = [&]() -> hipError_t {
-                hipError_t e = hipSuccess;
-                if (ns) e = hg::launch_batch_gradient(b->variant, t, a, b->block[t], b->stream);
+                hipError_t e = art_visc_all(b, dslot);
+                if (e == hipSuccess && ns) e = hg::launch_batch_gradient(b->variant, t, a, b->block[t], b->stream);
                 if (e == hipSuccess) e = hg::launch_batch_stage(b->variant, b->rs, t, a, b->block[t], b->stream);
                 return e;
             };
@@ -186,12 +201,11 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     for (int c = 0; c < ncases; ++c) {
         hg_handle *h = cases[c];
         if (!h) return hg::handle_error("hg_batch_create: null case handle");
-        if (h->ph.ad)   // the batched step has no per-case indicator launch
-            return hg::handle_error("hg_batch_create: cases with artificial dissipation are stepped on their own");
+        // artificial dissipation all or none: it decides whether the gradient pass runs every stage
         if (h->ph.ns != h0->ph.ns || h->ph.viscous != h0->ph.viscous || h->mixed != h0->mixed ||
-            h->opt.scheme != h0->opt.scheme)
+            h->opt.scheme != h0->opt.scheme || h->ph.ad != h0->ph.ad)
             return hg::handle_error("hg_batch_create: the cases of a batch must share equations, viscous method, "
-                                    "precision and time scheme");
+                                    "precision, time scheme and artificial dissipation");
         for (int k = 0; k < c; ++k)
             if (cases[k] == h) return hg::handle_error("hg_batch_create: a case appears twice");
     }
@@ -295,6 +309,7 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
         slots[(size_t)c * b->nslots + 2 * b->sch.nst] = d;
     }
     HB_CHECK(b->upload(&b->slots, slots));
+    b->hslots = slots;
     HB_CHECK(hipDeviceSynchronize());
     *out = b.release();
     return 0;

@@ -278,7 +278,8 @@ int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile);
  * Navier-Stokes) over a work list of element tiles of all cases, the whole step captured
  * in a HIP graph.  The workgroup tile and the order of the work list are chosen by a
  * timing autotuner at the first step.  All cases of a batch share the equations
- * (Euler / Navier-Stokes BR1 / Navier-Stokes IP), the precision and the time scheme.
+ * (Euler / Navier-Stokes BR1 / Navier-Stokes IP), the precision, the time scheme and
+ * whether artificial dissipation is on (its indicator runs once per case and stage).
  * No reference counterpart: the reference runs one case per process
  * (TimeIntegrator_t%dt per case, DGTimeIntegrator.f90:159-255). */
 typedef struct {

@@ -104,3 +104,19 @@ def test_boundary_condition_kinds_in_a_batch():
              _newdirichlet(0.3, 1.5, "br1", mode=2, outp=2)]
     single, batched, _ = run_pair(cases, "fp64", 3)
     assert_same(single, batched)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_artificial_dissipation_in_a_batch(precision):
+    """Jumps-based "Physical" dissipation in a batch: each case's indicator launch before the
+    batched gradient pass; BR1 cases of different orders and boundaries, bit for bit the
+    single-handle steps.  A batch mixing cases with and without it is refused."""
+    from test_gpu_parity import CASES as P
+    from horses2d_amd import gpu
+    cases = [P["ns_br1_ad_tgv"](), P["ns_br1_ad_channel"](), P["ns_br1_ad_annulus"]()]
+    single, batched, _ = run_pair(cases, precision, 3)
+    assert_same(single, batched)
+    plain = [P["ns_br1_ad_tgv"](), C.ns_tgv_case(6, 4, Re=1600.0)]
+    handles = [gpu.DGSEM(c, precision) for c in plain]
+    with pytest.raises(RuntimeError, match="artificial dissipation"):
+        gpu.Batch(handles)
