@@ -314,6 +314,9 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     if (opt->precision != HG_PRECISION_FP64 && opt->precision != HG_PRECISION_MIXED)
         return set_err("hg_create: unknown precision");
     if (opt->scheme < 0 || opt->scheme > 2) return set_err("hg_create: unknown scheme");
+    if (opt->navier_stokes && opt->viscous == HG_VISCOUS_IP &&
+        (opt->ip_method < HG_IP_SIPG || opt->ip_method > HG_IP_IIPG))
+        return set_err("hg_create: unknown interior penalty method (SIPG, NIPG or IIPG)");
     std::unique_ptr<hg_handle> h(new hg_handle());
     std::string err = hg::build_mesh(*mesh, h->mesh);
     if (!err.empty()) return set_err("hg_create: " + err);
@@ -345,7 +348,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     ph.ns = opt->navier_stokes ? 1 : 0;
     if (opt->ip_method == HG_IP_SIPG) { ph.ip_eps = -1.0; ph.ip_gradflux = 1; }
     else if (opt->ip_method == HG_IP_NIPG) { ph.ip_eps = 1.0; ph.ip_gradflux = 1; }
-    else { ph.ip_eps = 0.0; ph.ip_gradflux = 0; }
+    else { ph.ip_eps = 0.0; ph.ip_gradflux = 0; }   // IIPG (DGViscousMethods.f90:362-378)
     if (ph.viscous != 2) ph.ip_gradflux = 0;
     // the stage kernel compiled for the interior solver, unless a zone asks for another one
     ph.rs_kernel = opt->riemann;

@@ -75,8 +75,19 @@ struct hg_batch {
     double *resid_part = nullptr;
     int tile = -1, order = -1;
     int parity = 0;
+    // what the slot table was built from, per case: the trace parity (h->cur) at parity 0 and
+    // the FP32 form's reference pressure (carried by value in every slot)
+    std::vector<int> cur0;
+    std::vector<double> pref0;
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
+    // kernel timing (hg_batch_profile_enable): HIP events around every stage (0), gradient (1)
+    // and residual (2) launch, on the batch stream; the step is then launched kernel by kernel
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    int64_t prof_n[3] = {0, 0, 0};
+    double prof_ms[3] = {0.0, 0.0, 0.0};
 
     template <typename T>
     hipError_t upload(T **dst, const std::vector<T> &src) {
@@ -120,24 +131,54 @@ hipError_t art_visc_all(hg_batch *b, int slot) {
     return hipSuccess;
 }
 
+hipEvent_t take_event(hg_batch *b) {
+    if (!b->ev_pool.empty()) {
+        hipEvent_t e = b->ev_pool.back();
+        b->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+// A launch, between two recorded events when kernel timing is on (kernel id `which`).
+template <typename F>
+hipError_t timed(hg_batch *b, int which, F &&launch) {
+    if (!b->prof) return launch();
+    hipEvent_t e0 = take_event(b), e1 = take_event(b);
+    if (!e0 || !e1) return hipErrorOutOfMemory;
+    hipError_t e = hipEventRecord(e0, b->stream);
+    if (e == hipSuccess) e = launch();
+    if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+    b->ev_pending.push_back({which, {e0, e1}});
+    return e;
+}
+
 hipError_t enqueue_step(hg_batch *b, int parity) {
     const bool ns = b->cases[0]->ph.ns != 0 && !b->cases[0]->ip_fused;
+    const int t = b->tile;
     for (int s = 0; s < b->sch.nst; ++s) {
         const int slot = parity * b->sch.nst + s;
-        hg::BatchArgs a = args_of(b, b->tile, b->order, slot);
+        hg::BatchArgs a = args_of(b, t, b->order, slot);
         hipError_t e;
         if ((e = art_visc_all(b, slot)) != hipSuccess) return e;
-        if (ns && (e = hg::launch_batch_gradient(b->variant, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess)
+        if (ns && (e = timed(b, 1, [&] { return hg::launch_batch_gradient(b->variant, t, a, b->block[t], b->stream); })) !=
+                      hipSuccess)
             return e;
-        if ((e = hg::launch_batch_stage(b->variant, b->rs, b->tile, a, b->block[b->tile], b->stream)) != hipSuccess) return e;
+        if ((e = timed(b, 0, [&] { return hg::launch_batch_stage(b->variant, b->rs, t, a, b->block[t], b->stream); })) !=
+            hipSuccess)
+            return e;
     }
-    return hg::launch_batch_resid(b->resid_part, b->ranges[b->tile], b->resid_out, (int)b->cases.size(), b->stream);
+    return timed(b, 2, [&] {
+        return hg::launch_batch_resid(b->resid_part, b->ranges[t], b->resid_out, (int)b->cases.size(), b->stream);
+    });
 }
 
 // Times derivative-only evaluations (plus the gradient pass) of every (tile, order)
 // candidate not fixed by the options and keeps the fastest.
 int autotune(hg_batch *b) {
-    const bool ns = b->cases[0]->ph.ns != 0;
+    // the gradient pass runs as in enqueue_step (interior penalty: the stage epilogue forms it)
+    const bool ns = b->cases[0]->ph.ns != 0 && !b->cases[0]->ip_fused;
     const int dslot = 2 * b->sch.nst;
     hipEvent_t ev[2];
     HB_CHECK(hipEventCreate(&ev[0]));
@@ -170,6 +211,55 @@ int autotune(hg_batch *b) {
     b->tile = bt;
     b->order = bo;
     return 0;
+}
+
+// The slot table from every case's current state: parity p, stage s -> p*nst + s, the
+// derivative-only slot 2*nst.  Records each case's trace parity and reference pressure.
+std::vector<hg::StageArgs> slot_table(hg_batch *b) {
+    std::vector<hg::StageArgs> slots(b->cases.size() * b->nslots);
+    b->cur0.assign(b->cases.size(), 0);
+    b->pref0.assign(b->cases.size(), 0.0);
+    for (int c = 0; c < (int)b->cases.size(); ++c) {
+        hg_handle *h = b->cases[c];
+        b->cur0[c] = h->cur;
+        b->pref0[c] = h->ph.pref;
+        for (int p = 0; p < 2; ++p)
+            for (int s = 0; s < b->sch.nst; ++s) {
+                hg::StageArgs a = hg::handle_args(h);
+                const int in = h->cur ^ p ^ (s & 1);
+                a.tr_in = h->tr[in];
+                a.tr_out = h->tr[in ^ 1];
+                a.dtr = h->dtr[h->ip_fused ? in : 0];
+                a.dtr_out = h->dtr[h->ip_fused ? (in ^ 1) : 0];
+                a.am = b->sch.am[s];
+                a.cq = b->sch.cq[s];
+                a.cg = b->sch.cg[s];
+                a.first_stage = s == 0 ? 1 : 0;
+                a.mode = 0;
+                a.do_resid = s == b->sch.nst - 1 ? 1 : 0;
+                a.resid_part = b->resid_part;
+                slots[(size_t)c * b->nslots + p * b->sch.nst + s] = a;
+            }
+        hg::StageArgs d = hg::handle_args(h);   // dtr, dtr_out: the current traces
+        d.tr_in = h->tr[h->cur];
+        d.tr_out = h->tr[h->cur ^ 1];
+        d.cq = 1.0;
+        d.first_stage = 1;
+        d.mode = 1;
+        d.do_resid = 0;
+        d.resid_part = b->resid_part;
+        slots[(size_t)c * b->nslots + 2 * b->sch.nst] = d;
+    }
+    return slots;
+}
+
+void drop_graphs(hg_batch *b) {
+    for (int p = 0; p < 2; ++p) {
+        if (b->exec[p]) hipGraphExecDestroy(b->exec[p]);
+        if (b->graph[p]) hipGraphDestroy(b->graph[p]);
+        b->exec[p] = nullptr;
+        b->graph[p] = nullptr;
+    }
 }
 
 int build_graphs(hg_batch *b) {
@@ -277,37 +367,7 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     for (int c = 0; c < ncases; ++c) rout[c] = cases[c]->resid;
     HB_CHECK(b->upload(&b->resid_out, rout));
 
-    // slot table: parity p, stage s -> p*nst + s ; derivative-only slot 2*nst
-    std::vector<hg::StageArgs> slots((size_t)ncases * b->nslots);
-    for (int c = 0; c < ncases; ++c) {
-        hg_handle *h = cases[c];
-        for (int p = 0; p < 2; ++p)
-            for (int s = 0; s < b->sch.nst; ++s) {
-                hg::StageArgs a = hg::handle_args(h);
-                const int in = h->cur ^ p ^ (s & 1);
-                a.tr_in = h->tr[in];
-                a.tr_out = h->tr[in ^ 1];
-                a.dtr = h->dtr[h->ip_fused ? in : 0];
-                a.dtr_out = h->dtr[h->ip_fused ? (in ^ 1) : 0];
-                a.am = b->sch.am[s];
-                a.cq = b->sch.cq[s];
-                a.cg = b->sch.cg[s];
-                a.first_stage = s == 0 ? 1 : 0;
-                a.mode = 0;
-                a.do_resid = s == b->sch.nst - 1 ? 1 : 0;
-                a.resid_part = b->resid_part;
-                slots[(size_t)c * b->nslots + p * b->sch.nst + s] = a;
-            }
-        hg::StageArgs d = hg::handle_args(h);   // dtr, dtr_out: the current traces
-        d.tr_in = h->tr[h->cur];
-        d.tr_out = h->tr[h->cur ^ 1];
-        d.cq = 1.0;
-        d.first_stage = 1;
-        d.mode = 1;
-        d.do_resid = 0;
-        d.resid_part = b->resid_part;
-        slots[(size_t)c * b->nslots + 2 * b->sch.nst] = d;
-    }
+    std::vector<hg::StageArgs> slots = slot_table(b.get());
     HB_CHECK(b->upload(&b->slots, slots));
     b->hslots = slots;
     HB_CHECK(hipDeviceSynchronize());
@@ -318,17 +378,43 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
 int hg_batch_rk_step(hg_batch *b, const double *t, const double *dt) {
     (void)t;  // no time-dependent boundary data on the GPU path
     if (!b || !dt) return hg::handle_error("hg_batch_rk_step: bad argument");
+    // A case stepped or uploaded on its own since the last batch step has another trace parity,
+    // possibly another reference pressure (FP32 form) and stale viscous traces (interior
+    // penalty): rebuild the slot table from the cases' current state.  The captured kernels
+    // read it from device memory, so the graphs stay valid, except the indicator launches of
+    // artificial dissipation, which take their slot by value: those graphs are captured again.
+    bool stale = false;
+    for (size_t c = 0; c < b->cases.size(); ++c) {
+        hg_handle *h = b->cases[c];
+        if (h->cur != (b->cur0[c] ^ b->parity) || h->ph.pref != b->pref0[c]) stale = true;
+        if (hg::ensure_vtr(h)) return 1;
+    }
     for (auto *h : b->cases) HB_CHECK(hipStreamSynchronize(h->stream));
+    if (stale) {
+        HB_CHECK(hipStreamSynchronize(b->stream));
+        std::vector<hg::StageArgs> slots = slot_table(b);
+        HB_CHECK(hipMemcpy(b->slots, slots.data(), slots.size() * sizeof(hg::StageArgs), hipMemcpyHostToDevice));
+        b->hslots = slots;
+        if (b->cases[0]->ph.ad) drop_graphs(b);
+        b->parity = 0;
+    }
     HB_CHECK(hipMemcpyAsync(b->dt, dt, b->cases.size() * sizeof(double), hipMemcpyHostToDevice, b->stream));
     if (b->tile < 0) {
         if (b->parity != 0) return hg::handle_error("hg_batch_rk_step: internal parity error");
         if (autotune(b)) return 1;
-        if (b->opt.use_graph && build_graphs(b)) return 1;
     }
-    if (b->opt.use_graph) HB_CHECK(hipGraphLaunch(b->exec[b->parity], b->stream));
+    const bool graph = b->opt.use_graph && !b->prof;
+    if (graph && !b->exec[0] && build_graphs(b)) {
+        drop_graphs(b);
+        return 1;
+    }
+    if (graph) HB_CHECK(hipGraphLaunch(b->exec[b->parity], b->stream));
     else HB_CHECK(enqueue_step(b, b->parity));
     b->parity ^= 1;
-    for (auto *h : b->cases) h->cur ^= 1;   // every scheme has an odd number of stages
+    for (auto *h : b->cases) {
+        h->cur ^= 1;   // every scheme has an odd number of stages
+        h->qdot_valid = false;
+    }
     return 0;
 }
 
@@ -345,12 +431,47 @@ int hg_batch_schedule(hg_batch *b, int32_t *tile, int32_t *order) {
     return 0;
 }
 
+int hg_batch_profile_enable(hg_batch *b, int32_t on) {
+    if (!b) return hg::handle_error("hg_batch_profile_enable: null batch");
+    HB_CHECK(hipStreamSynchronize(b->stream));
+    for (auto &p : b->ev_pending) {
+        b->ev_pool.push_back(p.second.first);
+        b->ev_pool.push_back(p.second.second);
+    }
+    b->ev_pending.clear();
+    b->prof = on != 0;
+    for (int k = 0; k < 3; ++k) {
+        b->prof_n[k] = 0;
+        b->prof_ms[k] = 0.0;
+    }
+    return 0;
+}
+
+int hg_batch_profile_read(hg_batch *b, int32_t which, int64_t *launches, double *ms) {
+    if (!b || which < 0 || which > 2) return hg::handle_error("hg_batch_profile_read: bad argument");
+    HB_CHECK(hipStreamSynchronize(b->stream));
+    for (auto &p : b->ev_pending) {
+        float t = 0.0f;
+        HB_CHECK(hipEventElapsedTime(&t, p.second.first, p.second.second));
+        b->prof_n[p.first] += 1;
+        b->prof_ms[p.first] += t;
+        b->ev_pool.push_back(p.second.first);
+        b->ev_pool.push_back(p.second.second);
+    }
+    b->ev_pending.clear();
+    if (launches) *launches = b->prof_n[which];
+    if (ms) *ms = b->prof_ms[which];
+    return 0;
+}
+
 int hg_batch_destroy(hg_batch *b) {
     if (!b) return 0;
     if (b->stream) hipStreamSynchronize(b->stream);
-    for (int p = 0; p < 2; ++p) {
-        if (b->exec[p]) hipGraphExecDestroy(b->exec[p]);
-        if (b->graph[p]) hipGraphDestroy(b->graph[p]);
+    drop_graphs(b);
+    for (auto &e : b->ev_pool) hipEventDestroy(e);
+    for (auto &p : b->ev_pending) {
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
     }
     for (void *p : b->bufs) hipFree(p);
     if (b->stream) hipStreamDestroy(b->stream);

@@ -61,7 +61,8 @@ EXPORTS = [
     "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_volume_integrals",
     "hg_surface_integrals", "hg_probes_set", "hg_probes_update", "hg_synchronize",
     "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_set_graph", "hg_batch_create", "hg_batch_rk_step",
-    "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_destroy",
+    "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_profile_enable", "hg_batch_profile_read",
+    "hg_batch_destroy",
 ]
 
 _lib = None
@@ -109,6 +110,8 @@ def load_library(path: str = LIB_PATH):
         "hg_batch_rk_step": ([vp, vp, vp], ctypes.c_int),
         "hg_batch_synchronize": ([vp], ctypes.c_int),
         "hg_batch_schedule": ([vp, vp, vp], ctypes.c_int),
+        "hg_batch_profile_enable": ([vp, i32], ctypes.c_int),
+        "hg_batch_profile_read": ([vp, i32, vp, vp], ctypes.c_int),
         "hg_batch_destroy": ([vp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
@@ -390,6 +393,17 @@ class Batch:
         t = ctypes.c_int32(); o = ctypes.c_int32()
         _check(self._L.hg_batch_schedule(self._b, ctypes.byref(t), ctypes.byref(o)), "hg_batch_schedule")
         return int(t.value), int(o.value)
+
+    def profile(self, on: bool = True):
+        """Kernel timing: launch the step kernel by kernel with HIP events around each launch."""
+        _check(self._L.hg_batch_profile_enable(self._b, 1 if on else 0), "hg_batch_profile_enable")
+
+    def profile_read(self, which: int = 0):
+        """(launches, total ms) of kernel ``which``: 0 stage, 1 gradient, 2 residual reduction."""
+        n = ctypes.c_int64(); ms = ctypes.c_double()
+        _check(self._L.hg_batch_profile_read(self._b, which, ctypes.byref(n), ctypes.byref(ms)),
+               "hg_batch_profile_read")
+        return int(n.value), float(ms.value)
 
     def close(self):
         if getattr(self, "_b", None) is not None and self._b.value:

@@ -289,13 +289,23 @@ typedef struct {
 } hg_batch_options;
 
 /* The batch keeps pointers to the handles (not copies); they must outlive it.  Between
- * hg_batch_rk_step and hg_batch_synchronize the cases must not be used directly. */
+ * hg_batch_rk_step and hg_batch_synchronize the cases must not be used directly.  A case
+ * stepped (hg_rk_step) or given a new state (hg_upload_state) on its own after
+ * hg_batch_synchronize is taken as it is at the next hg_batch_rk_step. */
 int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_options *opt, hg_batch **out);
 /* One RK step of every case: t[c], dt[c] per case (hg_rk_step semantics per case). */
 int hg_batch_rk_step(hg_batch *b, const double *t, const double *dt);
 int hg_batch_synchronize(hg_batch *b);
 /* The autotuned (or fixed) tile and order; -1 before the first step. */
 int hg_batch_schedule(hg_batch *b, int32_t *tile, int32_t *order);
+/* Kernel-level timing of the batched step: with on = 1, hg_batch_rk_step launches the step
+ * kernel by kernel (no graph replay; same results) with HIP events around every launch on the
+ * batch's stream.  hg_batch_profile_read returns the recorded launches and their total
+ * milliseconds for kernel `which` (0: batched stage kernel, 1: batched gradient kernel,
+ * 2: per-case residual reduction) since the last hg_batch_profile_enable.  No reference
+ * counterpart (measurement only). */
+int hg_batch_profile_enable(hg_batch *b, int32_t on);
+int hg_batch_profile_read(hg_batch *b, int32_t which, int64_t *launches, double *ms);
 int hg_batch_destroy(hg_batch *b);
 
 #ifdef __cplusplus

@@ -22,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <omp.h>
 
 #define NCONS 4
 #define MAXP 17
@@ -2208,4 +2209,11 @@ int or_viscous_flux(double Re, double Pr, int n, const double *q, const double *
     Phys ph; phys_init(&ph, 1.0, Re, Pr);
     for (int i = 0; i < n; ++i) viscous_flux(&ph, q + 4 * i, dq + 8 * i, f + 4 * i, g + 4 * i);
     return 0;
+}
+
+/* Threads of the OpenMP element loops (for timing the restatement at a stated core count);
+ * returns the count now in effect. */
+int or_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }

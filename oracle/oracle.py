@@ -92,11 +92,18 @@ def lib():
         _lib.or_riemann.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int] + [ctypes.c_void_p] * 4
         _lib.or_inviscid_flux.argtypes = [ctypes.c_double, ctypes.c_int] + [ctypes.c_void_p] * 3
         _lib.or_viscous_flux.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int] + [ctypes.c_void_p] * 4
+        _lib.or_set_threads.argtypes = [ctypes.c_int]
+        _lib.or_set_threads.restype = ctypes.c_int
     return _lib
 
 
 def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's OpenMP loops (n <= 0 leaves them); returns the count in effect."""
+    return int(lib().or_set_threads(int(n)))
 
 
 def spectral(N: int):

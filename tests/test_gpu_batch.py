@@ -120,3 +120,40 @@ def test_artificial_dissipation_in_a_batch(precision):
     handles = [gpu.DGSEM(c, precision) for c in plain]
     with pytest.raises(RuntimeError, match="artificial dissipation"):
         gpu.Batch(handles)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_steps_after_direct_steps_and_uploads(precision):
+    """A case stepped on its own, or given a new state, between batch steps: the next batch step
+    starts from that case's current trace parity, viscous traces (interior penalty) and, in the
+    FP32 form, its new reference pressure -- bit for bit the single-handle sequence."""
+    from horses2d_amd import gpu
+    cases = [C.ns_kh_case(4, 4), C.ns_kh_case(3, 5)]
+    single = make(cases, precision)
+    batched = make(cases, precision)
+    dts = np.array([d.estimate_dt(c.dt_max, c.cfl) for d, c in zip(single, cases)])
+    for d, c in zip(batched, cases):
+        d.estimate_dt(c.dt_max, c.cfl)
+    b = gpu.Batch(batched)
+
+    def batch_step():
+        b.rk_step(0.0, dts)
+        b.synchronize()
+        for d, dt in zip(single, dts):
+            d.rk_step(0.0, dt)
+
+    batch_step()
+    batch_step()
+    # case 0 steps alone (its trace parity flips)
+    single[0].rk_step(0.0, dts[0])
+    batched[0].rk_step(0.0, dts[0])
+    batch_step()
+    assert_same(single, batched)
+    # case 1 gets a new state with a higher pressure (another FP32 reference pressure)
+    q = single[1].state().reshape(-1, 4).copy()
+    q[:, 3] *= 1.5
+    single[1].upload(q.reshape(-1))
+    batched[1].upload(q.reshape(-1))
+    batch_step()
+    batch_step()
+    assert_same(single, batched)
