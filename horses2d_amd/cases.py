@@ -43,9 +43,9 @@ RIEMANN = {"roe": 0, "hll": 1, "exact": 2, "lf": 3}
 VISCOUS = {"none": 0, "br1": 1, "ip": 2}
 IP_METHOD = {"sipg": 0, "nipg": 1, "iipg": 2}
 SCHEME = {"euler": 0, "rk3": 1, "rk5": 2}
-# "Artificial dissipation (0/1)" with "type" Physical and "indicator" Jumps-based
+# "Artificial dissipation (0/1)" with "type" Physical or Laplacian and "indicator" Jumps-based
 # (Setup.f90:333-349, DGArtificialDissipation.f90:100-137)
-ARTIFICIAL_DISSIPATION = {"none": 0, "physical": 1}
+ARTIFICIAL_DISSIPATION = {"none": 0, "physical": 1, "laplacian": 2}
 
 
 @dataclasses.dataclass
@@ -114,6 +114,10 @@ class Case:
     ic: Optional[Callable] = None        # ic(x, y) -> (..., 4) dimensionless state
     artificial_dissipation: str = "none"  # ARTIFICIAL_DISSIPATION key
     ad_intensity: float = 1.0            # "Artificial dissipation intensity" (Ceps)
+    mode: str = "transient"              # "Integration mode": transient (to t_end) or steady
+    residual_target: float = 0.0         # steady: stop when max |residual| < this
+    t_end: float = 1.0e30                # transient: "Simulation time"
+    t0: float = 0.0                      # "Initial time"
 
     def initial_state(self, coords: np.ndarray) -> np.ndarray:
         """Evaluate the IC at node coordinates ``coords`` (nnodes, 2) -> flat Q (4*nnodes)."""

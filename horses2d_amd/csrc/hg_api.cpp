@@ -81,7 +81,9 @@ hg::StageArgs base_args(hg_handle *h) {
     a.dlbw = h->dlbw;
     a.resid_part = h->resid_part;
     a.ph = h->ph;
-    a.mu_a = h->ph.ad ? h->mu_a : nullptr;
+    a.mu_a = (h->ph.ad && !h->lap) ? h->mu_a : nullptr;
+    a.lap = h->lap ? h->mu_a : nullptr;
+    a.earea = h->earea;
     a.ip_epi = h->ip_fused ? 1 : 0;
     return a;
 }
@@ -358,10 +360,12 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
             Z.type == HG_BC_PRESSUREINLET || Z.type == HG_BC_NEWDIRICHLET)
             ph.xbc = 1;
     }
-    if (opt->artificial_dissipation != HG_AD_NONE && opt->artificial_dissipation != HG_AD_PHYSICAL)
+    if (opt->artificial_dissipation != HG_AD_NONE && opt->artificial_dissipation != HG_AD_PHYSICAL &&
+        opt->artificial_dissipation != HG_AD_LAPLACIAN)
         return set_err("hg_create: unknown artificial dissipation type");
     // DGArtificialDissipation is a Navier-Stokes module
-    ph.ad = (ph.ns && opt->artificial_dissipation == HG_AD_PHYSICAL) ? 1 : 0;
+    ph.ad = (ph.ns && opt->artificial_dissipation != HG_AD_NONE) ? 1 : 0;
+    h->lap = ph.ad && opt->artificial_dissipation == HG_AD_LAPLACIAN;
     ph.ad_ceps = opt->ad_intensity;
     if (ph.ad) ph.xbc = 1;
     // the run-time-solver form and the forms with those kinds exist for the wide tile only
@@ -497,7 +501,8 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
         const size_t rs = h->mixed ? sizeof(float) : sizeof(double);
         // interior penalty forms the next stage's viscous traces in the stage epilogue, unless
         // they depend on the next stage's artificial viscosity (all elements' jumps first)
-        h->ip_fused = ph.viscous == HG_VISCOUS_IP && !ph.ad;
+        // (the Laplacian type leaves mu alone: its viscous traces do not wait for the indicator)
+        h->ip_fused = ph.viscous == HG_VISCOUS_IP && (!ph.ad || h->lap);
         for (int b = 0; b < (h->ip_fused ? 2 : 1); ++b)
             HG_CHECK(h->alloc_raw(&h->dtr[b], (size_t)K * 4 * P * hg::VT * rs));
     }
@@ -535,6 +540,11 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(upload(h.get(), &h->forder, forder));
     HG_CHECK(upload(h.get(), &h->egeo, egeo));
     HG_CHECK(upload(h.get(), &h->einvh, einvh));
+    {
+        std::vector<double> earea(E);
+        for (int k = 0; k < E; ++k) earea[k] = m.ed[k].area;
+        HG_CHECK(upload(h.get(), &h->earea, earea));
+    }
     HG_CHECK(upload(h.get(), &h->zone, zone));
     HG_CHECK(upload(h.get(), &h->hatD, hat));
     HG_CHECK(upload(h.get(), &h->D, Dm));
@@ -837,7 +847,8 @@ int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
         } else {
             hg::StageArgs g = base_args(h);
             g.tr_in = h->tr[h->cur];
-            if (h->ph.ad) HG_CHECK(hg::launch_art_visc(h->P, h->mixed, g, h->evol, h->w, h->mu_a, h->stream));
+            if (h->ph.ad && !h->lap)
+                HG_CHECK(hg::launch_art_visc(h->P, h->mixed, g, h->evol, h->w, h->mu_a, h->stream));
             HG_CHECK(hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream));
         }
         vtr = hg::vtr_current(h);
@@ -845,7 +856,8 @@ int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
     if (n > 0)
         HG_CHECK(hg::launch_surface_integrals(h->mixed, h->bslots + first, n, h->P, h->tr[h->cur], vtr, h->face,
                                               h->egeo, h->w, h->ph.gm1, h->mixed ? h->ph.Eref : 0.0,
-                                              h->ph.ad ? h->mu_a : nullptr, h->ph.mu, h->monitor_out, h->stream));
+                                              (h->ph.ad && !h->lap) ? h->mu_a : nullptr, h->ph.mu, h->monitor_out,
+                                              h->stream));
     else
         HG_CHECK(hipMemsetAsync(h->monitor_out, 0, 9 * sizeof(double), h->stream));
     HG_CHECK(hipMemcpyAsync(out, h->monitor_out, 9 * sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -293,7 +293,7 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
         if (!h) return hg::handle_error("hg_batch_create: null case handle");
         // artificial dissipation all or none: it decides whether the gradient pass runs every stage
         if (h->ph.ns != h0->ph.ns || h->ph.viscous != h0->ph.viscous || h->mixed != h0->mixed ||
-            h->opt.scheme != h0->opt.scheme || h->ph.ad != h0->ph.ad)
+            h->opt.scheme != h0->opt.scheme || h->ph.ad != h0->ph.ad || h->lap != h0->lap)
             return hg::handle_error("hg_batch_create: the cases of a batch must share equations, viscous method, "
                                     "precision, time scheme and artificial dissipation");
         for (int k = 0; k < c; ++k)

@@ -41,7 +41,7 @@ struct PhysParams {
     int xbc;        // a zone is an Euler wall, Dirichlet, pressure outlet or inlet, or artificial
                     // dissipation is on: the kernels compiled with those (stage: the run-time-
                     // solver form) are launched
-    int ad;         // artificial dissipation: "Physical" type, jumps-based indicator
+    int ad;         // artificial dissipation (jumps-based indicator): "Physical" or "Laplacian" type
     double ad_ceps; // its intensity
 };
 
@@ -74,6 +74,8 @@ struct StageArgs {
     int do_resid;
     PhysParams ph;
     const double *mu_a;         // artificial viscosity per element (null: none)
+    const double *lap;          // Laplacian dissipation: the indicator's viscosity per element (null: none)
+    const double *earea;        // per edge: its Area (the Laplacian face flux)
     int ip_epi;                 // interior penalty: the stage epilogue forms the next viscous traces
 };
 

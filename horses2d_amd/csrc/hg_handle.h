@@ -60,8 +60,10 @@ struct hg_handle {
     int2 *probe_ev = nullptr;
     double *probe_l = nullptr, *probe_out = nullptr;
     bool qdot_valid = false;   // QDot holds hg_time_derivative of the current state
-    // artificial dissipation: per-element viscosity (written before each stage) and volume
-    double *mu_a = nullptr, *evol = nullptr;
+    // artificial dissipation: per-element viscosity (written before each stage) and volume;
+    // lap: the Laplacian type (the viscosity enters its own fluxes, not mu); per-edge Area
+    double *mu_a = nullptr, *evol = nullptr, *earea = nullptr;
+    bool lap = false;
     double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
     int cur = 0;
     int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet

@@ -250,6 +250,13 @@ __device__ __forceinline__ R face_mu(const StageArgs &a, const Phys<R> &ph, int 
     return (R)(a.ph.mu + ma);
 }
 
+// Edge viscosity of the Laplacian dissipation (JumpsBasedEdgeViscosity,
+// DGArtificialDissipation.f90:340-363): the mean of the two elements' (boundary and periodic
+// faces: the own element's).
+__device__ __forceinline__ double face_lap(const StageArgs &a, int fe, int4 fr) {
+    return (fr.x & 15) <= FK_INT_R ? (a.lap[fe] + a.lap[fr.y >> 2]) / 2.0 : a.lap[fe] / 1.0;
+}
+
 // Face-node geometry and the neighbour's Q trace (periodic and interior faces).
 template <int P, typename R, typename TR>
 __device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, int fk, int4 &fr, R &nx, R &ny, R &dS,
@@ -454,7 +461,7 @@ struct StageSmem {
     alignas(16) R sU[IP ? T::NF * 4 : 1];
     alignas(16) R sV[IP ? T::NF * 4 : 1];
     alignas(16) R sHat[T::PP];
-    alignas(16) R sD[IP ? T::PP : 1];
+    alignas(16) R sD[NS ? T::PP : 1];   // IP gradient; BR1: the Laplacian dissipation's interior gradient
     alignas(16) R sLbw[2 * P];
     alignas(16) R sDlbw[2 * P];
     alignas(16) R sLbR[IP ? 2 * P : 1];   // lb in the compute type (IP epilogue, FP32 traces)
@@ -490,7 +497,7 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     }
     for (int x = t; x < PP; x += TPB) {
         sHat[x] = (R)a.hatD[x];
-        if (IP) sD[x] = (R)a.D[x];
+        if (IP || (BR1 && XB)) sD[x] = (R)a.D[x];
     }
     const int ne = min(EPB, a.K - e0);
     const int64_t nb0 = (int64_t)e0 * PP;
@@ -534,10 +541,15 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
 
     // ---- own nodal gradient (Navier-Stokes), kept in registers for the node phase
     R dq[NS ? 8 : 1];
+    // BR1 with the Laplacian dissipation: the element's interior gradient as well (its volume
+    // flux, DGArtificialDissipation.f90:207-222); interior penalty: that is dq itself
+    constexpr bool LAPG = BR1 && XB;
+    R dqi[LAPG ? 8 : 1];
     if (NS && node_active) {
         const int i = n % P, j = n / P;
         if (BR1) br1_node_gradient<P, R, 4, 4, true>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
         else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
+        if (LAPG && a.lap) ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dqi);
     }
     // the neighbour's viscous flux trace, issued before the barrier and consumed in the face phase
     R vn[NS ? VT : 1];
@@ -577,8 +589,16 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             viscous_flux(ph, q, dq, fv, gv);
             // e % mu_a (DGViscous_BR1.f90:307, DGViscous_IP.f90:84)
             const R mue = (XB && a.mu_a) ? (R)(a.ph.mu + a.mu_a[e]) : ph.mu;
+            // Laplacian dissipation: F = Fi - Fv - Fa, Fa = mu_e times the interior gradient
+            R fa[4] = {R(0), R(0), R(0), R(0)}, ga[4] = {R(0), R(0), R(0), R(0)};
+            if (XB && a.lap) {
+                const R lm = (R)a.lap[e];
+                const R *di = LAPG ? dqi : dq;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) { f[c] = (f[c] - mue * fv[c]) - R(0); gg[c] = (gg[c] - mue * gv[c]) - R(0); }
+                for (int c = 0; c < 4; ++c) { fa[c] = lm * di[c]; ga[c] = lm * di[4 + c]; }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) { f[c] = (f[c] - mue * fv[c]) - fa[c]; gg[c] = (gg[c] - mue * gv[c]) - ga[c]; }
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -621,8 +641,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             for (int c = 0; c < 4; ++c) F[c] = Fi[c] * dS;
         } else {
             R Fv[4] = {R(0), R(0), R(0), R(0)};
+            R Fa[4] = {R(0), R(0), R(0), R(0)};   // Laplacian dissipation: -mu Area (QL - QR)
             R G8[8] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
             const R muf = XB ? face_mu<R>(a, ph, fe, fr) : ph.mu;
+            const R lapA = (XB && a.lap) ? (R)(face_lap(a, fe, fr) * a.earea[fr.w]) : R(0);
             const R pen = IP ? ph.sigma0 * muf * (R)(P - 1) * (R)(P - 1) * invh : R(0);
             if (kind <= FK_PERIODIC) {
                 const R *uL = (kind == FK_INT_R) ? qo : qs, *uR = (kind == FK_INT_R) ? qs : qo;
@@ -640,6 +662,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
                 if (IP) {
 #pragma unroll
                     for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (uL[c] - uR[c]);
+                }
+                if (XB && a.lap) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) Fa[c] = -(lapA * (uL[c] - uR[c]));
                 }
                 if (ipgrad) {
                     // IP_GradientRiemannSolver (DGViscous_IP.f90:216-249): this side's share
@@ -667,6 +693,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
 #pragma unroll
                     for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (qs[c] - uSB[c]);
                 }
+                if (XB && a.lap) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) Fa[c] = -(lapA * (qs[c] - uSB[c]));
+                }
                 if (ipgrad) {
                     R fgr[8], f4[4], g4[4];
 #pragma unroll
@@ -680,10 +710,10 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             }
             if (kind <= FK_INT_R) {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] - Fv[c]) - R(0)) * dS;
+                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] - Fv[c]) - Fa[c]) * dS;
             } else {
 #pragma unroll
-                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] * dS - Fv[c] * dS) - R(0));
+                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] * dS - Fv[c] * dS) - Fa[c] * dS);
             }
             if (ipgrad) {
                 // contravariant gradient-flux components at the face (DGWeakIntegrals.f90:268-495)

@@ -20,7 +20,7 @@ module horses_gpu_bindings
    integer(c_int32_t), parameter, public :: HG_NEWDIRICHLET_SPECIFY_SPEED = 1, &
                                             HG_NEWDIRICHLET_SPECIFY_TOTAL_PRESSURE = 2, &
                                             HG_NEWDIRICHLET_STATIC_PRESSURE = 1, HG_NEWDIRICHLET_TOTAL_PRESSURE = 2
-   integer(c_int32_t), parameter, public :: HG_AD_NONE = 0, HG_AD_PHYSICAL = 1
+   integer(c_int32_t), parameter, public :: HG_AD_NONE = 0, HG_AD_PHYSICAL = 1, HG_AD_LAPLACIAN = 2
    integer(c_int32_t), parameter, public :: HG_PROBE_RHO = 0, HG_PROBE_RHOU = 1, HG_PROBE_RHOV = 2, &
                                             HG_PROBE_RHOE = 3, HG_PROBE_RHOT = 4, HG_PROBE_RHOUT = 5, &
                                             HG_PROBE_RHOVT = 6, HG_PROBE_RHOET = 7, HG_PROBE_U = 8, &
@@ -70,11 +70,22 @@ module horses_gpu_bindings
       real(c_double)     :: ad_intensity
    end type hg_options
 
+   ! batched form: tile -1 autotune / 0 narrow / 1 wide; order -1 autotune / 0 case-major /
+   ! 1 costliest tiles first / 2 cases interleaved; use_graph 1 = replay a captured step
+   type, bind(C), public :: hg_batch_options
+      integer(c_int32_t) :: tile
+      integer(c_int32_t) :: order
+      integer(c_int32_t) :: use_graph
+   end type hg_batch_options
+
    public :: hg_create, hg_destroy, hg_ndof, hg_get_node_coords, hg_upload_state, hg_download_state
    public :: hg_download_qdot, hg_time_derivative, hg_rk_step, hg_residuals, hg_estimate_dt
    public :: hg_reset_dt_state, hg_conservation_sums, hg_volume_integrals, hg_surface_integrals
    public :: hg_probes_set, hg_probes_update
    public :: hg_synchronize, hg_last_error
+   public :: hg_set_tiles, hg_get_tiles, hg_set_graph, hg_profile_enable, hg_profile_read
+   public :: hg_batch_create, hg_batch_rk_step, hg_batch_synchronize, hg_batch_schedule
+   public :: hg_batch_profile_enable, hg_batch_profile_read, hg_batch_destroy
    public :: hgf_check
 
    interface
@@ -203,6 +214,87 @@ module horses_gpu_bindings
          character(kind=c_char) :: buf(*)
          integer(c_int32_t), value :: n
       end function hg_last_error
+
+      ! performance controls (no reference counterpart): workgroup tiles, graph replay,
+      ! kernel timing (which: 0 stage kernel, 1 gradient kernel)
+      integer(c_int) function hg_set_tiles(handle, stage_tile, gradient_tile) bind(C, name="hg_set_tiles")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value        :: handle
+         integer(c_int32_t), value :: stage_tile, gradient_tile
+      end function hg_set_tiles
+
+      integer(c_int) function hg_get_tiles(handle, stage_tile, gradient_tile) bind(C, name="hg_get_tiles")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value              :: handle
+         integer(c_int32_t), intent(out) :: stage_tile, gradient_tile
+      end function hg_get_tiles
+
+      integer(c_int) function hg_set_graph(handle, on) bind(C, name="hg_set_graph")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value        :: handle
+         integer(c_int32_t), value :: on
+      end function hg_set_graph
+
+      integer(c_int) function hg_profile_enable(handle, on) bind(C, name="hg_profile_enable")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value        :: handle
+         integer(c_int32_t), value :: on
+      end function hg_profile_enable
+
+      integer(c_int) function hg_profile_read(handle, which, launches, ms) bind(C, name="hg_profile_read")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+         type(c_ptr), value              :: handle
+         integer(c_int32_t), value       :: which
+         integer(c_int64_t), intent(out) :: launches
+         real(c_double), intent(out)     :: ms
+      end function hg_profile_read
+
+      ! batched form: many handles (own mesh, order, dt) advanced by one launch per RK stage;
+      ! cases(:) holds the handles (they must outlive the batch)
+      integer(c_int) function hg_batch_create(cases, ncases, opt, batch) bind(C, name="hg_batch_create")
+         import :: c_int, c_ptr, c_int32_t, hg_batch_options
+         type(c_ptr), intent(in)            :: cases(*)
+         integer(c_int32_t), value          :: ncases
+         type(hg_batch_options), intent(in) :: opt
+         type(c_ptr), intent(out)           :: batch
+      end function hg_batch_create
+
+      ! one RK step of every case: t(c), dt(c) per case
+      integer(c_int) function hg_batch_rk_step(batch, t, dt) bind(C, name="hg_batch_rk_step")
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value         :: batch
+         real(c_double), intent(in) :: t(*), dt(*)
+      end function hg_batch_rk_step
+
+      integer(c_int) function hg_batch_synchronize(batch) bind(C, name="hg_batch_synchronize")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function hg_batch_synchronize
+
+      integer(c_int) function hg_batch_schedule(batch, tile, order) bind(C, name="hg_batch_schedule")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value              :: batch
+         integer(c_int32_t), intent(out) :: tile, order
+      end function hg_batch_schedule
+
+      integer(c_int) function hg_batch_profile_enable(batch, on) bind(C, name="hg_batch_profile_enable")
+         import :: c_int, c_ptr, c_int32_t
+         type(c_ptr), value        :: batch
+         integer(c_int32_t), value :: on
+      end function hg_batch_profile_enable
+
+      integer(c_int) function hg_batch_profile_read(batch, which, launches, ms) bind(C, name="hg_batch_profile_read")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+         type(c_ptr), value              :: batch
+         integer(c_int32_t), value       :: which
+         integer(c_int64_t), intent(out) :: launches
+         real(c_double), intent(out)     :: ms
+      end function hg_batch_profile_read
+
+      integer(c_int) function hg_batch_destroy(batch) bind(C, name="hg_batch_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function hg_batch_destroy
    end interface
 
 contains

@@ -122,6 +122,14 @@ def load_library(path: str = LIB_PATH):
     return L
 
 
+def _almost_equal(a: float, b: float) -> bool:
+    """AlmostEqual (Generic/Utilities.f90:37-63): within two machine epsilons, relative."""
+    eps = 2.0 * np.finfo(np.float64).eps
+    if a == 0.0 or b == 0.0:
+        return abs(a - b) <= eps
+    return abs(b - a) <= eps * max(abs(a), abs(b))
+
+
 def last_error() -> str:
     buf = ctypes.create_string_buffer(1024)
     load_library().hg_last_error(buf, 1024)
@@ -347,22 +355,43 @@ class DGSEM:
     def handle(self):
         return self._h
 
-    def integrate(self, steps: int, dt_max: Optional[float] = None, t_end: float = 1.0e30,
-                  monitor: bool = False):
-        """TimeIntegrator_Integrate (DGTimeIntegrator.f90:159-255), transient mode."""
-        dt_max = self.case.dt_max if dt_max is None else dt_max
-        dt = self.estimate_dt(dt_max, self.case.cfl)
+    def integrate(self, steps: Optional[int] = None, dt_max: Optional[float] = None,
+                  t_end: Optional[float] = None, monitor: bool = False, mode: Optional[str] = None,
+                  residual_target: Optional[float] = None):
+        """TimeIntegrator_Integrate (DGTimeIntegrator.f90:159-255) after NewTimeIntegrator's time-step
+        estimate (:86-157).  Transient: the last step is clamped to end at t_end and the loop
+        stops there.  Steady: after each step the residuals (max |QDot| of the last stage,
+        Monitors%UpdateValues) are read and the loop stops once their maximum is below the
+        residual target.  The time step is re-estimated every 100 iterations.  Defaults come
+        from the case (steps, dt cap, mode, target, t_end); returns the residual history
+        (every iteration when ``monitor`` or steady, else empty); ``self.iter`` is the last
+        iteration, as the reference's Integrator % iter."""
+        case = self.case
+        steps = case.steps if steps is None else steps
+        dt_max = case.dt_max if dt_max is None else dt_max
+        t_end = case.t_end if t_end is None else t_end
+        steady = (case.mode if mode is None else mode) == "steady"
+        target = case.residual_target if residual_target is None else residual_target
+        dt = self.estimate_dt(dt_max, case.cfl)
         history = []
-        for it in range(self.iter + 1, self.iter + steps + 1):
-            if self.t + dt >= t_end:
+        first = self.iter + 1
+        for it in range(first, first + steps):
+            self.iter = it
+            if not steady and self.t + dt >= t_end:
                 dt = t_end - self.t
             self.rk_step(self.t, dt)
             self.t = self.t + dt
-            self.iter = it
-            if monitor:
-                history.append(self.residuals())
+            if not steady and _almost_equal(self.t, t_end):
+                if monitor:
+                    history.append(self.residuals())
+                break
+            res = self.residuals() if (monitor or steady) else None
+            if res is not None:
+                history.append(res)
             if it % 100 == 0:
-                dt = self.estimate_dt(dt_max, self.case.cfl)
+                dt = self.estimate_dt(dt_max, case.cfl)
+            if steady and np.max(np.abs(res)) < target:
+                break
         self.dt = dt
         return np.array(history)
 

@@ -234,7 +234,8 @@ def read_p_refinement(path: str, pref_zones: Optional[np.ndarray], K: int, N: in
     return np.asarray(order, dtype=np.int32)[pref_zones]
 
 
-def case_from_hiocase(case_path: str, mesh_path: Optional[str] = None, ic=None, steps: Optional[int] = None):
+def case_from_hiocase(case_path: str, mesh_path: Optional[str] = None, ic=None, steps: Optional[int] = None,
+                      navier_stokes: Optional[bool] = None):
     """A ``cases.Case`` from a .HiOCase file (and its mesh, by default the file's "Mesh file"
     relative to the case file's directory, then its parent).  Keys: Reference pressure /
     Temperature, Reynolds length / number, Prandtl number, Mach number, Default polynomial
@@ -279,19 +280,29 @@ def case_from_hiocase(case_path: str, mesh_path: Optional[str] = None, ic=None, 
     if scheme not in SCHEME_NAMES:
         raise ValueError(f"Integration scheme '{scheme}' is not supported")
     # artificial dissipation (Setup.f90:333-349; DGArtificialDissipation.f90:100-137): type
-    # "Physical" (default) with the "Jumps-based" indicator (default); another type name is the
-    # base class, which adds nothing; "Laplacian" and "Residuals-based" are not provided
+    # "Physical" (default) or "Laplacian" with the "Jumps-based" indicator (default); another
+    # type name is the base class, which adds nothing; "Residuals-based" leaves the viscosity
+    # function unassociated in the reference and is not provided
     ad_on = _int(rv("Artificial dissipation (0/1)")) == 1
     ad_type = rv("Artificial dissipation type") or "Physical"
     ad_ind = rv("Artificial dissipation indicator") or "Jumps-based"
     ad_ceps = _real(rv("Artificial dissipation intensity"))
-    if ad_on and ad_type == "Laplacian":
-        raise ValueError("the Laplacian artificial dissipation is not supported")
-    if ad_on and ad_type == "Physical" and ad_ind != "Jumps-based":
-        raise ValueError(f"artificial dissipation indicator '{ad_ind}' is not supported")
+    ad_kind = "none"
+    if ad_on and ad_type in ("Physical", "Laplacian"):
+        if ad_ind != "Jumps-based":
+            raise ValueError(f"artificial dissipation indicator '{ad_ind}' is not supported")
+        ad_kind = ad_type.lower()
     cfl = _real(rv("CFL Number"))
     dt = _real(rv("Time step"))
     iters = _int(rv("Number of iterations"))
+    # TimeIntegrator_NewTimeIntegrator (DGTimeIntegrator.f90:86-157): "Steady" stops at the
+    # residual target, "Transient" at the simulation time
+    mode = (rv("Integration mode") or "Transient").strip()
+    if mode not in ("Steady", "Transient"):
+        raise ValueError(f"Integration mode '{mode}' is not supported")
+    target = _real(rv("Residual target"))
+    t_end = _real(rv("Simulation time"))
+    t0 = _real(rv("Initial time"))
 
     m = read_hiomesh(mesh_path, L)
     nmark = int(m["bmarker"].max()) if len(m["bmarker"]) else 0
@@ -299,15 +310,20 @@ def case_from_hiocase(case_path: str, mesh_path: Optional[str] = None, ic=None, 
     mesh = C.HiOMesh(m["points"], m["quads"], m["bedges"], m["bmarker"], zones,
                      m.get("cedges"), m.get("cxy"))
     orders = read_p_refinement(case_path, m.get("pref_zones"), len(m["quads"]), N)
-    ns = Re > 0.0
+    # the binary decides the equations (HORSES2D.NS / HORSES2D.Euler, Solver/Makefile:163-186);
+    # by default Navier-Stokes when the case has a viscosity or artificial dissipation
+    ns = (Re > 0.0 or ad_kind != "none") if navier_stokes is None else bool(navier_stokes)
     case = C.Case(os.path.splitext(os.path.basename(case_path))[0], mesh, orders, Mach, Re, Pr, ns,
                   riemann=RIEMANN_NAMES[rs], viscous=(visc.lower() if ns else "none"),
                   ip_method=ipm if ipm in C.IP_METHOD else "sipg",
                   sigma0=1.0 if sigma0 is None else sigma0, cfl=0.1 if cfl is None else cfl,
                   dt_max=0.01 if dt is None else dt, steps=steps or iters or 1,
                   scheme=SCHEME_NAMES[scheme], ic=ic,
-                  artificial_dissipation="physical" if (ad_on and ad_type == "Physical") else "none",
-                  ad_intensity=1.0 if ad_ceps is None else ad_ceps)
+                  artificial_dissipation=ad_kind if ns else "none",
+                  ad_intensity=1.0 if ad_ceps is None else ad_ceps, mode=mode.lower(),
+                  residual_target=0.0 if (mode == "Transient" or target is None) else target,
+                  t_end=1.0e30 if (mode == "Steady" or t_end is None) else t_end,
+                  t0=0.0 if t0 is None else t0)
     ic_name = rv("Initial condition") or ""
     if case.ic is None and ic_name == "Uniform":
         case.ic = C.uniform(C.freestream_state(Mach))

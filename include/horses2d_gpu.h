@@ -66,12 +66,14 @@ typedef struct hg_batch hg_batch;
 #define HG_PRECISION_FP64 0
 #define HG_PRECISION_MIXED 1 /* FP64 state/RK/reductions, FP32 fluxes and contractions */
 
-/* Artificial dissipation (Approximation/DGArtificialDissipation.f90): none, or the "Physical"
- * type (the element and edge viscosities mu_a added to mu in every viscous flux) with the
- * "Jumps-based" indicator.  The "Laplacian" type and the "Residuals-based" indicator (which
- * the reference leaves without a viscosity function) are not provided. */
+/* Artificial dissipation (Approximation/DGArtificialDissipation.f90), with the "Jumps-based"
+ * indicator: none, the "Physical" type (the element and edge viscosities mu_a added to mu in
+ * every viscous flux, :252-275) or the "Laplacian" type (a volume flux mu times the element's
+ * interior gradient and a face flux -mu Area (QL - QR), :207-249).  The "Residuals-based"
+ * indicator (which the reference leaves without a viscosity function) is not provided. */
 #define HG_AD_NONE 0
 #define HG_AD_PHYSICAL 1
+#define HG_AD_LAPLACIAN 2
 
 #define HG_SCHEME_EULER 0
 #define HG_SCHEME_RK3 1

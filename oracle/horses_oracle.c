@@ -62,6 +62,8 @@
 
 enum { RS_ROE = 0, RS_HLL = 1, RS_EXACT = 2, RS_LF = 3 };
 enum { VISC_NONE = 0, VISC_BR1 = 1, VISC_IP = 2 };
+/* artificial dissipation type (DGArtificialDissipation.f90:100-137) */
+enum { AD_NONE = 0, AD_PHYSICAL = 1, AD_LAPLACIAN = 2 };
 
 static char g_err[512];
 static int fail(const char *m) { snprintf(g_err, sizeof g_err, "%s", m); return -1; }
@@ -581,6 +583,7 @@ typedef struct {
     double *X, *dX, *n, *dS; /* X,dX,n: 2*P (straight: dX,n,dS only [0]) */
     double Area, invh;
     double mu_a;            /* artificial viscosity of the edge (ComputeEdgeViscosity) */
+    double lap;             /* Laplacian dissipation: the edge's EdgeViscosity (mean of its elements') */
     /* boundary data */
     int inviscidBCType, rs, associated, partner;
     int vbc[MAXP];
@@ -623,7 +626,8 @@ typedef struct {
     Phys ph;
     int ns, riemann, viscous;
     double sigma0, ip_eps; int ip_gradflux;
-    int ad; double ad_ceps; double *mu_e;   /* artificial dissipation: on/off, Ceps, per element */
+    int ad; double ad_ceps; double *mu_e;   /* artificial dissipation: type (AD_*), Ceps, per element mu_a */
+    double *lap_e;                          /* Laplacian dissipation: per element ElementViscosity */
     int nnodes; double *pts;
     int K; Elem *el;
     int E; Edge *ed;
@@ -826,7 +830,7 @@ static void mesh_free(Mesh *m) {
     for (int i = 0; i < m->E; ++i) { free(m->ed[i].X); free(m->ed[i].dX); free(m->ed[i].n); free(m->ed[i].dS); }
     for (int z = 0; z < m->nmarkers; ++z) free(m->zones[z].edges);
     free(m->el); free(m->ed); free(m->zones); free(m->pts);
-    free(m->Q); free(m->QDot); free(m->dQ); free(m->G); free(m->mu_e);
+    free(m->Q); free(m->QDot); free(m->dQ); free(m->G); free(m->mu_e); free(m->lap_e);
     free(m);
 }
 
@@ -1478,6 +1482,19 @@ static void gradient_br1(Mesh *m) {
 }
 
 /* QuadElement_QuadProcedures.incf:42-88 (IP interior gradient) */
+/* QuadElement_ComputeInteriorGradient (QuadElement_QuadProcedures.incf:42-88) at node (i, j):
+ * dq[eq + 4 d], the broken gradient (D Q, Q D^T) mapped with Ja and divided by jac. */
+static void interior_gradient(const Elem *e, const Spec *s, const double *Q, int i, int j, double *dq) {
+    int P = e->P, PP = P * P, k = i + P * j;
+    for (int eq = 0; eq < 4; ++eq) {
+        double dxi = 0.0, deta = 0.0;
+        for (int l = 0; l < P; ++l) dxi = dxi + s->D[i][l] * Q[QIDX(e, eq, l, j)];
+        for (int l = 0; l < P; ++l) deta = deta + s->D[j][l] * Q[QIDX(e, eq, i, l)];
+        dq[eq] = (dxi * e->Ja[0 * PP + k] + deta * e->Ja[1 * PP + k]) / e->jac[k];
+        dq[eq + 4] = (dxi * e->Ja[2 * PP + k] + deta * e->Ja[3 * PP + k]) / e->jac[k];
+    }
+}
+
 static void gradient_ip(Mesh *m) {
 #pragma omp parallel for schedule(static)
     for (int eid = 0; eid < m->K; ++eid) {
@@ -1629,9 +1646,13 @@ static double edge_jumps(Mesh *m, Edge *ed, int var) {
 /* Jumps-based artificial viscosity (DGArtificialDissipation.f90:277-363): per element
  * mu = Ceps sqrt(Volume)/(N+1) G(log10(max(sum J_rho, sum J_rhoE) / sum Area)), G the smooth
  * switch of half-width k = 5 around s0 = log10((N+1)^-4) + 1; per edge the mean of its
- * elements' values (one element for boundary edges, periodic ones included). */
+ * elements' values (one element for boundary edges, periodic ones included).  The "Physical"
+ * type adds them to mu in every viscous flux (e % mu_a, ed % mu_a); the "Laplacian" type keeps
+ * those at zero (it has no ComputeElementViscosity, :146-171) and uses the values in its own
+ * volume and face fluxes (:207-249). */
 static void artificial_viscosity(Mesh *m) {
     if (!m->ad) return;
+    double *mu_e = m->ad == AD_LAPLACIAN ? m->lap_e : m->mu_e;
     for (int eid = 0; eid < m->K; ++eid) {
         Elem *e = &m->el[eid];
         double gr = 0.0, ge = 0.0, area = 0.0;
@@ -1651,12 +1672,21 @@ static void artificial_viscosity(Mesh *m) {
             else if (gk < s0 + k) G = 0.5 * sin(0.5 * 3.141592653589793 * (gk - s0) / k) + 0.5;
             else G = 1.0;
         }
-        m->mu_e[eid] = m->ad_ceps * (sqrt(e->Volume) / (double)(e->N + 1)) * G;
+        mu_e[eid] = m->ad_ceps * (sqrt(e->Volume) / (double)(e->N + 1)) * G;
     }
     for (int k = 0; k < m->E; ++k) {
         Edge *ed = &m->ed[k];
-        ed->mu_a = ed->bdry ? m->mu_e[ed->quads[0]] : (m->mu_e[ed->quads[0]] + m->mu_e[ed->quads[1]]) / 2.0;
+        double mu = ed->bdry ? mu_e[ed->quads[0]] / 1.0 : (mu_e[ed->quads[0]] + mu_e[ed->quads[1]]) / 2.0;
+        if (m->ad == AD_LAPLACIAN) ed->lap = mu;
+        else ed->mu_a = mu;
     }
+}
+
+/* LaplaceDissipation_ComputeFaceFluxes (DGArtificialDissipation.f90:224-249): -mu Area (QL - QR)
+ * with mu the edge's viscosity; zero without the Laplacian type. */
+static void laplace_face(const Mesh *m, const Edge *ed, const double *qL, const double *qR, double *Fa) {
+    for (int eq = 0; eq < 4; ++eq)
+        Fa[eq] = m->ad == AD_LAPLACIAN ? -((ed->lap * ed->Area) * (qL[eq] - qR[eq])) : 0.0;
 }
 
 /* DGSpatialDiscretizationMethods.f90:160-235 computeQDot (+ newTimeStep :118-158) */
@@ -1690,7 +1720,16 @@ static void time_derivative(Mesh *m) {
                         for (int eq = 0; eq < 4; ++eq) dq[eq + 4 * d] = m->dQ[DQIDX(e, eq, i, j, d)];
                     viscous_flux(ph, q, dq, fv, gv);
                     const double mu = ph->mu + m->mu_e[eid];   /* e % mu_a, DGViscous_BR1.f90:307 */
-                    for (int eq = 0; eq < 4; ++eq) { f[eq] = (f[eq] - mu * fv[eq]) - 0.0; g[eq] = (g[eq] - mu * gv[eq]) - 0.0; }
+                    /* Laplacian: Fa = mu * the element's interior gradient (:207-222, ComputeInteriorGradient,
+                     * QuadElement_QuadProcedures.incf:42-88: the IP gradient itself) */
+                    double fa[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                    if (m->ad == AD_LAPLACIAN) {
+                        double dqi[8];
+                        if (m->viscous == VISC_IP) for (int c = 0; c < 8; ++c) dqi[c] = dq[c];
+                        else interior_gradient(e, s, m->Q, i, j, dqi);
+                        for (int c = 0; c < 8; ++c) fa[c] = m->lap_e[eid] * dqi[c];
+                    }
+                    for (int eq = 0; eq < 4; ++eq) { f[eq] = (f[eq] - mu * fv[eq]) - fa[eq]; g[eq] = (g[eq] - mu * gv[eq]) - fa[eq + 4]; }
                 }
                 for (int eq = 0; eq < 4; ++eq) {
                     cF[eq + 4 * k] = f[eq] * e->Ja[0 * PP + k] + g[eq] * e->Ja[2 * PP + k];
@@ -1731,7 +1770,9 @@ static void time_derivative(Mesh *m) {
                             dR[eq + 4 * d] = ed->tdq[1][eq + 4 * ir + 4 * P * d];
                         }
                     visc_interior(m, ed, qL, qR, dL, dR, nn, Fv);
-                    for (int eq = 0; eq < 4; ++eq) Fs[eq] = ((Fi[eq] - Fv[eq]) - 0.0) * dS;
+                    double Fa[4];
+                    laplace_face(m, ed, qL, qR, Fa);
+                    for (int eq = 0; eq < 4; ++eq) Fs[eq] = ((Fi[eq] - Fv[eq]) - Fa[eq]) * dS;
                     if (gradfl) {
                         double gl[8], gr[8];
                         ip_grad_interior(m, ed, qL, qR, nn, gl, gr);
@@ -1762,7 +1803,7 @@ static void time_derivative(Mesh *m) {
                 double dS = ed->curved ? ed->dS[i] : ed->dS[0];
                 const double *q = ed->tq[0] + 4 * i;
                 const double *qb = bdry_state(m, ed, i);
-                double Fi[4], Fv[4] = {0, 0, 0, 0}, Gv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                double Fi[4], Fv[4] = {0, 0, 0, 0}, Fa[4] = {0, 0, 0, 0}, Gv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 if (ed->inviscidBCType == WEAK_PRESCRIBED) {   /* Fi = FB (:1194-1198) */
                     for (int eq = 0; eq < 4; ++eq) Fi[eq] = ed->FB[4 * i + eq];
                 } else {
@@ -1779,7 +1820,8 @@ static void time_derivative(Mesh *m) {
                         double dqb[8];
                         partner_dq(m, ed, i, dqb);
                         visc_interior(m, ed, q, qb, dq, dqb, nn, Fv);
-                        for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] * dS;
+                        laplace_face(m, ed, q, qb, Fa);
+                        for (int eq = 0; eq < 4; ++eq) { Fv[eq] = Fv[eq] * dS; Fa[eq] = Fa[eq] * dS; }
                         if (gradfl) {
                             double gl[8], gr[8];
                             ip_grad_interior(m, ed, q, qb, nn, gl, gr);
@@ -1799,7 +1841,8 @@ static void time_derivative(Mesh *m) {
                             double pen = m->sigma0 * (ph->mu + ed->mu_a) * (double)ed->N * (double)ed->N * ed->invh;
                             for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] - pen * (q[eq] - usb[eq]);
                         }
-                        for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] * dS;
+                        laplace_face(m, ed, q, usb, Fa);
+                        for (int eq = 0; eq < 4; ++eq) { Fv[eq] = Fv[eq] * dS; Fa[eq] = Fa[eq] * dS; }
                         if (gradfl) {
                             double fg[8];
                             for (int d = 0; d < 2; ++d)
@@ -1818,7 +1861,8 @@ static void time_derivative(Mesh *m) {
                             double pen = m->sigma0 * (ph->mu + ed->mu_a) * (double)ed->N * (double)ed->N * ed->invh;
                             for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] - pen * (q[eq] - usb[eq]);
                         }
-                        for (int eq = 0; eq < 4; ++eq) Fv[eq] = Fv[eq] * dS;
+                        laplace_face(m, ed, q, usb, Fa);
+                        for (int eq = 0; eq < 4; ++eq) { Fv[eq] = Fv[eq] * dS; Fa[eq] = Fa[eq] * dS; }
                         if (gradfl) {
                             double fg[8];
                             for (int d = 0; d < 2; ++d)
@@ -1829,7 +1873,7 @@ static void time_derivative(Mesh *m) {
                         }
                     } /* NEUMANN: Fv = 0, Gv = 0 */
                 }
-                for (int eq = 0; eq < 4; ++eq) FL[eq + 4 * i] = m->ns ? ((Fi[eq] - Fv[eq]) - 0.0) : Fi[eq];
+                for (int eq = 0; eq < 4; ++eq) FL[eq + 4 * i] = m->ns ? ((Fi[eq] - Fv[eq]) - Fa[eq]) : Fi[eq];
                 for (int d = 0; d < 2; ++d)
                     for (int eq = 0; eq < 4; ++eq) GL[eq + 4 * i + 4 * P * d] = Gv[eq + 4 * d];
             }
@@ -1855,7 +1899,7 @@ typedef struct {
     double Mach, Re, Pr;
     int ns, riemann, viscous, ip_method; /* ip_method: 0 SIPG, 1 NIPG, 2 IIPG */
     double sigma0;
-    int ad;            /* artificial dissipation: 0 none, 1 "Physical" with the jumps-based indicator */
+    int ad;            /* artificial dissipation (AD_*): none, "Physical" or "Laplacian", jumps-based indicator */
     double ad_ceps;    /* "Artificial dissipation intensity" */
 } OrOptions;
 
@@ -1872,6 +1916,7 @@ void *or_create(const MeshIn *in, const OrOptions *o) {
     m->ad = m->ns ? o->ad : 0;   /* the module is Navier-Stokes only */
     m->ad_ceps = o->ad_ceps;
     m->mu_e = (double *)calloc(m->K + 1, sizeof(double));
+    m->lap_e = (double *)calloc(m->K + 1, sizeof(double));
     for (int k = 0; k < m->E; ++k) m->ed[k].mu_a = 0.0;
     return m;
 }

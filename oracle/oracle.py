@@ -284,17 +284,28 @@ class Oracle:
         return r
 
     def integrate(self, steps: int, t0: float = 0.0, dt_max: float = 1.0e30, t_end: float = 1.0e30,
-                  scheme: str = "rk3"):
-        """TimeIntegrator_Integrate (DGTimeIntegrator.f90:159-255), transient mode."""
+                  scheme: str = "rk3", mode: str = "transient", residual_target: float = 0.0):
+        """TimeIntegrator_Integrate (DGTimeIntegrator.f90:159-255): returns (t, dts, iterations,
+        residual history).  Transient: the last step ends at t_end; steady: stop after the first
+        iteration whose max residual (of its last stage) is below the target."""
         t = t0
         dt = self.estimate_dt(dt_max, self.case.cfl)
-        dts = []
+        dts, hist = [], []
+        steady = mode == "steady"
+        eps = 2.0 * np.finfo(np.float64).eps
+        it = 0
         for it in range(1, steps + 1):
-            if t + dt >= t_end:
+            if not steady and t + dt >= t_end:
                 dt = t_end - t
             self.step(t, dt, scheme)
             dts.append(dt)
             t = t + dt
+            res = self.residuals()
+            hist.append(res)
+            if not steady and abs(t_end - t) <= eps * max(abs(t), abs(t_end)):
+                break
             if it % 100 == 0:
                 dt = self.estimate_dt(dt_max, self.case.cfl)
-        return t, np.array(dts)
+            if steady and np.max(np.abs(res)) < residual_target:
+                break
+        return t, np.array(dts), it, np.array(hist)

@@ -1,0 +1,143 @@
+"""End-to-end pins against the reference's own benchmark goldens: the numbers each
+TestCases/Benchmark/Simulations/*/SETUP/ProblemFile.f90 ``Finalize`` checks after a full run.
+
+Ramp (Ramp/SETUP/ProblemFile.f90:79-89): supersonic flow (Mach 2) over a ramp, 180 straight
+quads of order 3, Euler walls and newDirichlet inflow/outflow, the Navier-Stokes equations at
+Re = 0 with interior penalty (IIPG), the Laplacian artificial dissipation with the jumps-based
+indicator (intensity 0.1), Williamson RK5, CFL 0.1, dt cap 2, steady mode with residual target
+1e-8.  Finalize requires the run to stop at iteration 9884 with the four residuals (max |QDot|
+of the last stage) within 1e-11 of its stored values.
+
+The case comes from committed fixtures (tests/golden/benchmark_ramp.json: the case file's
+parameters as horses2d_amd.hio reads them; mesh_ramp.npz: the HiOMesh arrays), written by
+tests/golden/make_golden.py from the reference tree.  Uniform initial condition
+(InitialConditions.f90:154-170).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from horses2d_amd import cases as C
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+REFERENCE = "/root/reference"
+
+
+def benchmark_case(name):
+    with open(os.path.join(GOLDEN, f"benchmark_{name}.json")) as f:
+        d = json.load(f)
+    p = d["case"]
+    m = np.load(os.path.join(GOLDEN, d["mesh"]), allow_pickle=False)
+    zones = [C.Zone(z["kind"], direction=z["direction"], partner=z["partner"], isothermal=z["isothermal"],
+                    q=tuple(z["q"]), Tw=z["Tw"], wall_velocity=tuple(z["wall_velocity"]), riemann=z["riemann"],
+                    outflow=z["outflow"], mode=z["mode"], outflow_pressure=z["outflow_pressure"]) for z in p["zones"]]
+    mesh = C.HiOMesh(m["points"], m["quads"], m["bedges"], m["bmarker"], zones,
+                     m["cedges"] if "cedges" in m else None, m["cxy"] if "cxy" in m else None)
+    assert len(p["N"]) == 1, "one polynomial order per mesh"
+    case = C.Case(name, mesh, np.full(mesh.K, p["N"][0], np.int32), p["Mach"], p["Re"], p["Pr"], True,
+                  riemann=p["riemann"], viscous=p["viscous"], ip_method=p["ip_method"], sigma0=p["sigma0"],
+                  cfl=p["cfl"], dt_max=p["dt_max"], steps=p["steps"], scheme=p["scheme"],
+                  ic=C.uniform(C.freestream_state(p["Mach"])), artificial_dissipation=p["artificial_dissipation"],
+                  ad_intensity=p["ad_intensity"], mode=p["mode"], residual_target=p["residual_target"],
+                  t_end=p["t_end"], t0=p["t0"])
+    return case, d["finalize"]
+
+
+def test_ramp_fixture_matches_the_case_file():
+    """The committed parameters are what the case reader makes of the reference's own files."""
+    path = os.path.join(REFERENCE, "TestCases/Benchmark/Simulations/Ramp/SETUP/Ramp.HiOCase")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    from horses2d_amd import hio
+    c = hio.case_from_hiocase(path, navier_stokes=True)
+    f, _ = benchmark_case("ramp")
+    for k in ("Mach", "Re", "viscous", "ip_method", "artificial_dissipation", "ad_intensity", "scheme", "mode",
+              "cfl", "dt_max", "residual_target", "steps"):
+        assert getattr(c, k) == getattr(f, k), k
+    np.testing.assert_array_equal(c.mesh.quads, f.mesh.quads)
+    np.testing.assert_array_equal(c.mesh.points, f.mesh.points)
+    assert [(z.kind, z.q) for z in c.mesh.zones] == [(z.kind, tuple(z.q)) for z in f.mesh.zones]
+
+
+def test_oracle_reproduces_the_ramp_golden():
+    """The oracle run to the steady stop: iteration 9884 and the residuals within Finalize's 1e-11."""
+    from oracle.oracle import Oracle
+    case, gold = benchmark_case("ramp")
+    orc = Oracle(case)
+    orc.set_state(case.initial_state(orc.coords()))
+    _, _, it, hist = orc.integrate(case.steps, dt_max=case.dt_max, scheme=case.scheme, mode=case.mode,
+                                   residual_target=case.residual_target)
+    print("oracle: iteration", it, "residuals", hist[-1], "golden", gold["residuals"])
+    assert it == gold["final_iteration"]
+    assert np.max(np.abs(hist[-1] - np.array(gold["residuals"]))) <= gold["residual_tolerance"]
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_the_ramp_golden():
+    """The FP64 GPU path (hg_rk_step replays of the RK5 step, hg_residuals after each, the host
+    loop of gpu.DGSEM.integrate) to the steady stop: iteration 9884, residuals within 1e-11."""
+    from horses2d_amd import gpu
+    case, gold = benchmark_case("ramp")
+    dg = gpu.DGSEM(case, "fp64")
+    hist = dg.integrate()
+    print("gpu: iteration", dg.iter, "residuals", hist[-1], "golden", gold["residuals"])
+    assert dg.iter == gold["final_iteration"]
+    assert np.max(np.abs(hist[-1] - np.array(gold["residuals"]))) <= gold["residual_tolerance"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_ramp_steps_match_the_oracle(precision):
+    """The first 60 RK5 steps (the shock forming, the dissipation switching on) against the
+    oracle: FP64 within 1e-12 (max-norm), the mixed form within 1e-6 (relative L2)."""
+    from horses2d_amd import gpu
+    from oracle.oracle import Oracle
+    case, _ = benchmark_case("ramp")
+    dg = gpu.DGSEM(case, precision)
+    orc = Oracle(case)
+    orc.set_state(case.initial_state(orc.coords()))
+    dt = orc.estimate_dt(case.dt_max, case.cfl)
+    assert dg.estimate_dt(case.dt_max, case.cfl) == pytest.approx(dt, rel=1e-14)
+    for it in range(60):
+        orc.step(0.0, dt, case.scheme)
+        dg.rk_step(0.0, dt)
+    a, b = dg.state(), orc.state()
+    if precision == "fp64":
+        e = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        assert e <= 1e-12, e
+        np.testing.assert_allclose(dg.residuals(), orc.residuals(), rtol=1e-10, atol=1e-10 * np.max(orc.residuals()))
+    else:
+        a = a.reshape(-1, 4); b = b.reshape(-1, 4)
+        err = np.sqrt(np.sum((a - b) ** 2, axis=0) / np.sum(b ** 2, axis=0))
+        print("mixed vs oracle after 60 steps", err)
+        assert np.all(err <= 1e-6), err
+
+
+def lift_drag(surface, Mach, reference_surface=1.0):
+    """The lift and drag monitors (IO/Monitors.f90:1130-1138, dynamic pressure :1013-1033) from
+    hg_surface_integrals' sums: (pressure force - sqrt(gamma) M viscous force) over the dynamic
+    pressure 1/2 rho V^2 S, which in the reference's dimensionless variables is gamma M^2 S / 2."""
+    F = surface[3:5] - surface[7:9] * np.sqrt(1.4) * Mach
+    q = 0.5 * 1.4 * Mach * Mach * reference_surface
+    return F[1] / q, F[0] / q
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_the_cylinder_mach2_golden():
+    """CylinderMach2 (CylinderMach2/SETUP/ProblemFile.f90:79-95): Mach 2 past a cylinder, 640
+    quads of order 5 with curved (order-10) wall edges, Euler walls, Dirichlet inflow, pressure
+    outlet, Navier-Stokes at Re = 0 with IIPG and the Laplacian dissipation (intensity 0.4),
+    RK5, CFL 0.05, dt cap 0.01, transient to t = 10.  Finalize: iteration 15599, residuals within
+    1e-9, lift and drag coefficients of the cylinder (zone 5) within 1e-12."""
+    from horses2d_amd import gpu
+    case, gold = benchmark_case("cylinder_mach2")
+    dg = gpu.DGSEM(case, "fp64")
+    hist = dg.integrate(monitor=True)
+    cl, cd = lift_drag(dg.surface_integrals(5), case.Mach)
+    print("gpu: iteration", dg.iter, "t", dg.t, "residuals", hist[-1], "cl", cl, "cd", cd, "golden", gold)
+    assert dg.iter == gold["final_iteration"]
+    assert np.max(np.abs(hist[-1] - np.array(gold["residuals"]))) <= gold["residual_tolerance"]
+    assert abs(cl - gold["cl"]) <= 1e-12 and abs(cd - gold["cd"]) <= 1e-12

@@ -157,3 +157,14 @@ def test_batch_steps_after_direct_steps_and_uploads(precision):
     batch_step()
     batch_step()
     assert_same(single, batched)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_laplacian_dissipation_in_a_batch(precision):
+    """The Laplacian dissipation (interior-penalty cases of different orders and boundaries,
+    the Ramp benchmark's Re = 0 form among them) in a batch: bit for bit the single-handle
+    steps."""
+    from test_gpu_parity import CASES as P
+    cases = [P["ns_iipg_lap_kh"](), P["ns_iipg_lap_newdirichlet_re0"]()]
+    single, batched, _ = run_pair(cases, precision, 3)
+    assert_same(single, batched)

@@ -1,7 +1,10 @@
 """The Fortran host (horses2d_amd/fortran/horses_gpu_driver.f90, ISO_C_BINDING module
-horses_gpu_bindings) drives the C ABI through the reference's time-loop semantics; its
-state after 20 RK3 iterations of the Euler vortex configuration matches the oracle."""
+horses_gpu_bindings) drives the C ABI through the reference's time-loop semantics
+(DGTimeIntegrator.f90:159-255): its state after 20 RK3 iterations of the Euler vortex
+configuration matches the oracle; in steady mode it stops at the same iteration as the
+oracle's loop; the batched form driven from Fortran (hg_batch_*) matches the oracle per case."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -14,18 +17,59 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "horses2d_amd", "bin", "horses_gpu_driver")
 
 
-def test_fortran_driver_matches_oracle(tmp_path):
+def run_driver(*args):
     if not os.path.exists(DRIVER):
         pytest.fail("Fortran driver not built: run __graft_entry__.build()")
-    out = tmp_path / "q.txt"
-    r = subprocess.run([DRIVER, "20", str(out)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    Qf = np.loadtxt(out)
+    return r.stdout
+
+
+def oracle_for(case):
     from oracle.oracle import Oracle
-    case = C.euler_vortex_case(16, 4)
     orc = Oracle(case)
     orc.set_state(case.initial_state(orc.coords()))
+    return orc
+
+
+def test_fortran_driver_matches_oracle(tmp_path):
+    out = tmp_path / "q.txt"
+    run_driver(20, out)
+    Qf = np.loadtxt(out)
+    orc = oracle_for(C.euler_vortex_case(16, 4))
     orc.integrate(20)
     Qo = orc.state()
     assert Qf.shape == Qo.shape
     assert np.max(np.abs(Qf - Qo)) / np.max(np.abs(Qo)) <= 1e-12
+
+
+def test_fortran_driver_steady_stop(tmp_path):
+    """Steady mode: the loop stops after the first iteration whose largest residual (of its
+    last stage) is below the target; the target sits inside the oracle's residual history."""
+    case = C.euler_vortex_case(16, 4)
+    _, _, _, hist = oracle_for(case).integrate(40)
+    m = np.max(np.abs(hist), axis=1)
+    target = float(np.sort(m)[len(m) // 2]) * (1 + 1e-9)
+    orc = oracle_for(case)
+    _, _, it, _ = orc.integrate(40, mode="steady", residual_target=target)
+    assert 1 < it < 40
+    out = tmp_path / "q.txt"
+    text = run_driver(40, out, repr(target))
+    assert int(re.search(r"iterations (\d+)", text).group(1)) == it
+    Qf, Qo = np.loadtxt(out), orc.state()
+    assert np.max(np.abs(Qf - Qo)) / np.max(np.abs(Qo)) <= 1e-12
+
+
+def test_fortran_driver_batched_form(tmp_path):
+    out = tmp_path / "q.txt"
+    run_driver("batch", 15, out)
+    Qf = np.loadtxt(out)
+    parts = []
+    for case in (C.euler_vortex_case(16, 4), C.euler_vortex_case(8, 6)):
+        orc = oracle_for(case)
+        orc.integrate(15)
+        parts.append(orc.state())
+    Qo = np.concatenate(parts)
+    assert Qf.shape == Qo.shape
+    for a, b in zip(np.split(Qf, [parts[0].size]), parts):
+        assert np.max(np.abs(a - b)) / np.max(np.abs(b)) <= 1e-12

@@ -58,6 +58,8 @@ CASES = {
     # artificial dissipation: the indicator from the FP32 traces (measured 3e-9 .. 6e-8)
     "ns_br1_ad_tgv": (lambda: _parity_case("ns_br1_ad_tgv"), 3e-7),
     "ns_sipg_ad_kh": (lambda: _parity_case("ns_sipg_ad_kh"), 3e-7),
+    "ns_iipg_lap_kh": (lambda: _parity_case("ns_iipg_lap_kh"), 3e-7),
+    "ns_br1_lap_annulus": (lambda: _parity_case("ns_br1_lap_annulus"), 3e-7),
 }
 
 

@@ -112,10 +112,15 @@ def _euler_wall_annulus(ns):
     return c
 
 
-def _with_density_step(case, x0, ceps=1.0):
-    """Artificial dissipation on (the "Physical" type, jumps-based indicator), and the case's
-    initial density raised by a quarter for x > x0 (velocity and pressure kept): a jump that
-    the element edges do not follow, so the indicator takes many values."""
+def _re0(case):
+    case.Re = 0.0
+    return case
+
+
+def _with_density_step(case, x0, ceps=1.0, kind="physical"):
+    """Artificial dissipation on (the "Physical" or "Laplacian" type, jumps-based indicator), and
+    the case's initial density raised by a quarter for x > x0 (velocity and pressure kept): a
+    jump that the element edges do not follow, so the indicator takes many values."""
     base = case.ic
 
     def ic(x, y):
@@ -125,7 +130,7 @@ def _with_density_step(case, x0, ceps=1.0):
         r = rho * np.where(x > x0, 1.25, 1.0)
         return C._pack(r, u, v, p)
     case.ic = ic
-    case.artificial_dissipation, case.ad_intensity = "physical", ceps
+    case.artificial_dissipation, case.ad_intensity = kind, ceps
     return case
 
 
@@ -164,6 +169,17 @@ CASES = {
     "ns_br1_ad_channel": lambda: _with_density_step(
         _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "br1", 1), 1.3),
     "ns_br1_ad_annulus": lambda: _with_density_step(C.ns_cylinder_case(4, 16, N=3, Re=200.0), 0.2),
+    # Laplacian dissipation (DGArtificialDissipation.f90:207-249): interior gradient volume flux,
+    # -mu Area (QL - QR) face flux; IIPG/SIPG/BR1, periodic, curved, walls and pressure boundaries
+    "ns_iipg_lap_kh": lambda: _with_density_step(_kh("iipg"), 0.47, 2.0, "laplacian"),
+    "ns_sipg_lap_channel": lambda: _with_density_step(
+        _channel(C.BC_PRESSUREINLET, C.BC_PRESSUREOUTLET, 0.3, 1.5, "ip", 1), 1.3, 1.0, "laplacian"),
+    "ns_br1_lap_tgv": lambda: _with_density_step(C.ns_tgv_case(6, 4, Re=1600.0), 2.9, 1.0, "laplacian"),
+    "ns_br1_lap_annulus": lambda: _with_density_step(C.ns_cylinder_case(4, 16, N=3, Re=200.0), 0.2, 1.0,
+                                                      "laplacian"),
+    # the Ramp benchmark's form: Navier-Stokes at Re = 0 (only the dissipation is viscous)
+    "ns_iipg_lap_newdirichlet_re0": lambda: _re0(_with_density_step(
+        _newdirichlet(1.6, 0.6, "ip", mode=1, ip="iipg"), 1.3, 0.5, "laplacian")),
 }
 
 

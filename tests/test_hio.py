@@ -209,12 +209,18 @@ def test_artificial_dissipation_keys(tmp_path):
         f.write(text.replace("Number of iterations : 77", "Number of iterations : 77" + extra))
     c = hio.case_from_hiocase(path)
     assert (c.artificial_dissipation, c.ad_intensity) == ("physical", 0.3)
-    for bad in ("   Artificial dissipation type : Laplacian\n",
-                "   Artificial dissipation indicator : Residuals-based\n"):
-        with open(path, "w") as f:
-            f.write(text.replace("Number of iterations : 77", "Number of iterations : 77" + extra + bad))
-        with pytest.raises(ValueError):
-            hio.case_from_hiocase(path)
+    lap = "   Artificial dissipation type : Laplacian\n"
+    with open(path, "w") as f:
+        f.write(text.replace("Number of iterations : 77", "Number of iterations : 77" + extra + lap))
+    assert hio.case_from_hiocase(path).artificial_dissipation == "laplacian"
+    # the dissipation module is Navier-Stokes only (DGArtificialDissipation.f90:22): the Euler
+    # binary runs without it
+    assert hio.case_from_hiocase(path, navier_stokes=False).artificial_dissipation == "none"
+    bad = "   Artificial dissipation indicator : Residuals-based\n"
+    with open(path, "w") as f:
+        f.write(text.replace("Number of iterations : 77", "Number of iterations : 77" + extra + bad))
+    with pytest.raises(ValueError):
+        hio.case_from_hiocase(path)
 
 
 def test_p_refinement_zones_map_to_orders(tmp_path):
