@@ -106,7 +106,8 @@ def traffic_of(name):
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("bytes_per_launch")
+        d = json.load(fh)
+    return d.get("bytes_per_launch", d.get("stage_kernel_bytes_per_launch"))
 
 
 # ---------------------------------------------------------------------------------------------
