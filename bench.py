@@ -182,7 +182,8 @@ def ensemble_forms(cases, steps, warmup):
                      "stage_kernel_launches": n_st, "stage_kernel_avg_ms": ms_st / max(n_st, 1),
                      "resid_kernel_avg_ms": ms_rs / max(n_rs, 1),
                      "schedule": {"tile": ["narrow", "wide"][tile],
-                                  "order": ["case-major", "costliest first", "interleaved"][order]}}
+                                  "order": ["case-major", "costliest first", "interleaved",
+                                            "one launch per order"][order]}}
         states[name] = q
 
     batched(hs, "batched_fp64")

@@ -69,6 +69,23 @@ struct hg_batch {
     int nitems[2] = {0, 0};
     int block[2] = {0, 0};
     int *sched[2][3] = {};                 // per tile, per order
+    // One launch per polynomial order P (points per direction; schedule order 3).  Candidates:
+    // per tile and P, the case-major positions in items[tile] of that order's work items.
+    // Chosen: a tile per order (ptile), one work list mixing the two tiles' items with unique
+    // residual slots (items_po, ranges_po, sched_po), and side streams on which the orders'
+    // launches of one stage run concurrently between a fork and a join event.
+    int *psched[2][12] = {};
+    int pnitems[2][12] = {};
+    std::vector<int> porders;              // the orders present, costliest first
+    int ptile[12] = {};
+    int4 *items_po = nullptr;
+    int2 *ranges_po = nullptr;
+    int *sched_po[12] = {};
+    int nitems_po[12] = {};
+    size_t max_items = 0;                  // capacity of resid_part, in work items
+    hipStream_t side[4] = {};
+    int nside = 0;
+    hipEvent_t fork = nullptr, join[4] = {};
     int2 *ranges[2] = {nullptr, nullptr};  // per tile: per case residual partial range
     double **resid_out = nullptr;          // per case: the handle's residual slots
     double *dt = nullptr;                  // per case
@@ -102,18 +119,100 @@ struct hg_batch {
 
 namespace {
 
+constexpr int kPerOrder = 3;   // schedule order: one launch per polynomial order
+
 hg::BatchArgs args_of(const hg_batch *b, int tile, int order, int slot) {
     hg::BatchArgs a{};
     a.slots = b->slots;
     a.nslots = b->nslots;
     a.slot = slot;
     a.items = b->items[tile];
-    a.sched = b->sched[tile][order];
+    a.sched = b->sched[tile][order == kPerOrder ? 0 : order];
     a.nitems = b->nitems[tile];
     a.dt = b->dt;
     a.dt_cq = b->sch.dt_cq;
     a.dt_cg = b->sch.dt_cg;
     return a;
+}
+
+// A candidate of the autotuner: order P's items of one tile's list.
+hg::BatchArgs args_of_order(const hg_batch *b, int tile, int P, int slot) {
+    hg::BatchArgs a = args_of(b, tile, 0, slot);
+    a.sched = b->psched[tile][P];
+    a.nitems = b->pnitems[tile][P];
+    return a;
+}
+
+// The chosen per-order schedule: order P's items of the mixed-tile list.
+hg::BatchArgs args_po(const hg_batch *b, int P, int slot) {
+    hg::BatchArgs a = args_of(b, 0, 0, slot);
+    a.items = b->items_po;
+    a.sched = b->sched_po[P];
+    a.nitems = b->nitems_po[P];
+    return a;
+}
+
+// One launch per order, each on a side stream: the side streams wait for what the batch stream
+// holds (fork), the batch stream for every side stream (join).  Captured into the step's graph,
+// the orders' kernels become parallel branches.
+template <typename F>
+hipError_t per_order(hg_batch *b, F &&launch) {
+    hipError_t e = hipEventRecord(b->fork, b->stream);
+    for (int k = 0; k < b->nside && e == hipSuccess; ++k) e = hipStreamWaitEvent(b->side[k], b->fork, 0);
+    for (size_t i = 0; i < b->porders.size() && e == hipSuccess; ++i)
+        e = launch(b->porders[i], b->side[i % b->nside]);
+    for (int k = 0; k < b->nside && e == hipSuccess; ++k) {
+        e = hipEventRecord(b->join[k], b->side[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join[k], 0);
+    }
+    return e;
+}
+
+// The gradient pass and the stage pass of one slot: one launch each over the whole work list,
+// or one per polynomial order (all gradient launches before the stage launches, so that each
+// pass is timed as one interval).
+hipError_t gradient_pass(hg_batch *b, int tile, int order, int slot) {
+    if (order != kPerOrder)
+        return hg::launch_batch_gradient(b->variant, tile, args_of(b, tile, order, slot), b->block[tile], b->stream);
+    return per_order(b, [&](int P, hipStream_t s) {
+        return hg::launch_batch_gradient_p(b->variant, P, b->ptile[P], args_po(b, P, slot), s);
+    });
+}
+
+hipError_t stage_pass(hg_batch *b, int tile, int order, int slot) {
+    if (order != kPerOrder)
+        return hg::launch_batch_stage(b->variant, b->rs, tile, args_of(b, tile, order, slot), b->block[tile], b->stream);
+    return per_order(b, [&](int P, hipStream_t s) {
+        return hg::launch_batch_stage_p(b->variant, b->rs, P, b->ptile[P], args_po(b, P, slot), s);
+    });
+}
+
+// The per-order work list for the chosen tiles: each case's element tiles in the tile of its
+// order, residual slots numbered over the whole list (case-major), per order the positions of
+// its items.  Replaces an earlier list (the autotuner runs once per batch).
+int build_per_order_list(hg_batch *b) {
+    std::vector<int4> items;
+    std::vector<int2> ranges;
+    std::vector<int> per[12];
+    for (int c = 0; c < (int)b->cases.size(); ++c) {
+        const hg_handle *h = b->cases[c];
+        const int epb = hg::tile_elems(h->P, b->ptile[h->P]);
+        const int first = (int)items.size();
+        for (int e0 = 0; e0 < h->K; e0 += epb) {
+            per[h->P].push_back((int)items.size());
+            items.push_back(make_int4(c, e0, h->P, (int)items.size()));
+        }
+        ranges.push_back(make_int2(first, (int)items.size()));
+    }
+    if (items.size() > b->max_items) return hg::handle_error("hg_batch: per-order work list exceeds the residual buffer");
+    HB_CHECK(b->upload(&b->items_po, items));
+    HB_CHECK(b->upload(&b->ranges_po, ranges));
+    for (int P = 0; P < 12; ++P) {
+        b->nitems_po[P] = (int)per[P].size();
+        b->sched_po[P] = nullptr;
+        if (!per[P].empty()) HB_CHECK(b->upload(&b->sched_po[P], per[P]));
+    }
+    return 0;
 }
 
 // One RK step of all cases for the given trace parity (launches only).  Interior penalty: no
@@ -159,23 +258,20 @@ hipError_t enqueue_step(hg_batch *b, int parity) {
     const int t = b->tile;
     for (int s = 0; s < b->sch.nst; ++s) {
         const int slot = parity * b->sch.nst + s;
-        hg::BatchArgs a = args_of(b, t, b->order, slot);
         hipError_t e;
         if ((e = art_visc_all(b, slot)) != hipSuccess) return e;
-        if (ns && (e = timed(b, 1, [&] { return hg::launch_batch_gradient(b->variant, t, a, b->block[t], b->stream); })) !=
-                      hipSuccess)
-            return e;
-        if ((e = timed(b, 0, [&] { return hg::launch_batch_stage(b->variant, b->rs, t, a, b->block[t], b->stream); })) !=
-            hipSuccess)
-            return e;
+        if (ns && (e = timed(b, 1, [&] { return gradient_pass(b, t, b->order, slot); })) != hipSuccess) return e;
+        if ((e = timed(b, 0, [&] { return stage_pass(b, t, b->order, slot); })) != hipSuccess) return e;
     }
     return timed(b, 2, [&] {
-        return hg::launch_batch_resid(b->resid_part, b->ranges[t], b->resid_out, (int)b->cases.size(), b->stream);
+        const int2 *ranges = b->order == kPerOrder ? b->ranges_po : b->ranges[t];
+        return hg::launch_batch_resid(b->resid_part, ranges, b->resid_out, (int)b->cases.size(), b->stream);
     });
 }
 
 // Times derivative-only evaluations (plus the gradient pass) of every (tile, order)
-// candidate not fixed by the options and keeps the fastest.
+// candidate not fixed by the options and keeps the fastest.  The per-order candidate first
+// picks each order's tile from its own launches, then is timed as it runs (concurrent orders).
 int autotune(hg_batch *b) {
     // the gradient pass runs as in enqueue_step (interior penalty: the stage epilogue forms it)
     const bool ns = b->cases[0]->ph.ns != 0 && !b->cases[0]->ip_fused;
@@ -183,27 +279,59 @@ int autotune(hg_batch *b) {
     hipEvent_t ev[2];
     HB_CHECK(hipEventCreate(&ev[0]));
     HB_CHECK(hipEventCreate(&ev[1]));
-    float best = 1e30f;
+    // ms of three runs of a launch sequence on the batch stream, after one untimed run
+    auto time_of = [&](auto &&run, float *ms) -> hipError_t {
+        hipError_t e = run();
+        if (e == hipSuccess) e = hipEventRecord(ev[0], b->stream);
+        for (int r = 0; r < 3 && e == hipSuccess; ++r) e = run();
+        if (e == hipSuccess) e = hipEventRecord(ev[1], b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, ev[0], ev[1]);
+        return e;
+    };
+    float best = 1e30f, ms = 0.f;
     int bt = 1, bo = 0;
     for (int t = 0; t < 2; ++t) {
         if (b->opt.tile >= 0 && t != b->opt.tile) continue;
-        for (int o = 0; o < 3; ++o) {
+        for (int o = 0; o < kPerOrder; ++o) {
             if (b->opt.order >= 0 && o != b->opt.order) continue;
-            hg::BatchArgs a = args_of(b, t, o, dslot);
-            auto run = [&]() -> hipError_t {
+            HB_CHECK(time_of([&]() -> hipError_t {
                 hipError_t e = art_visc_all(b, dslot);
-                if (e == hipSuccess && ns) e = hg::launch_batch_gradient(b->variant, t, a, b->block[t], b->stream);
-                if (e == hipSuccess) e = hg::launch_batch_stage(b->variant, b->rs, t, a, b->block[t], b->stream);
+                if (e == hipSuccess && ns) e = gradient_pass(b, t, o, dslot);
+                if (e == hipSuccess) e = stage_pass(b, t, o, dslot);
                 return e;
-            };
-            HB_CHECK(run());
-            HB_CHECK(hipEventRecord(ev[0], b->stream));
-            for (int r = 0; r < 3; ++r) HB_CHECK(run());
-            HB_CHECK(hipEventRecord(ev[1], b->stream));
-            HB_CHECK(hipEventSynchronize(ev[1]));
-            float ms = 0.f;
-            HB_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+            }, &ms));
             if (ms < best) { best = ms; bt = t; bo = o; }
+        }
+    }
+    if (b->opt.order < 0 || b->opt.order == kPerOrder) {
+        for (int P : b->porders) {
+            float pbest = 1e30f;
+            for (int t = 0; t < 2; ++t) {
+                if (b->opt.tile >= 0 && t != b->opt.tile) continue;
+                const hg::BatchArgs a = args_of_order(b, t, P, dslot);
+                HB_CHECK(time_of([&]() -> hipError_t {
+                    hipError_t e = ns ? hg::launch_batch_gradient_p(b->variant, P, t, a, b->stream) : hipSuccess;
+                    if (e == hipSuccess) e = hg::launch_batch_stage_p(b->variant, b->rs, P, t, a, b->stream);
+                    return e;
+                }, &ms));
+                if (ms < pbest) { pbest = ms; b->ptile[P] = t; }
+            }
+        }
+        if (build_per_order_list(b)) return 1;
+        HB_CHECK(time_of([&]() -> hipError_t {
+            hipError_t e = art_visc_all(b, dslot);
+            if (e == hipSuccess && ns) e = gradient_pass(b, 0, kPerOrder, dslot);
+            if (e == hipSuccess) e = stage_pass(b, 0, kPerOrder, dslot);
+            return e;
+        }, &ms));
+        if (ms < best) {
+            best = ms;
+            bo = kPerOrder;
+            // reported tile: the one holding most of the work items
+            int n[2] = {0, 0};
+            for (int P : b->porders) n[b->ptile[P]] += b->nitems_po[P];
+            bt = n[1] >= n[0] ? 1 : 0;
         }
     }
     hipEventDestroy(ev[0]);
@@ -285,8 +413,8 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     *out = nullptr;
     hg_batch_options o{-1, -1, 1};
     if (opt) o = *opt;
-    if (o.tile < -1 || o.tile > 1 || o.order < -1 || o.order > 2)
-        return hg::handle_error("hg_batch_create: tile must be -1..1 and order -1..2");
+    if (o.tile < -1 || o.tile > 1 || o.order < -1 || o.order > kPerOrder)
+        return hg::handle_error("hg_batch_create: tile must be -1..1 and order -1..3");
     hg_handle *h0 = cases[0];
     for (int c = 0; c < ncases; ++c) {
         hg_handle *h = cases[c];
@@ -358,7 +486,23 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
         HB_CHECK(b->upload(&b->sched[t][0], s0));
         HB_CHECK(b->upload(&b->sched[t][1], s1));
         HB_CHECK(b->upload(&b->sched[t][2], s2));
+        // per order: the case-major list restricted to that order
+        std::vector<int> per[12];
+        for (int i : s0) per[items[t][i].z].push_back(i);
+        for (int P = 0; P < 12; ++P) {
+            b->pnitems[t][P] = (int)per[P].size();
+            if (!per[P].empty()) HB_CHECK(b->upload(&b->psched[t][P], per[P]));
+        }
     }
+    for (int P = 11; P >= 0; --P)
+        if (b->pnitems[0][P] > 0) b->porders.push_back(P);
+    b->max_items = max_items;
+    b->nside = std::min<int>(4, (int)b->porders.size());
+    for (int k = 0; k < b->nside; ++k) {
+        HB_CHECK(hipStreamCreateWithFlags(&b->side[k], hipStreamNonBlocking));
+        HB_CHECK(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming));
+    }
+    HB_CHECK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     std::vector<double> zeros(4 * std::max<size_t>(max_items, 1), 0.0);
     HB_CHECK(b->upload(&b->resid_part, zeros));
     std::vector<double> dt0(ncases, 0.0);
@@ -474,6 +618,11 @@ int hg_batch_destroy(hg_batch *b) {
         hipEventDestroy(p.second.second);
     }
     for (void *p : b->bufs) hipFree(p);
+    for (int k = 0; k < 4; ++k) {
+        if (b->side[k]) hipStreamDestroy(b->side[k]);
+        if (b->join[k]) hipEventDestroy(b->join[k]);
+    }
+    if (b->fork) hipEventDestroy(b->fork);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return 0;

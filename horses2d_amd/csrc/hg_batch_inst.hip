@@ -30,6 +30,10 @@ hipError_t HG_STAGE_NAME(int tile, const BatchArgs &b, int block, hipStream_t st
     return batch_stage_variant<kBvNS, kBvIP, HG_RS1 - 1, BvR, BvR>(tile, b, block, st);
 }
 
+hipError_t HG_CAT(HG_STAGE_NAME, _p)(int P, int tile, const BatchArgs &b, hipStream_t st) {
+    return batch_stage_p_variant<kBvNS, kBvIP, HG_RS1 - 1, BvR, BvR>(P, tile, b, st);
+}
+
 #if HG_RS1 == 0
 hipError_t HG_CAT(batch_gradient_v, HG_BV)(int tile, const BatchArgs &b, int block, hipStream_t st) {
     if constexpr (!kBvNS) {
@@ -37,6 +41,15 @@ hipError_t HG_CAT(batch_gradient_v, HG_BV)(int tile, const BatchArgs &b, int blo
         return hipErrorInvalidValue;
     } else {
         return batch_gradient_variant<!kBvIP, BvR, BvR>(tile, b, block, st);
+    }
+}
+
+hipError_t HG_CAT(HG_CAT(batch_gradient_v, HG_BV), _p)(int P, int tile, const BatchArgs &b, hipStream_t st) {
+    if constexpr (!kBvNS) {
+        (void)P; (void)tile; (void)b; (void)st;
+        return hipErrorInvalidValue;
+    } else {
+        return batch_gradient_p_variant<!kBvIP, BvR, BvR>(P, tile, b, st);
     }
 }
 #endif

@@ -120,6 +120,95 @@ __global__ __launch_bounds__(256) void k_batch_gradient(const StageArgs *__restr
 
 #undef HG_BATCH_CASE
 
+// One order per launch: the work items of the cases of order P only, with that order's own LDS
+// tile and register allocation (the union kernels above reserve the largest order's LDS and the
+// register count of the costliest order for every item).  Same stage_body / gradient_body, so
+// the same bits.
+template <int P, int TILE, bool NS, bool IP, int RS, typename R, typename TR>
+__global__ __launch_bounds__((Tile<P, TileOf<P, TILE>::EPB>::TPB)) void k_batch_stage_p(
+    const StageArgs *__restrict__ slots, const int4 *__restrict__ items, const int *__restrict__ sched,
+    const double *__restrict__ dt, BatchArgs b) {
+    constexpr int EPB = TileOf<P, TILE>::EPB;
+    __shared__ StageSmem<P, EPB, NS, IP, R> S;
+    const int k = xcd_tile(blockIdx.x, gridDim.x);
+    if (k < b.nitems) {
+        const int4 it = uniform_item(items, sched, k);
+        const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
+        double cq = a.cq, cg = a.cg;
+        if (dt) {
+            const double d = dt[it.x];
+            if (b.dt_cq) cq *= d;
+            if (b.dt_cg) cg *= d;
+        }
+        stage_body<P, EPB, NS, IP, RS, R, TR>(a, it.y, it.w, cq, cg, S);
+    }
+}
+
+template <int P, int TILE, bool BR1, typename R, typename TR>
+__global__ __launch_bounds__((Tile<P, TileOf<P, TILE>::EPB>::TPB)) void k_batch_gradient_p(
+    const StageArgs *__restrict__ slots, const int4 *__restrict__ items, const int *__restrict__ sched, BatchArgs b) {
+    constexpr int EPB = TileOf<P, TILE>::EPB;
+    __shared__ GradSmem<P, EPB, BR1, R> S;
+    const int k = xcd_tile(blockIdx.x, gridDim.x);
+    if (k < b.nitems) {
+        const int4 it = uniform_item(items, sched, k);
+        const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
+        gradient_body<P, EPB, BR1, R, TR, true>(a, it.y, S);
+    }
+}
+
+template <int P, bool NS, bool IP, int RS, typename R, typename TR>
+hipError_t batch_stage_p_launch(int tile, const BatchArgs &b, hipStream_t st) {
+    if (tile == 0)
+        hipLaunchKernelGGL((k_batch_stage_p<P, 0, NS, IP, RS, R, TR>), dim3(b.nitems),
+                           dim3(Tile<P, TileOf<P, 0>::EPB>::TPB), 0, st, b.slots, b.items, b.sched, b.dt, b);
+    else
+        hipLaunchKernelGGL((k_batch_stage_p<P, 1, NS, IP, RS, R, TR>), dim3(b.nitems),
+                           dim3(Tile<P, TileOf<P, 1>::EPB>::TPB), 0, st, b.slots, b.items, b.sched, b.dt, b);
+    return hipGetLastError();
+}
+
+template <int P, bool BR1, typename R, typename TR>
+hipError_t batch_gradient_p_launch(int tile, const BatchArgs &b, hipStream_t st) {
+    if (tile == 0)
+        hipLaunchKernelGGL((k_batch_gradient_p<P, 0, BR1, R, TR>), dim3(b.nitems),
+                           dim3(Tile<P, TileOf<P, 0>::EPB>::TPB), 0, st, b.slots, b.items, b.sched, b);
+    else
+        hipLaunchKernelGGL((k_batch_gradient_p<P, 1, BR1, R, TR>), dim3(b.nitems),
+                           dim3(Tile<P, TileOf<P, 1>::EPB>::TPB), 0, st, b.slots, b.items, b.sched, b);
+    return hipGetLastError();
+}
+
+#define HG_BATCH_ORDERS(CALL) \
+    switch (P) {                  \
+    case 2: return CALL(2);       \
+    case 3: return CALL(3);       \
+    case 4: return CALL(4);       \
+    case 5: return CALL(5);       \
+    case 6: return CALL(6);       \
+    case 7: return CALL(7);       \
+    case 8: return CALL(8);       \
+    case 9: return CALL(9);       \
+    case 10: return CALL(10);     \
+    case 11: return CALL(11);     \
+    default: return hipErrorInvalidValue; \
+    }
+
+template <bool NS, bool IP, int RS, typename R, typename TR>
+hipError_t batch_stage_p_variant(int P, int tile, const BatchArgs &b, hipStream_t st) {
+#define HG_CALL(PV) batch_stage_p_launch<PV, NS, IP, RS, R, TR>(tile, b, st)
+    HG_BATCH_ORDERS(HG_CALL)
+#undef HG_CALL
+}
+
+template <bool BR1, typename R, typename TR>
+hipError_t batch_gradient_p_variant(int P, int tile, const BatchArgs &b, hipStream_t st) {
+#define HG_CALL(PV) batch_gradient_p_launch<PV, BR1, R, TR>(tile, b, st)
+    HG_BATCH_ORDERS(HG_CALL)
+#undef HG_CALL
+}
+#undef HG_BATCH_ORDERS
+
 // Launchers of one physics/precision variant (instantiated by hg_batch_inst.hip); grid = nitems.
 template <bool NS, bool IP, int RS, typename R, typename TR>
 hipError_t batch_stage_variant(int tile, const BatchArgs &b, int block, hipStream_t st) {

@@ -93,6 +93,9 @@ struct BatchArgs {
 // variant = 2 * physics (0 Euler, 1 Navier-Stokes BR1, 2 Navier-Stokes IP) + (1 if mixed)
 hipError_t launch_batch_stage(int variant, int rs, int tile, const BatchArgs &b, int block, hipStream_t s);
 hipError_t launch_batch_gradient(int variant, int tile, const BatchArgs &b, int block, hipStream_t s);
+// One order per launch: the work items (b.items, b.sched, b.nitems) of order P only.
+hipError_t launch_batch_stage_p(int variant, int rs, int P, int tile, const BatchArgs &b, hipStream_t s);
+hipError_t launch_batch_gradient_p(int variant, int P, int tile, const BatchArgs &b, hipStream_t s);
 // out[c][0..4) = max over the residual partials [ranges[c].x, ranges[c].y) (one workgroup per case)
 hipError_t launch_batch_resid(const double *part, const int2 *ranges, double *const *out, int ncases, hipStream_t s);
 int tile_threads(int P, int tile);  // workgroup size of a tile
@@ -101,14 +104,16 @@ int tile_elems(int P, int tile);    // elements per tile
 // Per-variant launchers (one object file per variant and Riemann solver, from
 // hg_batch_inst.hip): batch_stage_v<V>_rs<RS + 1>, batch_gradient_v<V>.
 #define HG_DECLARE_BATCH_STAGE(V, RS1) \
-    hipError_t batch_stage_v##V##_rs##RS1(int tile, const BatchArgs &b, int block, hipStream_t s);
+    hipError_t batch_stage_v##V##_rs##RS1(int tile, const BatchArgs &b, int block, hipStream_t s); \
+    hipError_t batch_stage_v##V##_rs##RS1##_p(int P, int tile, const BatchArgs &b, hipStream_t s);
 #define HG_DECLARE_BATCH_VARIANT(V)                                                            \
     HG_DECLARE_BATCH_STAGE(V, 0)                                                               \
     HG_DECLARE_BATCH_STAGE(V, 1)                                                               \
     HG_DECLARE_BATCH_STAGE(V, 2)                                                               \
     HG_DECLARE_BATCH_STAGE(V, 3)                                                               \
     HG_DECLARE_BATCH_STAGE(V, 4)                                                               \
-    hipError_t batch_gradient_v##V(int tile, const BatchArgs &b, int block, hipStream_t s);
+    hipError_t batch_gradient_v##V(int tile, const BatchArgs &b, int block, hipStream_t s);               \
+    hipError_t batch_gradient_v##V##_p(int P, int tile, const BatchArgs &b, hipStream_t s);
 HG_DECLARE_BATCH_VARIANT(0)
 HG_DECLARE_BATCH_VARIANT(1)
 HG_DECLARE_BATCH_VARIANT(2)

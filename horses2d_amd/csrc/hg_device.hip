@@ -424,6 +424,38 @@ hipError_t launch_batch_gradient(int variant, int tile, const BatchArgs &b, int 
     }
 }
 
+hipError_t launch_batch_stage_p(int variant, int rs, int P, int tile, const BatchArgs &b, hipStream_t st) {
+#define HG_RS(V)                                                        \
+    switch (rs) {                                                       \
+    case -1: return batch_stage_v##V##_rs0_p(P, tile, b, st);           \
+    case 0: return batch_stage_v##V##_rs1_p(P, tile, b, st);            \
+    case 1: return batch_stage_v##V##_rs2_p(P, tile, b, st);            \
+    case 2: return batch_stage_v##V##_rs3_p(P, tile, b, st);            \
+    case 3: return batch_stage_v##V##_rs4_p(P, tile, b, st);            \
+    default: return hipErrorInvalidValue;                               \
+    }
+    switch (variant) {
+    case 0: HG_RS(0)
+    case 1: HG_RS(1)
+    case 2: HG_RS(2)
+    case 3: HG_RS(3)
+    case 4: HG_RS(4)
+    case 5: HG_RS(5)
+    default: return hipErrorInvalidValue;
+    }
+#undef HG_RS
+}
+
+hipError_t launch_batch_gradient_p(int variant, int P, int tile, const BatchArgs &b, hipStream_t st) {
+    switch (variant) {
+    case 2: return batch_gradient_v2_p(P, tile, b, st);
+    case 3: return batch_gradient_v3_p(P, tile, b, st);
+    case 4: return batch_gradient_v4_p(P, tile, b, st);
+    case 5: return batch_gradient_v5_p(P, tile, b, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_batch_resid(const double *part, const int2 *ranges, double *const *out, int ncases, hipStream_t st) {
     hipLaunchKernelGGL(k_batch_resid, dim3(ncases), dim3(256), 0, st, part, ranges, out);
     return hipGetLastError();

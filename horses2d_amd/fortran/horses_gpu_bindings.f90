@@ -71,7 +71,8 @@ module horses_gpu_bindings
    end type hg_options
 
    ! batched form: tile -1 autotune / 0 narrow / 1 wide; order -1 autotune / 0 case-major /
-   ! 1 costliest tiles first / 2 cases interleaved; use_graph 1 = replay a captured step
+   ! 1 costliest tiles first / 2 cases interleaved / 3 one launch per polynomial order;
+   ! use_graph 1 = replay a captured step
    type, bind(C), public :: hg_batch_options
       integer(c_int32_t) :: tile
       integer(c_int32_t) :: order
