@@ -371,6 +371,8 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     // the run-time-solver form and the forms with those kinds exist for the wide tile only
     if (ph.rs_kernel < 0 || ph.xbc) h->tile[0] = 1;
 
+    HG_CHECK(hipGetDevice(&h->device));
+    if (hg::noalloc_setup(h->device)) return 1;
     HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // ---- host-side tables
     const int E = (int)m.ed.size();

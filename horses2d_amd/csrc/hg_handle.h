@@ -24,6 +24,7 @@ struct hg_handle {
     int K = 0, N = 0, P = 0;
     int64_t nodes = 0, ndof = 0;
     bool mixed = false;
+    int device = -1;   // the HIP device current at hg_create (the handle's stream and buffers)
     hg_options opt{};
     hg_physics phys{};
     hg::PhysParams ph{};
@@ -108,4 +109,6 @@ void *vtr_next(hg_handle *h);
 int ensure_vtr(hg_handle *h);
 // Records the message returned by hg_last_error and returns 1.
 int handle_error(const std::string &m);
+// Locates the allocation-free entry points' scratch array on a device (once per device).
+int noalloc_setup(int device);
 }  // namespace hg

@@ -58,7 +58,8 @@ EXPORTS = [
     "hg_create", "hg_destroy", "hg_ndof", "hg_get_node_coords", "hg_get_connectivity", "hg_num_edges",
     "hg_build_connectivity", "hg_build_geometry",
     "hg_upload_state", "hg_download_state", "hg_download_qdot", "hg_time_derivative", "hg_rk_step",
-    "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_volume_integrals",
+    "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_conservation_sums_noalloc",
+    "hg_volume_integrals",
     "hg_surface_integrals", "hg_probes_set", "hg_probes_update", "hg_synchronize",
     "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_set_graph", "hg_batch_create", "hg_batch_rk_step",
     "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_profile_enable", "hg_batch_profile_read",
@@ -95,6 +96,7 @@ def load_library(path: str = LIB_PATH):
         "hg_estimate_dt": ([vp, dbl, dbl, vp], ctypes.c_int),
         "hg_reset_dt_state": ([vp], ctypes.c_int),
         "hg_conservation_sums": ([vp, vp], ctypes.c_int),
+        "hg_conservation_sums_noalloc": ([vp, vp], ctypes.c_int),
         "hg_volume_integrals": ([vp, vp], ctypes.c_int),
         "hg_surface_integrals": ([vp, i32, vp], ctypes.c_int),
         "hg_probes_set": ([vp, i32, vp, vp, vp, vp], ctypes.c_int),

@@ -204,6 +204,16 @@ int hg_reset_dt_state(hg_handle *h);
  * a monitor for the mixed-precision form). */
 int hg_conservation_sums(hg_handle *h, double out[4]);
 
+/* hg_conservation_sums without allocating or synchronising: the same sums, by the same kernels
+ * in the same order, written to out_dev[4] (device memory) on the handle's stream.  No
+ * counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The per-workgroup partial sums
+ * live in a library-owned device array of 2 MiB per GPU (kept until the process ends), which
+ * limits the mesh to 65536 workgroups of the sum kernel (the 512 x 512 order-7 grid); larger
+ * meshes get an error.  Calls that overlap on one GPU share that array and race.  Read out_dev
+ * after hg_synchronize (or an event on the handle's stream). */
+int hg_conservation_sums_noalloc(hg_handle *h, double *out_dev);
+
 /* Volume monitors of the current state, QuadMesh_Integrals.incf:22-65
  * (Compute_volumeIntegral): out[0] "One" = sum_e w^T J w (the domain area), out[1]
  * "dSnorm" = sqrt(sum_e w^T ((p / rho^gamma - 1)^2 J) w).  FP64 in both forms. */
