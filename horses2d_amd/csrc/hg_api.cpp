@@ -7,6 +7,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -85,6 +86,9 @@ hg::StageArgs base_args(hg_handle *h) {
     a.lap = h->lap ? h->mu_a : nullptr;
     a.earea = h->earea;
     a.ip_epi = h->ip_fused ? 1 : 0;
+    a.mustar = h->mustar;
+    a.mflux = h->mflux;
+    a.mdq = h->mdq;
     return a;
 }
 
@@ -207,24 +211,29 @@ StepCoefs step_coefs(int scheme) {
     return r;
 }
 
+// parts: 1 the gradient pass (with the indicator), 2 the stage kernel (with the residual and
+// the trace-parity flip), 3 both.
 int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode,
-              const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0) {
+              const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0, int parts = 3) {
     if (autotune(h)) return 1;
-    // artificial viscosity of the stage's state, before anything evaluates a viscous flux
-    // (DGSpatial_QDotVolumeLoop computes it per element first, :255)
-    if (h->ph.ad) {
-        hg::StageArgs v = base_args(h);
-        v.tr_in = h->tr[h->cur];
-        HG_CHECK(hg::launch_art_visc(h->P, h->mixed, v, h->evol, h->w, h->mu_a, h->stream));
+    if (parts & 1) {
+        // artificial viscosity of the stage's state, before anything evaluates a viscous flux
+        // (DGSpatial_QDotVolumeLoop computes it per element first, :255)
+        if (h->ph.ad) {
+            hg::StageArgs v = base_args(h);
+            v.tr_in = h->tr[h->cur];
+            HG_CHECK(hg::launch_art_visc(h->P, h->mixed, v, h->evol, h->w, h->mu_a, h->stream));
+        }
+        // BR1: the viscous face-flux traces of the current state, every stage; interior penalty:
+        // only when the previous stage's epilogue did not write them (after an upload)
+        if (h->ph.ns && (!h->ip_fused || !h->vtr_valid)) {
+            hg::StageArgs g = base_args(h);
+            g.tr_in = h->tr[h->cur];
+            HG_CHECK(timed(h, 1, [&] { return hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream); }));
+            if (h->ip_fused) h->vtr_valid = true;
+        }
     }
-    // BR1: the viscous face-flux traces of the current state, every stage; interior penalty:
-    // only when the previous stage's epilogue did not write them (after an upload)
-    if (h->ph.ns && (!h->ip_fused || !h->vtr_valid)) {
-        hg::StageArgs g = base_args(h);
-        g.tr_in = h->tr[h->cur];
-        HG_CHECK(timed(h, 1, [&] { return hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream); }));
-        if (h->ip_fused) h->vtr_valid = true;
-    }
+    if (!(parts & 2)) return 0;
     hg::StageArgs a = base_args(h);
     a.tr_in = h->tr[h->cur];
     a.tr_out = h->tr[h->cur ^ 1];
@@ -277,6 +286,100 @@ int build_graphs(hg_handle *h, const StepCoefs &k) {
     return 0;
 }
 
+// ---- meshes with several polynomial orders: one sub-handle per order, coupled by the mortar
+// kernels (p-non-conforming edges, QuadElement_EdgeProcedures.incf:749-1206)
+
+// The mortar kernels' arguments at the sub-handles' current trace parity.
+hg::MortarArgs mortar_args(hg_handle *h) {
+    hg::MortarArgs a{};
+    a.edges = h->medges;
+    a.nedges = h->nmortar;
+    a.mats = h->mmats;
+    a.ph = h->sub[0]->ph;
+    for (size_t g = 0; g < h->sub.size(); ++g) {
+        hg_handle *s = h->sub[g];
+        a.tr[g] = (const double *)s->tr[s->cur];
+        a.mdq[g] = s->mdq;
+        a.mustar[g] = s->mustar;
+        a.mflux[g] = s->mflux;
+    }
+    return a;
+}
+
+// One RK stage (or the time derivative, mode 1) of every sub-handle: BR1 uStar of the mortar
+// edges, each sub-handle's gradient pass (which also keeps its mortar faces' gradient traces),
+// the mortar edges' fluxes, each sub-handle's stage kernel.
+int composite_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode,
+                    const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0) {
+    if (h->ph.ns) {
+        HG_CHECK(hg::launch_mortar_ustar(mortar_args(h), h->stream));
+        for (auto *s : h->sub)
+            if (run_stage(s, am, cq, cg, first, last, mode, dt_dev, dt_cq, dt_cg, 1)) return 1;
+    }
+    HG_CHECK(hg::launch_mortar_flux(mortar_args(h), h->stream));
+    for (auto *s : h->sub)
+        if (run_stage(s, am, cq, cg, first, last, mode, dt_dev, dt_cq, dt_cg, 2)) return 1;
+    if (mode == 0) h->cur ^= 1;
+    return 0;
+}
+
+int build_composite_graphs(hg_handle *h, const StepCoefs &k) {
+    const int cur0 = h->cur;
+    for (int p = 0; p < 2; ++p) {
+        h->cur = p;
+        for (auto *s : h->sub) s->cur = p;
+        HG_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int bad = 0;
+        for (int st = 0; st < k.nst && !bad; ++st)
+            bad = composite_stage(h, k.am[st], k.cq[st], k.cg[st], st == 0, st == k.nst - 1, 0, h->dt_dev, k.dt_cq,
+                                  k.dt_cg);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        h->cur = cur0;
+        for (auto *s : h->sub) s->cur = cur0;
+        if (bad) {
+            if (g) hipGraphDestroy(g);
+            return 1;
+        }
+        HG_CHECK(e);
+        h->graph[p] = g;
+        HG_CHECK(hipGraphInstantiate(&h->gexec[p], g, nullptr, nullptr, 0));
+    }
+    return 0;
+}
+
+int composite_rk_step(hg_handle *h, double dt) {
+    const StepCoefs k = step_coefs(h->opt.scheme);
+    for (auto *s : h->sub) {
+        s->qdot_valid = false;
+        if (autotune(s)) return 1;
+    }
+    if (h->use_graph && !h->prof) {
+        if (!h->dt_valid || dt != h->dt_last) {
+            h->dt_last = dt;
+            h->dt_valid = true;
+            HG_CHECK(hipMemcpyAsync(h->dt_dev, &h->dt_last, sizeof(double), hipMemcpyHostToDevice, h->stream));
+        }
+        if (!h->gexec[0] && build_composite_graphs(h, k)) {
+            drop_graphs(h);
+            return 1;
+        }
+        HG_CHECK(hipGraphLaunch(h->gexec[h->cur], h->stream));
+        if (k.nst & 1) {
+            h->cur ^= 1;
+            for (auto *s : h->sub) s->cur ^= 1;
+        }
+    } else {
+        for (int st = 0; st < k.nst; ++st) {
+            const double cq = k.dt_cq ? k.cq[st] * dt : k.cq[st];
+            const double cg = k.dt_cg ? k.cg[st] * dt : k.cg[st];
+            if (composite_stage(h, k.am[st], cq, cg, st == 0, st == k.nst - 1, 0)) return 1;
+        }
+    }
+    for (auto *s : h->sub) hg::mark_last_evaluation(s, s->cur ^ 1);
+    return 0;
+}
+
 }  // namespace
 
 namespace hg {
@@ -295,38 +398,21 @@ int ensure_vtr(hg_handle *h) {
     return 0;
 }
 int handle_error(const std::string &m) { return set_err(m); }
+void mark_last_evaluation(hg_handle *h, int tr) {
+    h->mon_tr = tr;
+    h->mon_vtr = h->ph.ns ? h->dtr[h->ip_fused ? tr : 0] : nullptr;
+}
 }  // namespace hg
 
-extern "C" {
-
-int hg_last_error(char *buf, int32_t n) {
-    if (!buf || n <= 0) return 1;
-    std::snprintf(buf, (size_t)n, "%s", g_err.c_str());
-    return 0;
-}
-
-int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options *opt, hg_handle **out) {
-    if (!mesh || !phys || !opt || !out) return set_err("hg_create: null argument");
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err("hg_create: no HIP device available");
-    if (opt->riemann < 0 || opt->riemann > 3) return set_err("hg_create: unknown Riemann solver");
-    if (opt->navier_stokes && (opt->viscous < 1 || opt->viscous > 2))
-        return set_err("hg_create: Navier-Stokes needs viscous = BR1 or IP");
-    if (opt->precision != HG_PRECISION_FP64 && opt->precision != HG_PRECISION_MIXED)
-        return set_err("hg_create: unknown precision");
-    if (opt->scheme < 0 || opt->scheme > 2) return set_err("hg_create: unknown scheme");
-    if (opt->navier_stokes && opt->viscous == HG_VISCOUS_IP &&
-        (opt->ip_method < HG_IP_SIPG || opt->ip_method > HG_IP_IIPG))
-        return set_err("hg_create: unknown interior penalty method (SIPG, NIPG or IIPG)");
-    std::unique_ptr<hg_handle> h(new hg_handle());
-    std::string err = hg::build_mesh(*mesh, h->mesh);
-    if (!err.empty()) return set_err("hg_create: " + err);
-    hg::Mesh &m = h->mesh;
-    h->K = (int)m.el.size();
-    h->N = m.el[0].N;
-    for (auto &e : m.el)
-        if (e.N != h->N) return set_err("hg_create: the GPU path needs one polynomial order per mesh");
+namespace {
+// Physics parameters, tables and device buffers of a handle over the elements `els` (global
+// ids, ascending, all of one order) of the mesh m: loc maps a global element to its index in
+// this handle (-1: another handle), mort a global edge to its index among the p-non-conforming
+// edges (-1: conforming).  A handle whose stream is already set shares it.
+int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const hg_options *opt,
+                 const std::vector<int> &els, const std::vector<int> &loc, const std::vector<int> &mort) {
+    h->K = (int)els.size();
+    h->N = m.el[els[0]].N;
     if (!hg::supported_order(h->N)) return set_err("hg_create: polynomial order not compiled (1..10)");
     h->P = h->N + 1;
     const int P = h->P, PP = P * P, K = h->K;
@@ -373,7 +459,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
 
     HG_CHECK(hipGetDevice(&h->device));
     if (hg::noalloc_setup(h->device)) return 1;
-    HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (!h->stream) HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     // ---- host-side tables
     const int E = (int)m.ed.size();
     std::vector<int> geo_off(E);
@@ -398,8 +484,9 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
         if (Z.type == HG_BC_VISCOUSWALL) { zone[8 * z] = Z.Tw; zone[8 * z + 1] = Z.vw[0]; zone[8 * z + 2] = Z.vw[1]; }
         else for (int c = 0; c < 4; ++c) zone[8 * z + c] = Z.q[c];   // the zone state
     }
+    bool has_mortar = false;
     for (int e = 0; e < K; ++e) {
-        const hg::Elem &el = m.el[e];
+        const hg::Elem &el = m.el[els[e]];
         int ord[4] = {0, 1, 2, 3};
         std::sort(ord, ord + 4, [&](int a, int b) { return el.edges[a] < el.edges[b]; });
         forder[e] = ord[0] | (ord[1] << 2) | (ord[2] << 4) | (ord[3] << 6);
@@ -408,9 +495,13 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
             const hg::Edge &ed = m.ed[k];
             const int side = el.qpos[f];
             int flags = 0, y = 0;
-            if (!ed.bdry) {
+            if (!ed.bdry && mort[k] >= 0) {
+                flags = hg::FK_MORTAR;   // evaluated by the mortar kernels
+                y = mort[k];
+                has_mortar = true;
+            } else if (!ed.bdry) {
                 flags = side == 0 ? hg::FK_INT_L : hg::FK_INT_R;
-                int other = ed.quads[1 - side];
+                int other = loc[ed.quads[1 - side]];
                 y = other * 4 + (ed.loc[1 - side] - 1);
                 if (ed.inverse) flags |= hg::FF_REV;
                 if (side == 1 && ed.inverse) flags |= hg::FF_SELFREV;
@@ -419,7 +510,9 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
                 if (Z.type == HG_BC_PERIODIC) {
                     const hg::Edge &pe = m.ed[ed.partner];
                     flags = hg::FK_PERIODIC;
-                    y = pe.quads[0] * 4 + (pe.loc[0] - 1);
+                    if (loc[pe.quads[0]] < 0)
+                        return set_err("hg_create: periodic edges between elements of different orders");
+                    y = loc[pe.quads[0]] * 4 + (pe.loc[0] - 1);
                     if (ed.inverse) flags |= hg::FF_REV;
                 } else {
                     switch (Z.type) {
@@ -453,7 +546,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     const hg::Spectral &sp = hg::spectral(h->N);
     if (ph.ip_gradflux) fmet.assign((size_t)K * 4 * P * 5, 0.0);
     for (int e = 0; e < K; ++e) {
-        const hg::Elem &el = m.el[e];
+        const hg::Elem &el = m.el[els[e]];
         for (int n = 0; n < PP; ++n) {
             size_t g = (size_t)e * PP + n;
             for (int c = 0; c < 4; ++c) metric[c * h->nodes + g] = el.Ja[c * PP + n];
@@ -536,25 +629,25 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
             h->metric_elem = 1;
         }
     }
-    HG_CHECK(upload_real(h.get(), &h->metric, h->metric_elem ? metric_e : metric, h->mixed));
-    if (!fmet.empty()) HG_CHECK(upload_real(h.get(), &h->fmet, fmet, h->mixed));
-    HG_CHECK(upload(h.get(), &h->face, face));
-    HG_CHECK(upload(h.get(), &h->forder, forder));
-    HG_CHECK(upload(h.get(), &h->egeo, egeo));
-    HG_CHECK(upload(h.get(), &h->einvh, einvh));
+    HG_CHECK(upload_real(h, &h->metric, h->metric_elem ? metric_e : metric, h->mixed));
+    if (!fmet.empty()) HG_CHECK(upload_real(h, &h->fmet, fmet, h->mixed));
+    HG_CHECK(upload(h, &h->face, face));
+    HG_CHECK(upload(h, &h->forder, forder));
+    HG_CHECK(upload(h, &h->egeo, egeo));
+    HG_CHECK(upload(h, &h->einvh, einvh));
     {
         std::vector<double> earea(E);
         for (int k = 0; k < E; ++k) earea[k] = m.ed[k].area;
-        HG_CHECK(upload(h.get(), &h->earea, earea));
+        HG_CHECK(upload(h, &h->earea, earea));
     }
-    HG_CHECK(upload(h.get(), &h->zone, zone));
-    HG_CHECK(upload(h.get(), &h->hatD, hat));
-    HG_CHECK(upload(h.get(), &h->D, Dm));
-    HG_CHECK(upload(h.get(), &h->lb, lb));
-    HG_CHECK(upload(h.get(), &h->lbw, lbw));
-    HG_CHECK(upload(h.get(), &h->dlbw, dlbw));
-    HG_CHECK(upload(h.get(), &h->w, wv));
-    HG_CHECK(upload(h.get(), &h->jac64, jac));
+    HG_CHECK(upload(h, &h->zone, zone));
+    HG_CHECK(upload(h, &h->hatD, hat));
+    HG_CHECK(upload(h, &h->D, Dm));
+    HG_CHECK(upload(h, &h->lb, lb));
+    HG_CHECK(upload(h, &h->lbw, lbw));
+    HG_CHECK(upload(h, &h->dlbw, dlbw));
+    HG_CHECK(upload(h, &h->w, wv));
+    HG_CHECK(upload(h, &h->jac64, jac));
     if (ph.ad) {
         // element volumes (BilinearForm_F(jac, w, w), QuadMesh_Construct.incf:266)
         std::vector<double> vol(K);
@@ -567,11 +660,11 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
             }
             vol[e] = v;
         }
-        HG_CHECK(upload(h.get(), &h->evol, vol));
+        HG_CHECK(upload(h, &h->evol, vol));
         HG_CHECK(h->alloc(&h->mu_a, K));
         HG_CHECK(hipMemset(h->mu_a, 0, K * sizeof(double)));
     }
-    HG_CHECK(upload(h.get(), &h->dx, dxe));
+    HG_CHECK(upload(h, &h->dx, dxe));
     HG_CHECK(h->alloc(&h->umax, K));
     HG_CHECK(h->alloc(&h->amax, K));
     HG_CHECK(h->alloc(&h->dtstate, 2));
@@ -585,8 +678,8 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
         std::vector<std::vector<int>> per(m.zones.size());
         for (size_t k = 0; k < m.ed.size(); ++k) {
             const hg::Edge &ed = m.ed[k];
-            if (ed.bdry && ed.marker >= 1 && ed.marker <= (int)m.zones.size())
-                per[ed.marker - 1].push_back(ed.quads[0] * 4 + (ed.loc[0] - 1));
+            if (ed.bdry && ed.marker >= 1 && ed.marker <= (int)m.zones.size() && loc[ed.quads[0]] >= 0)
+                per[ed.marker - 1].push_back(loc[ed.quads[0]] * 4 + (ed.loc[0] - 1));
         }
         std::vector<int> flat;
         h->bslot_off.assign(1, 0);
@@ -594,7 +687,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
             flat.insert(flat.end(), v.begin(), v.end());
             h->bslot_off.push_back((int)flat.size());
         }
-        HG_CHECK(upload(h.get(), &h->bslots, flat));
+        HG_CHECK(upload(h, &h->bslots, flat));
         HG_CHECK(h->alloc(&h->monitor_out, 12));
     }
     HG_CHECK(h->alloc(&h->dt_dev, 1));
@@ -604,7 +697,170 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     HG_CHECK(hipMemset(h->Q, 0, h->ndof * sizeof(double)));
     HG_CHECK(hipMemset(h->G, 0, h->ndof * sizeof(double)));
     HG_CHECK(hipMemset(h->QDot, 0, h->ndof * sizeof(double)));
+    if (has_mortar) {
+        // per face node: the projected flux (and for BR1 uStar and this side's gradient trace)
+        const size_t fn = (size_t)K * 4 * P;
+        HG_CHECK(h->alloc(&h->mflux, 4 * fn));
+        HG_CHECK(hipMemset(h->mflux, 0, 4 * fn * sizeof(double)));
+        if (ph.ns) {
+            HG_CHECK(h->alloc(&h->mustar, 4 * fn));
+            HG_CHECK(h->alloc(&h->mdq, 8 * fn));
+            HG_CHECK(hipMemset(h->mustar, 0, 4 * fn * sizeof(double)));
+            HG_CHECK(hipMemset(h->mdq, 0, 8 * fn * sizeof(double)));
+        }
+    }
     HG_CHECK(hipDeviceSynchronize());
+    return 0;
+}
+
+
+// A mesh with several polynomial orders (p-refinement zones, MeshFile.f90:143-180): one
+// sub-handle per order over its elements, the p-non-conforming edges between them (edge order
+// the higher of its two elements', QuadMesh_Construct.incf:104-118) in the mortar tables, and
+// the whole mesh's time-step scan.  The FP64 Euler and BR1 forms without artificial
+// dissipation, straight p-non-conforming edges.
+int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt, const std::vector<int> &orders) {
+    const hg::Mesh &m = h->mesh;
+    if (opt->precision != HG_PRECISION_FP64)
+        return set_err("hg_create: a mesh with several polynomial orders runs in the FP64 form only");
+    if (opt->navier_stokes && opt->viscous != HG_VISCOUS_BR1)
+        return set_err("hg_create: a mesh with several polynomial orders needs the Euler or the BR1 form");
+    if (opt->navier_stokes && opt->artificial_dissipation != HG_AD_NONE)
+        return set_err("hg_create: a mesh with several polynomial orders runs without artificial dissipation");
+    if ((int)orders.size() > hg::kMaxSub) return set_err("hg_create: too many polynomial orders in one mesh");
+    const int K = (int)m.el.size(), E = (int)m.ed.size();
+    std::vector<int> group(K), loc(K);
+    h->sub_els.assign(orders.size(), {});
+    for (int e = 0; e < K; ++e) {
+        group[e] = (int)(std::lower_bound(orders.begin(), orders.end(), m.el[e].N) - orders.begin());
+        loc[e] = (int)h->sub_els[group[e]].size();
+        h->sub_els[group[e]].push_back(e);
+    }
+    std::vector<int> mort(E, -1);
+    std::vector<hg::MortarEdge> medges;
+    std::vector<double> mats;
+    for (int k = 0; k < E; ++k) {
+        const hg::Edge &ed = m.ed[k];
+        if (ed.bdry || m.el[ed.quads[0]].N == m.el[ed.quads[1]].N) continue;
+        if (ed.curved) return set_err("hg_create: curved edges between elements of different orders are not supported");
+        hg::MortarEdge r{};
+        for (int side = 0; side < 2; ++side) {
+            const int q = ed.quads[side];
+            r.sub[side] = group[q];
+            r.slot[side] = loc[q] * 4 + (ed.loc[side] - 1);
+            r.Ps[side] = m.el[q].P;
+        }
+        r.P = ed.N + 1;
+        r.low = r.Ps[0] < r.Ps[1] ? 0 : 1;
+        r.inverse = ed.inverse;
+        r.nx = ed.n[0];
+        r.ny = ed.n[1];
+        r.dS = ed.dS[0];
+        r.mat = (int)mats.size();
+        const int Pl = r.Ps[r.low];
+        mats.resize(mats.size() + 2 * (size_t)r.P * Pl);
+        hg::mortar_matrices(Pl - 1, r.P - 1, mats.data() + r.mat, mats.data() + r.mat + (size_t)r.P * Pl);
+        mort[k] = (int)medges.size();
+        medges.push_back(r);
+    }
+    HG_CHECK(hipGetDevice(&h->device));
+    if (hg::noalloc_setup(h->device)) return 1;
+    HG_CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    for (size_t g = 0; g < orders.size(); ++g) {
+        hg_handle *s = new hg_handle();
+        s->stream = h->stream;
+        s->shared_stream = true;
+        h->sub.push_back(s);
+        std::vector<int> lg(K, -1);
+        for (int e : h->sub_els[g]) lg[e] = loc[e];
+        if (setup_handle(s, m, phys, opt, h->sub_els[g], lg, mort)) return 1;
+    }
+    h->opt = *opt;
+    h->phys = *phys;
+    h->ph = h->sub[0]->ph;
+    h->mixed = false;
+    h->K = K;
+    h->N = orders.back();
+    h->P = h->N + 1;
+    h->nodes = m.nodes_total;
+    h->ndof = 4 * h->nodes;
+    h->el_off.resize(K);
+    for (int e = 0; e < K; ++e) h->el_off[e] = m.el[e].off;
+    h->nmortar = (int)medges.size();
+    HG_CHECK(upload(h, &h->medges, medges));
+    HG_CHECK(upload(h, &h->mmats, mats));
+    for (size_t g = 0; g < orders.size(); ++g) HG_CHECK(upload(h, &h->sub_gmap[g], h->sub_els[g]));
+    // the time-step scan over the whole mesh in element order: N + 1 and the shortest side per
+    // element (as each sub-handle has them)
+    std::vector<double> n1(K), dx(K);
+    for (int e = 0; e < K; ++e) {
+        const hg::Elem &el = m.el[e];
+        n1[e] = (double)(el.N + 1);
+        double best = 1e300;
+        for (int c = 0; c < 4; ++c) {
+            int a = el.nodes[c], b = el.nodes[(c + 1) % 4];
+            double ddx = m.pts[2 * a] - m.pts[2 * b], ddy = m.pts[2 * a + 1] - m.pts[2 * b + 1];
+            best = std::fmin(best, std::sqrt(ddx * ddx + ddy * ddy));
+        }
+        dx[e] = best;
+    }
+    HG_CHECK(upload(h, &h->n1e, n1));
+    HG_CHECK(upload(h, &h->dx, dx));
+    HG_CHECK(h->alloc(&h->umax, K));
+    HG_CHECK(h->alloc(&h->amax, K));
+    HG_CHECK(h->alloc(&h->dtstate, 2));
+    HG_CHECK(h->alloc(&h->dtout, 1));
+    HG_CHECK(h->alloc(&h->dt_dev, 1));
+    HG_CHECK(hipMemset(h->dtstate, 0, 2 * sizeof(double)));
+    HG_CHECK(hipDeviceSynchronize());
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hg_last_error(char *buf, int32_t n) {
+    if (!buf || n <= 0) return 1;
+    std::snprintf(buf, (size_t)n, "%s", g_err.c_str());
+    return 0;
+}
+
+int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options *opt, hg_handle **out) {
+    if (!mesh || !phys || !opt || !out) return set_err("hg_create: null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_err("hg_create: no HIP device available");
+    if (opt->riemann < 0 || opt->riemann > 3) return set_err("hg_create: unknown Riemann solver");
+    if (opt->navier_stokes && (opt->viscous < 1 || opt->viscous > 2))
+        return set_err("hg_create: Navier-Stokes needs viscous = BR1 or IP");
+    if (opt->precision != HG_PRECISION_FP64 && opt->precision != HG_PRECISION_MIXED)
+        return set_err("hg_create: unknown precision");
+    if (opt->scheme < 0 || opt->scheme > 2) return set_err("hg_create: unknown scheme");
+    if (opt->navier_stokes && opt->viscous == HG_VISCOUS_IP &&
+        (opt->ip_method < HG_IP_SIPG || opt->ip_method > HG_IP_IIPG))
+        return set_err("hg_create: unknown interior penalty method (SIPG, NIPG or IIPG)");
+    std::unique_ptr<hg_handle> h(new hg_handle());
+    std::string err = hg::build_mesh(*mesh, h->mesh);
+    if (!err.empty()) return set_err("hg_create: " + err);
+    const hg::Mesh &m = h->mesh;
+    std::vector<int> orders;
+    for (const auto &e : m.el) orders.push_back(e.N);
+    std::sort(orders.begin(), orders.end());
+    orders.erase(std::unique(orders.begin(), orders.end()), orders.end());
+    for (int N : orders)
+        if (!hg::supported_order(N)) return set_err("hg_create: polynomial order not compiled (1..10)");
+    if (orders.size() > 1) {
+        if (create_composite(h.get(), phys, opt, orders)) {
+            const std::string msg = g_err;
+            hg_destroy(h.release());   // the sub-handles made so far
+            return set_err(msg);
+        }
+    } else {
+        std::vector<int> all(m.el.size()), mort(m.ed.size(), -1);
+        std::iota(all.begin(), all.end(), 0);
+        if (setup_handle(h.get(), m, phys, opt, all, all, mort)) return 1;
+    }
     *out = h.release();
     return 0;
 }
@@ -612,13 +868,18 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
 int hg_destroy(hg_handle *h) {
     if (!h) return 0;
     if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto *s : h->sub) {
+        s->stream = nullptr;   // shared with this handle
+        hg_destroy(s);
+    }
+    h->sub.clear();
     drop_graphs(h);
     for (auto &b : h->bufs) hipFree(b.p);
     for (void *p : {(void *)h->probe_ev, (void *)h->probe_l, (void *)h->probe_out})
         if (p) hipFree(p);
     for (auto &e : h->ev_pool) hipEventDestroy(e);
     for (auto &p : h->ev_pending) { hipEventDestroy(p.second.first); hipEventDestroy(p.second.second); }
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->stream && !h->shared_stream) hipStreamDestroy(h->stream);
     delete h;
     return 0;
 }
@@ -703,6 +964,19 @@ int hg_build_geometry(const hg_mesh_desc *mesh, int64_t nnodes, double *X, doubl
 int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     if (!h || !Q) return set_err("hg_upload_state: null argument");
     if (ndof != h->ndof) return set_err("hg_upload_state: ndof mismatch");
+    if (!h->sub.empty()) {
+        // each order's elements, from the reference layout, to its sub-handle
+        for (size_t g = 0; g < h->sub.size(); ++g) {
+            hg_handle *s = h->sub[g];
+            const size_t n = 4 * (size_t)s->P * s->P;
+            std::vector<double> q(s->ndof);
+            for (size_t e = 0; e < h->sub_els[g].size(); ++e)
+                std::memcpy(q.data() + e * n, Q + 4 * h->el_off[h->sub_els[g][e]], n * sizeof(double));
+            if (hg_upload_state(s, q.data(), s->ndof)) return 1;
+        }
+        h->cur = 0;
+        return 0;
+    }
     const double pref_old = h->ph.pref;
     if (h->mixed) {
         // Reference pressure of the FP32 form: the mean nodal pressure of the uploaded state.
@@ -725,13 +999,28 @@ int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     HG_CHECK(hg::launch_prolong(h->P, h->mixed, a, h->stream));
     h->vtr_valid = false;
     h->qdot_valid = false;
+    h->mon_tr = -1;
     HG_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// The per-order arrays of a mesh with several orders into the reference layout.
+static int composite_gather(hg_handle *h, bool qdot, double *out) {
+    for (size_t g = 0; g < h->sub.size(); ++g) {
+        hg_handle *s = h->sub[g];
+        const size_t n = 4 * (size_t)s->P * s->P;
+        std::vector<double> q(s->ndof);
+        if ((qdot ? hg_download_qdot(s, q.data(), s->ndof) : hg_download_state(s, q.data(), s->ndof))) return 1;
+        for (size_t e = 0; e < h->sub_els[g].size(); ++e)
+            std::memcpy(out + 4 * h->el_off[h->sub_els[g][e]], q.data() + e * n, n * sizeof(double));
+    }
     return 0;
 }
 
 int hg_download_state(hg_handle *h, double *Q, int64_t ndof) {
     if (!h || !Q) return set_err("hg_download_state: null argument");
     if (ndof != h->ndof) return set_err("hg_download_state: ndof mismatch");
+    if (!h->sub.empty()) return composite_gather(h, false, Q);
     HG_CHECK(hipMemcpyAsync(Q, h->Q, ndof * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
@@ -740,6 +1029,7 @@ int hg_download_state(hg_handle *h, double *Q, int64_t ndof) {
 int hg_download_qdot(hg_handle *h, double *QDot, int64_t ndof) {
     if (!h || !QDot) return set_err("hg_download_qdot: null argument");
     if (ndof != h->ndof) return set_err("hg_download_qdot: ndof mismatch");
+    if (!h->sub.empty()) return composite_gather(h, true, QDot);
     HG_CHECK(hipMemcpyAsync(QDot, h->QDot, ndof * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
@@ -748,14 +1038,24 @@ int hg_download_qdot(hg_handle *h, double *QDot, int64_t ndof) {
 int hg_time_derivative(hg_handle *h, double t) {
     (void)t;  // no time-dependent boundary data on the GPU path
     if (!h) return set_err("hg_time_derivative: null handle");
+    if (!h->sub.empty()) {
+        if (composite_stage(h, 0.0, 1.0, 0.0, true, true, 1)) return 1;
+        for (auto *s : h->sub) {
+            s->qdot_valid = true;
+            hg::mark_last_evaluation(s, s->cur);
+        }
+        return 0;
+    }
     if (run_stage(h, 0.0, 1.0, 0.0, true, true, 1)) return 1;
     h->qdot_valid = true;
+    hg::mark_last_evaluation(h, h->cur);
     return 0;
 }
 
 int hg_rk_step(hg_handle *h, double t, double dt) {
     (void)t;  // no time-dependent boundary data on the GPU path
     if (!h) return set_err("hg_rk_step: null handle");
+    if (!h->sub.empty()) return composite_rk_step(h, dt);
     h->qdot_valid = false;
     const StepCoefs k = step_coefs(h->opt.scheme);
     if (h->use_graph && !h->prof) {
@@ -773,6 +1073,7 @@ int hg_rk_step(hg_handle *h, double t, double dt) {
         }
         HG_CHECK(hipGraphLaunch(h->gexec[h->cur], h->stream));
         if (k.nst & 1) h->cur ^= 1;
+        hg::mark_last_evaluation(h, h->cur ^ 1);   // the last stage read the other buffer
         return 0;
     }
     for (int s = 0; s < k.nst; ++s) {
@@ -780,6 +1081,7 @@ int hg_rk_step(hg_handle *h, double t, double dt) {
         const double cg = k.dt_cg ? k.cg[s] * dt : k.cg[s];
         if (run_stage(h, k.am[s], cq, cg, s == 0, s == k.nst - 1, 0)) return 1;
     }
+    hg::mark_last_evaluation(h, h->cur ^ 1);
     return 0;
 }
 
@@ -791,6 +1093,17 @@ int hg_set_graph(hg_handle *h, int32_t on) {
 
 int hg_residuals(hg_handle *h, double res[4]) {
     if (!h || !res) return set_err("hg_residuals: null argument");
+    if (!h->sub.empty()) {
+        // max over the orders (a NaN stays)
+        for (int c = 0; c < 4; ++c) res[c] = 0.0;
+        for (auto *s : h->sub) {
+            double r[4];
+            if (hg_residuals(s, r)) return 1;
+            for (int c = 0; c < 4; ++c)
+                if (r[c] > res[c] || std::isnan(r[c])) res[c] = std::isnan(res[c]) ? res[c] : r[c];
+        }
+        return 0;
+    }
     HG_CHECK(hipMemcpyAsync(res, h->resid, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
@@ -798,10 +1111,22 @@ int hg_residuals(hg_handle *h, double res[4]) {
 
 int hg_estimate_dt(hg_handle *h, double dt_max, double cfl, double *dt) {
     if (!h || !dt) return set_err("hg_estimate_dt: null argument");
-    HG_CHECK(hg::launch_dt_elem(h->P, h->Q, h->K, h->ph.gamma, h->ph.gm1, h->umax, h->amax, h->stream));
     const double coef = h->phys.mach * std::sqrt(h->ph.gamma) * cfl;
-    HG_CHECK(hg::launch_dt_scan(h->K, h->umax, h->amax, h->dx, coef, (double)(h->N + 1), dt_max, h->dtstate, h->dtout,
-                                h->stream));
+    if (!h->sub.empty()) {
+        // per-element maxima of each order, into mesh order, then one scan over all elements
+        for (size_t g = 0; g < h->sub.size(); ++g) {
+            hg_handle *s = h->sub[g];
+            HG_CHECK(hg::launch_dt_elem(s->P, s->Q, s->K, s->ph.gamma, s->ph.gm1, s->umax, s->amax, h->stream));
+            HG_CHECK(hg::launch_scatter(s->umax, h->sub_gmap[g], s->K, h->umax, h->stream));
+            HG_CHECK(hg::launch_scatter(s->amax, h->sub_gmap[g], s->K, h->amax, h->stream));
+        }
+        HG_CHECK(hg::launch_dt_scan(h->K, h->umax, h->amax, h->dx, coef, 0.0, dt_max, h->dtstate, h->dtout, h->stream,
+                                    h->n1e));
+    } else {
+        HG_CHECK(hg::launch_dt_elem(h->P, h->Q, h->K, h->ph.gamma, h->ph.gm1, h->umax, h->amax, h->stream));
+        HG_CHECK(hg::launch_dt_scan(h->K, h->umax, h->amax, h->dx, coef, (double)(h->N + 1), dt_max, h->dtstate,
+                                    h->dtout, h->stream));
+    }
     HG_CHECK(hipMemcpyAsync(dt, h->dtout, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
     return 0;
@@ -816,6 +1141,15 @@ int hg_reset_dt_state(hg_handle *h) {
 
 int hg_conservation_sums(hg_handle *h, double out[4]) {
     if (!h || !out) return set_err("hg_conservation_sums: null argument");
+    if (!h->sub.empty()) {
+        for (int c = 0; c < 4; ++c) out[c] = 0.0;
+        for (auto *s : h->sub) {
+            double v[4];
+            if (hg_conservation_sums(s, v)) return 1;
+            for (int c = 0; c < 4; ++c) out[c] += v[c];
+        }
+        return 0;
+    }
     HG_CHECK(hg::launch_cons_sums(h->P, h->Q, h->jac64, h->w, h->K, h->cons_partial, h->cons_nb, h->cons_out, h->stream));
     HG_CHECK(hipMemcpyAsync(out, h->cons_out, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HG_CHECK(hipStreamSynchronize(h->stream));
@@ -824,6 +1158,18 @@ int hg_conservation_sums(hg_handle *h, double out[4]) {
 
 int hg_volume_integrals(hg_handle *h, double out[2]) {
     if (!h || !out) return set_err("hg_volume_integrals: null argument");
+    if (!h->sub.empty()) {
+        double one = 0.0, sq = 0.0;
+        for (auto *s : h->sub) {
+            double v[2];
+            if (hg_volume_integrals(s, v)) return 1;
+            one += v[0];
+            sq += v[1] * v[1];
+        }
+        out[0] = one;
+        out[1] = std::sqrt(sq);
+        return 0;
+    }
     HG_CHECK(hg::launch_volume_integrals(h->P, h->Q, h->jac64, h->w, h->K, h->ph.gamma, h->ph.gm1, h->cons_partial,
                                          h->cons_nb, h->monitor_out, h->stream));
     double v[4];
@@ -836,13 +1182,25 @@ int hg_volume_integrals(hg_handle *h, double out[2]) {
 
 int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
     if (!h || !out) return set_err("hg_surface_integrals: null argument");
+    if (!h->sub.empty()) {
+        for (int c = 0; c < 9; ++c) out[c] = 0.0;
+        for (auto *s : h->sub) {
+            double v[9];
+            if (hg_surface_integrals(s, marker, v)) return 1;
+            for (int c = 0; c < 9; ++c) out[c] += v[c];
+        }
+        return 0;
+    }
     const int nz = (int)h->bslot_off.size() - 1;
     if (marker < 1 || marker > nz) return set_err("hg_surface_integrals: marker out of range");
     const int first = h->bslot_off[marker - 1], n = h->bslot_off[marker] - first;
-    const void *vtr = nullptr;
-    if (h->ph.ns) {
-        // the viscous face-flux traces of the current state (BR1: those of the last gradient
-        // launch belong to the last stage's input, so the gradient kernel runs once more)
+    const void *vtr = nullptr, *tr = h->tr[h->cur];
+    if (h->mon_tr >= 0) {
+        // the edge storage of the last derivative evaluation
+        tr = h->tr[h->mon_tr];
+        vtr = h->mon_vtr;
+    } else if (h->ph.ns) {
+        // no evaluation since the upload: the viscous face-flux traces of the uploaded state
         if (autotune(h)) return 1;
         if (h->ip_fused) {
             if (hg::ensure_vtr(h)) return 1;
@@ -854,9 +1212,12 @@ int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]) {
             HG_CHECK(hg::launch_gradient(h->P, h->tile[1], h->mixed, g, h->stream));
         }
         vtr = hg::vtr_current(h);
+        hg::mark_last_evaluation(h, h->cur);
+    } else {
+        hg::mark_last_evaluation(h, h->cur);
     }
     if (n > 0)
-        HG_CHECK(hg::launch_surface_integrals(h->mixed, h->bslots + first, n, h->P, h->tr[h->cur], vtr, h->face,
+        HG_CHECK(hg::launch_surface_integrals(h->mixed, h->bslots + first, n, h->P, tr, vtr, h->face,
                                               h->egeo, h->w, h->ph.gm1, h->mixed ? h->ph.Eref : 0.0,
                                               (h->ph.ad && !h->lap) ? h->mu_a : nullptr, h->ph.mu, h->monitor_out,
                                               h->stream));
@@ -872,6 +1233,30 @@ int hg_probes_set(hg_handle *h, int32_t n, const int32_t *elem, const double *xi
     if (!h) return set_err("hg_probes_set: null handle");
     if (n < 0) return set_err("hg_probes_set: negative probe count");
     if (n > 0 && (!elem || !xi || !eta || !var)) return set_err("hg_probes_set: null argument");
+    if (!h->sub.empty()) {
+        // each probe to the sub-handle of its element's order, with that handle's element index
+        std::vector<int> group(h->K, -1), loc(h->K, -1);
+        for (size_t g = 0; g < h->sub.size(); ++g)
+            for (size_t e = 0; e < h->sub_els[g].size(); ++e) {
+                group[h->sub_els[g][e]] = (int)g;
+                loc[h->sub_els[g][e]] = (int)e;
+            }
+        for (int k = 0; k < n; ++k)
+            if (elem[k] < 0 || elem[k] >= h->K) return set_err("hg_probes_set: element id out of range");
+        h->sub_probes.assign(h->sub.size(), {});
+        for (int k = 0; k < n; ++k) h->sub_probes[group[elem[k]]].push_back(k);
+        for (size_t g = 0; g < h->sub.size(); ++g) {
+            const auto &ix = h->sub_probes[g];
+            std::vector<int32_t> el(ix.size()), vr(ix.size());
+            std::vector<double> x(ix.size()), y(ix.size());
+            for (size_t j = 0; j < ix.size(); ++j) {
+                el[j] = loc[elem[ix[j]]]; vr[j] = var[ix[j]]; x[j] = xi[ix[j]]; y[j] = eta[ix[j]];
+            }
+            if (hg_probes_set(h->sub[g], (int32_t)ix.size(), el.data(), x.data(), y.data(), vr.data())) return 1;
+        }
+        h->nprobes = n;
+        return 0;
+    }
     const int P = h->P;
     std::vector<int2> ev(n);
     std::vector<double> l((size_t)n * 2 * P);
@@ -912,6 +1297,16 @@ int hg_probes_update(hg_handle *h, double *out) {
     if (!h) return set_err("hg_probes_update: null handle");
     if (h->nprobes == 0) return 0;
     if (!out) return set_err("hg_probes_update: null argument");
+    if (!h->sub.empty()) {
+        for (size_t g = 0; g < h->sub.size(); ++g) {
+            const auto &ix = h->sub_probes[g];
+            if (ix.empty()) continue;
+            std::vector<double> v(ix.size());
+            if (hg_probes_update(h->sub[g], v.data())) return 1;
+            for (size_t j = 0; j < ix.size(); ++j) out[ix[j]] = v[j];
+        }
+        return 0;
+    }
     for (int v : h->probe_var)
         if (v >= HG_PROBE_RHOT && v <= HG_PROBE_RHOET && !h->qdot_valid)
             return set_err("hg_probes_update: rhot/rhout/rhovt/rhoet probes need hg_time_derivative after the "
@@ -938,11 +1333,14 @@ int hg_set_tiles(hg_handle *h, int32_t stage_tile, int32_t gradient_tile) {
     h->tile[0] = stage_tile;
     h->tile[1] = gradient_tile;
     drop_graphs(h);
+    for (auto *s : h->sub)
+        if (hg_set_tiles(s, stage_tile, gradient_tile)) return 1;
     return 0;
 }
 
 int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile) {
     if (!h) return set_err("hg_get_tiles: null handle");
+    if (!h->sub.empty()) return hg_get_tiles(h->sub.back(), stage_tile, gradient_tile);
     if (stage_tile) *stage_tile = h->tile[0];
     if (gradient_tile) *gradient_tile = h->tile[1];
     return 0;
@@ -950,6 +1348,8 @@ int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile) {
 
 int hg_profile_enable(hg_handle *h, int32_t on) {
     if (!h) return set_err("hg_profile_enable: null handle");
+    for (auto *s : h->sub)
+        if (hg_profile_enable(s, on)) return 1;
     h->prof = on != 0;
     h->prof_n[0] = h->prof_n[1] = 0;
     h->prof_ms[0] = h->prof_ms[1] = 0.0;
@@ -958,6 +1358,21 @@ int hg_profile_enable(hg_handle *h, int32_t on) {
 
 int hg_profile_read(hg_handle *h, int32_t which, int64_t *launches, double *ms) {
     if (!h || which < 0 || which > 1) return set_err("hg_profile_read: bad argument");
+    if (!h->sub.empty()) {
+        // the launches of every order together
+        int64_t n = 0;
+        double t = 0.0;
+        for (auto *s : h->sub) {
+            int64_t ns = 0;
+            double ts = 0.0;
+            if (hg_profile_read(s, which, &ns, &ts)) return 1;
+            n += ns;
+            t += ts;
+        }
+        if (launches) *launches = n;
+        if (ms) *ms = t;
+        return 0;
+    }
     HG_CHECK(hipStreamSynchronize(h->stream));
     for (auto &p : h->ev_pending) {
         float t = 0.0f;

@@ -419,6 +419,7 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     for (int c = 0; c < ncases; ++c) {
         hg_handle *h = cases[c];
         if (!h) return hg::handle_error("hg_batch_create: null case handle");
+        if (!h->sub.empty()) return hg::handle_error("hg_batch_create: a case with several polynomial orders");
         // artificial dissipation all or none: it decides whether the gradient pass runs every stage
         if (h->ph.ns != h0->ph.ns || h->ph.viscous != h0->ph.viscous || h->mixed != h0->mixed ||
             h->opt.scheme != h0->opt.scheme || h->ph.ad != h0->ph.ad || h->lap != h0->lap)
@@ -558,6 +559,7 @@ int hg_batch_rk_step(hg_batch *b, const double *t, const double *dt) {
     for (auto *h : b->cases) {
         h->cur ^= 1;   // every scheme has an odd number of stages
         h->qdot_valid = false;
+        hg::mark_last_evaluation(h, h->cur ^ 1);
     }
     return 0;
 }

@@ -18,6 +18,8 @@ enum FaceKind : int {
     FK_POUTLET = 7,   // pressure outlet (PressureOutlet.incf:151-450)
     FK_PINLET = 8,    // pressure inlet, prescribed flux (PressureInlet.incf:134-410)
     FK_NEWDIRICHLET = 9, // characteristic Dirichlet state (newDirichlet.incf:146-433)
+    FK_MORTAR = 10,   // p-non-conforming interior edge: the flux (and BR1 uStar) of this side
+                      // come projected from the mortar kernels (Edge_ProjectFluxesType3-6)
 };
 constexpr int FF_CURVED = 1 << 4;    // per-node normals and dS
 constexpr int FF_REV = 1 << 5;       // neighbor trace index runs backwards
@@ -77,7 +79,43 @@ struct StageArgs {
     const double *lap;          // Laplacian dissipation: the indicator's viscosity per element (null: none)
     const double *earea;        // per edge: its Area (the Laplacian face flux)
     int ip_epi;                 // interior penalty: the stage epilogue forms the next viscous traces
+    // p-non-conforming faces (FK_MORTAR), per element face node [(e*4+f)*P + k] (FP64):
+    const double *mustar;       // BR1 uStar of this side, projected and signed [4]
+    const double *mflux;        // (Fi - Fv) dS of this side, projected and signed [4]
+    double *mdq;                // this side's gradient trace [8], written by the gradient kernel
 };
+
+// p-non-conforming edges of a handle whose elements have several orders: the elements of each
+// order form a sub-handle; an edge between two of them is evaluated on its own nodes (the
+// higher order) by the mortar kernels.
+constexpr int kMaxSub = 12;
+struct MortarEdge {
+    int sub[2];      // sub-handle of the LEFT and RIGHT element
+    int slot[2];     // their face slots (local element * 4 + face)
+    int Ps[2];       // their points per direction
+    int P;           // the edge's points per direction (the higher)
+    int low;         // the side of lower order (0 LEFT, 1 RIGHT)
+    int inverse;     // the RIGHT element runs the edge backwards
+    int mat;         // offset of T_forward [P][Plow] and then T_backward [Plow][P] in the matrix pool
+    int pad;
+    double nx, ny, dS;   // straight edge: normal (LEFT to RIGHT) and length
+};
+struct MortarArgs {
+    const MortarEdge *edges;
+    int nedges;
+    const double *mats;
+    const double *tr[kMaxSub];   // Q traces of the current stage, per sub-handle
+    const double *mdq[kMaxSub];  // gradient traces at mortar faces, per sub-handle
+    double *mustar[kMaxSub];
+    double *mflux[kMaxSub];
+    PhysParams ph;
+};
+// BR1 uStar of every mortar edge (from the sub-handles' current Q traces), to each side.
+hipError_t launch_mortar_ustar(const MortarArgs &m, hipStream_t s);
+// The numerical flux (Fi - Fv) dS of every mortar edge on its nodes, to each side.
+hipError_t launch_mortar_flux(const MortarArgs &m, hipStream_t s);
+// dst[map[i]] = src[i], i < n (a sub-handle's per-element values into the whole mesh's order).
+hipError_t launch_scatter(const double *src, const int *map, int n, double *dst, hipStream_t s);
 
 // The batched form: one launch per RK stage over a work list of element tiles of many cases.
 struct BatchArgs {
@@ -137,8 +175,10 @@ hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream
 // Time-step estimate: per element maxima, then an ordered prefix-max scan.
 hipError_t launch_dt_elem(int P, const double *Q, int K, double gamma, double gm1, double *umax, double *amax,
                           hipStream_t s);
+// N1: N + 1 of every element, or per element n1e[e] when n1e is set (several orders).
 hipError_t launch_dt_scan(int K, const double *umax, const double *amax, const double *dx, double coef, double N1,
-                          double dt_max, double *state /* [umax, amax] */, double *dt_out, hipStream_t s);
+                          double dt_max, double *state /* [umax, amax] */, double *dt_out, hipStream_t s,
+                          const double *n1e = nullptr);
 // Conservation sums (deterministic two-pass).
 // Volume monitors [One, dSnorm^2, 0, 0] (QuadMesh_Integrals.incf:22-65) into out[4].
 hipError_t launch_volume_integrals(int P, const double *Q, const double *jac, const double *w, int K, double gamma,

@@ -60,7 +60,8 @@ __global__ __launch_bounds__(1024) void k_resid_final(const double *part, int nb
 // previous call, and dt is the minimum over elements of cfl*dx/(umax+amax)/(N+1) taken
 // with the maxima as they stood at that element.  Exclusive prefix max over 1024 chunks.
 __global__ __launch_bounds__(1024) void k_dt_scan(int K, const double *umax, const double *amax, const double *dx,
-                                                  double coef, double N1, double dt_max, double *state, double *dt_out) {
+                                                  double coef, double N1, double dt_max, double *state, double *dt_out,
+                                                  const double *n1e) {
     __shared__ double su[1024], sa[1024], sd[1024];
     const int t = threadIdx.x, T = blockDim.x;
     const int chunk = (K + T - 1) / T, b = t * chunk, end = min(K, b + chunk);
@@ -81,7 +82,7 @@ __global__ __launch_bounds__(1024) void k_dt_scan(int K, const double *umax, con
     double ru = su[t], ra = sa[t], dt = dt_max;
     for (int e = b; e < end; ++e) {
         ru = fmax(ru, umax[e]); ra = fmax(ra, amax[e]);
-        const double de = coef * dx[e] / (ru + ra) / N1;
+        const double de = coef * dx[e] / (ru + ra) / (n1e ? n1e[e] : N1);
         dt = fmin(dt, de);
     }
     sd[t] = dt;
@@ -323,8 +324,9 @@ hipError_t launch_dt_elem(int P, const double *Q, int K, double gamma, double gm
 }
 
 hipError_t launch_dt_scan(int K, const double *umax, const double *amax, const double *dx, double coef, double N1,
-                          double dt_max, double *state, double *dt_out, hipStream_t st) {
-    hipLaunchKernelGGL(k_dt_scan, dim3(1), dim3(1024), 0, st, K, umax, amax, dx, coef, N1, dt_max, state, dt_out);
+                          double dt_max, double *state, double *dt_out, hipStream_t st, const double *n1e) {
+    hipLaunchKernelGGL(k_dt_scan, dim3(1), dim3(1024), 0, st, K, umax, amax, dx, coef, N1, dt_max, state, dt_out,
+                       n1e);
     return hipGetLastError();
 }
 

@@ -61,11 +61,31 @@ struct hg_handle {
     int2 *probe_ev = nullptr;
     double *probe_l = nullptr, *probe_out = nullptr;
     bool qdot_valid = false;   // QDot holds hg_time_derivative of the current state
+    // the edge storage the surface monitors read: the Q trace buffer and viscous face-flux
+    // traces of the last derivative evaluation (-1: none since the last upload)
+    int mon_tr = -1;
+    void *mon_vtr = nullptr;
     // artificial dissipation: per-element viscosity (written before each stage) and volume;
     // lap: the Laplacian type (the viscosity enters its own fluxes, not mu); per-edge Area
     double *mu_a = nullptr, *evol = nullptr, *earea = nullptr;
     bool lap = false;
     double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
+    // p-non-conforming faces of this handle: per face node the mortar kernels' projected flux,
+    // BR1 uStar, and this side's gradient trace (null when every edge is conforming)
+    double *mflux = nullptr, *mustar = nullptr, *mdq = nullptr;
+    // A mesh with several polynomial orders: one sub-handle per order (its elements in mesh
+    // order, sharing this handle's stream) coupled by the mortar kernels; this handle keeps the
+    // whole mesh, the time-step scan over all elements in mesh order and the mortar tables.
+    std::vector<hg_handle *> sub;
+    std::vector<std::vector<int>> sub_els;   // per sub-handle: its elements' mesh ids
+    std::vector<int64_t> el_off;             // per mesh element: node offset in the reference layout
+    int *sub_gmap[hg::kMaxSub] = {};         // per sub-handle (device): local element -> mesh element
+    hg::MortarEdge *medges = nullptr;
+    double *mmats = nullptr;
+    int nmortar = 0;
+    double *n1e = nullptr;                   // per mesh element: N + 1 (time-step scan)
+    std::vector<std::vector<int>> sub_probes;   // per sub-handle: the probe indices it holds
+    bool shared_stream = false;              // a sub-handle: the stream belongs to its parent
     int cur = 0;
     int tile[2] = {-1, -1};  // workgroup tile of the stage and gradient kernels, -1 = not chosen yet
     // one RK step captured as a HIP graph per trace parity; the step size is read from dt_dev
@@ -107,6 +127,9 @@ StageArgs handle_args(hg_handle *h);
 void *vtr_current(hg_handle *h);
 void *vtr_next(hg_handle *h);
 int ensure_vtr(hg_handle *h);
+// Records which trace buffers hold the last derivative evaluation's edge storage (the Q trace
+// buffer `tr` and, for Navier-Stokes, its viscous face-flux traces), for the surface monitors.
+void mark_last_evaluation(hg_handle *h, int tr);
 // Records the message returned by hg_last_error and returns 1.
 int handle_error(const std::string &m);
 // Locates the allocation-free entry points' scratch array on a device (once per device).

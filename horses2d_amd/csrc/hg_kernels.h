@@ -119,6 +119,10 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
 #pragma unroll
             for (int c = 0; c < 4; ++c) us[c] = -us[c];
         }
+    } else if (kind == FK_MORTAR) {
+        // projected and signed by the mortar kernel (load_face put it in qo)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) us[c] = qo[c];
     } else {
         R uB[4];
         int neu;
@@ -271,6 +275,11 @@ __device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, in
         const TR *nbt = (const TR *)a.tr_in + nslot * 4;
 #pragma unroll
         for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
+    } else if ((fr.x & 15) == FK_MORTAR && a.mustar) {
+        // p-non-conforming edge: this side's projected uStar in place of a neighbour trace
+        const double *u = a.mustar + ((int64_t)(fe * 4 + ff) * P + fk) * 4;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) qo[c] = (R)u[c];
     }
 }
 
@@ -306,10 +315,11 @@ __device__ __forceinline__ void own_trace(const StageArgs &a, const double *sQe,
 // tile (node stride DS), times mu, stored in the VT layout (ViscousFluxes.f90:39-80, adiabatic
 // walls :279-318).  The gradient trace is accumulated in FP64 in the FP64 form and in FP32 in
 // the FP32 form (lbR: lb in the compute type).
+// mdq: where a p-non-conforming face keeps its gradient trace for the mortar kernel (or null).
 template <int P, int DS, typename R, typename TR>
 __device__ __forceinline__ void viscous_face_trace(const StageArgs &a, const Phys<R> &ph, int flags, const R *qs,
                                                    const R *sDQe, const double *sLb, const R *lbR, int ff, int fk,
-                                                   TR *o, R muf) {
+                                                   TR *o, R muf, double *mdq = nullptr) {
     R ds[8], fv[4], gv[4];
     if constexpr (sizeof(R) == 4) {
         face_trace_r<P, DS, 8>(sDQe, lbR, ff, fk, ds);
@@ -318,6 +328,10 @@ __device__ __forceinline__ void viscous_face_trace(const StageArgs &a, const Phy
         face_trace<P, DS, 8>(sDQe, sLb, ff, fk, tg);
 #pragma unroll
         for (int c = 0; c < 8; ++c) ds[c] = (R)(TR)tg[c];
+    }
+    if (mdq) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) mdq[c] = (double)ds[c];
     }
     if (is_adiabatic(flags)) adiabatic_flux(ph, qs, ds, fv, gv);
     else viscous_flux(ph, qs, ds, fv, gv);
@@ -428,8 +442,9 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
     __syncthreads();
     if (face_active) {
         const R muf = XB ? face_mu<R>(a, ph, fe, fr) : ph.mu;
+        double *mdq = ((fr.x & 15) == FK_MORTAR && a.mdq) ? a.mdq + ((int64_t)(fe * 4 + ff) * P + fk) * 8 : nullptr;
         viscous_face_trace<P, DS, R, TR>(a, ph, fr.x, qs, sDQ + fs * PP * DS, sLb, sLbw + 2 * P, ff, fk,
-                                         (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT, muf);
+                                         (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT, muf, mdq);
     }
 }
 
@@ -610,127 +625,138 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     // ---- face phase: Riemann fluxes of the element's own four faces
     if (face_active) {
         const int kind = fr.x & 15;
-        R vs[NS ? VT : 1];   // own viscous flux trace
-        if (NS) {
-            const TR *vt = (const TR *)a.dtr + ((int64_t)(fe * 4 + ff) * P + fk) * VT;
+        if (kind == FK_MORTAR) {
+            // p-non-conforming edge: (Fi - Fv) dS projected to this side by the mortar kernel
+            const double *mf = a.mflux + ((int64_t)(fe * 4 + ff) * P + fk) * 4;
 #pragma unroll
-            for (int c = 0; c < (NS ? VT : 1); ++c) vs[c] = (R)vt[c];
-        }
-        int rs = a.ph.riemann;
-        R uSB[4] = {R(0), R(0), R(0), R(0)};
-        int neumann = 0, prescribed = 0;
-        if (kind > FK_PERIODIC) {
-            prescribed = ghost_state<R, XB>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB,
-                                                  neumann);
-            const int zr = (fr.x >> FF_RS_SHIFT) & 15;
-            if (zr) rs = zr - 1;
-        }
-        R Fi[4];
-        if (XB && prescribed) {
-            // WEAK_PRESCRIBED (DGSpatialDiscretizationMethods.f90:1194-1198): the inviscid
-            // normal flux of the boundary's state
-            R fp[4], gp[4];
-            inviscid_flux(ph, qo, fp, gp);
-#pragma unroll
-            for (int c = 0; c < 4; ++c) Fi[c] = fp[c] * nx + gp[c] * ny;
-        } else if (kind == FK_INT_R) riemann<RS, R>(rs, ph, qo, qs, nx, ny, Fi);
-        else riemann<RS, R>(rs, ph, qs, qo, nx, ny, Fi);
-        R F[4];
-        if (!NS) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) F[c] = Fi[c] * dS;
-        } else {
-            R Fv[4] = {R(0), R(0), R(0), R(0)};
-            R Fa[4] = {R(0), R(0), R(0), R(0)};   // Laplacian dissipation: -mu Area (QL - QR)
-            R G8[8] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
-            const R muf = XB ? face_mu<R>(a, ph, fe, fr) : ph.mu;
-            const R lapA = (XB && a.lap) ? (R)(face_lap(a, fe, fr) * a.earea[fr.w]) : R(0);
-            const R pen = IP ? ph.sigma0 * muf * (R)(P - 1) * (R)(P - 1) * invh : R(0);
-            if (kind <= FK_PERIODIC) {
-                const R *uL = (kind == FK_INT_R) ? qo : qs, *uR = (kind == FK_INT_R) ? qs : qo;
-                const R *vL = (kind == FK_INT_R) ? vn : vs, *vR = (kind == FK_INT_R) ? vs : vn;
-                R fL[4], gL[4], fR[4], gR[4];
-                viscous_trace_flux(vL, fL, gL);
-                viscous_trace_flux(vR, fR, gR);
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    R acc = R(0);
-                    acc = acc + R(0.5) * (fL[c] + fR[c]) * nx;
-                    acc = acc + R(0.5) * (gL[c] + gR[c]) * ny;
-                    Fv[c] = acc;
-                }
-                if (IP) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (uL[c] - uR[c]);
-                }
-                if (XB && a.lap) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) Fa[c] = -(lapA * (uL[c] - uR[c]));
-                }
-                if (ipgrad) {
-                    // IP_GradientRiemannSolver (DGViscous_IP.f90:216-249): this side's share
-                    R fgr[8], f4[4], g4[4];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) { fgr[c] = (uL[c] - uR[c]) * nx; fgr[4 + c] = (uL[c] - uR[c]) * ny; }
-                    const R *own = (kind == FK_INT_R) ? uR : uL;
-                    viscous_flux(ph, own, fgr, f4, g4);
-                    const R cf = (kind == FK_INT_R ? -R(0.5) : R(0.5)) * ph.ip_eps * muf;
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
-                }
-            } else if (!neumann) {
-                const bool adiab = is_adiabatic(fr.x);
-                R fv4[4], gv4[4];
-                viscous_trace_flux(vs, fv4, gv4);   // adiabatic or not, as the gradient kernel chose
-#pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    R acc = R(0);
-                    acc = acc + fv4[c] * nx;
-                    acc = acc + gv4[c] * ny;
-                    Fv[c] = acc;
-                }
-                if (IP) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (qs[c] - uSB[c]);
-                }
-                if (XB && a.lap) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) Fa[c] = -(lapA * (qs[c] - uSB[c]));
-                }
-                if (ipgrad) {
-                    R fgr[8], f4[4], g4[4];
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) { fgr[c] = (qs[c] - uSB[c]) * nx; fgr[4 + c] = (qs[c] - uSB[c]) * ny; }
-                    if (adiab) adiabatic_flux(ph, qs, fgr, f4, g4);
-                    else viscous_flux(ph, qs, fgr, f4, g4);
-                    const R cf = ph.ip_eps * muf;
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
-                }
-            }
-            if (kind <= FK_INT_R) {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] - Fv[c]) - Fa[c]) * dS;
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] * dS - Fv[c] * dS) - Fa[c] * dS);
-            }
+            for (int c = 0; c < 4; ++c) sF[t * 4 + c] = (R)mf[c];
             if (ipgrad) {
-                // contravariant gradient-flux components at the face (DGWeakIntegrals.f90:268-495)
-                const R *fm = (const R *)a.fmet + ((int64_t)(fe * 4 + ff) * P + fk) * 5;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    sU[t * 4 + c] = (G8[c] * fm[0] + G8[4 + c] * fm[1]) * fm[4];   // fm[4] = 1/jac at the face node
-                    sV[t * 4 + c] = (G8[c] * fm[2] + G8[4 + c] * fm[3]) * fm[4];
+                for (int c = 0; c < 4; ++c) { sU[t * 4 + c] = R(0); sV[t * 4 + c] = R(0); }
+            }
+        } else {
+            R vs[NS ? VT : 1];   // own viscous flux trace
+            if (NS) {
+                const TR *vt = (const TR *)a.dtr + ((int64_t)(fe * 4 + ff) * P + fk) * VT;
+#pragma unroll
+                for (int c = 0; c < (NS ? VT : 1); ++c) vs[c] = (R)vt[c];
+            }
+            int rs = a.ph.riemann;
+            R uSB[4] = {R(0), R(0), R(0), R(0)};
+            int neumann = 0, prescribed = 0;
+            if (kind > FK_PERIODIC) {
+                prescribed = ghost_state<R, XB>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB,
+                                                      neumann);
+                const int zr = (fr.x >> FF_RS_SHIFT) & 15;
+                if (zr) rs = zr - 1;
+            }
+            R Fi[4];
+            if (XB && prescribed) {
+                // WEAK_PRESCRIBED (DGSpatialDiscretizationMethods.f90:1194-1198): the inviscid
+                // normal flux of the boundary's state
+                R fp[4], gp[4];
+                inviscid_flux(ph, qo, fp, gp);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) Fi[c] = fp[c] * nx + gp[c] * ny;
+            } else if (kind == FK_INT_R) riemann<RS, R>(rs, ph, qo, qs, nx, ny, Fi);
+            else riemann<RS, R>(rs, ph, qs, qo, nx, ny, Fi);
+            R F[4];
+            if (!NS) {
+#pragma unroll
+                for (int c = 0; c < 4; ++c) F[c] = Fi[c] * dS;
+            } else {
+                R Fv[4] = {R(0), R(0), R(0), R(0)};
+                R Fa[4] = {R(0), R(0), R(0), R(0)};   // Laplacian dissipation: -mu Area (QL - QR)
+                R G8[8] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+                const R muf = XB ? face_mu<R>(a, ph, fe, fr) : ph.mu;
+                const R lapA = (XB && a.lap) ? (R)(face_lap(a, fe, fr) * a.earea[fr.w]) : R(0);
+                const R pen = IP ? ph.sigma0 * muf * (R)(P - 1) * (R)(P - 1) * invh : R(0);
+                if (kind <= FK_PERIODIC) {
+                    const R *uL = (kind == FK_INT_R) ? qo : qs, *uR = (kind == FK_INT_R) ? qs : qo;
+                    const R *vL = (kind == FK_INT_R) ? vn : vs, *vR = (kind == FK_INT_R) ? vs : vn;
+                    R fL[4], gL[4], fR[4], gR[4];
+                    viscous_trace_flux(vL, fL, gL);
+                    viscous_trace_flux(vR, fR, gR);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        R acc = R(0);
+                        acc = acc + R(0.5) * (fL[c] + fR[c]) * nx;
+                        acc = acc + R(0.5) * (gL[c] + gR[c]) * ny;
+                        Fv[c] = acc;
+                    }
+                    if (IP) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (uL[c] - uR[c]);
+                    }
+                    if (XB && a.lap) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) Fa[c] = -(lapA * (uL[c] - uR[c]));
+                    }
+                    if (ipgrad) {
+                        // IP_GradientRiemannSolver (DGViscous_IP.f90:216-249): this side's share
+                        R fgr[8], f4[4], g4[4];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) { fgr[c] = (uL[c] - uR[c]) * nx; fgr[4 + c] = (uL[c] - uR[c]) * ny; }
+                        const R *own = (kind == FK_INT_R) ? uR : uL;
+                        viscous_flux(ph, own, fgr, f4, g4);
+                        const R cf = (kind == FK_INT_R ? -R(0.5) : R(0.5)) * ph.ip_eps * muf;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
+                    }
+                } else if (!neumann) {
+                    const bool adiab = is_adiabatic(fr.x);
+                    R fv4[4], gv4[4];
+                    viscous_trace_flux(vs, fv4, gv4);   // adiabatic or not, as the gradient kernel chose
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        R acc = R(0);
+                        acc = acc + fv4[c] * nx;
+                        acc = acc + gv4[c] * ny;
+                        Fv[c] = acc;
+                    }
+                    if (IP) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) Fv[c] = Fv[c] - pen * (qs[c] - uSB[c]);
+                    }
+                    if (XB && a.lap) {
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) Fa[c] = -(lapA * (qs[c] - uSB[c]));
+                    }
+                    if (ipgrad) {
+                        R fgr[8], f4[4], g4[4];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) { fgr[c] = (qs[c] - uSB[c]) * nx; fgr[4 + c] = (qs[c] - uSB[c]) * ny; }
+                        if (adiab) adiabatic_flux(ph, qs, fgr, f4, g4);
+                        else viscous_flux(ph, qs, fgr, f4, g4);
+                        const R cf = ph.ip_eps * muf;
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) { G8[c] = cf * f4[c] * dS; G8[4 + c] = cf * g4[c] * dS; }
+                    }
+                }
+                if (kind <= FK_INT_R) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] - Fv[c]) - Fa[c]) * dS;
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) F[c] = ((Fi[c] * dS - Fv[c] * dS) - Fa[c] * dS);
+                }
+                if (ipgrad) {
+                    // contravariant gradient-flux components at the face (DGWeakIntegrals.f90:268-495)
+                    const R *fm = (const R *)a.fmet + ((int64_t)(fe * 4 + ff) * P + fk) * 5;
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        sU[t * 4 + c] = (G8[c] * fm[0] + G8[4 + c] * fm[1]) * fm[4];   // fm[4] = 1/jac at the face node
+                        sV[t * 4 + c] = (G8[c] * fm[2] + G8[4 + c] * fm[3]) * fm[4];
+                    }
                 }
             }
-        }
-        if (kind == FK_INT_R) {
+            if (kind == FK_INT_R) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) F[c] = -F[c];
-        }
+                for (int c = 0; c < 4; ++c) F[c] = -F[c];
+            }
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sF[t * 4 + c] = F[c];
+            for (int c = 0; c < 4; ++c) sF[t * 4 + c] = F[c];
+        }
     }
     __syncthreads();
 

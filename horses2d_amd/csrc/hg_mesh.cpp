@@ -217,8 +217,11 @@ void set_curve(const Mesh &m, Edge &e, const double *cpts, int order) {
     }
 }
 
-// Edge point and tangent at the element's local index (QuadElement_EdgeMappings.incf:52-177,262-442)
-void edge_point(const Mesh &m, const Edge &e, int idx, int dir, double *X, double *dX) {
+// Edge point and tangent at the local index of an element of order elN
+// (QuadElement_EdgeMappings.incf:52-177,262-442): an interior edge is evaluated at the element's
+// node (QuadElement_QuadMappings.incf:55-62), which differs from the edge's nodes when the edge
+// takes the higher order of its two elements.
+void edge_point(const Mesh &m, const Edge &e, int elN, int idx, int dir, double *X, double *dX) {
     const Spectral &s = spectral(e.N);
     if (e.bdry) {
         int c = dir == 1 ? idx : e.N - idx;
@@ -227,7 +230,8 @@ void edge_point(const Mesh &m, const Edge &e, int idx, int dir, double *X, doubl
         dX[0] = (double)dir * e.dX[2 * cd]; dX[1] = (double)dir * e.dX[2 * cd + 1];
         return;
     }
-    double x = dir == -1 ? 1.0 - s.xi[idx] : s.xi[idx];
+    const double xe = spectral(elN).xi[idx];
+    double x = dir == -1 ? 1.0 - xe : xe;
     if (!e.curved) {
         const double *a = &m.pts[2 * e.nodes[0]], *b = &m.pts[2 * e.nodes[1]];
         for (int d = 0; d < 2; ++d) { X[d] = a[d] * (1.0 - x) + b[d] * x; dX[d] = (double)dir * (b[d] - a[d]); }
@@ -252,7 +256,7 @@ void element_mappings(const Mesh &m, Elem &el) {
         bool horiz = (pos == 1 || pos == 3);
         for (int a = 0; a < P; ++a) {
             double X[2], dX[2];
-            edge_point(m, e, a, el.dir[pos - 1], X, dX);
+            edge_point(m, e, el.N, a, el.dir[pos - 1], X, dX);
             for (int b = 0; b < P; ++b) {
                 int i = horiz ? a : b, j = horiz ? b : a;
                 for (int d = 0; d < 2; ++d) { XE[at(d, i, j, pos - 1)] = X[d]; DXE[at(d, i, j, pos - 1)] = dX[d]; }
@@ -303,6 +307,14 @@ const Spectral &spectral(int N) {
         make_spectral(*table[N], N);
     }
     return *table[N];
+}
+
+void mortar_matrices(int NLow, int N, double *Tf, double *Tb) {
+    const Spectral &sl = spectral(NLow), &sh = spectral(N);
+    const int Pl = NLow + 1, P = N + 1;
+    interpolation_matrix(NLow, N, sl.xi, sl.wb, sh.xi, Tf);
+    for (int i = 0; i < Pl; ++i)
+        for (int l = 0; l < P; ++l) Tb[i * P + l] = ((1.0 / sl.w[i]) * Tf[l * Pl + i]) * sh.w[l];
 }
 
 double lagrange_at(const Spectral &s, double x, double *lj) {
@@ -378,9 +390,11 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
         Edge &ed = m.ed[k];
         if (ed.bdry) { ed.N = m.el[ed.quads[0]].N; ed.NCurve = ed.N; }
         else {
-            int a = m.el[ed.quads[0]].N, b = m.el[ed.quads[1]].N;
-            if (a != b) return "p-non-conforming edges are not supported";
-            ed.N = a; ed.NCurve = a > 1 ? a / 2 : a;
+            // the higher order of the two elements; the curve at half the lower one
+            // (QuadMesh_Construct.incf:104-118)
+            const int a = m.el[ed.quads[0]].N, b = m.el[ed.quads[1]].N, lo = std::min(a, b);
+            ed.N = std::max(a, b);
+            ed.NCurve = lo > 1 ? lo / 2 : lo;
         }
         ed.curved = curve_of[k] >= 0;
         set_curve(m, ed, ed.curved ? &cxy[(size_t)2 * in.np1 * curve_of[k]] : nullptr, in.np1 - 1);

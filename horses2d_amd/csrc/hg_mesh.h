@@ -23,6 +23,11 @@ struct Spectral {
 };
 const Spectral &spectral(int N);
 double lagrange_at(const Spectral &s, double x, double *lj);  // barycentric l_j(x)
+// p-non-conforming edge of orders NLow < N (Edge_LinkWithElements, QuadElement_EdgeProcedures.incf:
+// 105-124): T_forward [(N+1) x (NLow+1)] (PolynomialInterpolationMatrix from the low order's
+// nodes to the edge's) and T_backward [(NLow+1) x (N+1)] = Minv_low T_forward^T M_edge
+// (TripleMatrixProduct with diagonal mass matrices: (1/w_low(i) T_forward(l, i)) w_edge(l)).
+void mortar_matrices(int NLow, int N, double *Tf, double *Tb);
 
 struct Edge {
     int nodes[2] = {-1, -1};   // after the linking inversions

@@ -54,6 +54,8 @@ int noalloc_setup(int device) {
 
 extern "C" int hg_conservation_sums_noalloc(hg_handle *h, double *out_dev) {
     if (!h || !out_dev) return hg::handle_error("hg_conservation_sums_noalloc: null argument");
+    if (!h->sub.empty())
+        return hg::handle_error("hg_conservation_sums_noalloc: one polynomial order per mesh (use hg_conservation_sums)");
     if ((size_t)h->cons_nb * 4 > kScratchDoubles)
         return hg::handle_error("hg_conservation_sums_noalloc: the mesh needs " + std::to_string(h->cons_nb) +
                                 " partial sums, above the scratch array's " + std::to_string(kScratchDoubles / 4));

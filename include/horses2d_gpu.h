@@ -219,9 +219,12 @@ int hg_conservation_sums_noalloc(hg_handle *h, double *out_dev);
  * "dSnorm" = sqrt(sum_e w^T ((p / rho^gamma - 1)^2 J) w).  FP64 in both forms. */
 int hg_volume_integrals(hg_handle *h, double out[2]);
 
-/* Surface monitors of the current state over the edges of boundary zone `marker`
- * (1-based, the HiOMesh marker), QuadMesh_Integrals.incf:67-346, from the element-side traces
- * (storage(1)): out[0] "Surface" sum w dS, out[1] "pressure" sum w p dS, out[2] "density"
+/* Surface monitors over the edges of boundary zone `marker` (1-based, the HiOMesh marker),
+ * QuadMesh_Integrals.incf:67-346, from the element-side edge storage (storage(1)) as the last
+ * derivative evaluation left it: after hg_rk_step (or hg_batch_rk_step) the traces and gradients
+ * of the last RK stage's input state, not of the updated state, as the reference's monitors read
+ * them after TimeStep; after hg_time_derivative those of the current state; after
+ * hg_upload_state they are formed from the uploaded state.  out[0] "Surface" sum w dS, out[1] "pressure" sum w p dS, out[2] "density"
  * sum w rho dS, out[3..4] "pressure" force sum w p n dS, out[5] "mass-flow" sum w (rho u.n) dS,
  * out[6] "flow" sum w (rho u.n)/rho dS, out[7..8] "viscous" force sum w (tau.n) dS (Navier-Stokes
  * only, tau as ComputeViscousTensor, ViscousFluxes.f90:416-461; zero for Euler).  Replaces the

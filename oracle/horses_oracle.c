@@ -576,6 +576,12 @@ typedef struct {
     int id, bdry, curved;
     int nodes[2];          /* node ids after linking inversions */
     int N, NLow, NCurve;
+    /* p-non-conforming edge (Edge_LinkWithElements, QuadElement_EdgeProcedures.incf:105-124,
+     * 154-168): the side (0 LEFT, 1 RIGHT) whose order NLow is below the edge's N, or -1;
+     * T_forward [i*(NLow+1) + l] (edge node i from the low side's node l) and
+     * T_backward [i*(N+1) + l] = Minv_low T_forward^T M_edge (low node i from edge node l) */
+    int low;
+    double *Tf, *Tb;
     int quads[2];          /* [0] LEFT element (lowest id) / boundary element, [1] RIGHT */
     int loc[2];            /* edge location 1..4 in quads[0], quads[1] */
     int inverse;
@@ -635,6 +641,7 @@ typedef struct {
     int64_t nodes_total, ndof;
     double *Q, *QDot, *dQ, *G;
     double dt_umax, dt_amax;  /* SAVEd running maxima of EstimateTimeStep */
+    int traces_valid;         /* the edge traces are those of the last derivative evaluation */
 } Mesh;
 
 /* Mesh input (the content of a HiOMesh file, coordinates already divided by
@@ -741,9 +748,11 @@ static void edge_set_curve(Mesh *m, Edge *ed, const double *cpts /* curved point
     }
 }
 
-/* Edge geometry seen by an element at local coordinate index iXi
- * (QuadElement_EdgeMappings.incf:52-177, 262-442). */
-static void edge_eval_X(Mesh *m, Edge *ed, int iXi, int dirn, double *X, double *dX) {
+/* Edge geometry seen by an element of order eN at its local coordinate index iXi
+ * (QuadElement_EdgeMappings.incf:52-177, 262-442): interior edges are evaluated at the
+ * element's node xi(iXi) (QuadElement_QuadMappings.incf:55-62), which is not one of the edge's
+ * nodes when the edge takes the higher order of its two elements. */
+static void edge_eval_X(Mesh *m, Edge *ed, int eN, int iXi, int dirn, double *X, double *dX) {
     const Spec *s = spec_get(ed->N);
     int P = ed->N + 1;
     if (ed->bdry) { /* getX/getdX */
@@ -753,7 +762,7 @@ static void edge_eval_X(Mesh *m, Edge *ed, int iXi, int dirn, double *X, double 
         else { dX[0] = (double)dirn * ed->dX[2 * ci]; dX[1] = (double)dirn * ed->dX[2 * ci + 1]; }
         return;
     }
-    double xi = s->xi[iXi];
+    double xi = spec_get(eN)->xi[iXi];
     double cxi = (dirn == BACKWARD) ? 1.0 - xi : xi;
     if (!ed->curved) {
         const double *n1 = m->pts + 2 * ed->nodes[0], *n2 = m->pts + 2 * ed->nodes[1];
@@ -783,7 +792,7 @@ static void elem_set_mappings(Mesh *m, Elem *e) {
         Edge *ed = &m->ed[e->edges[pos - 1]];
         for (int i = 0; i < P; ++i) {
             double X[2], dX[2];
-            edge_eval_X(m, ed, i, e->dir[pos - 1], X, dX);
+            edge_eval_X(m, ed, e->N, i, e->dir[pos - 1], X, dX);
             for (int j = 0; j < P; ++j)
                 for (int d = 0; d < 2; ++d) { XE(d, i, j, pos - 1) = X[d]; DXE(d, i, j, pos - 1) = dX[d]; }
         }
@@ -792,7 +801,7 @@ static void elem_set_mappings(Mesh *m, Elem *e) {
         Edge *ed = &m->ed[e->edges[pos - 1]];
         for (int j = 0; j < P; ++j) {
             double X[2], dX[2];
-            edge_eval_X(m, ed, j, e->dir[pos - 1], X, dX);
+            edge_eval_X(m, ed, e->N, j, e->dir[pos - 1], X, dX);
             for (int i = 0; i < P; ++i)
                 for (int d = 0; d < 2; ++d) { XE(d, i, j, pos - 1) = X[d]; DXE(d, i, j, pos - 1) = dX[d]; }
         }
@@ -827,7 +836,10 @@ static void elem_set_mappings(Mesh *m, Elem *e) {
 static void mesh_free(Mesh *m) {
     if (!m) return;
     for (int e = 0; e < m->K; ++e) { free(m->el[e].X); free(m->el[e].Ja); free(m->el[e].jac); }
-    for (int i = 0; i < m->E; ++i) { free(m->ed[i].X); free(m->ed[i].dX); free(m->ed[i].n); free(m->ed[i].dS); }
+    for (int i = 0; i < m->E; ++i) {
+        free(m->ed[i].X); free(m->ed[i].dX); free(m->ed[i].n); free(m->ed[i].dS);
+        free(m->ed[i].Tf); free(m->ed[i].Tb);
+    }
     for (int z = 0; z < m->nmarkers; ++z) free(m->zones[z].edges);
     free(m->el); free(m->ed); free(m->zones); free(m->pts);
     free(m->Q); free(m->QDot); free(m->dQ); free(m->G); free(m->mu_e); free(m->lap_e);
@@ -917,12 +929,24 @@ static Mesh *mesh_build(const MeshIn *in) {
     /* edges: order, curves, construct */
     for (int k = 0; k < E; ++k) {
         Edge *ed = &m->ed[k];
+        ed->low = -1;
         if (ed->bdry) { ed->N = m->el[ed->quads[0]].N; ed->NLow = ed->N; ed->NCurve = ed->N; }
         else {
             int a = m->el[ed->quads[0]].N, b = m->el[ed->quads[1]].N;
             ed->N = a > b ? a : b; ed->NLow = a < b ? a : b;
             ed->NCurve = ed->NLow > 1 ? ed->NLow / 2 : ed->NLow;
-            if (a != b) { mesh_free(m); free(curve_of); free(cxy); fail("p-non-conforming edges are not supported"); return NULL; }
+            if (a != b) {
+                /* PolynomialInterpolationMatrix(NLow, N, ...) and TripleMatrixProduct(Minv_low,
+                 * T_forward^T, M_edge): (1/w_low(i) * T_forward(l, i)) * w_edge(l) */
+                const Spec *sl = spec_get(ed->NLow), *sh = spec_get(ed->N);
+                int Pl = ed->NLow + 1, Ph = ed->N + 1;
+                ed->low = a < b ? 0 : 1;
+                ed->Tf = (double *)malloc(sizeof(double) * Pl * Ph);
+                ed->Tb = (double *)malloc(sizeof(double) * Pl * Ph);
+                interp_matrix(ed->NLow, ed->N, sl->xi, sl->wb, sh->xi, ed->Tf);
+                for (int i = 0; i < Pl; ++i)
+                    for (int l = 0; l < Ph; ++l) ed->Tb[i * Ph + l] = ((1.0 / sl->w[i]) * ed->Tf[l * Pl + i]) * sh->w[l];
+            }
         }
         ed->curved = curve_of[k] >= 0;
         int P = ed->N + 1;
@@ -1390,6 +1414,53 @@ static const double *bdry_state(Mesh *m, Edge *ed, int i) {
     return ed->associated ? partner_q(m, ed, i) : ed->uB + 4 * i;
 }
 
+/* Edge_ProjectSolution / Edge_ProjectSolutionAndGradient, Types 1-6
+ * (QuadElement_EdgeProcedures.incf:749-1079): side `side`'s trace values of one plane
+ * (src[eq + 4 l], the side's own nodes) at the edge's nodes i, in the edge's (LEFT) orientation.
+ * The low side is interpolated with T_forward; a RIGHT side of an inverse edge is read (Type 2)
+ * or written (Type 6) backwards. */
+static void edge_side_values(const Edge *ed, int side, const double *src, double *out /* 4 (N+1) */) {
+    int P = ed->N + 1, Pl = ed->NLow + 1;
+    for (int i = 0; i < P; ++i) {
+        for (int eq = 0; eq < 4; ++eq) {
+            double v;
+            if (side != ed->low) v = src[eq + 4 * ((side == 1 && ed->inverse) ? P - 1 - i : i)];
+            else {
+                int row = (side == 1 && ed->inverse) ? P - 1 - i : i;
+                double acc = 0.0;
+                for (int l = 0; l < Pl; ++l) acc = acc + ed->Tf[row * Pl + l] * src[eq + 4 * l];
+                v = acc;
+            }
+            out[eq + 4 * i] = v;
+        }
+    }
+}
+
+/* Edge_ProjectFluxes, Types 1-6 (QuadElement_EdgeProcedures.incf:1081-1206): a flux at the
+ * edge's nodes (F[eq + 4 i]) to side `side`'s own nodes, negated for the RIGHT side; the low
+ * side takes T_backward F. */
+static void edge_flux_to_side(const Edge *ed, int side, const double *F, double *out) {
+    int P = ed->N + 1, Pl = ed->NLow + 1;
+    if (side != ed->low) {
+        for (int i = 0; i < P; ++i) {
+            int o = (side == 1 && ed->inverse) ? P - 1 - i : i;
+            for (int eq = 0; eq < 4; ++eq) out[eq + 4 * o] = side == 0 ? F[eq + 4 * i] : -F[eq + 4 * i];
+        }
+        return;
+    }
+    for (int i = 0; i < Pl; ++i) {
+        int o = (side == 1 && ed->inverse) ? Pl - 1 - i : i;
+        for (int eq = 0; eq < 4; ++eq) {
+            double acc = 0.0;
+            for (int l = 0; l < P; ++l) {
+                if (side == 0) acc = acc + ed->Tb[i * P + l] * F[eq + 4 * l];
+                else acc = acc - ed->Tb[i * P + l] * F[eq + 4 * l];
+            }
+            out[eq + 4 * o] = acc;
+        }
+    }
+}
+
 /* DGWeakIntegrals.f90:503-553 Vector_StdVolumeGreen + BR1 face loop
  * (DGViscous_BR1.f90:48-280) + DGViscousMethods.f90:538-616,786-917. */
 static void gradient_br1(Mesh *m) {
@@ -1414,7 +1485,34 @@ static void gradient_br1(Mesh *m) {
         Edge *ed = &m->ed[k];
         int P = ed->N + 1;
         double FuL[2][4 * MAXP], FuR[2][4 * MAXP];
-        if (!ed->bdry) {
+        if (!ed->bdry && ed->low >= 0) {
+            /* p-non-conforming: BaseClass_ComputeSolutionRiemann_Interior / _CurvedInterior
+             * (DGViscousMethods.f90:538-616): uStar on the edge's nodes from the projected
+             * traces; straight edges project uStar and then scale by dS n, curved edges scale
+             * first and project each direction */
+            double qL[4 * MAXP], qR[4 * MAXP], us[4 * MAXP];
+            edge_side_values(ed, 0, ed->tq[0], qL);
+            edge_side_values(ed, 1, ed->tq[1], qR);
+            for (int c = 0; c < 4 * P; ++c) us[c] = 0.5 * (qL[c] + qR[c]);
+            int PL = m->el[ed->quads[0]].P, PR = m->el[ed->quads[1]].P;
+            if (!ed->curved) {
+                double uL[4 * MAXP], uR[4 * MAXP];
+                edge_flux_to_side(ed, 0, us, uL);
+                edge_flux_to_side(ed, 1, us, uR);
+                for (int d = 0; d < 2; ++d) {
+                    for (int c = 0; c < 4 * PL; ++c) FuL[d][c] = uL[c] * ed->dS[0] * ed->n[d];
+                    for (int c = 0; c < 4 * PR; ++c) FuR[d][c] = uR[c] * ed->dS[0] * ed->n[d];
+                }
+            } else {
+                for (int d = 0; d < 2; ++d) {
+                    double fs[4 * MAXP];
+                    for (int i = 0; i < P; ++i)
+                        for (int eq = 0; eq < 4; ++eq) fs[eq + 4 * i] = us[eq + 4 * i] * ed->dS[i] * ed->n[2 * i + d];
+                    edge_flux_to_side(ed, 0, fs, FuL[d]);
+                    edge_flux_to_side(ed, 1, fs, FuR[d]);
+                }
+            }
+        } else if (!ed->bdry) {
             for (int i = 0; i < P; ++i) {
                 const double *qL = ed->tq[0] + 4 * i;
                 const double *qR = ed->tq[1] + 4 * (ed->inverse ? P - 1 - i : i);
@@ -1453,11 +1551,11 @@ static void gradient_br1(Mesh *m) {
         for (int side = 0; side < (ed->bdry ? 1 : 2); ++side) {
             Elem *e = &m->el[ed->quads[side]];
             const Spec *s = spec_get(e->N);
-            int loc = ed->loc[side];
+            int loc = ed->loc[side], Pe = e->P;
             double(*Fu)[4 * MAXP] = side == 0 ? FuL : FuR;
             for (int d = 0; d < 2; ++d)
-                for (int jj = 0; jj < P; ++jj)
-                    for (int ii = 0; ii < P; ++ii)
+                for (int jj = 0; jj < Pe; ++jj)
+                    for (int ii = 0; ii < Pe; ++ii)
                         for (int eq = 0; eq < 4; ++eq) {
                             double v;
                             switch (loc) {
@@ -1753,7 +1851,40 @@ static void time_derivative(Mesh *m) {
         double FL[4 * MAXP], FR[4 * MAXP];
         double GL[8 * MAXP], GR[8 * MAXP];
         int gradfl = m->ns && m->viscous == VISC_IP && m->ip_gradflux;
-        if (!ed->bdry) {
+        if (!ed->bdry && ed->low >= 0) {
+            /* p-non-conforming (ComputeRiemannSolver_InteriorEdge / _InteriorCurvedEdge
+             * :545-759): the Riemann problem on the edge's nodes from the projected traces and
+             * gradients, FStar = (Fi - Fv - Fa) dS, returned to each side by ProjectFluxes */
+            double qL[4 * MAXP], qR[4 * MAXP], dL[2][4 * MAXP], dR[2][4 * MAXP], Fs[4 * MAXP];
+            int PL = m->el[ed->quads[0]].P, PR = m->el[ed->quads[1]].P;
+            edge_side_values(ed, 0, ed->tq[0], qL);
+            edge_side_values(ed, 1, ed->tq[1], qR);
+            if (m->ns)
+                for (int d = 0; d < 2; ++d) {
+                    edge_side_values(ed, 0, ed->tdq[0] + 4 * PL * d, dL[d]);
+                    edge_side_values(ed, 1, ed->tdq[1] + 4 * PR * d, dR[d]);
+                }
+            for (int i = 0; i < P; ++i) {
+                const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+                double dS = ed->curved ? ed->dS[i] : ed->dS[0];
+                double Fi[4];
+                rdefault(ph, qL + 4 * i, qR + 4 * i, nn, Fi);
+                if (m->ns) {
+                    double a[8], b[8], Fv[4], Fa[4];
+                    for (int d = 0; d < 2; ++d)
+                        for (int eq = 0; eq < 4; ++eq) { a[eq + 4 * d] = dL[d][eq + 4 * i]; b[eq + 4 * d] = dR[d][eq + 4 * i]; }
+                    visc_interior(m, ed, qL + 4 * i, qR + 4 * i, a, b, nn, Fv);
+                    laplace_face(m, ed, qL + 4 * i, qR + 4 * i, Fa);
+                    for (int eq = 0; eq < 4; ++eq) Fs[eq + 4 * i] = ((Fi[eq] - Fv[eq]) - Fa[eq]) * dS;
+                } else {
+                    for (int eq = 0; eq < 4; ++eq) Fs[eq + 4 * i] = Fi[eq] * dS;
+                }
+            }
+            edge_flux_to_side(ed, 0, Fs, FL);
+            edge_flux_to_side(ed, 1, Fs, FR);
+            std_face(m, &m->el[ed->quads[0]], ed->loc[0], FL);
+            std_face(m, &m->el[ed->quads[1]], ed->loc[1], FR);
+        } else if (!ed->bdry) {
             /* ComputeRiemannSolver_InteriorEdge :545-653 / InteriorCurvedEdge :655-759 */
             for (int i = 0; i < P; ++i) {
                 int ir = ed->inverse ? P - 1 - i : i;
@@ -1890,6 +2021,7 @@ static void time_derivative(Mesh *m) {
             for (int i = 0; i < P; ++i)
                 for (int eq = 0; eq < 4; ++eq) m->QDot[QIDX(e, eq, i, j)] = m->QDot[QIDX(e, eq, i, j)] / e->jac[i + P * j];
     }
+    m->traces_valid = 1;
 }
 
 /* ======================================================================== */
@@ -1915,6 +2047,12 @@ void *or_create(const MeshIn *in, const OrOptions *o) {
     if (!riemann_fn(m->riemann)) { mesh_free(m); fail("unknown Riemann solver"); return NULL; }
     m->ad = m->ns ? o->ad : 0;   /* the module is Navier-Stokes only */
     m->ad_ceps = o->ad_ceps;
+    for (int k = 0; k < m->E; ++k)
+        if (m->ed[k].low >= 0 && ((m->ns && m->viscous == VISC_IP) || m->ad)) {
+            mesh_free(m);
+            fail("p-non-conforming edges: the Euler and BR1 forms without artificial dissipation only");
+            return NULL;
+        }
     m->mu_e = (double *)calloc(m->K + 1, sizeof(double));
     m->lap_e = (double *)calloc(m->K + 1, sizeof(double));
     for (int k = 0; k < m->E; ++k) m->ed[k].mu_a = 0.0;
@@ -1924,7 +2062,12 @@ void or_destroy(void *h) { mesh_free((Mesh *)h); }
 int64_t or_ndof(void *h) { return ((Mesh *)h)->ndof; }
 int or_num_edges(void *h) { return ((Mesh *)h)->E; }
 
-int or_set_state(void *h, const double *Q) { Mesh *m = h; memcpy(m->Q, Q, sizeof(double) * m->ndof); return 0; }
+int or_set_state(void *h, const double *Q) {
+    Mesh *m = h;
+    memcpy(m->Q, Q, sizeof(double) * m->ndof);
+    m->traces_valid = 0;
+    return 0;
+}
 int or_get_state(void *h, double *Q) { Mesh *m = h; memcpy(Q, m->Q, sizeof(double) * m->ndof); return 0; }
 int or_get_qdot(void *h, double *Q) { Mesh *m = h; memcpy(Q, m->QDot, sizeof(double) * m->ndof); return 0; }
 int or_get_grad(void *h, double *dQ) { Mesh *m = h; memcpy(dQ, m->dQ, sizeof(double) * 2 * m->ndof); return 0; }
@@ -1980,6 +2123,20 @@ int or_get_edge_geometry(void *h, int maxP, double *nx, double *ny, double *dS, 
             nx[k * maxP + i] = ed->n[2 * c]; ny[k * maxP + i] = ed->n[2 * c + 1]; dS[k * maxP + i] = ed->dS[c];
         }
         invh[k] = ed->invh;
+    }
+    return 0;
+}
+
+/* The p-non-conforming data of edge k: low side (-1 conforming), orders, T_forward and T_backward. */
+int or_edge_mortar(void *h, int k, int *low, int *N, int *NLow, double *Tf, double *Tb) {
+    Mesh *m = (Mesh *)h;
+    if (k < 0 || k >= m->E) return fail("bad edge");
+    Edge *ed = &m->ed[k];
+    *low = ed->low; *N = ed->N; *NLow = ed->NLow;
+    if (ed->low >= 0) {
+        int n = (ed->N + 1) * (ed->NLow + 1);
+        memcpy(Tf, ed->Tf, sizeof(double) * n);
+        memcpy(Tb, ed->Tb, sizeof(double) * n);
     }
     return 0;
 }
@@ -2162,12 +2319,19 @@ int or_surface_integrals(void *h, int marker, double *out) {
     Mesh *m = h;
     const Phys *ph = &m->ph;
     if (marker < 1 || marker > m->nmarkers) return fail("or_surface_integrals: bad marker");
-    prolong(m, 0);
-    if (m->ns) {
-        update_bcs(m);
-        if (m->viscous == VISC_BR1) gradient_br1(m);
-        else gradient_ip(m);
-        prolong(m, 1);
+    /* The integrals read the edge storage (QuadMesh_Integrals.incf:109-314), i.e. the traces and
+     * gradients of the last derivative evaluation: after a time step those of its last stage's
+     * input state, not of the updated state.  Only when the state was set after the last
+     * evaluation are they formed from the current state. */
+    if (!m->traces_valid) {
+        prolong(m, 0);
+        if (m->ns) {
+            update_bcs(m);
+            if (m->viscous == VISC_BR1) gradient_br1(m);
+            else gradient_ip(m);
+            prolong(m, 1);
+        }
+        m->traces_valid = 1;
     }
     for (int k = 0; k < 9; ++k) out[k] = 0.0;
     const Zone *Z = &m->zones[marker - 1];

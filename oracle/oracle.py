@@ -85,6 +85,7 @@ def lib():
         _lib.or_get_connectivity.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
         _lib.or_get_edge_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
         _lib.or_time_derivative.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        _lib.or_edge_mortar.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.or_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _lib.or_estimate_dt.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
         _lib.or_reset_dt_state.argtypes = [ctypes.c_void_p]
@@ -224,6 +225,15 @@ class Oracle:
 
     def grad(self):
         Q = np.zeros(2 * self.ndof); lib().or_get_grad(self.h, _ptr(Q)); return Q
+
+    def edge_mortar(self, k: int):
+        """(low side or -1, N, NLow, T_forward (N+1, NLow+1), T_backward (NLow+1, N+1)) of edge k."""
+        low, N, NL = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        Tf = np.zeros(17 * 17); Tb = np.zeros(17 * 17)
+        rc = lib().or_edge_mortar(self.h, k, ctypes.byref(low), ctypes.byref(N), ctypes.byref(NL), _ptr(Tf), _ptr(Tb))
+        assert rc == 0
+        n, nl = N.value + 1, NL.value + 1
+        return low.value, N.value, NL.value, Tf[:n * nl].reshape(n, nl), Tb[:n * nl].reshape(nl, n)
 
     def time_derivative(self, t: float = 0.0):
         lib().or_time_derivative(self.h, t)

@@ -31,6 +31,7 @@ MESHES = {
     "kelvin_helmholtz": "TestCases/Cases/KelvinHelmholtz/MESH/KelvinHelmholtz.HiOMesh",
     "ramp": "TestCases/Benchmark/Simulations/Ramp/MESH/Ramp.HiOMesh",
     "cylinder_benchmark": "TestCases/Benchmark/Simulations/CylinderMach2/MESH/Cylinder.HiOMesh",
+    "cylinder_re100": "TestCases/Benchmark/Simulations/CylinderRe100/MESH/Cylinder.HiOMesh",
 }
 
 # benchmark case -> (case file, mesh fixture, problem file)
@@ -40,6 +41,8 @@ BENCHMARKS = {
     "cylinder_mach2": ("TestCases/Benchmark/Simulations/CylinderMach2/SETUP/CylinderSupersonic.HiOCase",
                        "cylinder_benchmark",
                        "TestCases/Benchmark/Simulations/CylinderMach2/SETUP/ProblemFile.f90"),
+    "cylinder_re100": ("TestCases/Benchmark/Simulations/CylinderRe100/SETUP/Cylinder.HiOCase", "cylinder_re100",
+                       "TestCases/Benchmark/Simulations/CylinderRe100/SETUP/ProblemFile.f90"),
 }
 
 
@@ -73,6 +76,8 @@ def case_parameters(case_file, mesh_file):
     c = hio.case_from_hiocase(case_file, mesh_path=mesh_file, navier_stokes=True)
     return {
         "Mach": c.Mach, "Re": c.Re, "Pr": c.Pr, "N": sorted(set(int(n) for n in c.N)),
+        # per element, when the p-refinement block gives the elements different orders
+        "orders": [int(n) for n in c.N] if len(set(int(n) for n in c.N)) > 1 else None,
         "riemann": c.riemann, "viscous": c.viscous, "ip_method": c.ip_method, "sigma0": c.sigma0,
         "artificial_dissipation": c.artificial_dissipation, "ad_intensity": c.ad_intensity,
         "scheme": c.scheme, "mode": c.mode, "cfl": c.cfl, "dt_max": c.dt_max, "steps": c.steps,

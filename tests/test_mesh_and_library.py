@@ -56,8 +56,30 @@ def periodic_reference_vortex_mesh():
                   ic=C.isentropic_vortex(0.5, 1.0, 0.5, 0.5))
 
 
+def mixed_order_channel():
+    """A skewed channel of random orders with rotated element node lists (p-non-conforming
+    edges: the elements' mappings evaluate each interior edge at their own nodes)."""
+    q = C.freestream_state(0.3, 0.1)
+    mesh = C.channel(5, 4, 3.0, 1.0, [C.Zone(C.BC_FARFIELD, q=tuple(q)) for _ in range(4)], skew=0.3)
+    rng = np.random.default_rng(8)
+    for e in range(mesh.K):
+        mesh.quads[e] = np.roll(mesh.quads[e], int(rng.integers(0, 4)))
+    return C.Case("mixed_orders", mesh, rng.integers(1, 9, size=mesh.K).astype(np.int32), 0.3, ic=C.uniform(q))
+
+
+def reference_cylinder_re100():
+    """The reference's CylinderRe100 mesh with its four p-refinement zones (orders 2, 4, 6, 8)."""
+    d = np.load(os.path.join(GOLDEN, "mesh_cylinder_re100.npz"), allow_pickle=False)
+    case = reference_mesh_case("cylinder_re100")
+    orders = {1: 2, 2: 8, 3: 6, 4: 4}   # CylinderRe100/SETUP/Cylinder.HiOCase p-Refinement block
+    case.N = np.array([orders[int(z)] for z in d["pref_zones"]], np.int32)
+    return case
+
+
 CASES = {
     "periodic_square": lambda: C.euler_vortex_case(7, 3),
+    "mixed_orders_channel": mixed_order_channel,
+    "ref_cylinder_re100": reference_cylinder_re100,
     "tgv": lambda: C.ns_tgv_case(5, 4),
     "annulus": lambda: C.ns_cylinder_case(3, 10, N=4),
     "ref_vortex_transport": periodic_reference_vortex_mesh,
