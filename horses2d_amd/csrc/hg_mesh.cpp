@@ -452,6 +452,9 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
     for (int k = 0; k < E; ++k) {
         Edge &ed = m.ed[k];
         if (!ed.bdry) continue;
+        // an element edge with no neighbour that is no boundary edge of the file: a subdivided
+        // edge (hanging node), which MeshFile.f90:364-490 merges; not built here
+        if (ed.marker == 0) return "subdivided edges (hanging nodes) are not supported";
         if (ed.marker < 1 || ed.marker > in.nmarkers) return "boundary edge without a valid marker";
         m.zones[ed.marker - 1].edges.push_back(k);
     }

@@ -1067,7 +1067,11 @@ static Mesh *mesh_build(const MeshIn *in) {
         Z1->associated = Z2->associated = 1;
     }
     for (int k = 0; k < E; ++k)
-        if (m->ed[k].bdry && m->ed[k].marker == 0) { mesh_free(m); fail("boundary edge without marker"); return NULL; }
+        if (m->ed[k].bdry && m->ed[k].marker == 0) {
+            /* an element edge with no neighbour and no boundary marker: a subdivided edge
+             * (MeshFile.f90:364-490), not restated */
+            mesh_free(m); fail("subdivided edges (hanging nodes) are not supported"); return NULL;
+        }
     m->Q = (double *)calloc(m->ndof, sizeof(double));
     m->QDot = (double *)calloc(m->ndof, sizeof(double));
     m->dQ = (double *)calloc(2 * m->ndof, sizeof(double));

@@ -166,3 +166,18 @@ def test_boundary_condition_kinds_accepted_and_refused(gpu_lib):
     for kind in (8, 10):
         with pytest.raises(RuntimeError, match="unsupported boundary condition"):
             gpu.build_connectivity(channel(kind))
+
+
+def test_subdivided_edges_are_refused_by_both_builders():
+    """A hanging node (one element's edge meets two elements, MeshFile.f90:364-490): not built;
+    the product's builder and the oracle stop with the same message."""
+    pts = np.array([[0, 0], [1, 0], [2, 0], [2, 1], [2, 2], [1, 2], [0, 2], [1, 1]], np.float64)
+    quads = np.array([[0, 1, 5, 6], [1, 2, 3, 7], [7, 3, 4, 5]], np.int32)
+    bedges = np.array([[0, 1], [1, 2], [2, 3], [3, 4], [4, 5], [5, 6], [6, 0]], np.int32)
+    q = C.freestream_state(0.3)
+    mesh = C.HiOMesh(pts, quads, bedges, np.ones(7, np.int32), [C.Zone(C.BC_FARFIELD, q=tuple(q))])
+    case = C.Case("hanging", mesh, np.full(3, 3, np.int32), 0.3, ic=C.uniform(q))
+    with pytest.raises(RuntimeError, match="subdivided edges"):
+        gpu.build_connectivity(case)
+    with pytest.raises(RuntimeError, match="subdivided edges"):
+        O.Oracle(case)
