@@ -249,7 +249,8 @@ int run_stage(hg_handle *h, double am, double cq, double cg, bool first, bool la
     a.dt_cg = dt_cg;
     HG_CHECK(timed(h, 0, [&] { return hg::launch_stage(h->P, h->tile[0], h->mixed, a, h->stream); }));
     if (last)
-        HG_CHECK(hg::launch_resid_final(h->resid_part, hg::stage_blocks(h->P, h->tile[0], h->K), h->resid, h->stream));
+        HG_CHECK(hg::launch_resid_final(h->resid_part, hg::stage_blocks(h->P, h->tile[0], h->K), h->resid, h->stream,
+                                        h->resid_scratch));
     if (mode == 0) h->cur ^= 1;
     return 0;
 }
@@ -693,6 +694,7 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
     HG_CHECK(h->alloc(&h->dt_dev, 1));
     HG_CHECK(h->alloc(&h->resid, 4));
     HG_CHECK(h->alloc(&h->resid_part, 4 * (size_t)K));   // at least one element per workgroup
+    HG_CHECK(h->alloc(&h->resid_scratch, 4 * (size_t)hg::kResidGroups));
     HG_CHECK(hipMemset(h->resid, 0, 4 * sizeof(double)));
     HG_CHECK(hipMemset(h->Q, 0, h->ndof * sizeof(double)));
     HG_CHECK(hipMemset(h->G, 0, h->ndof * sizeof(double)));

@@ -169,8 +169,10 @@ hipError_t launch_stage(int P, int tile, bool mixed, const StageArgs &a, hipStre
 hipError_t launch_gradient(int P, int tile, bool mixed, const StageArgs &a, hipStream_t s);
 // Workgroups of the stage kernel (= residual partials it writes) for a tile choice.
 int stage_blocks(int P, int tile, int K);
-// Reduces nb residual partials to out[4] (NaN propagates).
-hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream_t s);
+// Reduces nb residual partials to out[4] (NaN propagates); with scratch (4 * kResidGroups
+// doubles) many partials are reduced by several workgroups first.
+constexpr int kResidGroups = 256;
+hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream_t s, double *scratch = nullptr);
 
 // Time-step estimate: per element maxima, then an ordered prefix-max scan.
 hipError_t launch_dt_elem(int P, const double *Q, int K, double gamma, double gm1, double *umax, double *amax,

@@ -4,6 +4,7 @@
 // hg_inst.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "hg_device.h"
@@ -14,21 +15,23 @@ namespace {
 
 __device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
 
-// max|QDot| per equation over the stage kernel's per-workgroup partials (4 doubles each), one
-// workgroup: every lane keeps four partials' loads in flight, then wavefront shuffles and LDS.
-// The maximum does not depend on the order (NaN wins every compare), so the result is exact.
+// max|QDot| per equation over the stage kernel's per-workgroup partials (4 doubles each):
+// workgroup g takes the partials g*1024 + t, stepping by the grid, every lane keeping four
+// partials' loads in flight, then wavefront shuffles and LDS; it writes its four maxima to
+// out[4 g].  The maximum does not depend on the order (NaN wins every compare), so the result
+// is exact whatever the grid.
 __global__ __launch_bounds__(1024) void k_resid_final(const double *part, int nb, double *out) {
     __shared__ double s[16][4];
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, stride = 1024 * gridDim.x;
     const double2 *p2 = (const double2 *)part;
     double m[4] = {0.0, 0.0, 0.0, 0.0};
-    int b = t;
-    for (; b + 3 * 1024 < nb; b += 4 * 1024) {
+    int b = blockIdx.x * 1024 + t;
+    for (; b + 3 * stride < nb; b += 4 * stride) {
         double2 v[8];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            v[2 * u] = p2[(int64_t)(b + u * 1024) * 2];
-            v[2 * u + 1] = p2[(int64_t)(b + u * 1024) * 2 + 1];
+            v[2 * u] = p2[(int64_t)(b + u * stride) * 2];
+            v[2 * u + 1] = p2[(int64_t)(b + u * stride) * 2 + 1];
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -36,7 +39,7 @@ __global__ __launch_bounds__(1024) void k_resid_final(const double *part, int nb
             m[2] = nan_max(m[2], v[2 * u + 1].x); m[3] = nan_max(m[3], v[2 * u + 1].y);
         }
     }
-    for (; b < nb; b += 1024) {
+    for (; b < nb; b += stride) {
         const double2 a = p2[(int64_t)b * 2], c = p2[(int64_t)b * 2 + 1];
         m[0] = nan_max(m[0], a.x); m[1] = nan_max(m[1], a.y); m[2] = nan_max(m[2], c.x); m[3] = nan_max(m[3], c.y);
     }
@@ -51,7 +54,7 @@ __global__ __launch_bounds__(1024) void k_resid_final(const double *part, int nb
     if (t < 4) {
         double v = s[0][t];
         for (int w = 1; w < 16; ++w) v = nan_max(v, s[w][t]);
-        out[t] = v;
+        out[blockIdx.x * 4 + t] = v;
     }
 }
 
@@ -257,8 +260,19 @@ hipError_t launch_gradient(int P, int tile, bool mixed, const StageArgs &a, hipS
 #undef HG_CALL
 }
 
-hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(1024), 0, st, part, nb, out);
+hipError_t launch_resid_final(const double *part, int nb, double *out, hipStream_t st, double *scratch) {
+    // one workgroup (one lane per partial) when that covers them; beyond, a first pass of up to
+    // kResidGroups workgroups into scratch, then one workgroup over their maxima (one workgroup
+    // over 64 k partials, the 256 x 256 order-7 mesh, took 37 us of a 250 us stage)
+    const int groups = std::min(kResidGroups, (nb + 1023) / 1024);
+    if (groups <= 1 || !scratch) {
+        hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(1024), 0, st, part, nb, out);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_resid_final, dim3(groups), dim3(1024), 0, st, part, nb, scratch);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(1024), 0, st, scratch, groups, out);
     return hipGetLastError();
 }
 

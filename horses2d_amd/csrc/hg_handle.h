@@ -70,6 +70,7 @@ struct hg_handle {
     double *mu_a = nullptr, *evol = nullptr, *earea = nullptr;
     bool lap = false;
     double *resid = nullptr, *resid_part = nullptr;  // max|QDot| of the last stage; per-workgroup partials
+    double *resid_scratch = nullptr;                 // first-pass maxima of a many-workgroup reduction
     // p-non-conforming faces of this handle: per face node the mortar kernels' projected flux,
     // BR1 uStar, and this side's gradient trace (null when every edge is conforming)
     double *mflux = nullptr, *mustar = nullptr, *mdq = nullptr;
