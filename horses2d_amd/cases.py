@@ -186,6 +186,59 @@ def channel(nx: int, ny: int, L: float, H: float, zones: List[Zone], skew: float
     return HiOMesh(points, quads, bedges, bmarker, list(zones))
 
 
+def refined_channel(nx: int, ny: int, L: float, H: float, zones: List[Zone], refine, skew: float = 0.0) -> HiOMesh:
+    """The channel of ``channel`` with the cells (i, j) in ``refine`` split into 2 x 2 quads: where
+    a split cell meets an unsplit one, the unsplit element's edge meets two elements and their
+    common node hangs on it (the subdivided edges of MeshFile.f90:364-490).  The smooth skew map
+    moves every node, the hanging ones too, so the reference's builder has to move them back
+    onto the big edge (QuadMesh_Construct.incf:168-173).  Markers 1 bottom, 2 right, 3 top,
+    4 left."""
+    refine = set((int(i), int(j)) for i, j in refine)
+    ids = {}
+
+    def node(a, b):          # lattice point (a, b) of the 2nx x 2ny half-cell lattice
+        if (a, b) not in ids:
+            ids[(a, b)] = len(ids)
+        return ids[(a, b)]
+
+    quads = []
+    for i in range(nx):
+        for j in range(ny):
+            a, b = 2 * i, 2 * j
+            if (i, j) in refine:
+                for di in (0, 1):
+                    for dj in (0, 1):
+                        quads.append((node(a + di, b + dj), node(a + di + 1, b + dj), node(a + di + 1, b + dj + 1),
+                                      node(a + di, b + dj + 1)))
+            else:
+                quads.append((node(a, b), node(a + 2, b), node(a + 2, b + 2), node(a, b + 2)))
+
+    def side(cells, first, step):
+        out = []
+        for c, (a, b) in zip(cells, first):
+            da, db = step
+            if c in refine:
+                out += [(node(a, b), node(a + da, b + db)), (node(a + da, b + db), node(a + 2 * da, b + 2 * db))]
+            else:
+                out.append((node(a, b), node(a + 2 * da, b + 2 * db)))
+        return out
+    bottom = side([(i, 0) for i in range(nx)], [(2 * i, 0) for i in range(nx)], (1, 0))
+    right = side([(nx - 1, j) for j in range(ny)], [(2 * nx, 2 * j) for j in range(ny)], (0, 1))
+    top = side([(i, ny - 1) for i in range(nx)], [(2 * i + 2, 2 * ny) for i in range(nx)], (-1, 0))
+    left = side([(0, j) for j in range(ny)], [(0, 2 * j + 2) for j in range(ny)], (0, -1))
+    lat = np.zeros((len(ids), 2))
+    for (a, b), k in ids.items():
+        lat[k] = (a, b)
+    X = L * lat[:, 0] / (2 * nx)
+    Y = H * lat[:, 1] / (2 * ny)
+    bump = np.sin(np.pi * X / L) * np.sin(np.pi * Y / H)
+    X = X + skew * (L / nx) * bump
+    Y = Y + skew * (H / ny) * bump * np.cos(np.pi * X / L)
+    bedges = np.array(bottom + right + top + left, dtype=np.int32)
+    bmarker = np.array([1] * len(bottom) + [2] * len(right) + [3] * len(top) + [4] * len(left), dtype=np.int32)
+    return HiOMesh(np.stack([X, Y], axis=1), np.array(quads, dtype=np.int32), bedges, bmarker, list(zones))
+
+
 def annulus(nr: int, nt: int, r0: float, r1: float, q_inf, wall_velocity=(0.0, 0.0),
             stretch: bool = True, np1: int = 3) -> HiOMesh:
     """O-grid annulus r in [r0, r1]: every azimuthal edge is a curved (quadratic) arc.
@@ -256,6 +309,27 @@ def isentropic_vortex(Mach: float, beta: float = 5.0, xc: float = 0.0, yc: float
         u = Mach * np.sqrt(GAMMA) - beta / (2.0 * np.pi) * dy * f
         v = beta / (2.0 * np.pi) * dx * f
         T = 1.0 - GM1 * beta * beta / (8.0 * GAMMA * np.pi * np.pi) * f * f
+        rho = T ** (1.0 / GM1)
+        return _pack(rho, u, v, rho * T)
+    return ic
+
+
+def gaussian_vortex(Mach: float, radius: float = 0.1, beta: float = 0.05, xc: float = 0.0, yc: float = 0.0):
+    """The vortex of the reference's VortexTransport benchmark
+    (TestCases/Benchmark/Simulations/VortexTransport/SETUP/ProblemFile.f90:36-76), in the
+    dimensionless variables its initial-condition driver stores (rho / rho_ref, rho u / (rho_ref
+    a_ref), rho e / p_ref): velocity V (1 - beta dy / R e^(-r2/2), beta dx / R e^(-r2/2)) with
+    r2 = (dx^2 + dy^2) / R^2, temperature T_ref (1 - gamma M^2 beta^2 / (2 cp) e^(-r2)) with the
+    dimensionless cp = gamma / (gamma - 1), isentropic density (T / T_ref)^(1 / (gamma - 1))."""
+    V = Mach * np.sqrt(GAMMA)
+    cp = GAMMA / GM1
+
+    def ic(x, y):
+        dx, dy = x - xc, y - yc
+        r2 = (dx * dx + dy * dy) / (radius * radius)
+        u = V * (1.0 - beta * dy / radius * np.exp(-0.5 * r2))
+        v = V * (beta * dx / radius * np.exp(-0.5 * r2))
+        T = 1.0 - GAMMA * Mach * Mach * beta * beta / (2.0 * cp) * np.exp(-r2)
         rho = T ** (1.0 / GM1)
         return _pack(rho, u, v, rho * T)
     return ic

@@ -572,9 +572,26 @@ static void adiabatic_viscous_flux(const Phys *ph, const double *q, const double
 /* ======================================================================== */
 /* Mesh                                                                      */
 /* ======================================================================== */
+/* A subdivided edge (SubdividedEdge_t): its two mortars (NORTH: hanging node to node 2, SOUTH:
+ * node 1 to the hanging node), their orders N_N, N_S, and the eight transformation matrices
+ * (SubdividedEdge_ComputeMortarsTransformationMatrices, QuadElement_EdgeProcedures.incf:530-634),
+ * row-major [rows][cols]; the RIGHT_SOUTH element's traces (the third storage). */
+typedef struct {
+    int NN, NS;
+    double *TLN_F, *TLS_F, *TLN_B, *TLS_B, *TRN_F, *TRN_B, *TRS_F, *TRS_B;
+    /* the mortars' normals and surface Jacobians per mortar node (QuadElement_EdgeMappings.incf:450-565):
+     * straight, the edge's n and 0.5 dS at every node; curved, from the edge's dS n interpolated at
+     * 0.5 + 0.5 xi_N and 0.5 xi_S */
+    double nN[2 * MAXP], nS[2 * MAXP], dSN[MAXP], dSS[MAXP];
+    double tq2[NCONS * MAXP];
+    double tdq2[2 * NCONS * MAXP];
+} SubData;
+
 typedef struct {
     int id, bdry, curved;
-    int nodes[2];          /* node ids after linking inversions */
+    int sub;               /* subdivided edge: nodes[2] the hanging node, quads LEFT / RIGHT_NORTH / RIGHT_SOUTH */
+    SubData *sd;
+    int nodes[3];          /* node ids after linking inversions */
     int N, NLow, NCurve;
     /* p-non-conforming edge (Edge_LinkWithElements, QuadElement_EdgeProcedures.incf:105-124,
      * 154-168): the side (0 LEFT, 1 RIGHT) whose order NLow is below the edge's N, or -1;
@@ -582,8 +599,8 @@ typedef struct {
      * T_backward [i*(N+1) + l] = Minv_low T_forward^T M_edge (low node i from edge node l) */
     int low;
     double *Tf, *Tb;
-    int quads[2];          /* [0] LEFT element (lowest id) / boundary element, [1] RIGHT */
-    int loc[2];            /* edge location 1..4 in quads[0], quads[1] */
+    int quads[3];          /* [0] LEFT element (lowest id) / boundary element, [1] RIGHT (subdivided: [1] NORTH, [2] SOUTH) */
+    int loc[3];            /* edge location 1..4 in quads[0..2] */
     int inverse;
     int marker;
     double *X, *dX, *n, *dS; /* X,dX,n: 2*P (straight: dX,n,dS only [0]) */
@@ -748,11 +765,29 @@ static void edge_set_curve(Mesh *m, Edge *ed, const double *cpts /* curved point
     }
 }
 
+/* EvaluateX of an edge at x, forwards: Edge_AnalyticalX from the nodes, CurvedEdge_InterpolantX
+ * from the edge's points (QuadElement_EdgeMappings.incf:52-78, 355-380). */
+static void edge_point_at(const Mesh *m, const Edge *ed, double x, double *X) {
+    if (!ed->curved) {
+        const double *n1 = m->pts + 2 * ed->nodes[0], *n2 = m->pts + 2 * ed->nodes[1];
+        for (int d = 0; d < 2; ++d) X[d] = n1[d] * (1.0 - x) + n2[d] * x;
+        return;
+    }
+    const Spec *s = spec_get(ed->N);
+    double lj[MAXP];
+    lagrange_barycentric(x, ed->N, s->xi, s->wb, lj);
+    for (int d = 0; d < 2; ++d) {
+        double a = 0.0;
+        for (int j = 0; j <= ed->N; ++j) a = a + ed->X[2 * j + d] * lj[j];
+        X[d] = a;
+    }
+}
+
 /* Edge geometry seen by an element of order eN at its local coordinate index iXi
  * (QuadElement_EdgeMappings.incf:52-177, 262-442): interior edges are evaluated at the
  * element's node xi(iXi) (QuadElement_QuadMappings.incf:55-62), which is not one of the edge's
  * nodes when the edge takes the higher order of its two elements. */
-static void edge_eval_X(Mesh *m, Edge *ed, int eN, int iXi, int dirn, double *X, double *dX) {
+static void edge_eval_X(Mesh *m, Edge *ed, int eN, int iXi, int dirn, int qpos, double *X, double *dX) {
     const Spec *s = spec_get(ed->N);
     int P = ed->N + 1;
     if (ed->bdry) { /* getX/getdX */
@@ -760,6 +795,29 @@ static void edge_eval_X(Mesh *m, Edge *ed, int eN, int iXi, int dirn, double *X,
         X[0] = ed->X[2 * ci]; X[1] = ed->X[2 * ci + 1];
         if (!ed->curved) { dX[0] = (double)dirn * ed->dX[0]; dX[1] = (double)dirn * ed->dX[1]; }
         else { dX[0] = (double)dirn * ed->dX[2 * ci]; dX[1] = (double)dirn * ed->dX[2 * ci + 1]; }
+        return;
+    }
+    if (ed->sub) {
+        /* a subdivided edge, always evaluated forwards: the big element on [0, 1], the NORTH
+         * element on [0.5, 1], the SOUTH one on [0, 0.5], a backward small element at its nodes
+         * reversed, dX scaled by the half and the direction (QuadElement_QuadMappings.incf:66-115) */
+        const double *xs = spec_get(eN)->xi;
+        double xv = xs[iXi], scal = 1.0;
+        if (qpos == 1) { xv = 0.5 + 0.5 * (dirn == BACKWARD ? xs[eN - iXi] : xs[iXi]); scal = dirn == BACKWARD ? -0.5 : 0.5; }
+        else if (qpos == 2) { xv = 0.5 * (dirn == BACKWARD ? xs[eN - iXi] : xs[iXi]); scal = dirn == BACKWARD ? -0.5 : 0.5; }
+        edge_point_at(m, ed, xv, X);
+        if (!ed->curved) {
+            const double *n1 = m->pts + 2 * ed->nodes[0], *n2 = m->pts + 2 * ed->nodes[1];
+            for (int d = 0; d < 2; ++d) dX[d] = scal * (n2[d] - n1[d]);
+        } else {
+            double lj[MAXP];
+            lagrange_barycentric(xv, ed->N, s->xi, s->wb, lj);
+            for (int d = 0; d < 2; ++d) {
+                double b = 0.0;
+                for (int j = 0; j < P; ++j) b = b + ed->dX[2 * j + d] * lj[j];
+                dX[d] = scal * b;
+            }
+        }
         return;
     }
     double xi = spec_get(eN)->xi[iXi];
@@ -792,7 +850,7 @@ static void elem_set_mappings(Mesh *m, Elem *e) {
         Edge *ed = &m->ed[e->edges[pos - 1]];
         for (int i = 0; i < P; ++i) {
             double X[2], dX[2];
-            edge_eval_X(m, ed, e->N, i, e->dir[pos - 1], X, dX);
+            edge_eval_X(m, ed, e->N, i, e->dir[pos - 1], e->qpos[pos - 1], X, dX);
             for (int j = 0; j < P; ++j)
                 for (int d = 0; d < 2; ++d) { XE(d, i, j, pos - 1) = X[d]; DXE(d, i, j, pos - 1) = dX[d]; }
         }
@@ -801,7 +859,7 @@ static void elem_set_mappings(Mesh *m, Elem *e) {
         Edge *ed = &m->ed[e->edges[pos - 1]];
         for (int j = 0; j < P; ++j) {
             double X[2], dX[2];
-            edge_eval_X(m, ed, e->N, j, e->dir[pos - 1], X, dX);
+            edge_eval_X(m, ed, e->N, j, e->dir[pos - 1], e->qpos[pos - 1], X, dX);
             for (int i = 0; i < P; ++i)
                 for (int d = 0; d < 2; ++d) { XE(d, i, j, pos - 1) = X[d]; DXE(d, i, j, pos - 1) = dX[d]; }
         }
@@ -839,11 +897,200 @@ static void mesh_free(Mesh *m) {
     for (int i = 0; i < m->E; ++i) {
         free(m->ed[i].X); free(m->ed[i].dX); free(m->ed[i].n); free(m->ed[i].dS);
         free(m->ed[i].Tf); free(m->ed[i].Tb);
+        if (m->ed[i].sd) {
+            SubData *d = m->ed[i].sd;
+            free(d->TLN_F); free(d->TLS_F); free(d->TLN_B); free(d->TLS_B);
+            free(d->TRN_F); free(d->TRN_B); free(d->TRS_F); free(d->TRS_B); free(d);
+        }
     }
     for (int z = 0; z < m->nmarkers; ++z) free(m->zones[z].edges);
     free(m->el); free(m->ed); free(m->zones); free(m->pts);
     free(m->Q); free(m->QDot); free(m->dQ); free(m->G); free(m->mu_e); free(m->lap_e);
     free(m);
+}
+
+/* The points_of_edges list (IntegerArrayLinkedList.f90) as arrays: entry k holds fnn[k] node
+ * ids fn[3k..] (2, or 3 for a merged subdivided edge) and a boundary attribute fat[k]. */
+static int entry_has(const int *fn, const int *fnn, int k, int node) {
+    for (int c = 0; c < fnn[k]; ++c)
+        if (fn[3 * k + c] == node) return 1;
+    return 0;
+}
+
+/* IntegerArrayLinkedList_SearchIfPresent (:489-567): the first `how` entries holding `node`,
+ * -1 padded; their attributes. */
+static void search_if_present(const int *fn, const int *fnn, const int *fat, int E, int node, int how, int *pos,
+                              int *att) {
+    int c = 0;
+    for (int k = 0; k < E && c < how; ++k)
+        if (entry_has(fn, fnn, k, node)) { pos[c] = k; if (att) att[c] = fat[k]; ++c; }
+    for (; c < how; ++c) { pos[c] = -1; if (att) att[c] = 0; }
+}
+
+/* IntegerArrayLinkedList_Search (:346-412) for a two-node entry, either order. */
+static int search_pair(const int *fn, const int *fnn, int E, int a, int b) {
+    for (int k = 0; k < E; ++k)
+        if (fnn[k] == 2 && ((fn[3 * k] == a && fn[3 * k + 1] == b) || (fn[3 * k] == b && fn[3 * k + 1] == a))) return k;
+    return -1;
+}
+
+static void remove_entry(int *fn, int *fnn, int *fat, int *E, int k) {
+    if (k < 0) return;
+    memmove(fn + 3 * k, fn + 3 * (k + 1), sizeof(int) * 3 * (*E - k - 1));
+    memmove(fnn + k, fnn + k + 1, sizeof(int) * (*E - k - 1));
+    memmove(fat + k, fat + k + 1, sizeof(int) * (*E - k - 1));
+    --*E;
+}
+
+/* mergeDividedFaces (MeshFile.f90:364-490), nodes in ascending order: a node with exactly three
+ * edges, none of them a boundary edge, whose neighbours are joined by a fourth edge is a hanging
+ * node; that edge and the two halves are removed and [n1, n2, hanging] appended. */
+static void merge_divided(int nnodes, int *fn, int *fnn, int *fat, int *E) {
+    for (int node = 0; node < nnodes; ++node) {
+        int e4[4], at[4];
+        search_if_present(fn, fnn, fat, *E, node, 4, e4, at);
+        if (e4[3] != -1 || e4[2] == -1 || at[0] || at[1] || at[2] || at[3]) continue;
+        int pts[3];
+        for (int j = 0; j < 3; ++j) {
+            const int *v = fn + 3 * e4[j];
+            pts[j] = v[0] != node ? v[0] : (v[1] != node ? v[1] : -1);
+        }
+        int done = 0;
+        for (int j = 0; j < 3 && !done; ++j) {
+            int ne[8];
+            search_if_present(fn, fnn, fat, *E, pts[j], 8, ne, NULL);
+            for (int k = 0; k < 8 && !done; ++k) {
+                if (ne[k] == -1) continue;
+                const int p0 = fn[3 * ne[k]], p1 = fn[3 * ne[k] + 1];
+                int point = -1;
+                if (p0 != pts[j] && p0 != node) point = p0;
+                else if (p1 != pts[j] && p1 != node) point = p1;
+                if (point != -1 && (point == pts[0] || point == pts[1] || point == pts[2])) {
+                    remove_entry(fn, fnn, fat, E, search_pair(fn, fnn, *E, p0, p1));
+                    remove_entry(fn, fnn, fat, E, search_pair(fn, fnn, *E, p0, node));
+                    remove_entry(fn, fnn, fat, E, search_pair(fn, fnn, *E, p1, node));
+                    fn[3 * *E] = p0; fn[3 * *E + 1] = p1; fn[3 * *E + 2] = node;
+                    fnn[*E] = 3; fat[*E] = 0; ++*E;
+                    done = 1;
+                }
+            }
+        }
+    }
+}
+
+/* SubdividedEdge_ConstructMortars / CurvedSubdividedEdge_ConstructMortars
+ * (QuadElement_EdgeMappings.incf:450-565), the edge's n and dS already oriented by the linking. */
+static void mortar_geometry(const Edge *ed, SubData *d) {
+    const Spec *sN = spec_get(d->NN), *sS = spec_get(d->NS), *s = spec_get(ed->N);
+    for (int half = 0; half < 2; ++half) {
+        const int NM = half == 0 ? d->NN : d->NS;
+        const Spec *sm = half == 0 ? sN : sS;
+        double *nrm = half == 0 ? d->nN : d->nS, *dS = half == 0 ? d->dSN : d->dSS;
+        for (int p = 0; p <= NM; ++p) {
+            if (!ed->curved) {
+                nrm[2 * p] = ed->n[0]; nrm[2 * p + 1] = ed->n[1];
+                dS[p] = 0.5 * ed->dS[0];
+                continue;
+            }
+            /* CurvedEdge_InterpolantdS: (dS n) interpolated at the mortar node's edge coordinate */
+            const double x = half == 0 ? 0.5 + 0.5 * sm->xi[p] : 0.5 * sm->xi[p];
+            double lj[MAXP], v[2];
+            lagrange_barycentric(x, ed->N, s->xi, s->wb, lj);
+            for (int c = 0; c < 2; ++c) {
+                double acc = 0.0;
+                for (int j = 0; j <= ed->N; ++j) acc = acc + (ed->dS[j] * ed->n[2 * j + c]) * lj[j];
+                v[c] = acc;
+            }
+            dS[p] = 0.5 * sqrt(v[0] * v[0] + v[1] * v[1]);
+            nrm[2 * p] = 0.5 * v[0] / dS[p];
+            nrm[2 * p + 1] = 0.5 * v[1] / dS[p];
+        }
+    }
+}
+
+/* SubdividedEdge_LinkWithElements (QuadElement_EdgeProcedures.incf:201-355) with
+ * SubdividedEdge_ConstructMortars (QuadElement_EdgeMappings.incf:450-497) and
+ * SubdividedEdge_ComputeMortarsTransformationMatrices (QuadElement_EdgeProcedures.incf:530-634).
+ * The big element is LEFT and forward; R1 (nodes 1 and h) and R2 (nodes 2 and h) become NORTH and
+ * SOUTH according to the inversion the LEFT element needs. */
+static int link_subdivided(Mesh *m, Edge *ed, int k) {
+    int needs, pos, dirn;
+    int ids[3] = {ed->nodes[0], ed->nodes[1], ed->nodes[2]};
+    Elem *L = &m->el[ed->quads[0]];
+    search_edge(L->nodes, ids, &needs, &pos, &dirn);
+    if (pos < 0) return fail("subdivided edge not found in its element");
+    L->edges[pos - 1] = k; L->dir[pos - 1] = FORWARD; L->qpos[pos - 1] = 0; ed->loc[0] = pos;
+    int rn, rs;
+    if (needs) {
+        rn = ed->quads[1]; rs = ed->quads[2];
+        edge_invert(ed, pos == EBOTTOM || pos == ERIGHT);
+        int t = ids[0]; ids[0] = ids[1]; ids[1] = t;
+    } else {
+        rn = ed->quads[2]; rs = ed->quads[1];
+        if (pos == ETOP || pos == ELEFT) { ed->n[0] = -ed->n[0]; ed->n[1] = -ed->n[1]; }
+    }
+    ed->quads[1] = rn; ed->quads[2] = rs;
+    SubData *d = (SubData *)calloc(1, sizeof(SubData));
+    ed->sd = d;
+    const int NL = L->N, NRN = m->el[rn].N, NRS = m->el[rs].N;
+    d->NN = NL > NRN ? NL : NRN;
+    d->NS = NL > NRS ? NL : NRS;
+    mortar_geometry(ed, d);
+    /* NORTH: [h, node 2]; SOUTH: [node 1, h] */
+    int sub_n[2] = {ids[2], ids[1]}, sub_s[2] = {ids[0], ids[2]};
+    Elem *RN = &m->el[rn], *RS = &m->el[rs];
+    int dn, ds;
+    search_edge(RN->nodes, sub_n, &needs, &pos, &dn);
+    if (pos < 0) return fail("subdivided edge not found in its north element");
+    RN->edges[pos - 1] = k; RN->dir[pos - 1] = dn; RN->qpos[pos - 1] = 1; ed->loc[1] = pos;
+    search_edge(RS->nodes, sub_s, &needs, &pos, &ds);
+    if (pos < 0) return fail("subdivided edge not found in its south element");
+    RS->edges[pos - 1] = k; RS->dir[pos - 1] = ds; RS->qpos[pos - 1] = 2; ed->loc[2] = pos;
+    /* the eight transformation matrices */
+    const Spec *sL = spec_get(NL), *sN = spec_get(d->NN), *sS = spec_get(d->NS), *sRN = spec_get(NRN), *sRS = spec_get(NRS);
+    const int PL = NL + 1, PN = d->NN + 1, PS = d->NS + 1, PRN = NRN + 1, PRS = NRS + 1;
+    double xs[MAXP], xn[MAXP];
+    for (int p = 0; p < PS; ++p) xs[p] = 0.5 * sS->xi[p];
+    for (int p = 0; p < PN; ++p) xn[p] = 0.5 + 0.5 * sN->xi[p];
+    d->TLS_F = (double *)malloc(sizeof(double) * PS * PL); interp_matrix(NL, d->NS, sL->xi, sL->wb, xs, d->TLS_F);
+    d->TLN_F = (double *)malloc(sizeof(double) * PN * PL); interp_matrix(NL, d->NN, sL->xi, sL->wb, xn, d->TLN_F);
+    d->TRS_F = (double *)malloc(sizeof(double) * PS * PRS); interp_matrix(NRS, d->NS, sRS->xi, sRS->wb, sS->xi, d->TRS_F);
+    d->TRN_F = (double *)malloc(sizeof(double) * PN * PRN); interp_matrix(NRN, d->NN, sRN->xi, sRN->wb, sN->xi, d->TRN_F);
+    d->TLS_B = (double *)malloc(sizeof(double) * PL * PS);
+    d->TLN_B = (double *)malloc(sizeof(double) * PL * PN);
+    d->TRS_B = (double *)malloc(sizeof(double) * PRS * PS);
+    d->TRN_B = (double *)malloc(sizeof(double) * PRN * PN);
+    for (int i = 0; i < PL; ++i) {
+        for (int l = 0; l < PS; ++l) d->TLS_B[i * PS + l] = ((1.0 / sL->w[i]) * d->TLS_F[l * PL + i]) * sS->w[l];
+        for (int l = 0; l < PN; ++l) d->TLN_B[i * PN + l] = ((1.0 / sL->w[i]) * d->TLN_F[l * PL + i]) * sN->w[l];
+    }
+    for (int i = 0; i < PRS; ++i)
+        for (int l = 0; l < PS; ++l) d->TRS_B[i * PS + l] = ((1.0 / sRS->w[i]) * d->TRS_F[l * PRS + i]) * sS->w[l];
+    for (int i = 0; i < PRN; ++i)
+        for (int l = 0; l < PN; ++l) d->TRN_B[i * PN + l] = ((1.0 / sRN->w[i]) * d->TRN_F[l * PRN + i]) * sN->w[l];
+    /* a small element running its half backwards: T_FWD times the reversal (its columns
+     * reversed), the reversal times T_BKW (its rows reversed) */
+    if (ds == BACKWARD) {
+        for (int i = 0; i < PS; ++i)
+            for (int l = 0; l < PRS / 2; ++l) {
+                double t = d->TRS_F[i * PRS + l]; d->TRS_F[i * PRS + l] = d->TRS_F[i * PRS + PRS - 1 - l]; d->TRS_F[i * PRS + PRS - 1 - l] = t;
+            }
+        for (int i = 0; i < PRS / 2; ++i)
+            for (int l = 0; l < PS; ++l) {
+                double t = d->TRS_B[i * PS + l]; d->TRS_B[i * PS + l] = d->TRS_B[(PRS - 1 - i) * PS + l]; d->TRS_B[(PRS - 1 - i) * PS + l] = t;
+            }
+    }
+    if (dn == BACKWARD) {
+        for (int i = 0; i < PN; ++i)
+            for (int l = 0; l < PRN / 2; ++l) {
+                double t = d->TRN_F[i * PRN + l]; d->TRN_F[i * PRN + l] = d->TRN_F[i * PRN + PRN - 1 - l]; d->TRN_F[i * PRN + PRN - 1 - l] = t;
+            }
+        for (int i = 0; i < PRN / 2; ++i)
+            for (int l = 0; l < PN; ++l) {
+                double t = d->TRN_B[i * PN + l]; d->TRN_B[i * PN + l] = d->TRN_B[(PRN - 1 - i) * PN + l]; d->TRN_B[(PRN - 1 - i) * PN + l] = t;
+            }
+    }
+    return 0;
 }
 
 /* MeshFile.f90:77-673 (connectivity), QuadMesh_Construct.incf:22-305
@@ -858,23 +1105,49 @@ static Mesh *mesh_build(const MeshIn *in) {
     m->el = (Elem *)calloc(in->K, sizeof(Elem));
     /* computeFaces: first appearance order over (element, face 1..4), O(E^2) search */
     int cap = 4 * in->K + 4;
-    int *fn = (int *)malloc(sizeof(int) * 2 * cap);
+    int *fn = (int *)malloc(sizeof(int) * 3 * cap), *fnn = (int *)malloc(sizeof(int) * cap);
+    int *fat = (int *)calloc(cap, sizeof(int));
     int E = 0;
     for (int e = 0; e < in->K; ++e)
         for (int f = 0; f < 4; ++f) {
             int face[2] = {in->quads[4 * e + f], in->quads[4 * e + (f + 1) % 4]};
             int found = 0;
             for (int k = 0; k < E; ++k)
-                if (same_face(fn + 2 * k, face)) { found = 1; break; }
-            if (!found) { fn[2 * E] = face[0]; fn[2 * E + 1] = face[1]; ++E; }
+                if (same_face(fn + 3 * k, face)) { found = 1; break; }
+            if (!found) { fn[3 * E] = face[0]; fn[3 * E + 1] = face[1]; fnn[E] = 2; ++E; }
         }
+    /* the entries that are boundary edges of the file carry the attribute (:337-360) */
+    for (int b = 0; b < in->nbedges; ++b) {
+        int k = search_pair(fn, fnn, E, in->bedges[2 * b], in->bedges[2 * b + 1]);
+        if (k >= 0) fat[k] = 1;
+    }
+    merge_divided(in->nnodes, fn, fnn, fat, &E);
     m->E = E;
     m->ed = (Edge *)calloc(E, sizeof(Edge));
-    /* computeElementOfEdges: elements containing both nodes, in element order */
+    /* computeElementOfEdges (:500-570): elements containing both nodes, in element order; a
+     * subdivided edge [n1, n2, h]: the element with n1 and n2, then n1 and h, then n2 and h
+     * (searchDividedEdgeInElements, :785-824) */
     for (int k = 0; k < E; ++k) {
         Edge *ed = &m->ed[k];
-        ed->id = k; ed->nodes[0] = fn[2 * k]; ed->nodes[1] = fn[2 * k + 1];
-        ed->quads[0] = ed->quads[1] = -1;
+        ed->id = k; ed->nodes[0] = fn[3 * k]; ed->nodes[1] = fn[3 * k + 1]; ed->nodes[2] = -1;
+        ed->quads[0] = ed->quads[1] = ed->quads[2] = -1;
+        ed->partner = -1;
+        if (fnn[k] == 3) {
+            ed->sub = 1;
+            ed->nodes[2] = fn[3 * k + 2];
+            for (int e = 0; e < in->K; ++e) {
+                int h0 = 0, h1 = 0, h2 = 0;
+                for (int c = 0; c < 4; ++c) {
+                    const int q = in->quads[4 * e + c];
+                    h0 |= q == ed->nodes[0]; h1 |= q == ed->nodes[1]; h2 |= q == ed->nodes[2];
+                }
+                if (h0 && h1) ed->quads[0] = e;
+                else if (h0 && h2) ed->quads[1] = e;
+                else if (h1 && h2) ed->quads[2] = e;
+                if (ed->quads[0] >= 0 && ed->quads[1] >= 0 && ed->quads[2] >= 0) break;
+            }
+            continue;
+        }
         int cnt = 0;
         for (int e = 0; e < in->K && cnt < 2; ++e) {
             int h0 = 0, h1 = 0;
@@ -882,9 +1155,8 @@ static Mesh *mesh_build(const MeshIn *in) {
             if (h0 && h1) ed->quads[cnt++] = e;
         }
         ed->bdry = (cnt == 1);
-        ed->partner = -1;
     }
-    free(fn);
+    free(fn); free(fnn); free(fat);
     /* computeFaceMarkers */
     for (int b = 0; b < in->nbedges; ++b)
         for (int k = 0; k < E; ++k)
@@ -930,7 +1202,22 @@ static Mesh *mesh_build(const MeshIn *in) {
     for (int k = 0; k < E; ++k) {
         Edge *ed = &m->ed[k];
         ed->low = -1;
-        if (ed->bdry) { ed->N = m->el[ed->quads[0]].N; ed->NLow = ed->N; ed->NCurve = ed->N; }
+        if (ed->sub) {
+            /* divided interior edge: the big (LEFT) element's order (QuadMesh_Construct.incf:121-139) */
+            if (ed->quads[0] < 0 || ed->quads[1] < 0 || ed->quads[2] < 0) {
+                mesh_free(m); free(curve_of); free(cxy); fail("subdivided edge without its three elements"); return NULL;
+            }
+            int a = m->el[ed->quads[0]].N, b = m->el[ed->quads[1]].N, c = m->el[ed->quads[2]].N;
+            ed->N = a;
+            ed->NLow = a < b ? (a < c ? a : c) : (b < c ? b : c);
+            ed->NCurve = ed->NLow > 1 ? ed->NLow / 2 : ed->NLow;
+            if (b != c) {
+                /* the reference's inversion matrix of the NORTH mortar is sized by the SOUTH element's
+                 * order (QuadElement_EdgeProcedures.incf:556-559): defined only when they agree */
+                mesh_free(m); free(curve_of); free(cxy);
+                fail("subdivided edges need the two small elements of one order"); return NULL;
+            }
+        } else if (ed->bdry) { ed->N = m->el[ed->quads[0]].N; ed->NLow = ed->N; ed->NCurve = ed->N; }
         else {
             int a = m->el[ed->quads[0]].N, b = m->el[ed->quads[1]].N;
             ed->N = a > b ? a : b; ed->NLow = a < b ? a : b;
@@ -955,12 +1242,28 @@ static Mesh *mesh_build(const MeshIn *in) {
         ed->n = (double *)calloc(2 * P, sizeof(double));
         ed->dS = (double *)calloc(P, sizeof(double));
         edge_set_curve(m, ed, ed->curved ? cxy + 2 * in->np1 * curve_of[k] : NULL, in->np1 - 1);
+        if (ed->sub) {
+            /* the hanging node moves onto the edge at 0.5 (QuadMesh_Construct.incf:168-173) */
+            double X[2];
+            edge_point_at(m, ed, 0.5, X);
+            m->pts[2 * ed->nodes[2]] = X[0];
+            m->pts[2 * ed->nodes[2] + 1] = X[1];
+        }
     }
     free(curve_of); free(cxy);
+    /* straight conforming interior edges again, with the moved nodes (:177-185) */
+    for (int k = 0; k < E; ++k) {
+        Edge *ed = &m->ed[k];
+        if (!ed->sub && !ed->bdry && !ed->curved) edge_set_curve(m, ed, NULL, 0);
+    }
     /* linking (QuadElement_EdgeProcedures.incf:30-199, 357-424) */
     for (int k = 0; k < E; ++k) {
         Edge *ed = &m->ed[k];
         int needs, pos, dirn;
+        if (ed->sub) {
+            if (link_subdivided(m, ed, k)) { mesh_free(m); return NULL; }
+            continue;
+        }
         Elem *L = &m->el[ed->quads[0]];
         search_edge(L->nodes, ed->nodes, &needs, &pos, &dirn);
         if (pos < 0) { mesh_free(m); fail("edge not found in element"); return NULL; }
@@ -999,7 +1302,11 @@ static Mesh *mesh_build(const MeshIn *in) {
         for (int i = 0; i <= ed->N; ++i) A = A + (ed->curved ? ed->dS[i] : ed->dS[0]) * s->w[i];
         ed->Area = A;
         if (ed->bdry) ed->invh = A / m->el[ed->quads[0]].Volume;
-        else {
+        else if (ed->sub) {
+            double v0 = m->el[ed->quads[0]].Volume, v1 = m->el[ed->quads[1]].Volume, v2 = m->el[ed->quads[2]].Volume;
+            double vm = v0 < v1 ? v0 : v1;
+            ed->invh = A / (vm < v2 ? vm : v2);
+        } else {
             double vl = m->el[ed->quads[0]].Volume, vr = m->el[ed->quads[1]].Volume;
             ed->invh = A / (vl < vr ? vl : vr);
         }
@@ -1068,9 +1375,9 @@ static Mesh *mesh_build(const MeshIn *in) {
     }
     for (int k = 0; k < E; ++k)
         if (m->ed[k].bdry && m->ed[k].marker == 0) {
-            /* an element edge with no neighbour and no boundary marker: a subdivided edge
-             * (MeshFile.f90:364-490), not restated */
-            mesh_free(m); fail("subdivided edges (hanging nodes) are not supported"); return NULL;
+            /* an element edge with no neighbour and no boundary marker, which the merging of
+             * subdivided edges (MeshFile.f90:364-490) did not absorb */
+            mesh_free(m); fail("an edge with neither a neighbour nor a boundary marker"); return NULL;
         }
     m->Q = (double *)calloc(m->ndof, sizeof(double));
     m->QDot = (double *)calloc(m->ndof, sizeof(double));
@@ -1107,8 +1414,10 @@ static void prolong(Mesh *m, int grad) {
                             else v = grad ? m->dQ[DQIDX(e, eq, l, i, d)] : m->Q[QIDX(e, eq, l, i)];
                             acc = acc + v * lb[l];
                         }
-                        if (grad) ed->tdq[side][eq + 4 * i + 4 * P * d] = acc;
-                        else ed->tq[side][eq + 4 * i] = acc;
+                        double *tq = side == 2 ? ed->sd->tq2 : ed->tq[side];
+                        double *tdq = side == 2 ? ed->sd->tdq2 : ed->tdq[side];
+                        if (grad) tdq[eq + 4 * i + 4 * P * d] = acc;
+                        else tq[eq + 4 * i] = acc;
                     }
         }
     }
@@ -1465,6 +1774,44 @@ static void edge_flux_to_side(const Edge *ed, int side, const double *F, double 
     }
 }
 
+/* A subdivided edge's mortar values from one side's traces, out[eq + 4 i] = sum_l T(i, l) src[eq + 4 l]
+ * (ComputeRiemannSolver_SubdividedEdge, DGSpatialDiscretizationMethods.f90:809-828). */
+static void mortar_in(const double *T, int PM, int Pe, const double *src, double *out) {
+    for (int i = 0; i < PM; ++i)
+        for (int eq = 0; eq < 4; ++eq) {
+            double acc = 0.0;
+            for (int l = 0; l < Pe; ++l) acc = acc + T[i * Pe + l] * src[eq + 4 * l];
+            out[eq + 4 * i] = acc;
+        }
+}
+
+/* Mortar values back to a side, accumulated into out (sign +1 or -1 per term):
+ * out[eq + 4 i] (+/-)= T(i, l) F[eq + 4 l] for l in order (:887-907). */
+static void mortar_out(const double *T, int Pe, int PM, const double *F, double sign, double *out) {
+    for (int i = 0; i < Pe; ++i)
+        for (int eq = 0; eq < 4; ++eq) {
+            double acc = out[eq + 4 * i];
+            for (int l = 0; l < PM; ++l) {
+                if (sign > 0.0) acc = acc + T[i * PM + l] * F[eq + 4 * l];
+                else acc = acc - T[i * PM + l] * F[eq + 4 * l];
+            }
+            out[eq + 4 * i] = acc;
+        }
+}
+
+/* The three traces of a subdivided edge on its NORTH and SOUTH mortars: q[0..3] = LN, LS, RN, RS. */
+static void subdivided_mortar_values(const Mesh *m, const Edge *ed, int grad, int d, double q[4][4 * MAXP]) {
+    const SubData *sd = ed->sd;
+    const int PL = m->el[ed->quads[0]].P, PN = m->el[ed->quads[1]].P, PS = m->el[ed->quads[2]].P;
+    const double *tl = grad ? ed->tdq[0] + 4 * PL * d : ed->tq[0];
+    const double *tn = grad ? ed->tdq[1] + 4 * PN * d : ed->tq[1];
+    const double *ts = grad ? sd->tdq2 + 4 * PS * d : sd->tq2;
+    mortar_in(sd->TLN_F, sd->NN + 1, PL, tl, q[0]);
+    mortar_in(sd->TLS_F, sd->NS + 1, PL, tl, q[1]);
+    mortar_in(sd->TRN_F, sd->NN + 1, PN, tn, q[2]);
+    mortar_in(sd->TRS_F, sd->NS + 1, PS, ts, q[3]);
+}
+
 /* DGWeakIntegrals.f90:503-553 Vector_StdVolumeGreen + BR1 face loop
  * (DGViscous_BR1.f90:48-280) + DGViscousMethods.f90:538-616,786-917. */
 static void gradient_br1(Mesh *m) {
@@ -1488,8 +1835,52 @@ static void gradient_br1(Mesh *m) {
     for (int k = 0; k < m->E; ++k) {
         Edge *ed = &m->ed[k];
         int P = ed->N + 1;
-        double FuL[2][4 * MAXP], FuR[2][4 * MAXP];
-        if (!ed->bdry && ed->low >= 0) {
+        double FuL[2][4 * MAXP], FuR[2][4 * MAXP], FuS[2][4 * MAXP];
+        if (ed->sub) {
+            /* BaseClass_ComputeSolutionRiemann_Subdivided / _CurvedSubdivided
+             * (DGViscousMethods.f90:618-780): uStar on each mortar; straight edges return
+             * 0.5 uStar to the big side and scale by the edge's dS n, curved ones scale each
+             * mortar node first and return the products */
+            const SubData *sd = ed->sd;
+            const int PL = m->el[ed->quads[0]].P, PN = m->el[ed->quads[1]].P, PS = m->el[ed->quads[2]].P;
+            const int PMN = sd->NN + 1, PMS = sd->NS + 1;
+            double q[4][4 * MAXP], uN[4 * MAXP], uS[4 * MAXP];
+            subdivided_mortar_values(m, ed, 0, 0, q);
+            for (int c = 0; c < 4 * PMN; ++c) uN[c] = 0.5 * (q[0][c] + q[2][c]);
+            for (int c = 0; c < 4 * PMS; ++c) uS[c] = 0.5 * (q[1][c] + q[3][c]);
+            if (!ed->curved) {
+                double TN[MAXP * MAXP], TS[MAXP * MAXP], uL[4 * MAXP], uRN[4 * MAXP], uRS[4 * MAXP];
+                for (int c = 0; c < PL * PMN; ++c) TN[c] = 0.5 * sd->TLN_B[c];
+                for (int c = 0; c < PL * PMS; ++c) TS[c] = 0.5 * sd->TLS_B[c];
+                for (int c = 0; c < 4 * PL; ++c) uL[c] = 0.0;
+                for (int c = 0; c < 4 * PN; ++c) uRN[c] = 0.0;
+                for (int c = 0; c < 4 * PS; ++c) uRS[c] = 0.0;
+                mortar_out(TN, PL, PMN, uN, 1.0, uL);
+                mortar_out(TS, PL, PMS, uS, 1.0, uL);
+                mortar_out(sd->TRN_B, PN, PMN, uN, 1.0, uRN);
+                mortar_out(sd->TRS_B, PS, PMS, uS, 1.0, uRS);
+                for (int d = 0; d < 2; ++d) {
+                    for (int c = 0; c < 4 * PL; ++c) FuL[d][c] = uL[c] * ed->dS[0] * ed->n[d];
+                    for (int c = 0; c < 4 * PN; ++c) FuR[d][c] = (-uRN[c]) * sd->dSN[0] * sd->nN[d];
+                    for (int c = 0; c < 4 * PS; ++c) FuS[d][c] = (-uRS[c]) * sd->dSS[0] * sd->nS[d];
+                }
+            } else {
+                for (int d = 0; d < 2; ++d) {
+                    double fN[4 * MAXP], fS[4 * MAXP];
+                    for (int i = 0; i < PMN; ++i)
+                        for (int eq = 0; eq < 4; ++eq) fN[eq + 4 * i] = uN[eq + 4 * i] * sd->dSN[i] * sd->nN[2 * i + d];
+                    for (int i = 0; i < PMS; ++i)
+                        for (int eq = 0; eq < 4; ++eq) fS[eq + 4 * i] = uS[eq + 4 * i] * sd->dSS[i] * sd->nS[2 * i + d];
+                    for (int c = 0; c < 4 * PL; ++c) FuL[d][c] = 0.0;
+                    for (int c = 0; c < 4 * PN; ++c) FuR[d][c] = 0.0;
+                    for (int c = 0; c < 4 * PS; ++c) FuS[d][c] = 0.0;
+                    mortar_out(sd->TLN_B, PL, PMN, fN, 1.0, FuL[d]);
+                    mortar_out(sd->TLS_B, PL, PMS, fS, 1.0, FuL[d]);
+                    mortar_out(sd->TRN_B, PN, PMN, fN, -1.0, FuR[d]);
+                    mortar_out(sd->TRS_B, PS, PMS, fS, -1.0, FuS[d]);
+                }
+            }
+        } else if (!ed->bdry && ed->low >= 0) {
             /* p-non-conforming: BaseClass_ComputeSolutionRiemann_Interior / _CurvedInterior
              * (DGViscousMethods.f90:538-616): uStar on the edge's nodes from the projected
              * traces; straight edges project uStar and then scale by dS n, curved edges scale
@@ -1552,11 +1943,11 @@ static void gradient_br1(Mesh *m) {
                 }
             }
         }
-        for (int side = 0; side < (ed->bdry ? 1 : 2); ++side) {
+        for (int side = 0; side < (ed->bdry ? 1 : (ed->sub ? 3 : 2)); ++side) {
             Elem *e = &m->el[ed->quads[side]];
             const Spec *s = spec_get(e->N);
             int loc = ed->loc[side], Pe = e->P;
-            double(*Fu)[4 * MAXP] = side == 0 ? FuL : FuR;
+            double(*Fu)[4 * MAXP] = side == 0 ? FuL : (side == 1 ? FuR : FuS);
             for (int d = 0; d < 2; ++d)
                 for (int jj = 0; jj < Pe; ++jj)
                     for (int ii = 0; ii < Pe; ++ii)
@@ -1855,7 +2246,48 @@ static void time_derivative(Mesh *m) {
         double FL[4 * MAXP], FR[4 * MAXP];
         double GL[8 * MAXP], GR[8 * MAXP];
         int gradfl = m->ns && m->viscous == VISC_IP && m->ip_gradflux;
-        if (!ed->bdry && ed->low >= 0) {
+        if (ed->sub) {
+            /* ComputeRiemannSolver_SubdividedEdge / _CurvedSubdividedEdge (:761-1140): the Riemann
+             * problem on each mortar, FStar = (Fi - Fv - Fa) dS_mortar, returned with T_BKW: the big
+             * side takes NORTH then SOUTH, each small side its own mortar, negated */
+            const SubData *sd = ed->sd;
+            const int PL = m->el[ed->quads[0]].P, PN = m->el[ed->quads[1]].P, PS = m->el[ed->quads[2]].P;
+            const int PM[2] = {sd->NN + 1, sd->NS + 1};
+            double q[4][4 * MAXP], dq[2][4][4 * MAXP], Fs[2][4 * MAXP], FRS[4 * MAXP];
+            subdivided_mortar_values(m, ed, 0, 0, q);
+            if (m->ns)
+                for (int d = 0; d < 2; ++d) subdivided_mortar_values(m, ed, 1, d, dq[d]);
+            for (int h = 0; h < 2; ++h) {
+                const double *nrm = h == 0 ? sd->nN : sd->nS, *dSm = h == 0 ? sd->dSN : sd->dSS;
+                const double *qa = q[h], *qb = q[2 + h];
+                for (int i = 0; i < PM[h]; ++i) {
+                    double Fi[4];
+                    rdefault(ph, qa + 4 * i, qb + 4 * i, nrm + 2 * i, Fi);
+                    if (m->ns) {
+                        double a[8], b[8], Fv[4];
+                        for (int d = 0; d < 2; ++d)
+                            for (int eq = 0; eq < 4; ++eq) {
+                                a[eq + 4 * d] = dq[d][h][eq + 4 * i];
+                                b[eq + 4 * d] = dq[d][2 + h][eq + 4 * i];
+                            }
+                        visc_interior(m, ed, qa + 4 * i, qb + 4 * i, a, b, nrm + 2 * i, Fv);
+                        for (int eq = 0; eq < 4; ++eq) Fs[h][eq + 4 * i] = (Fi[eq] - Fv[eq]) * dSm[i];
+                    } else {
+                        for (int eq = 0; eq < 4; ++eq) Fs[h][eq + 4 * i] = Fi[eq] * dSm[i];
+                    }
+                }
+            }
+            for (int c = 0; c < 4 * PL; ++c) FL[c] = 0.0;
+            for (int c = 0; c < 4 * PN; ++c) FR[c] = 0.0;
+            for (int c = 0; c < 4 * PS; ++c) FRS[c] = 0.0;
+            mortar_out(sd->TLN_B, PL, PM[0], Fs[0], 1.0, FL);
+            mortar_out(sd->TLS_B, PL, PM[1], Fs[1], 1.0, FL);
+            mortar_out(sd->TRN_B, PN, PM[0], Fs[0], -1.0, FR);
+            mortar_out(sd->TRS_B, PS, PM[1], Fs[1], -1.0, FRS);
+            std_face(m, &m->el[ed->quads[0]], ed->loc[0], FL);
+            std_face(m, &m->el[ed->quads[1]], ed->loc[1], FR);
+            std_face(m, &m->el[ed->quads[2]], ed->loc[2], FRS);
+        } else if (!ed->bdry && ed->low >= 0) {
             /* p-non-conforming (ComputeRiemannSolver_InteriorEdge / _InteriorCurvedEdge
              * :545-759): the Riemann problem on the edge's nodes from the projected traces and
              * gradients, FStar = (Fi - Fv - Fa) dS, returned to each side by ProjectFluxes */
@@ -2052,9 +2484,10 @@ void *or_create(const MeshIn *in, const OrOptions *o) {
     m->ad = m->ns ? o->ad : 0;   /* the module is Navier-Stokes only */
     m->ad_ceps = o->ad_ceps;
     for (int k = 0; k < m->E; ++k)
-        if (m->ed[k].low >= 0 && ((m->ns && m->viscous == VISC_IP) || m->ad)) {
+        if ((m->ed[k].low >= 0 || m->ed[k].sub) && ((m->ns && m->viscous == VISC_IP) || m->ad)) {
             mesh_free(m);
-            fail("p-non-conforming edges: the Euler and BR1 forms without artificial dissipation only");
+            fail(m->ed[k].sub ? "subdivided edges: the Euler and BR1 forms without artificial dissipation only"
+                              : "p-non-conforming edges: the Euler and BR1 forms without artificial dissipation only");
             return NULL;
         }
     m->mu_e = (double *)calloc(m->K + 1, sizeof(double));
@@ -2142,6 +2575,28 @@ int or_edge_mortar(void *h, int k, int *low, int *N, int *NLow, double *Tf, doub
         memcpy(Tf, ed->Tf, sizeof(double) * n);
         memcpy(Tb, ed->Tb, sizeof(double) * n);
     }
+    return 0;
+}
+
+/* A subdivided edge's linking and mortar matrices: info = {N_left, N_north, N_south, N of the
+ * NORTH and SOUTH elements, their directions, the three elements, their edge locations}; mats =
+ * TLN_F, TLS_F, TLN_B, TLS_B, TRN_F, TRS_F, TRN_B, TRS_B, each row-major in a MAXP*MAXP slot.
+ * Returns 1 for an edge that is not subdivided. */
+int or_edge_subdivided(void *h, int k, int *info, double *mats) {
+    Mesh *m = h;
+    if (k < 0 || k >= m->E) return fail("edge index out of range");
+    const Edge *ed = &m->ed[k];
+    if (!ed->sub) return 1;
+    const SubData *d = ed->sd;
+    const Elem *L = &m->el[ed->quads[0]], *RN = &m->el[ed->quads[1]], *RS = &m->el[ed->quads[2]];
+    info[0] = L->N; info[1] = d->NN; info[2] = d->NS; info[3] = RN->N; info[4] = RS->N;
+    info[5] = RN->dir[ed->loc[1] - 1]; info[6] = RS->dir[ed->loc[2] - 1];
+    for (int c = 0; c < 3; ++c) { info[7 + c] = ed->quads[c]; info[10 + c] = ed->loc[c]; }
+    const double *src[8] = {d->TLN_F, d->TLS_F, d->TLN_B, d->TLS_B, d->TRN_F, d->TRS_F, d->TRN_B, d->TRS_B};
+    const int rows[8] = {d->NN + 1, d->NS + 1, L->P, L->P, d->NN + 1, d->NS + 1, RN->P, RS->P};
+    const int cols[8] = {L->P, L->P, d->NN + 1, d->NS + 1, RN->P, RS->P, d->NN + 1, d->NS + 1};
+    for (int t = 0; t < 8; ++t)
+        memcpy(mats + (size_t)t * MAXP * MAXP, src[t], sizeof(double) * rows[t] * cols[t]);
     return 0;
 }
 

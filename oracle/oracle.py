@@ -18,6 +18,12 @@ on every C function).  What pins each part (see also tests/test_oracle_*.py):
 * RK3 -- the stability polynomial 1 + z + z^2/2 + z^3/6 on y' = lambda y.
 * Connectivity -- a brute-force O(E^2) restatement of MeshFile.f90; compared bit-exactly
   with the product's O(E) builder.
+* Mortars -- p-non-conforming edges (test_oracle_mortar.py: exact interpolation, T_BKW T_FWD = I,
+  free stream, conservation) and subdivided edges (test_oracle_subdivided.py: the same on each
+  half, the hanging nodes moved back onto the big edge); end to end, the reference's
+  CylinderRe100 (orders 2-8) and VortexTransport (34 subdivided edges, 10 curved) goldens.
+* Whole runs -- the reference's Ramp, CylinderRe100 and VortexTransport Finalize values
+  (tests/test_benchmark_goldens.py).
 * Lax-Friedrichs flux and the far-field viscous treatment -- ABSENT from the reference;
   parity unpinned (consistency / antisymmetry / dissipation-sign checks only).
 """
@@ -86,6 +92,7 @@ def lib():
         _lib.or_get_edge_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
         _lib.or_time_derivative.argtypes = [ctypes.c_void_p, ctypes.c_double]
         _lib.or_edge_mortar.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
+        _lib.or_edge_subdivided.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _lib.or_estimate_dt.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
         _lib.or_reset_dt_state.argtypes = [ctypes.c_void_p]
@@ -234,6 +241,25 @@ class Oracle:
         assert rc == 0
         n, nl = N.value + 1, NL.value + 1
         return low.value, N.value, NL.value, Tf[:n * nl].reshape(n, nl), Tb[:n * nl].reshape(nl, n)
+
+    def edge_subdivided(self, k: int):
+        """A subdivided edge's linking and its eight mortar matrices (None for other edges): the
+        orders of the big element and the mortars, the small elements' directions (1 forward, -1
+        backward), the elements LEFT / NORTH / SOUTH and their edge locations."""
+        info = np.zeros(13, np.int32)
+        mats = np.zeros(8 * 17 * 17)
+        rc = lib().or_edge_subdivided(self.h, k, _ptr(info), _ptr(mats))
+        if rc == 1:
+            return None
+        assert rc == 0
+        NL, NN, NS, NRN, NRS = (int(v) + 1 for v in info[:5])
+        shapes = {"TLN_F": (NN, NL), "TLS_F": (NS, NL), "TLN_B": (NL, NN), "TLS_B": (NL, NS),
+                  "TRN_F": (NN, NRN), "TRS_F": (NS, NRS), "TRN_B": (NRN, NN), "TRS_B": (NRS, NS)}
+        out = {"N_left": NL - 1, "N_north": NN - 1, "N_south": NS - 1, "dir_north": int(info[5]),
+               "dir_south": int(info[6]), "elements": info[7:10].tolist(), "locations": info[10:13].tolist()}
+        for t, (name, (r, c)) in enumerate(shapes.items()):
+            out[name] = mats[t * 289: t * 289 + r * c].reshape(r, c).copy()
+        return out
 
     def time_derivative(self, t: float = 0.0):
         lib().or_time_derivative(self.h, t)

@@ -32,6 +32,7 @@ MESHES = {
     "ramp": "TestCases/Benchmark/Simulations/Ramp/MESH/Ramp.HiOMesh",
     "cylinder_benchmark": "TestCases/Benchmark/Simulations/CylinderMach2/MESH/Cylinder.HiOMesh",
     "cylinder_re100": "TestCases/Benchmark/Simulations/CylinderRe100/MESH/Cylinder.HiOMesh",
+    "vortex_transport_benchmark": "TestCases/Benchmark/Simulations/VortexTransport/MESH/VortexTransport.HiOMesh",
 }
 
 # benchmark case -> (case file, mesh fixture, problem file)
@@ -43,6 +44,9 @@ BENCHMARKS = {
                        "TestCases/Benchmark/Simulations/CylinderMach2/SETUP/ProblemFile.f90"),
     "cylinder_re100": ("TestCases/Benchmark/Simulations/CylinderRe100/SETUP/Cylinder.HiOCase", "cylinder_re100",
                        "TestCases/Benchmark/Simulations/CylinderRe100/SETUP/ProblemFile.f90"),
+    "vortex_transport": ("TestCases/Benchmark/Simulations/VortexTransport/SETUP/VortexTransport.HiOCase",
+                         "vortex_transport_benchmark",
+                         "TestCases/Benchmark/Simulations/VortexTransport/SETUP/ProblemFile.f90"),
 }
 
 
@@ -98,9 +102,12 @@ def finalize_values(problem_file):
     m = re.search(r"residuals\(4\)\s*=\s*\[([^\]]*)\]", text)
     if m:
         out["residuals"] = [float(v) for v in re.findall(num, m.group(1))]
-    m = re.search(r"finalIteration\s*=\s*(\d+)", text)
+    m = re.search(r"(?:finalIteration|expectedIter)\s*=\s*(\d+)", text)
     if m:
         out["final_iteration"] = int(m.group(1))
+    m = re.search(r"expectedErrors\(NCONS\)\s*=\s*&?\s*\[([^\]]*)\]", text)
+    if m:   # errors against the analytical solution (VortexTransport)
+        out["errors"] = [float(v) for v in re.findall(num, m.group(1))]
     for k in ("cl", "cd"):
         m = re.search(r"::\s*" + k + r"\s*=\s*" + num, text)
         if m:

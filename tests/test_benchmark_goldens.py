@@ -16,10 +16,19 @@ Williamson RK5, CFL 0.1, dt cap 0.01, steady mode for 250 iterations.  Finalize 
 four residuals within 1e-11 and the lift and drag coefficients of the wall within 1e-12; the
 surface monitors read the edge storage of the last stage's derivative evaluation.
 
+VortexTransport (VortexTransport/SETUP/ProblemFile.f90:107-150), reproduced by the oracle: a
+vortex carried through a periodic square at Mach 0.6 (Euler: the case has Re = 0 and no
+artificial dissipation), 144 quads of orders 5, 7 and 12 with 34 subdivided edges (10 of them
+curved: hanging nodes, two mortars per edge), Roe (the case file's "Inviscid Riemann Flux"
+label is not the one the solver reads, so its default applies), Williamson RK5, CFL 0.4, dt cap
+0.01, transient to t = 2.  Finalize requires the run to end at iteration 3237 with the max-norm
+errors against the initial vortex within 1e-12 of its stored values.
+
 The cases come from committed fixtures (tests/golden/benchmark_*.json: the case file's
 parameters as horses2d_amd.hio reads them; mesh_*.npz: the HiOMesh arrays), written by
 tests/golden/make_golden.py from the reference tree.  Uniform initial condition
-(InitialConditions.f90:154-170).
+(InitialConditions.f90:154-170), except VortexTransport's problem-file vortex
+(horses2d_amd.cases.gaussian_vortex).
 """
 import json
 import os
@@ -45,10 +54,14 @@ def benchmark_case(name):
     mesh = C.HiOMesh(m["points"], m["quads"], m["bedges"], m["bmarker"], zones,
                      m["cedges"] if "cedges" in m else None, m["cxy"] if "cxy" in m else None)
     orders = np.array(p["orders"], np.int32) if p.get("orders") else np.full(mesh.K, p["N"][0], np.int32)
-    case = C.Case(name, mesh, orders, p["Mach"], p["Re"], p["Pr"], True,
+    # the reference's benchmark build solves the Navier-Stokes equations; at Re = 0 without artificial
+    # dissipation every viscous term is a product with mu = 0, the Euler form's computation
+    ns = p["Re"] > 0.0 or p["artificial_dissipation"] != "none"
+    ic = C.gaussian_vortex(p["Mach"]) if name == "vortex_transport" else C.uniform(C.freestream_state(p["Mach"]))
+    case = C.Case(name, mesh, orders, p["Mach"], p["Re"], p["Pr"], ns,
                   riemann=p["riemann"], viscous=p["viscous"], ip_method=p["ip_method"], sigma0=p["sigma0"],
                   cfl=p["cfl"], dt_max=p["dt_max"], steps=p["steps"], scheme=p["scheme"],
-                  ic=C.uniform(C.freestream_state(p["Mach"])), artificial_dissipation=p["artificial_dissipation"],
+                  ic=ic, artificial_dissipation=p["artificial_dissipation"],
                   ad_intensity=p["ad_intensity"], mode=p["mode"], residual_target=p["residual_target"],
                   t_end=p["t_end"], t0=p["t0"])
     return case, d["finalize"]
@@ -206,3 +219,37 @@ def test_gpu_reproduces_the_cylinder_re100_golden():
     assert dg.iter == case.steps
     assert np.max(np.abs(hist[-1] - np.array(gold["residuals"]))) <= gold["residual_tolerance"]
     assert abs(cl - gold["cl"]) <= 1e-12 and abs(cd - gold["cd"]) <= 1e-12
+
+
+def vortex_errors(state, case, coords):
+    """Finalize's errors (VortexTransport/SETUP/ProblemFile.f90:107-131): the max over the nodes of
+    |initial vortex - Q| per variable."""
+    return np.max(np.abs(case.initial_state(coords).reshape(-1, 4) - state.reshape(-1, 4)), axis=0)
+
+
+def test_vortex_transport_fixture_matches_the_case_file():
+    path = os.path.join(REFERENCE, "TestCases/Benchmark/Simulations/VortexTransport/SETUP/VortexTransport.HiOCase")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    from horses2d_amd import hio
+    c = hio.case_from_hiocase(path, navier_stokes=True)
+    f, gold = benchmark_case("vortex_transport")
+    for k in ("Mach", "Re", "riemann", "scheme", "mode", "cfl", "dt_max", "t_end"):
+        assert getattr(c, k) == getattr(f, k), k
+    np.testing.assert_array_equal(c.N, f.N)
+    assert not f.navier_stokes and gold["final_iteration"] == 3237
+
+
+def test_oracle_reproduces_the_vortex_transport_golden():
+    """3237 RK5 steps to t = 2 across the subdivided and p-non-conforming edges: the final
+    iteration exactly, the errors within Finalize's 1e-12."""
+    from oracle.oracle import Oracle
+    case, gold = benchmark_case("vortex_transport")
+    orc = Oracle(case)
+    X = orc.coords()
+    orc.set_state(case.initial_state(X))
+    t, _, it, _ = orc.integrate(case.steps, dt_max=case.dt_max, t_end=case.t_end, scheme=case.scheme, mode=case.mode)
+    err = vortex_errors(orc.state(), case, X)
+    print("oracle: iteration", it, "t", t, "errors", err, "golden", gold)
+    assert it == gold["final_iteration"] and t == case.t_end
+    assert np.max(np.abs(err - np.array(gold["errors"]))) <= gold["residual_tolerance"]

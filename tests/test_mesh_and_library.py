@@ -168,9 +168,10 @@ def test_boundary_condition_kinds_accepted_and_refused(gpu_lib):
             gpu.build_connectivity(channel(kind))
 
 
-def test_subdivided_edges_are_refused_by_both_builders():
-    """A hanging node (one element's edge meets two elements, MeshFile.f90:364-490): not built;
-    the product's builder and the oracle stop with the same message."""
+def test_subdivided_edges_product_refuses_oracle_builds():
+    """A hanging node (one element's edge meets two elements, MeshFile.f90:364-490): the
+    product's builder refuses it with a message; the oracle merges the edge and links its three
+    elements (big LEFT, NORTH, SOUTH)."""
     pts = np.array([[0, 0], [1, 0], [2, 0], [2, 1], [2, 2], [1, 2], [0, 2], [1, 1]], np.float64)
     quads = np.array([[0, 1, 5, 6], [1, 2, 3, 7], [7, 3, 4, 5]], np.int32)
     bedges = np.array([[0, 1], [1, 2], [2, 3], [3, 4], [4, 5], [5, 6], [6, 0]], np.int32)
@@ -179,5 +180,6 @@ def test_subdivided_edges_are_refused_by_both_builders():
     case = C.Case("hanging", mesh, np.full(3, 3, np.int32), 0.3, ic=C.uniform(q))
     with pytest.raises(RuntimeError, match="subdivided edges"):
         gpu.build_connectivity(case)
-    with pytest.raises(RuntimeError, match="subdivided edges"):
-        O.Oracle(case)
+    orc = O.Oracle(case)
+    subs = [s for s in (orc.edge_subdivided(k) for k in range(orc.E)) if s is not None]
+    assert len(subs) == 1 and sorted(subs[0]["elements"]) == [0, 1, 2] and subs[0]["elements"][0] == 0

@@ -1,8 +1,8 @@
 """Pins of the CPU oracle against checks that do not restate its formulas.
 
 Each test names the reference function it pins.  The reference's own end-to-end goldens
-(TestCases/Benchmark/*/SETUP/ProblemFile.f90 ``Finalize``) need RK5 with p-non-conforming and
-subdivided mortars, which are not on this path; see the oracle module docstring.
+(TestCases/Benchmark/*/SETUP/ProblemFile.f90 ``Finalize``) are in tests/test_benchmark_goldens.py;
+the mortar pins in tests/test_oracle_mortar.py and tests/test_oracle_subdivided.py.
 """
 import numpy as np
 import pytest
