@@ -295,6 +295,9 @@ hg::MortarArgs mortar_args(hg_handle *h) {
     hg::MortarArgs a{};
     a.edges = h->medges;
     a.nedges = h->nmortar;
+    a.subs = h->sedges;
+    a.nsubs = h->nsubedge;
+    a.sgeo = h->sgeo;
     a.mats = h->mmats;
     a.ph = h->sub[0]->ph;
     for (size_t g = 0; g < h->sub.size(); ++g) {
@@ -314,10 +317,12 @@ int composite_stage(hg_handle *h, double am, double cq, double cg, bool first, b
                     const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0) {
     if (h->ph.ns) {
         HG_CHECK(hg::launch_mortar_ustar(mortar_args(h), h->stream));
+        HG_CHECK(hg::launch_sub_ustar(mortar_args(h), h->stream));
         for (auto *s : h->sub)
             if (run_stage(s, am, cq, cg, first, last, mode, dt_dev, dt_cq, dt_cg, 1)) return 1;
     }
     HG_CHECK(hg::launch_mortar_flux(mortar_args(h), h->stream));
+    HG_CHECK(hg::launch_sub_flux(mortar_args(h), h->stream));
     for (auto *s : h->sub)
         if (run_stage(s, am, cq, cg, first, last, mode, dt_dev, dt_cq, dt_cg, 2)) return 1;
     if (mode == 0) h->cur ^= 1;
@@ -477,6 +482,17 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
         }
         einvh[k] = ed.invh;
     }
+    // a subdivided edge's small sides see their mortar's normal and half the edge's length (the
+    // BR1 gradient lift of a straight edge, DGViscousMethods.f90:693-698)
+    std::vector<int> half_off(E, -1);
+    for (int k = 0; k < E; ++k) {
+        const hg::Edge &ed = m.ed[k];
+        if (ed.hang < 0) continue;
+        half_off[k] = (int)(egeo.size() / 3);
+        egeo.push_back(ed.n[0]);
+        egeo.push_back(ed.n[1]);
+        egeo.push_back(0.5 * ed.dS[0]);
+    }
     std::vector<int4> face(4 * (size_t)K);
     std::vector<int> forder(K);
     std::vector<double> zone(8 * std::max<size_t>(m.zones.size(), 1), 0.0);
@@ -496,10 +512,12 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
             const hg::Edge &ed = m.ed[k];
             const int side = el.qpos[f];
             int flags = 0, y = 0;
+            int goff = geo_off[k];
             if (!ed.bdry && mort[k] >= 0) {
                 flags = hg::FK_MORTAR;   // evaluated by the mortar kernels
                 y = mort[k];
                 has_mortar = true;
+                if (ed.hang >= 0 && side > 0) goff = half_off[k];
             } else if (!ed.bdry) {
                 flags = side == 0 ? hg::FK_INT_L : hg::FK_INT_R;
                 int other = loc[ed.quads[1 - side]];
@@ -537,8 +555,10 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
                 }
                 if (Z.riemann >= 0) flags |= (Z.riemann + 1) << hg::FF_RS_SHIFT;
             }
-            if (ed.curved) flags |= hg::FF_CURVED;
-            face[4 * (size_t)e + f] = make_int4(flags, y, geo_off[k], k);
+            // the mortar kernels hold a curved subdivided edge's geometry; its faces keep the
+            // edge's first node's (read by no kernel: the BR1 form refuses such edges)
+            if (ed.curved && ed.hang < 0) flags |= hg::FF_CURVED;
+            face[4 * (size_t)e + f] = make_int4(flags, y, goff, k);
         }
     }
     // metric planes and per-face-node metric for the IP gradient face terms
@@ -724,11 +744,11 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
 int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt, const std::vector<int> &orders) {
     const hg::Mesh &m = h->mesh;
     if (opt->precision != HG_PRECISION_FP64)
-        return set_err("hg_create: a mesh with several polynomial orders runs in the FP64 form only");
+        return set_err("hg_create: a mesh with several polynomial orders or subdivided edges runs in the FP64 form only");
     if (opt->navier_stokes && opt->viscous != HG_VISCOUS_BR1)
-        return set_err("hg_create: a mesh with several polynomial orders needs the Euler or the BR1 form");
+        return set_err("hg_create: a mesh with several polynomial orders or subdivided edges needs the Euler or the BR1 form");
     if (opt->navier_stokes && opt->artificial_dissipation != HG_AD_NONE)
-        return set_err("hg_create: a mesh with several polynomial orders runs without artificial dissipation");
+        return set_err("hg_create: a mesh with several polynomial orders or subdivided edges runs without artificial dissipation");
     if ((int)orders.size() > hg::kMaxSub) return set_err("hg_create: too many polynomial orders in one mesh");
     const int K = (int)m.el.size(), E = (int)m.ed.size();
     std::vector<int> group(K), loc(K);
@@ -743,7 +763,7 @@ int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt
     std::vector<double> mats;
     for (int k = 0; k < E; ++k) {
         const hg::Edge &ed = m.ed[k];
-        if (ed.bdry || m.el[ed.quads[0]].N == m.el[ed.quads[1]].N) continue;
+        if (ed.bdry || ed.hang >= 0 || m.el[ed.quads[0]].N == m.el[ed.quads[1]].N) continue;
         if (ed.curved) return set_err("hg_create: curved edges between elements of different orders are not supported");
         hg::MortarEdge r{};
         for (int side = 0; side < 2; ++side) {
@@ -764,6 +784,39 @@ int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt
         hg::mortar_matrices(Pl - 1, r.P - 1, mats.data() + r.mat, mats.data() + r.mat + (size_t)r.P * Pl);
         mort[k] = (int)medges.size();
         medges.push_back(r);
+    }
+    // subdivided edges (SubdividedEdge_LinkWithElements, QuadElement_EdgeProcedures.incf:201-355)
+    std::vector<hg::SubMortarEdge> sedges;
+    std::vector<double> sgeo;
+    for (int k = 0; k < E; ++k) {
+        const hg::Edge &ed = m.ed[k];
+        if (ed.hang < 0) continue;
+        if (ed.curved && opt->navier_stokes)
+            return set_err("hg_create: curved subdivided edges are supported in the Euler form only");
+        const hg::SubEdge &d = *ed.sd;
+        hg::SubMortarEdge r{};
+        for (int side = 0; side < 3; ++side) {
+            const int q = ed.quads[side];
+            r.sub[side] = group[q];
+            r.slot[side] = loc[q] * 4 + (ed.loc[side] - 1);
+            r.Ps[side] = m.el[q].P;
+        }
+        r.PM[0] = d.NN + 1;
+        r.PM[1] = d.NS + 1;
+        r.mat = (int)mats.size();
+        for (const std::vector<double> *t : {&d.TLN_F, &d.TLS_F, &d.TLN_B, &d.TLS_B, &d.TRN_F, &d.TRS_F, &d.TRN_B, &d.TRS_B})
+            mats.insert(mats.end(), t->begin(), t->end());
+        r.geo = (int)(sgeo.size() / 3);
+        for (int h = 0; h < 2; ++h) {
+            const std::vector<double> &nrm = h == 0 ? d.nN : d.nS, &dS = h == 0 ? d.dSN : d.dSS;
+            for (int p = 0; p < r.PM[h]; ++p) {
+                sgeo.push_back(nrm[2 * p]);
+                sgeo.push_back(nrm[2 * p + 1]);
+                sgeo.push_back(dS[p]);
+            }
+        }
+        mort[k] = (int)(medges.size() + sedges.size());
+        sedges.push_back(r);
     }
     HG_CHECK(hipGetDevice(&h->device));
     if (hg::noalloc_setup(h->device)) return 1;
@@ -789,7 +842,10 @@ int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt
     h->el_off.resize(K);
     for (int e = 0; e < K; ++e) h->el_off[e] = m.el[e].off;
     h->nmortar = (int)medges.size();
+    h->nsubedge = (int)sedges.size();
     HG_CHECK(upload(h, &h->medges, medges));
+    HG_CHECK(upload(h, &h->sedges, sedges));
+    HG_CHECK(upload(h, &h->sgeo, sgeo));
     HG_CHECK(upload(h, &h->mmats, mats));
     for (size_t g = 0; g < orders.size(); ++g) HG_CHECK(upload(h, &h->sub_gmap[g], h->sub_els[g]));
     // the time-step scan over the whole mesh in element order: N + 1 and the shortest side per
@@ -852,7 +908,10 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     orders.erase(std::unique(orders.begin(), orders.end()), orders.end());
     for (int N : orders)
         if (!hg::supported_order(N)) return set_err("hg_create: polynomial order not compiled (1..10)");
-    if (orders.size() > 1) {
+    // several orders or subdivided edges: sub-handles coupled by the mortar kernels
+    bool subdivided = false;
+    for (const auto &e : m.ed) subdivided = subdivided || e.hang >= 0;
+    if (orders.size() > 1 || subdivided) {
         if (create_composite(h.get(), phys, opt, orders)) {
             const std::string msg = g_err;
             hg_destroy(h.release());   // the sub-handles made so far

@@ -100,9 +100,22 @@ struct MortarEdge {
     int pad;
     double nx, ny, dS;   // straight edge: normal (LEFT to RIGHT) and length
 };
+// A subdivided edge (one element's edge meeting two elements, MeshFile.f90:364-490): its big
+// (LEFT), NORTH and SOUTH elements and the two mortars between them.
+struct SubMortarEdge {
+    int sub[3];      // sub-handle of the LEFT, NORTH and SOUTH elements
+    int slot[3];     // their face slots (local element * 4 + face)
+    int Ps[3];       // their points per direction
+    int PM[2];       // points of the NORTH and SOUTH mortars
+    int mat;         // T_LN_F, T_LS_F, T_LN_B, T_LS_B, T_RN_F, T_RS_F, T_RN_B, T_RS_B in the pool, row-major
+    int geo;         // (nx, ny, dS) per mortar node in the geometry pool: NORTH then SOUTH
+};
 struct MortarArgs {
     const MortarEdge *edges;
     int nedges;
+    const SubMortarEdge *subs;
+    int nsubs;
+    const double *sgeo;
     const double *mats;
     const double *tr[kMaxSub];   // Q traces of the current stage, per sub-handle
     const double *mdq[kMaxSub];  // gradient traces at mortar faces, per sub-handle
@@ -114,6 +127,9 @@ struct MortarArgs {
 hipError_t launch_mortar_ustar(const MortarArgs &m, hipStream_t s);
 // The numerical flux (Fi - Fv) dS of every mortar edge on its nodes, to each side.
 hipError_t launch_mortar_flux(const MortarArgs &m, hipStream_t s);
+// The same two steps for the subdivided edges (two mortars each).
+hipError_t launch_sub_ustar(const MortarArgs &m, hipStream_t s);
+hipError_t launch_sub_flux(const MortarArgs &m, hipStream_t s);
 // dst[map[i]] = src[i], i < n (a sub-handle's per-element values into the whole mesh's order).
 hipError_t launch_scatter(const double *src, const int *map, int n, double *dst, hipStream_t s);
 

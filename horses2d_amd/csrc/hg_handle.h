@@ -84,6 +84,9 @@ struct hg_handle {
     hg::MortarEdge *medges = nullptr;
     double *mmats = nullptr;
     int nmortar = 0;
+    hg::SubMortarEdge *sedges = nullptr;     // subdivided edges and their mortars' geometry
+    double *sgeo = nullptr;
+    int nsubedge = 0;
     double *n1e = nullptr;                   // per mesh element: N + 1 (time-step scan)
     std::vector<std::vector<int>> sub_probes;   // per sub-handle: the probe indices it holds
     bool shared_stream = false;              // a sub-handle: the stream belongs to its parent

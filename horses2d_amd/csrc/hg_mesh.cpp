@@ -10,6 +10,8 @@
 // and the edge curves of QuadElement_EdgeMappings.incf:22-442.
 #include "hg_mesh.h"
 
+#include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -217,12 +219,53 @@ void set_curve(const Mesh &m, Edge &e, const double *cpts, int order) {
     }
 }
 
+// EvaluateX forwards at x: Edge_AnalyticalX from the nodes, CurvedEdge_InterpolantX from the
+// edge's points (QuadElement_EdgeMappings.incf:52-78,355-380).
+void point_at(const Mesh &m, const Edge &e, double x, double *X) {
+    if (!e.curved) {
+        const double *a = &m.pts[2 * e.nodes[0]], *b = &m.pts[2 * e.nodes[1]];
+        for (int d = 0; d < 2; ++d) X[d] = a[d] * (1.0 - x) + b[d] * x;
+        return;
+    }
+    const Spectral &s = spectral(e.N);
+    double lj[kMaxP];
+    lagrange_bary(x, e.N, s.xi, s.wb, lj);
+    for (int d = 0; d < 2; ++d) {
+        double p = 0.0;
+        for (int j = 0; j <= e.N; ++j) p += e.X[2 * j + d] * lj[j];
+        X[d] = p;
+    }
+}
+
 // Edge point and tangent at the local index of an element of order elN
 // (QuadElement_EdgeMappings.incf:52-177,262-442): an interior edge is evaluated at the element's
 // node (QuadElement_QuadMappings.incf:55-62), which differs from the edge's nodes when the edge
-// takes the higher order of its two elements.
-void edge_point(const Mesh &m, const Edge &e, int elN, int idx, int dir, double *X, double *dX) {
+// takes the higher order of its two elements.  A subdivided edge is evaluated forwards, on
+// [0.5, 1] for its NORTH element (qpos 1) and [0, 0.5] for its SOUTH one (qpos 2), at the
+// element's nodes reversed when it runs its half backwards, dX scaled by the half and the
+// direction (QuadElement_QuadMappings.incf:66-115).
+void edge_point(const Mesh &m, const Edge &e, int elN, int idx, int dir, int qpos, double *X, double *dX) {
     const Spectral &s = spectral(e.N);
+    if (e.hang >= 0) {
+        const double *xs = spectral(elN).xi;
+        double xv = xs[idx], scal = 1.0;
+        if (qpos == 1) { xv = 0.5 + 0.5 * (dir == -1 ? xs[elN - idx] : xs[idx]); scal = dir == -1 ? -0.5 : 0.5; }
+        else if (qpos == 2) { xv = 0.5 * (dir == -1 ? xs[elN - idx] : xs[idx]); scal = dir == -1 ? -0.5 : 0.5; }
+        point_at(m, e, xv, X);
+        if (!e.curved) {
+            const double *a = &m.pts[2 * e.nodes[0]], *b = &m.pts[2 * e.nodes[1]];
+            for (int d = 0; d < 2; ++d) dX[d] = scal * (b[d] - a[d]);
+        } else {
+            double lj[kMaxP];
+            lagrange_bary(xv, e.N, s.xi, s.wb, lj);
+            for (int d = 0; d < 2; ++d) {
+                double q = 0.0;
+                for (int j = 0; j <= e.N; ++j) q += e.dX[2 * j + d] * lj[j];
+                dX[d] = scal * q;
+            }
+        }
+        return;
+    }
     if (e.bdry) {
         int c = dir == 1 ? idx : e.N - idx;
         X[0] = e.X[2 * c]; X[1] = e.X[2 * c + 1];
@@ -256,7 +299,7 @@ void element_mappings(const Mesh &m, Elem &el) {
         bool horiz = (pos == 1 || pos == 3);
         for (int a = 0; a < P; ++a) {
             double X[2], dX[2];
-            edge_point(m, e, el.N, a, el.dir[pos - 1], X, dX);
+            edge_point(m, e, el.N, a, el.dir[pos - 1], el.qpos[pos - 1], X, dX);
             for (int b = 0; b < P; ++b) {
                 int i = horiz ? a : b, j = horiz ? b : a;
                 for (int d = 0; d < 2; ++d) { XE[at(d, i, j, pos - 1)] = X[d]; DXE[at(d, i, j, pos - 1)] = dX[d]; }
@@ -296,6 +339,179 @@ void element_mappings(const Mesh &m, Elem &el) {
     el.volume = vol;
 }
 
+// mergeDividedFaces (MeshFile.f90:364-490) on the edge list.  Entries are in list order (their
+// ids: a removal keeps the order, a merged entry is appended), each two node ids, or three after
+// a merge, and the file's boundary attribute.  For the nodes in ascending order: a node in
+// exactly three entries, none of them boundary edges, two of whose neighbours are joined by an
+// entry of their own (the first eight entries of each neighbour are looked at) hangs on that
+// edge: the edge and its two halves are removed and [p0, p1, node] is appended.
+struct EdgeList {
+    std::vector<std::array<int, 3>> v;
+    std::vector<char> attr, alive;
+    std::vector<std::vector<int>> of_node;                          // entry ids, ascending
+    std::unordered_map<uint64_t, std::vector<int>, PairHash> pair;  // two-node entries
+};
+
+void merge_divided(int nnodes, EdgeList &L) {
+    auto remove = [&](int id) {
+        if (id < 0) return;
+        L.alive[id] = 0;
+        for (int c = 0; c < 3; ++c) {
+            const int n = L.v[id][c];
+            if (n < 0) continue;
+            auto &o = L.of_node[n];
+            o.erase(std::find(o.begin(), o.end(), id));
+        }
+        if (L.v[id][2] < 0) {
+            auto &p = L.pair[key(L.v[id][0], L.v[id][1])];
+            p.erase(std::find(p.begin(), p.end(), id));
+        }
+    };
+    auto first_pair = [&](int a, int b) {
+        auto it = L.pair.find(key(a, b));
+        return (it == L.pair.end() || it->second.empty()) ? -1 : it->second.front();
+    };
+    for (int node = 0; node < nnodes; ++node) {
+        if (L.of_node[node].size() != 3) continue;
+        const int inc[3] = {L.of_node[node][0], L.of_node[node][1], L.of_node[node][2]};
+        if (L.attr[inc[0]] || L.attr[inc[1]] || L.attr[inc[2]]) continue;
+        int pts[3];
+        for (int j = 0; j < 3; ++j) {
+            const auto &v = L.v[inc[j]];
+            pts[j] = v[0] != node ? v[0] : (v[1] != node ? v[1] : -1);
+        }
+        bool done = false;
+        for (int j = 0; j < 3 && !done; ++j) {
+            if (pts[j] < 0) continue;
+            const auto &on = L.of_node[pts[j]];
+            const std::vector<int> ne(on.begin(), on.begin() + std::min<size_t>(8, on.size()));
+            for (size_t k = 0; k < ne.size() && !done; ++k) {
+                const int p0 = L.v[ne[k]][0], p1 = L.v[ne[k]][1];
+                int point = -1;
+                if (p0 != pts[j] && p0 != node) point = p0;
+                else if (p1 != pts[j] && p1 != node) point = p1;
+                if (point == -1 || (point != pts[0] && point != pts[1] && point != pts[2])) continue;
+                remove(first_pair(p0, p1));
+                remove(first_pair(p0, node));
+                remove(first_pair(p1, node));
+                const int id = (int)L.v.size();
+                L.v.push_back({p0, p1, node});
+                L.attr.push_back(0);
+                L.alive.push_back(1);
+                L.of_node[p0].push_back(id);
+                L.of_node[p1].push_back(id);
+                L.of_node[node].push_back(id);
+                done = true;
+            }
+        }
+    }
+}
+
+// The mortars' normals and surface Jacobians (SubdividedEdge_ConstructMortars /
+// CurvedSubdividedEdge_ConstructMortars, QuadElement_EdgeMappings.incf:450-565) from the edge's
+// n and dS as the linking left them: straight, n and dS / 2 at every mortar node; curved, dS n
+// interpolated at 0.5 + 0.5 xi_N and 0.5 xi_S, its length halved and the normal its direction.
+void mortar_geometry(const Edge &e, SubEdge &d) {
+    const Spectral &s = spectral(e.N);
+    for (int half = 0; half < 2; ++half) {
+        const int NM = half == 0 ? d.NN : d.NS;
+        const Spectral &sm = spectral(NM);
+        std::vector<double> &nrm = half == 0 ? d.nN : d.nS, &dS = half == 0 ? d.dSN : d.dSS;
+        nrm.assign(2 * (NM + 1), 0.0);
+        dS.assign(NM + 1, 0.0);
+        for (int p = 0; p <= NM; ++p) {
+            if (!e.curved) {
+                nrm[2 * p] = e.n[0]; nrm[2 * p + 1] = e.n[1];
+                dS[p] = 0.5 * e.dS[0];
+                continue;
+            }
+            const double x = half == 0 ? 0.5 + 0.5 * sm.xi[p] : 0.5 * sm.xi[p];
+            double lj[kMaxP], v[2];
+            lagrange_bary(x, e.N, s.xi, s.wb, lj);
+            for (int c = 0; c < 2; ++c) {
+                double acc = 0.0;
+                for (int j = 0; j <= e.N; ++j) acc += (e.dS[j] * e.n[2 * j + c]) * lj[j];
+                v[c] = acc;
+            }
+            dS[p] = 0.5 * std::sqrt(v[0] * v[0] + v[1] * v[1]);
+            nrm[2 * p] = 0.5 * v[0] / dS[p];
+            nrm[2 * p + 1] = 0.5 * v[1] / dS[p];
+        }
+    }
+}
+
+// SubdividedEdge_LinkWithElements (QuadElement_EdgeProcedures.incf:201-355) with
+// SubdividedEdge_ComputeMortarsTransformationMatrices (:530-748).  The big element is LEFT and
+// forward; the element with node 1 and the hanging node (quads[1] before linking) becomes NORTH
+// when LEFT needs the edge inverted, SOUTH otherwise.
+std::string link_subdivided(Mesh &m, Edge &e, int k) {
+    int ids[3] = {e.nodes[0], e.nodes[1], e.hang};
+    Elem &L = m.el[e.quads[0]];
+    int pos, dir;
+    locate(L.nodes, ids, pos, dir);
+    if (pos < 0) return "subdivided edge not found in its element";
+    L.edges[pos - 1] = k; L.dir[pos - 1] = 1; L.qpos[pos - 1] = 0; e.loc[0] = pos;
+    int rn, rs;
+    if (dir != 1) {
+        rn = e.quads[1]; rs = e.quads[2];
+        invert(e, pos == 1 || pos == 2);
+        std::swap(ids[0], ids[1]);
+    } else {
+        rn = e.quads[2]; rs = e.quads[1];
+        if (pos == 3 || pos == 4) {
+            const int np = e.curved ? e.N + 1 : 1;
+            for (int i = 0; i < 2 * np; ++i) e.n[i] = -e.n[i];
+        }
+    }
+    e.quads[1] = rn; e.quads[2] = rs;
+    e.sd = std::make_shared<SubEdge>();
+    SubEdge &d = *e.sd;
+    const int NL = L.N, NRN = m.el[rn].N, NRS = m.el[rs].N;
+    d.NN = std::max(NL, NRN);
+    d.NS = std::max(NL, NRS);
+    mortar_geometry(e, d);
+    const int sub_n[2] = {ids[2], ids[1]}, sub_s[2] = {ids[0], ids[2]};
+    int dn, ds;
+    Elem &RN = m.el[rn], &RS = m.el[rs];
+    locate(RN.nodes, sub_n, pos, dn);
+    if (pos < 0) return "subdivided edge not found in its north element";
+    RN.edges[pos - 1] = k; RN.dir[pos - 1] = dn; RN.qpos[pos - 1] = 1; e.loc[1] = pos;
+    locate(RS.nodes, sub_s, pos, ds);
+    if (pos < 0) return "subdivided edge not found in its south element";
+    RS.edges[pos - 1] = k; RS.dir[pos - 1] = ds; RS.qpos[pos - 1] = 2; e.loc[2] = pos;
+    const Spectral &sL = spectral(NL), &sN = spectral(d.NN), &sS = spectral(d.NS), &sRN = spectral(NRN),
+                   &sRS = spectral(NRS);
+    const int PL = NL + 1, PN = d.NN + 1, PS = d.NS + 1, PRN = NRN + 1, PRS = NRS + 1;
+    double xs[kMaxP], xn[kMaxP];
+    for (int p = 0; p < PS; ++p) xs[p] = 0.5 * sS.xi[p];
+    for (int p = 0; p < PN; ++p) xn[p] = 0.5 + 0.5 * sN.xi[p];
+    d.TLS_F.assign(PS * PL, 0.0); interpolation_matrix(NL, d.NS, sL.xi, sL.wb, xs, d.TLS_F.data());
+    d.TLN_F.assign(PN * PL, 0.0); interpolation_matrix(NL, d.NN, sL.xi, sL.wb, xn, d.TLN_F.data());
+    d.TRS_F.assign(PS * PRS, 0.0); interpolation_matrix(NRS, d.NS, sRS.xi, sRS.wb, sS.xi, d.TRS_F.data());
+    d.TRN_F.assign(PN * PRN, 0.0); interpolation_matrix(NRN, d.NN, sRN.xi, sRN.wb, sN.xi, d.TRN_F.data());
+    // T_BKW = Minv T_FWD^T M with diagonal mass matrices: (1/w_el(i) T_FWD(l, i)) w_mortar(l)
+    d.TLS_B.assign(PL * PS, 0.0); d.TLN_B.assign(PL * PN, 0.0);
+    d.TRS_B.assign(PRS * PS, 0.0); d.TRN_B.assign(PRN * PN, 0.0);
+    for (int i = 0; i < PL; ++i) {
+        for (int l = 0; l < PS; ++l) d.TLS_B[i * PS + l] = ((1.0 / sL.w[i]) * d.TLS_F[l * PL + i]) * sS.w[l];
+        for (int l = 0; l < PN; ++l) d.TLN_B[i * PN + l] = ((1.0 / sL.w[i]) * d.TLN_F[l * PL + i]) * sN.w[l];
+    }
+    for (int i = 0; i < PRS; ++i)
+        for (int l = 0; l < PS; ++l) d.TRS_B[i * PS + l] = ((1.0 / sRS.w[i]) * d.TRS_F[l * PRS + i]) * sS.w[l];
+    for (int i = 0; i < PRN; ++i)
+        for (int l = 0; l < PN; ++l) d.TRN_B[i * PN + l] = ((1.0 / sRN.w[i]) * d.TRN_F[l * PRN + i]) * sN.w[l];
+    // a small element running its half backwards: T_FWD's columns and T_BKW's rows reversed
+    auto reverse = [](std::vector<double> &F, std::vector<double> &B, int Pm, int Pe) {
+        for (int i = 0; i < Pm; ++i)
+            for (int l = 0; l < Pe / 2; ++l) std::swap(F[i * Pe + l], F[i * Pe + Pe - 1 - l]);
+        for (int i = 0; i < Pe / 2; ++i)
+            for (int l = 0; l < Pm; ++l) std::swap(B[i * Pm + l], B[(Pe - 1 - i) * Pm + l]);
+    };
+    if (ds == -1) reverse(d.TRS_F, d.TRS_B, PS, PRS);
+    if (dn == -1) reverse(d.TRN_F, d.TRN_B, PN, PRN);
+    return "";
+}
+
 }  // namespace
 
 const Spectral &spectral(int N) {
@@ -329,37 +545,92 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
     for (int e = 0; e < K; ++e)
         for (int c = 0; c < 4; ++c)
             if (in.quads[4 * e + c] < 0 || in.quads[4 * e + c] >= in.nnodes) return "node id out of range";
-    // faces in first-appearance order
+    // faces in first-appearance order with their elements in element order
     std::unordered_map<uint64_t, int, PairHash> face_id;
     face_id.reserve((size_t)4 * K);
-    m.ed.clear();
-    m.ed.reserve((size_t)2 * K + 16);
+    std::vector<std::array<int, 2>> fnodes, fown;
+    fnodes.reserve((size_t)2 * K + 16);
+    fown.reserve((size_t)2 * K + 16);
     for (int e = 0; e < K; ++e)
         for (int f = 0; f < 4; ++f) {
             int a = in.quads[4 * e + f], b = in.quads[4 * e + (f + 1) % 4];
             auto it = face_id.find(key(a, b));
             int id;
             if (it == face_id.end()) {
-                id = (int)m.ed.size();
+                id = (int)fnodes.size();
                 face_id.emplace(key(a, b), id);
-                Edge ed;
-                ed.nodes[0] = a; ed.nodes[1] = b;
-                m.ed.push_back(ed);
+                fnodes.push_back({a, b});
+                fown.push_back({-1, -1});
             } else {
                 id = it->second;
             }
-            Edge &ed = m.ed[id];
-            if (ed.quads[0] < 0) ed.quads[0] = e;
-            else if (ed.quads[1] < 0 && ed.quads[0] != e) ed.quads[1] = e;
-            else if (ed.quads[0] != e && ed.quads[1] != e) return "edge shared by more than two elements";
+            auto &o = fown[id];
+            if (o[0] < 0) o[0] = e;
+            else if (o[1] < 0 && o[0] != e) o[1] = e;
+            else if (o[0] != e && o[1] != e) return "edge shared by more than two elements";
         }
-    const int E = (int)m.ed.size();
-    for (auto &ed : m.ed) ed.bdry = ed.quads[1] < 0;
+    const int F = (int)fnodes.size();
+    std::vector<int> fmark(F, 0);
     for (int b = 0; b < in.nbedges; ++b) {
         auto it = face_id.find(key(in.bedges[2 * b], in.bedges[2 * b + 1]));
         if (it == face_id.end()) return "boundary edge not found";
-        m.ed[it->second].marker = in.bmarker[b];
+        fmark[it->second] = in.bmarker[b];
     }
+    // subdivided edges: merged only where a node lies on exactly three faces, none of them on the
+    // boundary (the merge's precondition), which conforming meshes rarely have
+    std::vector<int> deg(in.nnodes, 0);
+    std::vector<char> onb(in.nnodes, 0);
+    for (int f = 0; f < F; ++f)
+        for (int c = 0; c < 2; ++c) {
+            ++deg[fnodes[f][c]];
+            if (fmark[f]) onb[fnodes[f][c]] = 1;
+        }
+    bool candidates = false;
+    for (int n = 0; n < in.nnodes && !candidates; ++n) candidates = deg[n] == 3 && !onb[n];
+    EdgeList L;
+    L.v.resize(F);
+    L.attr.resize(F);
+    L.alive.assign(F, 1);
+    for (int f = 0; f < F; ++f) { L.v[f] = {fnodes[f][0], fnodes[f][1], -1}; L.attr[f] = fmark[f] != 0; }
+    if (candidates) {
+        L.of_node.resize(in.nnodes);
+        for (int f = 0; f < F; ++f) {
+            L.of_node[fnodes[f][0]].push_back(f);
+            L.of_node[fnodes[f][1]].push_back(f);
+            L.pair[key(fnodes[f][0], fnodes[f][1])].push_back(f);
+        }
+        merge_divided(in.nnodes, L);
+    }
+    // the edges in list order (computeElementOfEdges and searchDividedEdgeInElements,
+    // MeshFile.f90:500-570,785-824: a subdivided edge's elements hold nodes 1 and 2, 1 and the
+    // hanging node, 2 and the hanging node)
+    m.ed.clear();
+    m.ed.reserve(L.v.size());
+    std::unordered_map<uint64_t, int, PairHash> edge_id;
+    edge_id.reserve(L.v.size());
+    for (size_t id = 0; id < L.v.size(); ++id) {
+        if (!L.alive[id]) continue;
+        Edge ed;
+        ed.nodes[0] = L.v[id][0]; ed.nodes[1] = L.v[id][1];
+        if (L.v[id][2] >= 0) {
+            ed.hang = L.v[id][2];
+            const uint64_t ks[3] = {key(ed.nodes[0], ed.nodes[1]), key(ed.nodes[0], ed.hang), key(ed.nodes[1], ed.hang)};
+            for (int c = 0; c < 3; ++c) {
+                auto it = face_id.find(ks[c]);
+                if (it == face_id.end()) return "subdivided edge without its three elements";
+                ed.quads[c] = fown[it->second][0];
+            }
+        } else {
+            ed.quads[0] = fown[id][0];
+            ed.quads[1] = fown[id][1];
+            ed.bdry = ed.quads[1] < 0;
+            ed.marker = fmark[id];
+        }
+        edge_id.emplace(key(ed.nodes[0], ed.nodes[1]), (int)m.ed.size());
+        m.ed.push_back(std::move(ed));
+    }
+    face_id.swap(edge_id);
+    const int E = (int)m.ed.size();
     std::vector<int> curve_of(E, -1);
     std::vector<double> cxy;
     if (in.ncurved > 0) {
@@ -388,7 +659,17 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
     m.nodes_total = off;
     for (int k = 0; k < E; ++k) {
         Edge &ed = m.ed[k];
-        if (ed.bdry) { ed.N = m.el[ed.quads[0]].N; ed.NCurve = ed.N; }
+        if (ed.hang >= 0) {
+            // the big element's order; the curve at half the lowest of the three
+            // (QuadMesh_Construct.incf:121-139)
+            const int a = m.el[ed.quads[0]].N, b = m.el[ed.quads[1]].N, c = m.el[ed.quads[2]].N;
+            const int lo = std::min(a, std::min(b, c));
+            ed.N = a;
+            ed.NCurve = lo > 1 ? lo / 2 : lo;
+            // the reference sizes the NORTH mortar's reversal by the SOUTH element's order
+            // (QuadElement_EdgeProcedures.incf:556-559): defined when the two agree
+            if (b != c) return "subdivided edges need the two small elements of one order";
+        } else if (ed.bdry) { ed.N = m.el[ed.quads[0]].N; ed.NCurve = ed.N; }
         else {
             // the higher order of the two elements; the curve at half the lower one
             // (QuadMesh_Construct.incf:104-118)
@@ -398,9 +679,26 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
         }
         ed.curved = curve_of[k] >= 0;
         set_curve(m, ed, ed.curved ? &cxy[(size_t)2 * in.np1 * curve_of[k]] : nullptr, in.np1 - 1);
+        if (ed.hang >= 0) {
+            // the hanging node moves onto the edge at 0.5 (QuadMesh_Construct.incf:168-173)
+            double X[2];
+            point_at(m, ed, 0.5, X);
+            m.pts[2 * ed.hang] = X[0];
+            m.pts[2 * ed.hang + 1] = X[1];
+        }
+    }
+    // straight conforming interior edges again, with the moved nodes (:177-185)
+    for (int k = 0; k < E; ++k) {
+        Edge &ed = m.ed[k];
+        if (ed.hang < 0 && !ed.bdry && !ed.curved) set_curve(m, ed, nullptr, in.np1 - 1);
     }
     for (int k = 0; k < E; ++k) {
         Edge &ed = m.ed[k];
+        if (ed.hang >= 0) {
+            const std::string err = link_subdivided(m, ed, k);
+            if (!err.empty()) return err;
+            continue;
+        }
         Elem &L = m.el[ed.quads[0]];
         int pos, dir;
         locate(L.nodes, ed.nodes, pos, dir);
@@ -431,6 +729,7 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
         ed.area = A;
         double v = m.el[ed.quads[0]].volume;
         if (!ed.bdry) v = std::fmin(v, m.el[ed.quads[1]].volume);
+        if (ed.hang >= 0) v = std::fmin(v, m.el[ed.quads[2]].volume);
         ed.invh = A / v;
     }
     // zones (Mesh_ConstructZones, QuadMesh.f90:215-309)
@@ -452,9 +751,9 @@ std::string build_mesh(const hg_mesh_desc &in, Mesh &m) {
     for (int k = 0; k < E; ++k) {
         Edge &ed = m.ed[k];
         if (!ed.bdry) continue;
-        // an element edge with no neighbour that is no boundary edge of the file: a subdivided
-        // edge (hanging node), which MeshFile.f90:364-490 merges; not built here
-        if (ed.marker == 0) return "subdivided edges (hanging nodes) are not supported";
+        // an element edge with no neighbour that is no boundary edge of the file, which the
+        // merging of subdivided edges did not absorb
+        if (ed.marker == 0) return "an edge with neither a neighbour nor a boundary marker";
         if (ed.marker < 1 || ed.marker > in.nmarkers) return "boundary edge without a valid marker";
         m.zones[ed.marker - 1].edges.push_back(k);
     }

@@ -2,6 +2,7 @@
 // with the reference's numbering, element/edge geometry and boundary zones.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -29,13 +30,28 @@ double lagrange_at(const Spectral &s, double x, double *lj);  // barycentric l_j
 // (TripleMatrixProduct with diagonal mass matrices: (1/w_low(i) T_forward(l, i)) w_edge(l)).
 void mortar_matrices(int NLow, int N, double *Tf, double *Tb);
 
+// A subdivided edge's two mortars (SubdividedEdge_ConstructMortars and
+// SubdividedEdge_ComputeMortarsTransformationMatrices, QuadElement_EdgeMappings.incf:450-565,
+// QuadElement_EdgeProcedures.incf:530-748): NORTH runs from the hanging node to node 2, SOUTH
+// from node 1 to the hanging node; orders NN = max(N_left, N_north), NS = max(N_left, N_south).
+// Matrices row-major: T_L?_F [mortar][left], T_L?_B [left][mortar], T_R?_F [mortar][small],
+// T_R?_B [small][mortar] (a small element running its half backwards has them reversed).
+// Normals and surface Jacobians per mortar node (straight: the edge's n and dS / 2 at all).
+struct SubEdge {
+    int NN = 0, NS = 0;
+    std::vector<double> TLN_F, TLS_F, TLN_B, TLS_B, TRN_F, TRS_F, TRN_B, TRS_B;
+    std::vector<double> nN, nS, dSN, dSS;
+};
+
 struct Edge {
     int nodes[2] = {-1, -1};   // after the linking inversions
-    int quads[2] = {-1, -1};   // LEFT (lowest id) / boundary element, RIGHT
-    int loc[2] = {0, 0};       // 1..4 = bottom, right, top, left
+    int quads[3] = {-1, -1, -1};   // LEFT (lowest id) / boundary element, RIGHT (subdivided: NORTH, SOUTH)
+    int loc[3] = {0, 0, 0};        // 1..4 = bottom, right, top, left
     int inverse = 0, marker = 0, bdry = 0, curved = 0;
     int N = 0, NCurve = 0;
     int partner = -1, associated = 0;
+    int hang = -1;                 // subdivided edge (MeshFile.f90:364-490): the hanging node
+    std::shared_ptr<SubEdge> sd;
     std::vector<double> X, dX, n, dS;  // straight: dX, n, dS use entry 0
     double area = 0.0, invh = 0.0;
 };
@@ -45,7 +61,7 @@ struct Elem {
     int nodes[4];
     int edges[4];   // per position 1..4
     int dir[4];     // edgesDirection (+1 forward / -1 backward)
-    int qpos[4];    // 0: LEFT/boundary side of the edge, 1: RIGHT side
+    int qpos[4];    // 0: LEFT/boundary side of the edge, 1: RIGHT (subdivided: NORTH) side, 2: SOUTH
     std::vector<double> X;    // [(i + P j)*2 + d]
     std::vector<double> Ja;   // [(r*2+c)*P*P + i + P j]
     std::vector<double> jac;  // [i + P j]

@@ -145,6 +145,149 @@ __global__ __launch_bounds__(64) void k_mortar_flux(MortarArgs m) {
     }
 }
 
+// ---- subdivided edges: the big (LEFT) element's edge meets a NORTH and a SOUTH element; the
+// Riemann problem is solved on two mortars (NORTH: the hanging node to node 2, SOUTH: node 1 to
+// the hanging node) from the three sides' traces interpolated with the T_FWD matrices, and
+// returned with the T_BKW matrices: the big side takes NORTH's terms then SOUTH's, each small
+// side its own mortar's, negated (ComputeRiemannSolver_SubdividedEdge /
+// _CurvedSubdividedEdge, DGSpatialDiscretizationMethods.f90:761-1140).  One workgroup per edge:
+// lanes 0-31 the NORTH mortar's nodes, lanes 32-63 the SOUTH one's; then lane j a side's node j.
+
+struct SubMats {
+    const double *LF[2], *LB[2], *RF[2], *RB[2];   // [0] NORTH, [1] SOUTH
+};
+
+__device__ inline SubMats sub_mats(const MortarArgs &m, const SubMortarEdge &se) {
+    const int PL = se.Ps[0], PN = se.PM[0], PS = se.PM[1];
+    const double *p = m.mats + se.mat;
+    SubMats t;
+    t.LF[0] = p; p += PN * PL;
+    t.LF[1] = p; p += PS * PL;
+    t.LB[0] = p; p += PL * PN;
+    t.LB[1] = p; p += PL * PS;
+    t.RF[0] = p; p += PN * se.Ps[1];
+    t.RF[1] = p; p += PS * se.Ps[2];
+    t.RB[0] = p; p += se.Ps[1] * PN;
+    t.RB[1] = p;
+    return t;
+}
+
+__device__ inline const double *sub_trace(const double *const *buf, const SubMortarEdge &se, int side, int nv) {
+    return buf[se.sub[side]] + (int64_t)se.slot[side] * se.Ps[side] * nv;
+}
+
+// Mortar node i of half h: the big side's (a) and the small side's (b) NV values there, sums
+// over the sides' nodes in ascending order (as the reference accumulates them).
+template <int NV>
+__device__ void sub_values(const SubMortarEdge &se, const SubMats &T, int h, int i, const double *big,
+                           const double *small, double *a, double *b) {
+    const int PL = se.Ps[0], Pe = se.Ps[1 + h];
+    const double *Tl = T.LF[h] + i * PL, *Tr = T.RF[h] + i * Pe;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        double acc = 0.0;
+        for (int l = 0; l < PL; ++l) acc = acc + Tl[l] * big[l * NV + c];
+        a[c] = acc;
+        acc = 0.0;
+        for (int l = 0; l < Pe; ++l) acc = acc + Tr[l] * small[l * NV + c];
+        b[c] = acc;
+    }
+}
+
+// Mortar values (sV[h][l*4 + c]) at side `side`'s node j: the big side sums NORTH's then SOUTH's
+// T_BKW terms (scaled by `bigw`), a small side subtracts its own mortar's.
+__device__ void sub_to_side(const SubMortarEdge &se, const SubMats &T, int side, int j, const double (*sV)[kEdgeMax * 4],
+                            double bigw, double *out) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        double acc = 0.0;
+        if (side == 0) {
+            for (int h = 0; h < 2; ++h) {
+                const double *Tb = T.LB[h] + j * se.PM[h];
+                for (int l = 0; l < se.PM[h]; ++l) acc = acc + (bigw * Tb[l]) * sV[h][l * 4 + c];
+            }
+        } else {
+            const int h = side - 1;
+            const double *Tb = T.RB[h] + j * se.PM[h];
+            for (int l = 0; l < se.PM[h]; ++l) acc = acc - Tb[l] * sV[h][l * 4 + c];
+        }
+        out[c] = acc;
+    }
+}
+
+// BaseClass_ComputeSolutionRiemann_Subdivided (DGViscousMethods.f90:618-700), straight edges:
+// uStar = (QL + QR)/2 on each mortar; the big side takes 0.5 T_BKW uStar of both (its gradient
+// lift multiplies by the whole edge's dS n), a small side -T_BKW uStar of its own (lifted with
+// its mortar's dS / 2 and n).
+__global__ __launch_bounds__(64) void k_sub_ustar(MortarArgs m) {
+    __shared__ double sU[2][kEdgeMax * 4];
+    const SubMortarEdge se = m.subs[blockIdx.x];
+    const SubMats T = sub_mats(m, se);
+    const int t = threadIdx.x, h = t >> 5, i = t & 31;
+    if (i < se.PM[h]) {
+        double a[4], b[4];
+        sub_values<4>(se, T, h, i, sub_trace(m.tr, se, 0, 4), sub_trace(m.tr, se, 1 + h, 4), a, b);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sU[h][i * 4 + c] = 0.5 * (a[c] + b[c]);
+    }
+    __syncthreads();
+    for (int side = 0; side < 3; ++side) {
+        if (t < se.Ps[side]) {
+            double u[4];
+            sub_to_side(se, T, side, t, sU, 0.5, u);
+            double *o = m.mustar[se.sub[side]] + ((int64_t)se.slot[side] * se.Ps[side] + t) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) o[c] = u[c];
+        }
+    }
+}
+
+// FStar = (Fi - Fv) dS_mortar at each mortar node (its own normal and surface Jacobian), Fi the
+// interior Riemann solver, Fv the BR1 average of both sides' viscous fluxes; to each side.
+__global__ __launch_bounds__(64) void k_sub_flux(MortarArgs m) {
+    __shared__ double sF[2][kEdgeMax * 4];
+    const SubMortarEdge se = m.subs[blockIdx.x];
+    const SubMats T = sub_mats(m, se);
+    const int t = threadIdx.x, h = t >> 5, i = t & 31;
+    const Phys<double> ph(m.ph);
+    if (i < se.PM[h]) {
+        const double *g = m.sgeo + 3 * ((int64_t)se.geo + (h ? se.PM[0] : 0) + i);
+        const double nx = g[0], ny = g[1], dS = g[2];
+        double qa[4], qb[4], Fi[4], F[4];
+        sub_values<4>(se, T, h, i, sub_trace(m.tr, se, 0, 4), sub_trace(m.tr, se, 1 + h, 4), qa, qb);
+        riemann<-1, double>(m.ph.riemann, ph, qa, qb, nx, ny, Fi);
+        if (m.ph.ns) {
+            double da[8], db[8], fa[4], ga[4], fb[4], gb[4];
+            sub_values<8>(se, T, h, i, sub_trace(m.mdq, se, 0, 8), sub_trace(m.mdq, se, 1 + h, 8), da, db);
+            viscous_flux(ph, qa, da, fa, ga);
+            viscous_flux(ph, qb, db, fb, gb);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double FL0 = ph.mu * fa[c], FL1 = ph.mu * ga[c], FR0 = ph.mu * fb[c], FR1 = ph.mu * gb[c];
+                double acc = 0.0;
+                acc = acc + 0.5 * (FL0 + FR0) * nx;
+                acc = acc + 0.5 * (FL1 + FR1) * ny;
+                F[c] = (Fi[c] - acc) * dS;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) F[c] = Fi[c] * dS;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sF[h][i * 4 + c] = F[c];
+    }
+    __syncthreads();
+    for (int side = 0; side < 3; ++side) {
+        if (t < se.Ps[side]) {
+            double f[4];
+            sub_to_side(se, T, side, t, sF, 1.0, f);
+            double *o = m.mflux[se.sub[side]] + ((int64_t)se.slot[side] * se.Ps[side] + t) * 4;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) o[c] = f[c];
+        }
+    }
+}
+
 __global__ void k_scatter(const double *src, const int *map, int n, double *dst) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[map[i]] = src[i];
@@ -161,6 +304,18 @@ hipError_t launch_mortar_ustar(const MortarArgs &m, hipStream_t st) {
 hipError_t launch_mortar_flux(const MortarArgs &m, hipStream_t st) {
     if (m.nedges <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_mortar_flux, dim3(m.nedges), dim3(64), 0, st, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub_ustar(const MortarArgs &m, hipStream_t st) {
+    if (m.nsubs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sub_ustar, dim3(m.nsubs), dim3(64), 0, st, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_sub_flux(const MortarArgs &m, hipStream_t st) {
+    if (m.nsubs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sub_flux, dim3(m.nsubs), dim3(64), 0, st, m);
     return hipGetLastError();
 }
 

@@ -168,18 +168,32 @@ def test_boundary_condition_kinds_accepted_and_refused(gpu_lib):
             gpu.build_connectivity(channel(kind))
 
 
-def test_subdivided_edges_product_refuses_oracle_builds():
-    """A hanging node (one element's edge meets two elements, MeshFile.f90:364-490): the
-    product's builder refuses it with a message; the oracle merges the edge and links its three
-    elements (big LEFT, NORTH, SOUTH)."""
+def test_subdivided_edges_both_builders_agree():
+    """Hanging nodes (one element's edge meets two elements, MeshFile.f90:364-490): both builders
+    merge the edge (numbered after the remaining edges, in the order of the hanging nodes), link
+    its three elements (big LEFT, NORTH, SOUTH) and move the hanging node onto the big edge: the
+    product's O(E) builder reproduces the oracle's connectivity and geometry bit for bit, on a
+    three-element mesh, skewed refined channels with rotated node lists, and the reference's
+    VortexTransport mesh (34 subdivided edges, 10 curved)."""
     pts = np.array([[0, 0], [1, 0], [2, 0], [2, 1], [2, 2], [1, 2], [0, 2], [1, 1]], np.float64)
     quads = np.array([[0, 1, 5, 6], [1, 2, 3, 7], [7, 3, 4, 5]], np.int32)
     bedges = np.array([[0, 1], [1, 2], [2, 3], [3, 4], [4, 5], [5, 6], [6, 0]], np.int32)
     q = C.freestream_state(0.3)
     mesh = C.HiOMesh(pts, quads, bedges, np.ones(7, np.int32), [C.Zone(C.BC_FARFIELD, q=tuple(q))])
     case = C.Case("hanging", mesh, np.full(3, 3, np.int32), 0.3, ic=C.uniform(q))
-    with pytest.raises(RuntimeError, match="subdivided edges"):
-        gpu.build_connectivity(case)
     orc = O.Oracle(case)
     subs = [s for s in (orc.edge_subdivided(k) for k in range(orc.E)) if s is not None]
     assert len(subs) == 1 and sorted(subs[0]["elements"]) == [0, 1, 2] and subs[0]["elements"][0] == 0
+    from test_oracle_subdivided import refined_case
+    from test_benchmark_goldens import benchmark_case
+    zones = [C.Zone(C.BC_FARFIELD, q=tuple(q)) for _ in range(4)]
+    cases = [case] + [refined_case(zones, seed) for seed in range(4)] + [benchmark_case("vortex_transport")[0]]
+    for c in cases:
+        orc = O.Oracle(c)
+        for a, b in zip(orc.connectivity(), gpu.build_connectivity(c)):
+            np.testing.assert_array_equal(a, b)
+        X, Ja, jac = gpu.build_geometry(c)
+        Ja_o, jac_o = orc.metrics()
+        np.testing.assert_array_equal(X, orc.coords())
+        np.testing.assert_array_equal(Ja, Ja_o)
+        np.testing.assert_array_equal(jac, jac_o)
