@@ -17,7 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libhorses2d_gpu.so")
 SOURCES = ["hg_mesh.cpp", "hg_api.cpp", "hg_device.hip", "hg_batch.hip", "hg_noalloc.hip", "hg_mortar.hip"]
-ORDERS = range(2, 12)   # points per direction P = N+1: one object per order from hg_inst.hip
+ORDERS = range(2, 14)   # points per direction P = N+1: one object per order from hg_inst.hip
 ARCH = os.environ.get("HG_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=on: multiply-adds are fused only within one source expression, decided before

@@ -419,7 +419,7 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
                  const std::vector<int> &els, const std::vector<int> &loc, const std::vector<int> &mort) {
     h->K = (int)els.size();
     h->N = m.el[els[0]].N;
-    if (!hg::supported_order(h->N)) return set_err("hg_create: polynomial order not compiled (1..10)");
+    if (!hg::supported_order(h->N)) return set_err("hg_create: polynomial order not compiled (1..12)");
     h->P = h->N + 1;
     const int P = h->P, PP = P * P, K = h->K;
     h->nodes = (int64_t)K * PP;
@@ -907,7 +907,7 @@ int hg_create(const hg_mesh_desc *mesh, const hg_physics *phys, const hg_options
     std::sort(orders.begin(), orders.end());
     orders.erase(std::unique(orders.begin(), orders.end()), orders.end());
     for (int N : orders)
-        if (!hg::supported_order(N)) return set_err("hg_create: polynomial order not compiled (1..10)");
+        if (!hg::supported_order(N)) return set_err("hg_create: polynomial order not compiled (1..12)");
     // several orders or subdivided edges: sub-handles coupled by the mortar kernels
     bool subdivided = false;
     for (const auto &e : m.ed) subdivided = subdivided || e.hang >= 0;

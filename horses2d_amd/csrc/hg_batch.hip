@@ -420,6 +420,7 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
         hg_handle *h = cases[c];
         if (!h) return hg::handle_error("hg_batch_create: null case handle");
         if (!h->sub.empty()) return hg::handle_error("hg_batch_create: a case with several polynomial orders");
+        if (h->P > 11) return hg::handle_error("hg_batch_create: the batched kernels take polynomial orders 1..10");
         // artificial dissipation all or none: it decides whether the gradient pass runs every stage
         if (h->ph.ns != h0->ph.ns || h->ph.viscous != h0->ph.viscous || h->mixed != h0->mixed ||
             h->opt.scheme != h0->opt.scheme || h->ph.ad != h0->ph.ad || h->lap != h0->lap)

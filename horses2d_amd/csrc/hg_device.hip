@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void k_batch_resid(const double *part, const i
 
 }  // namespace
 
-bool supported_order(int N) { return N >= 1 && N <= 10; }
+bool supported_order(int N) { return N >= 1 && N <= 12; }
 
 #define HG_DISPATCH_P(P_, CALL)          \
     switch (P_) {                        \
@@ -225,6 +225,8 @@ bool supported_order(int N) { return N >= 1 && N <= 10; }
     case 9: return CALL(9);              \
     case 10: return CALL(10);            \
     case 11: return CALL(11);            \
+    case 12: return CALL(12);            \
+    case 13: return CALL(13);            \
     default: return hipErrorInvalidValue; \
     }
 
@@ -289,6 +291,8 @@ int tile_threads(int P, int tile) {
     case 9: return HG_CALL(9);
     case 10: return HG_CALL(10);
     case 11: return HG_CALL(11);
+    case 12: return HG_CALL(12);
+    case 13: return HG_CALL(13);
     default: return 0;
     }
 #undef HG_CALL
@@ -307,6 +311,8 @@ int tile_elems(int P, int tile) {
     case 9: return HG_CALL(9);
     case 10: return HG_CALL(10);
     case 11: return HG_CALL(11);
+    case 12: return HG_CALL(12);
+    case 13: return HG_CALL(13);
     default: return 0;
     }
 #undef HG_CALL
@@ -325,6 +331,8 @@ int stage_blocks(int P, int tile, int K) {
     case 9: return HG_CALL(9);
     case 10: return HG_CALL(10);
     case 11: return HG_CALL(11);
+    case 12: return HG_CALL(12);
+    case 13: return HG_CALL(13);
     default: return 0;
     }
 #undef HG_CALL
@@ -365,6 +373,8 @@ static int cons_blocks(int P, int K) {
     case 9: return HG_CALL(9);
     case 10: return HG_CALL(10);
     case 11: return HG_CALL(11);
+    case 12: return HG_CALL(12);
+    case 13: return HG_CALL(13);
     default: return 0;
     }
 #undef HG_CALL

@@ -1,5 +1,5 @@
 // One polynomial order's kernels for one set of equations: compiled once per order and
-// physics with -DHG_P=<N+1> (2..11) and -DHG_PHYS=<0 Euler | 1 Navier-Stokes BR1 | 2 IP>, so
+// physics with -DHG_P=<N+1> (2..13) and -DHG_PHYS=<0 Euler | 1 Navier-Stokes BR1 | 2 IP>, so
 // that the objects build in parallel.  The order's gradient, trace and reduction kernels
 // live in the HG_PHYS=0 object.
 #include "hg_kernels.h"

@@ -16,7 +16,8 @@ Williamson RK5, CFL 0.1, dt cap 0.01, steady mode for 250 iterations.  Finalize 
 four residuals within 1e-11 and the lift and drag coefficients of the wall within 1e-12; the
 surface monitors read the edge storage of the last stage's derivative evaluation.
 
-VortexTransport (VortexTransport/SETUP/ProblemFile.f90:107-150), reproduced by the oracle: a
+VortexTransport (VortexTransport/SETUP/ProblemFile.f90:107-150), reproduced by the oracle and
+the GPU path: a
 vortex carried through a periodic square at Mach 0.6 (Euler: the case has Re = 0 and no
 artificial dissipation), 144 quads of orders 5, 7 and 12 with 34 subdivided edges (10 of them
 curved: hanging nodes, two mortars per edge), Roe (the case file's "Inviscid Riemann Flux"
@@ -252,4 +253,19 @@ def test_oracle_reproduces_the_vortex_transport_golden():
     err = vortex_errors(orc.state(), case, X)
     print("oracle: iteration", it, "t", t, "errors", err, "golden", gold)
     assert it == gold["final_iteration"] and t == case.t_end
+    assert np.max(np.abs(err - np.array(gold["errors"]))) <= gold["residual_tolerance"]
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_the_vortex_transport_golden():
+    """The FP64 GPU path (one sub-handle per order, the p-non-conforming and subdivided mortar
+    kernels between them, RK5 replayed as a graph) through the same 3237 steps: Finalize's bars."""
+    from horses2d_amd import gpu
+    case, gold = benchmark_case("vortex_transport")
+    dg = gpu.DGSEM(case, "fp64")
+    X = dg.coords()
+    dg.integrate()
+    err = vortex_errors(dg.state(), case, X)
+    print("gpu: iteration", dg.iter, "t", dg.t, "errors", err, "golden", gold)
+    assert dg.iter == gold["final_iteration"] and dg.t == case.t_end
     assert np.max(np.abs(err - np.array(gold["errors"]))) <= gold["residual_tolerance"]

@@ -1,6 +1,7 @@
 """Meshes with subdivided edges (one element's edge meets two elements whose common node hangs on
 it, MeshFile.f90:364-490): the FP64 GPU path against the oracle, through the C ABI, at the parity
-bar of the conforming cases (1e-12 relative, max-norm).  The reference solves such an edge's
+bar of the conforming cases (1e-12 relative, max-norm; for QDot, twice the oracle's own
+sensitivity to last-place state changes where that is larger).  The reference solves such an edge's
 Riemann problem on two mortars from the three sides' traces and returns it with the mortar
 matrices (DGSpatialDiscretizationMethods.f90:761-1140, DGViscousMethods.f90:618-780); the
 library does it in the subdivided-mortar kernels between the per-order sub-handles.  The small
@@ -40,11 +41,10 @@ def br1_case(seed, riemann="hll"):
 
 
 def vortex_transport_case():
-    """The reference's VortexTransport benchmark mesh (34 subdivided edges, 10 curved), its order
-    12 zone taken down to 10, the highest order the kernels are compiled for."""
+    """The reference's VortexTransport benchmark case: 144 quads of orders 5, 7 and 12, 34
+    subdivided edges (10 curved), periodic boundaries, Euler."""
     from test_benchmark_goldens import benchmark_case
     case, _ = benchmark_case("vortex_transport")
-    case.N = np.minimum(case.N, 10).astype(np.int32)
     return case
 
 
