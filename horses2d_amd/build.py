@@ -61,6 +61,7 @@ def _compile(unit, incremental: bool = False) -> str:
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FORTRAN = os.path.join(PKG, "fortran")
 DRIVER = os.path.join(PKG, "bin", "horses_gpu_driver")
+EVAL = os.path.join(PKG, "bin", "contraction_eval")
 
 
 def build_fortran(verbose: bool = True) -> str:
@@ -101,6 +102,12 @@ def build(verbose: bool = True, incremental: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    # the measurement of the volume contraction on the vector units against the matrix cores
+    os.makedirs(os.path.dirname(EVAL), exist_ok=True)
+    cmd = [HIPCC, "-O3", "-std=c++17"] + DEVICE + ["-o", EVAL, os.path.join(CSRC, "hg_contraction_eval.hip")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     if os.path.exists(FLANG):
         build_fortran(verbose)
     return LIB
