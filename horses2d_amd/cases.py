@@ -63,6 +63,7 @@ class Zone:
     outflow: int = 0            # pressure outlet: OUTFLOW[...]
     mode: int = 1               # newDirichlet: NEWDIRICHLET_MODE[...]
     outflow_pressure: int = 1   # newDirichlet: NEWDIRICHLET_OUTFLOW[...]
+    function: str = ""          # "Use function": the problem-file function giving the state per node
 
     def ints(self):
         if self.kind == BC_PERIODIC:
@@ -118,6 +119,8 @@ class Case:
     residual_target: float = 0.0         # steady: stop when max |residual| < this
     t_end: float = 1.0e30                # transient: "Simulation time"
     t0: float = 0.0                      # "Initial time"
+    p_ref: float = 101325.0              # "Reference pressure" (Pa)
+    T_ref: float = 273.15                # "Reference Temperature" (K)
 
     def initial_state(self, coords: np.ndarray) -> np.ndarray:
         """Evaluate the IC at node coordinates ``coords`` (nnodes, 2) -> flat Q (4*nnodes)."""
@@ -362,6 +365,34 @@ def shear_layer(Mach: float, delta: float = 1.0 / 28.0, eps: float = 1.0e-2, see
         rho = np.ones_like(x)
         p = np.ones_like(x)
         return _pack(rho, u, v, p)
+    return ic
+
+
+def couette_flow(Mach: float, Pr: float, p_ref: float = 101325.0, T_ref: float = 273.15, perturb: bool = False):
+    """The Couette profile of the reference's Couette benchmark
+    (TestCases/Benchmark/Simulations/Couette/SETUP/ProblemFile.f90: BoundaryConditionFunction1
+    :194-218, UserDefinedInitialCondition :32-61 with ``perturb``: rho u times 1.005 where
+    x^2 + y^2 < 0.005), evaluated like the problem file in dimensional variables of air
+    (R = 287.15, gamma = 1.4, PhysicsNS.f90:97-98) with the reference values of the
+    dimensionless-tau form (PhysicsNS.f90:423-437: rho_ref = p_ref / (R T_ref), a_ref =
+    sqrt(p_ref / rho_ref), V = sqrt(gamma) Mach a_ref) and made dimensionless by
+    getDimensionlessVariables (VariableConversion.f90:32-43): T = T_ref - V^2 Pr / (8 cp)
+    (y + 1)(y - 1), rho = p_ref / (R T), rho u = rho V (y + 1) / 2, rho v = 0,
+    rho e = cv rho T + (rho u)^2 / (2 rho)."""
+    R, gamma = 287.15, 1.4
+    cp, cv = R * 3.5, R * 2.5
+    rho_ref = p_ref / (R * T_ref)
+    a_ref = np.sqrt(p_ref / rho_ref)
+    V = np.sqrt(gamma) * Mach * a_ref
+
+    def ic(x, y):
+        T = T_ref - (V * V) * Pr / (8.0 * cp) * (y + 1.0) * (y - 1.0)
+        rho = p_ref / (R * T)
+        rhou = rho * V * 0.5 * (y + 1.0)
+        rhoe = cv * rho * T + 0.5 * rhou * rhou / rho
+        if perturb:
+            rhou = np.where(x * x + y * y < 0.005, 1.005 * rhou, rhou)
+        return np.stack([rho / rho_ref, rhou / (rho_ref * a_ref), np.zeros_like(rho), rhoe / p_ref], axis=-1)
     return ic
 
 

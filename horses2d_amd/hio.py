@@ -158,8 +158,6 @@ def zone_state(path: str, in_label: str, Mach: float, T_ref: float, p_ref: float
     T = _real(read_value_in_region(path, "Temperature", in_label))
     M = _real(read_value_in_region(path, "Mach", in_label))
     aoa = _real(read_value_in_region(path, "Angle of attack", in_label))
-    if read_value_in_region(path, "Use function", in_label):
-        raise ValueError(f"{in_label}: states from problem-file functions are not supported")
     p = p_ref if p is None else p
     T = T_ref if T is None else T
     M = Mach if M is None else M
@@ -207,6 +205,14 @@ def read_zone(path: str, marker: int, Mach: float, T_ref: float, p_ref: float) -
         z.wall_velocity = (v * d[0], v * d[1])
     elif kind in (C.BC_FARFIELD, C.BC_DIRICHLET, C.BC_PRESSUREOUTLET, C.BC_PRESSUREINLET, C.BC_NEWDIRICHLET):
         z.q = zone_state(path, lab, Mach, T_ref, p_ref)
+        # "Use function" (AssociateBoundaryConditionFunction, BoundaryConditionFunctionDrivers.incf:
+        # 22-63): the state comes node by node from a problem-file function, which the caller
+        # evaluates at the zone's boundary nodes (DGSEM.set_boundary_state); q stays the default
+        fn = read_value_in_region(path, "Use function", lab)
+        if fn:
+            if fn not in ("Initial condition", "BC1", "BC2", "BC3", "BC4"):
+                raise ValueError(f"zone {marker}: unknown boundary-state function '{fn}'")
+            z.function = fn
         if kind == C.BC_PRESSUREOUTLET:
             # PressureOutletBC_Construct (PressureOutlet.incf:66-72): default partially non reflective
             of = read_value_in_region(path, "Outflow type", lab)
@@ -324,6 +330,7 @@ def case_from_hiocase(case_path: str, mesh_path: Optional[str] = None, ic=None, 
                   residual_target=0.0 if (mode == "Transient" or target is None) else target,
                   t_end=1.0e30 if (mode == "Steady" or t_end is None) else t_end,
                   t0=0.0 if t0 is None else t0)
+    case.p_ref, case.T_ref = p_ref, T_ref
     ic_name = rv("Initial condition") or ""
     if case.ic is None and ic_name == "Uniform":
         case.ic = C.uniform(C.freestream_state(Mach))

@@ -612,6 +612,7 @@ typedef struct {
     int vbc[MAXP];
     double uB[NCONS * MAXP], uSB[NCONS * MAXP];
     double FB[NCONS * MAXP];   /* WEAK_PRESCRIBED: F.n per node (times dS when applied) */
+    double *fq;                /* boundary-state function values per node [4 P] (or NULL: the zone state) */
     /* traces: storage(LEFT)=tq[0], storage(RIGHT)=tq[1] (boundary: tq[0]) */
     double tq[2][NCONS * MAXP];
     double tdq[2][2 * NCONS * MAXP]; /* [eq + 4*i + 4*P*dim] */
@@ -896,7 +897,7 @@ static void mesh_free(Mesh *m) {
     for (int e = 0; e < m->K; ++e) { free(m->el[e].X); free(m->el[e].Ja); free(m->el[e].jac); }
     for (int i = 0; i < m->E; ++i) {
         free(m->ed[i].X); free(m->ed[i].dX); free(m->ed[i].n); free(m->ed[i].dS);
-        free(m->ed[i].Tf); free(m->ed[i].Tb);
+        free(m->ed[i].Tf); free(m->ed[i].Tb); free(m->ed[i].fq);
         if (m->ed[i].sd) {
             SubData *d = m->ed[i].sd;
             free(d->TLN_F); free(d->TLS_F); free(d->TLN_B); free(d->TLS_B);
@@ -1426,18 +1427,16 @@ static void prolong(Mesh *m, int grad) {
 /* BoundaryConditions: FarfieldBC_UpdateSolution (Farfield.incf:138-376),
  * ViscousWall_UpdateSolution (ViscousWall.incf:158-254). Periodic edges read
  * the partner trace directly (QuadMesh_Zones.incf:111-117). */
-static void update_bcs(Mesh *m) {
+/* The boundary states of edge ed's nodes i0 .. i1-1 from zone Z's state Z->q. */
+static void update_edge_bc(Mesh *m, const Zone *Z, Edge *ed, int i0, int i1) {
     const Phys *ph = &m->ph;
-    for (int z = 0; z < m->nmarkers; ++z) {
-        Zone *Z = &m->zones[z];
-        for (int a = 0; a < Z->nedges; ++a) {
-            Edge *ed = &m->ed[Z->edges[a]];
-            int P = ed->N + 1;
+    {
+        {
             if (Z->type == BC_FARFIELD) {
                 const double *q = Z->q;
                 double rhoI = q[0], uI = q[1] / rhoI, vI = q[2] / rhoI;
                 double pI = ph->gm1 * (q[3] - 0.5 * q[1] * uI - 0.5 * q[2] * vI);
-                for (int i = 0; i < P; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
                     const double *Qt = ed->tq[0] + 4 * i;
                     double rhoL = Qt[0], uL = Qt[1] / rhoL, vL = Qt[2] / rhoL;
@@ -1479,7 +1478,7 @@ static void update_bcs(Mesh *m) {
                 /* EulerWall_UpdateSolution (EulerWall.incf:96-159): the wall state q (normal
                  * momentum removed, energy from p and the tangential momentum) gives the
                  * prescribed flux FB = sum_d F_d(q) n_d dS; uB mirrors the normal momentum */
-                for (int i = 0; i < P; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
                     const double dS = ed->curved ? ed->dS[i] : ed->dS[0];
                     const double *Qt = ed->tq[0] + 4 * i;
@@ -1506,7 +1505,7 @@ static void update_bcs(Mesh *m) {
             } else if (Z->type == BC_DIRICHLET) {
                 /* DirichletBC_Associate (Dirichlet.incf:78-104): uB = the zone state;
                  * DirichletBC_UpdateSolution (:106-337): uSB and the viscous type per node */
-                for (int i = 0; i < P; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
                     const double *Qt = ed->tq[0] + 4 * i;
                     double *uB = ed->uB + 4 * i, *uSB = ed->uSB + 4 * i;
@@ -1541,7 +1540,7 @@ static void update_bcs(Mesh *m) {
                 double rhoI = q[0], uI = q[1] / rhoI, vI = q[2] / rhoI;
                 double pI = ph->gm1 * (q[3] - 0.5 * q[1] * uI - 0.5 * q[2] * vI);
                 double nrm = sqrt(uI * uI + vI * vI), nIx = uI / nrm, nIy = vI / nrm;
-                for (int i = 0; i < P; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
                     const double *Qt = ed->tq[0] + 4 * i;
                     double *uB = ed->uB + 4 * i, *uSB = ed->uSB + 4 * i;
@@ -1594,7 +1593,7 @@ static void update_bcs(Mesh *m) {
                 double MExtSq = (uE * uE + vE * vE) / (ph->gamma * pE) * q[0];
                 double sExt = pE / pow(q[0], ph->gamma);
                 double ptExt = pE * pow(1.0 + 0.5 * ph->gm1 * MExtSq, ph->cp);
-                for (int i = 0; i < P; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
                     const double *Qt = ed->tq[0] + 4 * i;
                     double *uB = ed->uB + 4 * i, *uSB = ed->uSB + 4 * i;
@@ -1642,7 +1641,7 @@ static void update_bcs(Mesh *m) {
                 double atExtSq = ph->gamma * pE / qe[0] * (1.0 + 0.5 * ph->gm1 * MExtSq);
                 double sExt = pE / pow(qe[0], ph->gamma);
                 double ptExt = pE * pow(1.0 + 0.5 * ph->gm1 * MExtSq, ph->cp);
-                for (int i = 0; i < P; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
                     const double dS = ed->curved ? ed->dS[i] : ed->dS[0];
                     const double *Qt = ed->tq[0] + 4 * i;
@@ -1688,7 +1687,7 @@ static void update_bcs(Mesh *m) {
                     for (int k = 0; k < 4; ++k) ed->uB[4 * i + k] = q[k];
                 }
             } else if (Z->type == BC_VISCOUSWALL) {
-                for (int i = 0; i < P; ++i) {
+                for (int i = i0; i < i1; ++i) {
                     const double *Qt = ed->tq[0] + 4 * i;
                     double *uB = ed->uB + 4 * i, *uSB = ed->uSB + 4 * i;
                     uB[0] = Qt[0];
@@ -1705,6 +1704,30 @@ static void update_bcs(Mesh *m) {
                     else
                         uSB[3] = Qt[3] - Ek + 0.5 * uSB[0] * (Z->vw[0] * Z->vw[0] + Z->vw[1] * Z->vw[1]);
                 }
+            }
+        }
+    }
+}
+
+/* Every boundary edge's states (DGSpatial_newTimeStep, DGSpatialDiscretizationMethods.f90:
+ * 118-150).  A zone whose state comes from a boundary-state function ("Use function",
+ * BoundaryConditionFunctionDrivers.incf:22-63) has it per edge node (ed->fq, set by the
+ * caller from the function at the edge's node coordinates): each node then sees the zone with
+ * its own state, as the reference evaluates q = BoundaryState(x(iXi), time) node by node. */
+static void update_bcs(Mesh *m) {
+    for (int z = 0; z < m->nmarkers; ++z) {
+        const Zone *Z = &m->zones[z];
+        for (int a = 0; a < Z->nedges; ++a) {
+            Edge *ed = &m->ed[Z->edges[a]];
+            const int P = ed->N + 1;
+            if (!ed->fq) {
+                update_edge_bc(m, Z, ed, 0, P);
+                continue;
+            }
+            for (int i = 0; i < P; ++i) {
+                Zone Zi = *Z;
+                for (int k = 0; k < 4; ++k) Zi.q[k] = ed->fq[4 * i + k];
+                update_edge_bc(m, &Zi, ed, i, i + 1);
             }
         }
     }
@@ -1769,6 +1792,29 @@ static void edge_flux_to_side(const Edge *ed, int side, const double *F, double 
                 if (side == 0) acc = acc + ed->Tb[i * P + l] * F[eq + 4 * l];
                 else acc = acc - ed->Tb[i * P + l] * F[eq + 4 * l];
             }
+            out[eq + 4 * o] = acc;
+        }
+    }
+}
+
+/* Edge_ProjectGradientFluxes, Types 1-6 (QuadElement_EdgeProcedures.incf:1208-1339): one
+ * direction of a side's gradient flux at the edge's nodes (G[eq + 4 i]) to that side's own
+ * nodes.  Unlike ProjectFluxes the RIGHT side is not negated (each side's flux was formed from
+ * its own state); the low side takes T_backward G, a RIGHT side of an inverse edge reversed. */
+static void edge_grad_to_side(const Edge *ed, int side, const double *G, double *out) {
+    int P = ed->N + 1, Pl = ed->NLow + 1;
+    if (side != ed->low) {
+        for (int i = 0; i < P; ++i) {
+            int o = (side == 1 && ed->inverse) ? P - 1 - i : i;
+            for (int eq = 0; eq < 4; ++eq) out[eq + 4 * o] = G[eq + 4 * i];
+        }
+        return;
+    }
+    for (int i = 0; i < Pl; ++i) {
+        int o = (side == 1 && ed->inverse) ? Pl - 1 - i : i;
+        for (int eq = 0; eq < 4; ++eq) {
+            double acc = 0.0;
+            for (int l = 0; l < P; ++l) acc = acc + ed->Tb[i * P + l] * G[eq + 4 * l];
             out[eq + 4 * o] = acc;
         }
     }
@@ -2009,8 +2055,10 @@ static void gradient_ip(Mesh *m) {
 
 /* Viscous Riemann solvers: BR1 (DGViscous_BR1.f90:330-441), IP
  * (DGViscous_IP.f90:88-214), with the viscosity mu + edge % mu_a. */
-static void visc_interior(Mesh *m, Edge *ed, const double *uL, const double *uR, const double *dL, const double *dR,
-                          const double *nn, double *Fs) {
+/* Nn: the order of the nodes the flux is evaluated on (the edge's, a subdivided edge's mortar's),
+ * which sets the IP penalty sigma0 mu Nn^2 / h. */
+static void visc_interior_n(Mesh *m, Edge *ed, int Nn, const double *uL, const double *uR, const double *dL,
+                            const double *dR, const double *nn, double *Fs) {
     const Phys *ph = &m->ph;
     const double mu = ph->mu + ed->mu_a;
     double fL[4], gL[4], fR[4], gR[4];
@@ -2024,9 +2072,13 @@ static void visc_interior(Mesh *m, Edge *ed, const double *uL, const double *uR,
         Fs[k] = acc;
     }
     if (m->viscous == VISC_IP) {
-        double pen = m->sigma0 * mu * (double)ed->N * (double)ed->N * ed->invh;
+        double pen = m->sigma0 * mu * (double)Nn * (double)Nn * ed->invh;
         for (int k = 0; k < 4; ++k) Fs[k] = Fs[k] - pen * (uL[k] - uR[k]);
     }
+}
+static void visc_interior(Mesh *m, Edge *ed, const double *uL, const double *uR, const double *dL, const double *dR,
+                          const double *nn, double *Fs) {
+    visc_interior_n(m, ed, ed->N, uL, uR, dL, dR, nn, Fs);
 }
 
 /* DGWeakIntegrals.f90:268-495 Scalar_StdGradientFace, subtracted from QDot */
@@ -2270,7 +2322,7 @@ static void time_derivative(Mesh *m) {
                                 a[eq + 4 * d] = dq[d][h][eq + 4 * i];
                                 b[eq + 4 * d] = dq[d][2 + h][eq + 4 * i];
                             }
-                        visc_interior(m, ed, qa + 4 * i, qb + 4 * i, a, b, nrm + 2 * i, Fv);
+                        visc_interior_n(m, ed, PM[h] - 1, qa + 4 * i, qb + 4 * i, a, b, nrm + 2 * i, Fv);
                         for (int eq = 0; eq < 4; ++eq) Fs[h][eq + 4 * i] = (Fi[eq] - Fv[eq]) * dSm[i];
                     } else {
                         for (int eq = 0; eq < 4; ++eq) Fs[h][eq + 4 * i] = Fi[eq] * dSm[i];
@@ -2287,6 +2339,38 @@ static void time_derivative(Mesh *m) {
             std_face(m, &m->el[ed->quads[0]], ed->loc[0], FL);
             std_face(m, &m->el[ed->quads[1]], ed->loc[1], FR);
             std_face(m, &m->el[ed->quads[2]], ed->loc[2], FRS);
+            if (gradfl) {
+                /* IP_GradientRiemannSolver on each mortar, times the mortar's dS (straight: its
+                 * node 0's, curved: each node's), returned with T_BKW to all three sides, the
+                 * small sides not negated (:906-940, :1093-1136) */
+                double Gm[2][2][8 * MAXP];   /* [mortar][big, small][eq + 4 i + 4 PM d] */
+                for (int h = 0; h < 2; ++h) {
+                    const double *nrm = h == 0 ? sd->nN : sd->nS, *dSm = h == 0 ? sd->dSN : sd->dSS;
+                    for (int i = 0; i < PM[h]; ++i) {
+                        double gl[8], gr[8];
+                        ip_grad_interior(m, ed, q[h] + 4 * i, q[2 + h] + 4 * i, nrm + 2 * i, gl, gr);
+                        const double sc = ed->curved ? dSm[i] : dSm[0];
+                        for (int d = 0; d < 2; ++d)
+                            for (int eq = 0; eq < 4; ++eq) {
+                                Gm[h][0][eq + 4 * i + 4 * PM[h] * d] = gl[eq + 4 * d] * sc;
+                                Gm[h][1][eq + 4 * i + 4 * PM[h] * d] = gr[eq + 4 * d] * sc;
+                            }
+                    }
+                }
+                double GRS[8 * MAXP];
+                for (int c = 0; c < 8 * PL; ++c) GL[c] = 0.0;
+                for (int c = 0; c < 8 * PN; ++c) GR[c] = 0.0;
+                for (int c = 0; c < 8 * PS; ++c) GRS[c] = 0.0;
+                for (int d = 0; d < 2; ++d) {
+                    mortar_out(sd->TLN_B, PL, PM[0], Gm[0][0] + 4 * PM[0] * d, 1.0, GL + 4 * PL * d);
+                    mortar_out(sd->TLS_B, PL, PM[1], Gm[1][0] + 4 * PM[1] * d, 1.0, GL + 4 * PL * d);
+                    mortar_out(sd->TRN_B, PN, PM[0], Gm[0][1] + 4 * PM[0] * d, 1.0, GR + 4 * PN * d);
+                    mortar_out(sd->TRS_B, PS, PM[1], Gm[1][1] + 4 * PM[1] * d, 1.0, GRS + 4 * PS * d);
+                }
+                std_gradient_face(m, &m->el[ed->quads[0]], ed->loc[0], GL);
+                std_gradient_face(m, &m->el[ed->quads[1]], ed->loc[1], GR);
+                std_gradient_face(m, &m->el[ed->quads[2]], ed->loc[2], GRS);
+            }
         } else if (!ed->bdry && ed->low >= 0) {
             /* p-non-conforming (ComputeRiemannSolver_InteriorEdge / _InteriorCurvedEdge
              * :545-759): the Riemann problem on the edge's nodes from the projected traces and
@@ -2320,6 +2404,31 @@ static void time_derivative(Mesh *m) {
             edge_flux_to_side(ed, 1, Fs, FR);
             std_face(m, &m->el[ed->quads[0]], ed->loc[0], FL);
             std_face(m, &m->el[ed->quads[1]], ed->loc[1], FR);
+            if (gradfl) {
+                /* IP_GradientRiemannSolver on the projected traces, each side's flux returned by
+                 * ProjectGradientFluxes: straight edges project and then multiply by dS(0),
+                 * curved edges multiply by dS(i) first (:638-651, :744-757) */
+                double Ge[2][8 * MAXP];   /* [side][eq + 4 i + 4 P d] on the edge's nodes */
+                for (int i = 0; i < P; ++i) {
+                    const double *nn = ed->curved ? ed->n + 2 * i : ed->n;
+                    double gl[8], gr[8];
+                    ip_grad_interior(m, ed, qL + 4 * i, qR + 4 * i, nn, gl, gr);
+                    for (int d = 0; d < 2; ++d)
+                        for (int eq = 0; eq < 4; ++eq) {
+                            Ge[0][eq + 4 * i + 4 * P * d] = ed->curved ? gl[eq + 4 * d] * ed->dS[i] : gl[eq + 4 * d];
+                            Ge[1][eq + 4 * i + 4 * P * d] = ed->curved ? gr[eq + 4 * d] * ed->dS[i] : gr[eq + 4 * d];
+                        }
+                }
+                for (int side = 0; side < 2; ++side) {
+                    const int Ps = side == 0 ? PL : PR;
+                    double *Gs = side == 0 ? GL : GR;
+                    for (int d = 0; d < 2; ++d) edge_grad_to_side(ed, side, Ge[side] + 4 * P * d, Gs + 4 * Ps * d);
+                    if (!ed->curved)
+                        for (int c = 0; c < 8 * Ps; ++c) Gs[c] = Gs[c] * ed->dS[0];
+                }
+                std_gradient_face(m, &m->el[ed->quads[0]], ed->loc[0], GL);
+                std_gradient_face(m, &m->el[ed->quads[1]], ed->loc[1], GR);
+            }
         } else if (!ed->bdry) {
             /* ComputeRiemannSolver_InteriorEdge :545-653 / InteriorCurvedEdge :655-759 */
             for (int i = 0; i < P; ++i) {
@@ -2483,13 +2592,14 @@ void *or_create(const MeshIn *in, const OrOptions *o) {
     if (!riemann_fn(m->riemann)) { mesh_free(m); fail("unknown Riemann solver"); return NULL; }
     m->ad = m->ns ? o->ad : 0;   /* the module is Navier-Stokes only */
     m->ad_ceps = o->ad_ceps;
-    for (int k = 0; k < m->E; ++k)
-        if ((m->ed[k].low >= 0 || m->ed[k].sub) && ((m->ns && m->viscous == VISC_IP) || m->ad)) {
+    for (int k = 0; k < m->E; ++k) {
+        const int sub = m->ed[k].sub != 0, pnc = m->ed[k].low >= 0;
+        if ((sub || pnc) && m->ad) {
+            fail(sub ? "subdivided edges: no artificial dissipation" : "p-non-conforming edges: no artificial dissipation");
             mesh_free(m);
-            fail(m->ed[k].sub ? "subdivided edges: the Euler and BR1 forms without artificial dissipation only"
-                              : "p-non-conforming edges: the Euler and BR1 forms without artificial dissipation only");
             return NULL;
         }
+    }
     m->mu_e = (double *)calloc(m->K + 1, sizeof(double));
     m->lap_e = (double *)calloc(m->K + 1, sizeof(double));
     for (int k = 0; k < m->E; ++k) m->ed[k].mu_a = 0.0;
@@ -2550,6 +2660,52 @@ int or_get_connectivity(void *h, int *edges, int *elem, int *partner) {
     return 0;
 }
 /* Edge geometry export: geo[k*(3*MAXPE)+...] normals and dS in edge node order, per node. */
+/* Node count of zone `marker`'s edges (ascending edge id, each edge's N + 1 nodes in its own
+ * order), or -1. */
+static int64_t zone_nodes(const Mesh *m, int marker) {
+    if (marker < 1 || marker > m->nmarkers) return -1;
+    const Zone *Z = &m->zones[marker - 1];
+    int64_t n = 0;
+    for (int a = 0; a < Z->nedges; ++a) n += m->ed[Z->edges[a]].N + 1;
+    return n;
+}
+/* The coordinates x[2 node + d] of those nodes (edge % x, where the reference evaluates a
+ * boundary-state function); x = NULL returns their number. */
+int64_t or_boundary_nodes(void *h, int marker, int64_t n, double *x) {
+    Mesh *m = h;
+    if (!x) return zone_nodes(m, marker) < 0 ? fail("or_boundary_nodes: marker out of range") : zone_nodes(m, marker);
+    if (zone_nodes(m, marker) != n) return fail("or_boundary_nodes: marker or node count mismatch");
+    const Zone *Z = &m->zones[marker - 1];
+    int64_t o = 0;
+    for (int a = 0; a < Z->nedges; ++a) {
+        const Edge *ed = &m->ed[Z->edges[a]];
+        for (int i = 0; i <= ed->N; ++i, ++o) { x[2 * o] = ed->X[2 * i]; x[2 * o + 1] = ed->X[2 * i + 1]; }
+    }
+    return 0;
+}
+/* Boundary-state function values q[4 node + eq] at those nodes (dimensionless), replacing the
+ * zone state of a far-field, Dirichlet, newDirichlet or pressure zone; q = NULL drops them. */
+int or_set_boundary_state(void *h, int marker, int64_t n, const double *q) {
+    Mesh *m = h;
+    if (zone_nodes(m, marker) < 0 || (q && zone_nodes(m, marker) != n))
+        return fail("or_set_boundary_state: marker or node count mismatch");
+    const Zone *Z = &m->zones[marker - 1];
+    if (q && Z->type != BC_FARFIELD && Z->type != BC_DIRICHLET && Z->type != BC_NEWDIRICHLET &&
+        Z->type != BC_PRESSUREOUTLET && Z->type != BC_PRESSUREINLET)
+        return fail("or_set_boundary_state: the zone's kind has no boundary state");
+    int64_t o = 0;
+    for (int a = 0; a < Z->nedges; ++a) {
+        Edge *ed = &m->ed[Z->edges[a]];
+        const int P = ed->N + 1;
+        if (!q) { free(ed->fq); ed->fq = NULL; continue; }
+        if (!ed->fq) ed->fq = (double *)malloc(sizeof(double) * 4 * P);
+        memcpy(ed->fq, q + 4 * o, sizeof(double) * 4 * P);
+        o += P;
+    }
+    m->traces_valid = 0;
+    return 0;
+}
+
 int or_get_edge_geometry(void *h, int maxP, double *nx, double *ny, double *dS, double *invh) {
     Mesh *m = h;
     for (int k = 0; k < m->E; ++k) {

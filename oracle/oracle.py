@@ -87,6 +87,9 @@ def lib():
         _lib.or_surface_integrals.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         _lib.or_boundary_states.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
         _lib.or_probes.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 5
+        _lib.or_boundary_nodes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p]
+        _lib.or_boundary_nodes.restype = ctypes.c_int64
+        _lib.or_set_boundary_state.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p]
         _lib.or_get_metrics.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _lib.or_get_connectivity.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 3
         _lib.or_get_edge_geometry.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 4
@@ -292,6 +295,28 @@ class Oracle:
         if lib().or_surface_integrals(self.h, int(marker), _ptr(r)) != 0:
             raise RuntimeError(lib().or_last_error().decode())
         return r
+
+    def boundary_nodes(self, marker: int):
+        """Coordinates [n, 2] of zone ``marker``'s edge nodes (ascending edge id, each edge's nodes
+        in its own order): where a boundary-state function is evaluated."""
+        n = lib().or_boundary_nodes(self.h, int(marker), 0, None)
+        if n < 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        x = np.zeros((int(n), 2))
+        if lib().or_boundary_nodes(self.h, int(marker), int(n), _ptr(x)) != 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return x
+
+    def set_boundary_state(self, marker: int, q):
+        """Boundary-state function values [n, 4] (dimensionless) at those nodes, in place of the
+        zone state; None drops them."""
+        if q is None:
+            if lib().or_set_boundary_state(self.h, int(marker), -1, None) != 0:
+                raise RuntimeError(lib().or_last_error().decode())
+            return
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 4)
+        if lib().or_set_boundary_state(self.h, int(marker), q.shape[0], _ptr(q)) != 0:
+            raise RuntimeError(lib().or_last_error().decode())
 
     def boundary_states(self, marker: int):
         """(trace, normal, uB, uSB, viscous type) per edge node of zone ``marker`` from the

@@ -33,6 +33,8 @@ MESHES = {
     "cylinder_benchmark": "TestCases/Benchmark/Simulations/CylinderMach2/MESH/Cylinder.HiOMesh",
     "cylinder_re100": "TestCases/Benchmark/Simulations/CylinderRe100/MESH/Cylinder.HiOMesh",
     "vortex_transport_benchmark": "TestCases/Benchmark/Simulations/VortexTransport/MESH/VortexTransport.HiOMesh",
+    "cylinder_re1000": "TestCases/Benchmark/Simulations/CylinderRe1000/MESH/CylinderInflation.HiOMesh",
+    "couette": "TestCases/Benchmark/Simulations/Couette/MESH/Couette.HiOMesh",
 }
 
 # benchmark case -> (case file, mesh fixture, problem file)
@@ -47,6 +49,11 @@ BENCHMARKS = {
     "vortex_transport": ("TestCases/Benchmark/Simulations/VortexTransport/SETUP/VortexTransport.HiOCase",
                          "vortex_transport_benchmark",
                          "TestCases/Benchmark/Simulations/VortexTransport/SETUP/ProblemFile.f90"),
+    "cylinder_re1000": ("TestCases/Benchmark/Simulations/CylinderRe1000/SETUP/CylinderRe1000.HiOCase",
+                        "cylinder_re1000",
+                        "TestCases/Benchmark/Simulations/CylinderRe1000/SETUP/ProblemFile.f90"),
+    "couette": ("TestCases/Benchmark/Simulations/Couette/SETUP/Couette.HiOCase", "couette",
+                "TestCases/Benchmark/Simulations/Couette/SETUP/ProblemFile.f90"),
 }
 
 
@@ -89,7 +96,8 @@ def case_parameters(case_file, mesh_file):
         "zones": [{"kind": z.kind, "q": list(z.q), "outflow": z.outflow, "mode": z.mode,
                    "outflow_pressure": z.outflow_pressure, "isothermal": z.isothermal, "Tw": z.Tw,
                    "wall_velocity": list(z.wall_velocity), "riemann": z.riemann,
-                   "direction": z.direction, "partner": z.partner} for z in c.mesh.zones],
+                   "direction": z.direction, "partner": z.partner, "function": z.function} for z in c.mesh.zones],
+        "p_ref": c.p_ref, "T_ref": c.T_ref,
     }
 
 
@@ -99,7 +107,7 @@ def finalize_values(problem_file):
     text = open(problem_file).read()
     num = r"([-+0-9.]+E[-+][0-9]+)_RP"
     out = {}
-    m = re.search(r"residuals\(4\)\s*=\s*\[([^\]]*)\]", text)
+    m = re.search(r"[Rr]esiduals\((?:4|NCONS)\)\s*&?\s*=\s*\[([^\]]*)\]", text)
     if m:
         out["residuals"] = [float(v) for v in re.findall(num, m.group(1))]
     m = re.search(r"(?:finalIteration|expectedIter)\s*=\s*(\d+)", text)

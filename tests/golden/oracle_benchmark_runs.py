@@ -20,15 +20,18 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 def main(name):
     from oracle.oracle import Oracle
-    from test_benchmark_goldens import benchmark_case, lift_drag
+    from test_benchmark_goldens import apply_boundary_functions, benchmark_case, lift_drag
     case, _ = benchmark_case(name)
     orc = Oracle(case)
+    apply_boundary_functions(orc, case)
     orc.set_state(case.initial_state(orc.coords()))
     t0 = time.time()
     t, dts, it, hist = orc.integrate(case.steps, dt_max=case.dt_max, t_end=case.t_end, scheme=case.scheme,
                                      mode=case.mode, residual_target=case.residual_target)
     out = {"final_iteration": int(it), "t": float(t), "residuals": [float(v) for v in hist[-1]],
            "first_dt": float(dts[0]), "last_dt": float(dts[-1]), "seconds": time.time() - t0}
+    if len(hist) >= 20:   # an early point of the history, for a short test to start the run from
+        out["residuals_at_20"] = [float(v) for v in hist[19]]
     zones = [z.kind for z in case.mesh.zones]
     if 3 in zones:   # lift and drag of the last Euler-wall zone (the body)
         marker = len(zones) - zones[::-1].index(3)

@@ -245,6 +245,18 @@ def test_unsupported_zones_are_refused(tmp_path):
         hio.case_from_hiocase(path)
 
 
+def test_use_function_zones_record_the_function(tmp_path):
+    """"Use function = BC1" (AssociateBoundaryConditionFunction, BoundaryConditionFunctionDrivers.
+    incf:22-63): the zone keeps its default state and names the function its caller evaluates."""
+    _, path = _box_files(tmp_path)
+    with open(path) as f:
+        text = f.read()
+    with open(path, "w") as f:
+        f.write(text.replace("   Angle of attack = 30.0", "   Use function = BC1"))
+    c = hio.case_from_hiocase(path)
+    assert [z.function for z in c.mesh.zones if z.function] == ["BC1"]
+
+
 def test_hiorst_round_trip_and_restart_initial_condition(tmp_path):
     from oracle.oracle import Oracle
     d, path = _box_files(tmp_path)

@@ -10,8 +10,10 @@ what the mathematics fixes, independent of the restatement.
   element node lists are rotated (every edge Type 3-6 occurs), Euler and BR1.
 * In a closed box of Euler walls the discrete mass and energy are conserved with mortar edges
   between orders.
-The whole path is pinned end to end by the reference's CylinderRe100 golden
-(tests/test_benchmark_goldens.py), a mesh of orders 2, 4, 6 and 8.
+The whole path is pinned end to end by the reference's goldens (tests/test_benchmark_goldens.py):
+CylinderRe100 (BR1, orders 2, 4, 6 and 8), CylinderRe1000 (interior penalty with the gradient
+fluxes of ProjectGradientFluxes, orders 4 to 10) and Couette (BR1 across curved edges between
+orders 5, 7 and 10, a function-given inflow).
 """
 import numpy as np
 import pytest
@@ -67,20 +69,43 @@ def test_projection_matrices():
     assert len(pairs) >= 10
 
 
-@pytest.mark.parametrize("ns", [False, True])
+@pytest.mark.parametrize("viscous", ["none", "br1", "sipg", "nipg"])
 @pytest.mark.parametrize("seed", [4, 9])
-def test_free_stream_is_preserved_across_orders(ns, seed):
+def test_free_stream_is_preserved_across_orders(viscous, seed):
     from oracle.oracle import Oracle
     q = C.freestream_state(0.3, 0.2)
     zones = [C.Zone(C.BC_FARFIELD, q=tuple(q)) for _ in range(4)]
     mesh = rotated_channel(6, 4, zones, 0.3, seed)
     N = np.random.default_rng(seed).integers(1, 9, size=mesh.K).astype(np.int32)
-    c = C.Case("fs", mesh, N, 0.3, 100.0 if ns else 0.0, 0.72, ns, riemann="roe", viscous="br1" if ns else "none",
-               ic=C.uniform(q))
+    ns = viscous != "none"
+    c = C.Case("fs", mesh, N, 0.3, 100.0 if ns else 0.0, 0.72, ns, riemann="roe",
+               viscous={"none": "none", "br1": "br1"}.get(viscous, "ip"),
+               ip_method=viscous if viscous in ("sipg", "nipg") else "sipg", sigma0=10.0, ic=C.uniform(q))
     orc = Oracle(c)
     assert mortar_edges(orc)
     orc.set_state(c.initial_state(orc.coords()))
-    assert np.max(np.abs(orc.time_derivative())) <= 1e-11
+    # the penalty sigma0 mu N^2 / h (about 60 here) multiplies the rounding of the trace jumps
+    assert np.max(np.abs(orc.time_derivative())) <= (1e-10 if viscous in ("sipg", "nipg") else 1e-11)
+
+
+@pytest.mark.parametrize("viscous", ["none", "br1", "ip"])
+def test_free_stream_is_preserved_across_curved_orders(viscous):
+    """The curved annulus with the orders changing from ring to ring (curved p-non-conforming
+    edges: n and dS per node, BR1 uStar dS n projected per direction), far field all around."""
+    from oracle.oracle import Oracle
+    c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
+    q = c.mesh.zones[1].q
+    c.mesh.zones[0] = C.Zone(C.BC_FARFIELD, q=q)
+    c.N = np.array([4 + (e % 4) % 3 for e in range(c.mesh.K)], np.int32)   # arcs keep curving: NLow >= 4
+    c.ic = C.uniform(np.array(q))
+    if viscous == "none":
+        c.navier_stokes, c.Re, c.viscous = False, 0.0, "none"
+    else:
+        c.viscous, c.sigma0 = viscous, 10.0
+    orc = Oracle(c)
+    assert mortar_edges(orc)
+    orc.set_state(c.initial_state(orc.coords()))
+    assert np.max(np.abs(orc.time_derivative())) <= 1e-10
 
 
 def test_closed_box_conserves_mass_and_energy_across_orders():

@@ -93,16 +93,19 @@ def test_mortar_matrices():
     assert n >= 20
 
 
-@pytest.mark.parametrize("ns", [False, True])
+@pytest.mark.parametrize("viscous", ["none", "br1", "ip"])
 @pytest.mark.parametrize("seed", [0, 1, 2])
-def test_free_stream_is_preserved(ns, seed):
+def test_free_stream_is_preserved(viscous, seed):
     from oracle.oracle import Oracle
     q = C.freestream_state(0.3, 0.2)
     zones = [C.Zone(C.BC_FARFIELD, q=tuple(q)) for _ in range(4)]
-    case = refined_case(zones, seed, ns)
+    case = refined_case(zones, seed, viscous != "none")
+    if viscous == "ip":
+        case.viscous, case.ip_method, case.sigma0 = "ip", "sipg", 10.0
     orc = Oracle(case)
     orc.set_state(case.initial_state(orc.coords()))
-    assert np.max(np.abs(orc.time_derivative())) <= 1e-11
+    # the penalty sigma0 mu N^2 / h (over 100 on the split cells) multiplies the rounding of the jumps
+    assert np.max(np.abs(orc.time_derivative())) <= (1e-10 if viscous == "ip" else 1e-11)
 
 
 def test_closed_box_conserves_mass_and_energy():
@@ -132,8 +135,8 @@ def test_unsupported_forms_are_refused():
     q = C.freestream_state(0.3, 0.2)
     zones = [C.Zone(C.BC_FARFIELD, q=tuple(q)) for _ in range(4)]
     c = refined_case(zones, 0, ns=True, lo=3, hi=4)   # one order: no p-non-conforming edges
-    c.viscous = "ip"
-    with pytest.raises(RuntimeError, match="subdivided edges: the Euler and BR1 forms"):
+    c.artificial_dissipation = "physical"
+    with pytest.raises(RuntimeError, match="subdivided edges: no artificial dissipation"):
         Oracle(c)
     c = refined_case(zones, 0)
     c.N = c.N.copy()
