@@ -89,6 +89,8 @@ hg::StageArgs base_args(hg_handle *h) {
     a.mustar = h->mustar;
     a.mflux = h->mflux;
     a.mdq = h->mdq;
+    a.mgrad = h->mgrad;
+    a.fstate = h->fstate;
     return a;
 }
 
@@ -306,6 +308,7 @@ hg::MortarArgs mortar_args(hg_handle *h) {
         a.mdq[g] = s->mdq;
         a.mustar[g] = s->mustar;
         a.mflux[g] = s->mflux;
+        a.mgrad[g] = s->mgrad;
     }
     return a;
 }
@@ -316,8 +319,10 @@ hg::MortarArgs mortar_args(hg_handle *h) {
 int composite_stage(hg_handle *h, double am, double cq, double cg, bool first, bool last, int mode,
                     const double *dt_dev = nullptr, int dt_cq = 0, int dt_cg = 0) {
     if (h->ph.ns) {
-        HG_CHECK(hg::launch_mortar_ustar(mortar_args(h), h->stream));
-        HG_CHECK(hg::launch_sub_ustar(mortar_args(h), h->stream));
+        if (h->ph.viscous == HG_VISCOUS_BR1) {
+            HG_CHECK(hg::launch_mortar_ustar(mortar_args(h), h->stream));
+            HG_CHECK(hg::launch_sub_ustar(mortar_args(h), h->stream));
+        }
         for (auto *s : h->sub)
             if (run_stage(s, am, cq, cg, first, last, mode, dt_dev, dt_cq, dt_cg, 1)) return 1;
     }
@@ -359,6 +364,9 @@ int composite_rk_step(hg_handle *h, double dt) {
     for (auto *s : h->sub) {
         s->qdot_valid = false;
         if (autotune(s)) return 1;
+        // interior penalty: the viscous traces (and the mortar faces' gradient traces) of the
+        // current state before a captured step, whose stages expect the previous epilogue's
+        if (hg::ensure_vtr(s)) return 1;
     }
     if (h->use_graph && !h->prof) {
         if (!h->dt_valid || dt != h->dt_last) {
@@ -518,6 +526,11 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
                 y = mort[k];
                 has_mortar = true;
                 if (ed.hang >= 0 && side > 0) goff = half_off[k];
+                // a curved mortar edge returns the BR1 lift terms per direction (bit 8 + f)
+                if (ed.curved) {
+                    forder[e] |= 1 << (8 + f);
+                    flags |= hg::FF_MCURVED;
+                }
             } else if (!ed.bdry) {
                 flags = side == 0 ? hg::FK_INT_L : hg::FK_INT_R;
                 int other = loc[ed.quads[1 - side]];
@@ -725,10 +738,14 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
         HG_CHECK(h->alloc(&h->mflux, 4 * fn));
         HG_CHECK(hipMemset(h->mflux, 0, 4 * fn * sizeof(double)));
         if (ph.ns) {
-            HG_CHECK(h->alloc(&h->mustar, 4 * fn));
+            HG_CHECK(h->alloc(&h->mustar, 8 * fn));
             HG_CHECK(h->alloc(&h->mdq, 8 * fn));
-            HG_CHECK(hipMemset(h->mustar, 0, 4 * fn * sizeof(double)));
+            HG_CHECK(hipMemset(h->mustar, 0, 8 * fn * sizeof(double)));
             HG_CHECK(hipMemset(h->mdq, 0, 8 * fn * sizeof(double)));
+            if (ph.ip_gradflux) {
+                HG_CHECK(h->alloc(&h->mgrad, 8 * fn));
+                HG_CHECK(hipMemset(h->mgrad, 0, 8 * fn * sizeof(double)));
+            }
         }
     }
     HG_CHECK(hipDeviceSynchronize());
@@ -739,14 +756,12 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
 // A mesh with several polynomial orders (p-refinement zones, MeshFile.f90:143-180): one
 // sub-handle per order over its elements, the p-non-conforming edges between them (edge order
 // the higher of its two elements', QuadMesh_Construct.incf:104-118) in the mortar tables, and
-// the whole mesh's time-step scan.  The FP64 Euler and BR1 forms without artificial
-// dissipation, straight p-non-conforming edges.
+// the whole mesh's time-step scan.  The FP64 form (Euler, BR1 or interior penalty) without
+// artificial dissipation; straight and curved mortar edges.
 int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt, const std::vector<int> &orders) {
     const hg::Mesh &m = h->mesh;
     if (opt->precision != HG_PRECISION_FP64)
         return set_err("hg_create: a mesh with several polynomial orders or subdivided edges runs in the FP64 form only");
-    if (opt->navier_stokes && opt->viscous != HG_VISCOUS_BR1)
-        return set_err("hg_create: a mesh with several polynomial orders or subdivided edges needs the Euler or the BR1 form");
     if (opt->navier_stokes && opt->artificial_dissipation != HG_AD_NONE)
         return set_err("hg_create: a mesh with several polynomial orders or subdivided edges runs without artificial dissipation");
     if ((int)orders.size() > hg::kMaxSub) return set_err("hg_create: too many polynomial orders in one mesh");
@@ -761,10 +776,10 @@ int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt
     std::vector<int> mort(E, -1);
     std::vector<hg::MortarEdge> medges;
     std::vector<double> mats;
+    std::vector<double> sgeo;   // per node (nx, ny, dS): curved p-non-conforming edges, subdivided edges' mortars
     for (int k = 0; k < E; ++k) {
         const hg::Edge &ed = m.ed[k];
         if (ed.bdry || ed.hang >= 0 || m.el[ed.quads[0]].N == m.el[ed.quads[1]].N) continue;
-        if (ed.curved) return set_err("hg_create: curved edges between elements of different orders are not supported");
         hg::MortarEdge r{};
         for (int side = 0; side < 2; ++side) {
             const int q = ed.quads[side];
@@ -778,6 +793,17 @@ int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt
         r.nx = ed.n[0];
         r.ny = ed.n[1];
         r.dS = ed.dS[0];
+        r.invh = ed.invh;
+        r.geo = -1;
+        if (ed.curved) {
+            // the edge's normals and surface Jacobians at its N + 1 nodes (CurvedEdge_t)
+            r.geo = (int)(sgeo.size() / 3);
+            for (int i = 0; i <= ed.N; ++i) {
+                sgeo.push_back(ed.n[2 * i]);
+                sgeo.push_back(ed.n[2 * i + 1]);
+                sgeo.push_back(ed.dS[i]);
+            }
+        }
         r.mat = (int)mats.size();
         const int Pl = r.Ps[r.low];
         mats.resize(mats.size() + 2 * (size_t)r.P * Pl);
@@ -787,14 +813,13 @@ int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt
     }
     // subdivided edges (SubdividedEdge_LinkWithElements, QuadElement_EdgeProcedures.incf:201-355)
     std::vector<hg::SubMortarEdge> sedges;
-    std::vector<double> sgeo;
     for (int k = 0; k < E; ++k) {
         const hg::Edge &ed = m.ed[k];
         if (ed.hang < 0) continue;
-        if (ed.curved && opt->navier_stokes)
-            return set_err("hg_create: curved subdivided edges are supported in the Euler form only");
         const hg::SubEdge &d = *ed.sd;
         hg::SubMortarEdge r{};
+        r.curved = ed.curved ? 1 : 0;
+        r.invh = ed.invh;
         for (int side = 0; side < 3; ++side) {
             const int q = ed.quads[side];
             r.sub[side] = group[q];
@@ -1238,6 +1263,106 @@ int hg_volume_integrals(hg_handle *h, double out[2]) {
     HG_CHECK(hipStreamSynchronize(h->stream));
     out[0] = v[0];
     out[1] = std::sqrt(v[1]);
+    return 0;
+}
+
+// ---- boundary states from problem-file functions
+// The zone's boundary edges in ascending edge id (the node order of the boundary-state calls).
+static std::vector<int> zone_edges(const hg::Mesh &m, int marker) {
+    std::vector<int> r;
+    for (size_t k = 0; k < m.ed.size(); ++k)
+        if (m.ed[k].bdry && m.ed[k].marker == marker) r.push_back((int)k);
+    return r;
+}
+
+int hg_boundary_node_count(hg_handle *h, int32_t marker, int64_t *n) {
+    if (!h || !n) return set_err("hg_boundary_node_count: null argument");
+    if (marker < 1 || marker > (int)h->mesh.zones.size()) return set_err("hg_boundary_node_count: marker out of range");
+    int64_t c = 0;
+    for (int k : zone_edges(h->mesh, marker)) c += h->mesh.ed[k].N + 1;
+    *n = c;
+    return 0;
+}
+
+int hg_boundary_nodes(hg_handle *h, int32_t marker, int64_t n, double *x) {
+    int64_t c = 0;
+    if (hg_boundary_node_count(h, marker, &c)) return 1;
+    if (!x || n != c) return set_err("hg_boundary_nodes: null argument or node count mismatch");
+    int64_t o = 0;
+    for (int k : zone_edges(h->mesh, marker)) {
+        const hg::Edge &ed = h->mesh.ed[k];
+        for (int i = 0; i <= ed.N; ++i, ++o) { x[2 * o] = ed.X[2 * i]; x[2 * o + 1] = ed.X[2 * i + 1]; }
+    }
+    return 0;
+}
+
+int hg_set_boundary_state(hg_handle *h, int32_t marker, int64_t n, const double *q) {
+    int64_t c = 0;
+    if (hg_boundary_node_count(h, marker, &c)) return 1;
+    if (q && n != c) return set_err("hg_set_boundary_state: node count mismatch");
+    const hg::Mesh &m = h->mesh;
+    const int type = m.zones[marker - 1].type;
+    if (type != HG_BC_FARFIELD && type != HG_BC_DIRICHLET && type != HG_BC_NEWDIRICHLET &&
+        type != HG_BC_PRESSUREOUTLET && type != HG_BC_PRESSUREINLET)
+        return set_err("hg_set_boundary_state: the zone's kind has no boundary state");
+    if (q)
+        for (int64_t i = 0; i < 4 * n; ++i)
+            if (!std::isfinite(q[i])) return set_err("hg_set_boundary_state: non-finite state");
+    // the handle (or sub-handle) and local element of every mesh element
+    const int K = (int)m.el.size();
+    std::vector<hg_handle *> owner(K, h);
+    std::vector<int> loc(K);
+    if (h->sub.empty()) {
+        std::iota(loc.begin(), loc.end(), 0);
+    } else {
+        for (size_t g = 0; g < h->sub.size(); ++g)
+            for (size_t e = 0; e < h->sub_els[g].size(); ++e) {
+                owner[h->sub_els[g][e]] = h->sub[g];
+                loc[h->sub_els[g][e]] = (int)e;
+            }
+    }
+    std::vector<hg_handle *> hs = h->sub.empty() ? std::vector<hg_handle *>{h} : h->sub;
+    for (hg_handle *s : hs) HG_CHECK(hipStreamSynchronize(s->stream));
+    // per target handle: its face records and (when states are set) its per-face-node states
+    std::vector<std::vector<int4>> faces(hs.size());
+    std::vector<std::vector<double>> fst(hs.size());
+    for (size_t g = 0; g < hs.size(); ++g) {
+        hg_handle *s = hs[g];
+        faces[g].resize(4 * (size_t)s->K);
+        HG_CHECK(hipMemcpy(faces[g].data(), s->face, faces[g].size() * sizeof(int4), hipMemcpyDeviceToHost));
+        const size_t fn = (size_t)s->K * 4 * s->P * 4;
+        fst[g].assign(fn, 0.0);
+        if (s->fstate) HG_CHECK(hipMemcpy(fst[g].data(), s->fstate, fn * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    int64_t o = 0;
+    for (int k : zone_edges(m, marker)) {
+        const hg::Edge &ed = m.ed[k];
+        const int e = ed.quads[0], f = ed.loc[0] - 1;
+        hg_handle *s = owner[e];
+        const size_t g = h->sub.empty() ? 0 : (size_t)(std::find(h->sub.begin(), h->sub.end(), s) - h->sub.begin());
+        int4 &fr = faces[g][4 * (size_t)loc[e] + f];
+        if (!q) {
+            fr.x &= ~hg::FF_FSTATE;
+            continue;
+        }
+        fr.x |= hg::FF_FSTATE;
+        // a boundary face is its edge's LEFT side: face node i is edge node i
+        for (int i = 0; i <= ed.N; ++i, ++o)
+            for (int v = 0; v < 4; ++v) fst[g][(((size_t)loc[e] * 4 + f) * s->P + i) * 4 + v] = q[4 * o + v];
+    }
+    for (size_t g = 0; g < hs.size(); ++g) {
+        hg_handle *s = hs[g];
+        HG_CHECK(hipMemcpy(s->face, faces[g].data(), faces[g].size() * sizeof(int4), hipMemcpyHostToDevice));
+        if (q) {
+            if (!s->fstate) HG_CHECK(s->alloc(&s->fstate, fst[g].size()));
+            HG_CHECK(hipMemcpy(s->fstate, fst[g].data(), fst[g].size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        drop_graphs(s);   // the captured kernels hold the state pointer by value
+        s->qdot_valid = false;
+        if (s->ip_fused) s->vtr_valid = false;   // the epilogue's viscous traces used the old states
+    }
+    drop_graphs(h);
+    h->qdot_valid = false;
     return 0;
 }
 

@@ -96,6 +96,7 @@ struct hg_batch {
     // the FP32 form's reference pressure (carried by value in every slot)
     std::vector<int> cur0;
     std::vector<double> pref0;
+    std::vector<const double *> fst0;   // the boundary-function state buffer (hg_set_boundary_state)
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
     // kernel timing (hg_batch_profile_enable): HIP events around every stage (0), gradient (1)
@@ -347,10 +348,12 @@ std::vector<hg::StageArgs> slot_table(hg_batch *b) {
     std::vector<hg::StageArgs> slots(b->cases.size() * b->nslots);
     b->cur0.assign(b->cases.size(), 0);
     b->pref0.assign(b->cases.size(), 0.0);
+    b->fst0.assign(b->cases.size(), nullptr);
     for (int c = 0; c < (int)b->cases.size(); ++c) {
         hg_handle *h = b->cases[c];
         b->cur0[c] = h->cur;
         b->pref0[c] = h->ph.pref;
+        b->fst0[c] = h->fstate;
         for (int p = 0; p < 2; ++p)
             for (int s = 0; s < b->sch.nst; ++s) {
                 hg::StageArgs a = hg::handle_args(h);
@@ -532,7 +535,7 @@ int hg_batch_rk_step(hg_batch *b, const double *t, const double *dt) {
     bool stale = false;
     for (size_t c = 0; c < b->cases.size(); ++c) {
         hg_handle *h = b->cases[c];
-        if (h->cur != (b->cur0[c] ^ b->parity) || h->ph.pref != b->pref0[c]) stale = true;
+        if (h->cur != (b->cur0[c] ^ b->parity) || h->ph.pref != b->pref0[c] || h->fstate != b->fst0[c]) stale = true;
         if (hg::ensure_vtr(h)) return 1;
     }
     for (auto *h : b->cases) HB_CHECK(hipStreamSynchronize(h->stream));

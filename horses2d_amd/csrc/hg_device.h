@@ -29,6 +29,9 @@ constexpr int FF_REFLECTIVE = 1 << 7;  // pressure outlet: reflective outflow (s
 constexpr int FF_TOTAL_INFLOW = 1 << 7; // newDirichlet: subsonic inflow at the total pressure (same bit)
 constexpr int FF_TOTAL_OUTFLOW = 1 << 12; // newDirichlet: subsonic outflow at the total pressure
 constexpr int FF_RS_SHIFT = 8;       // (zone Riemann solver + 1), 0 = interior method
+constexpr int FF_FSTATE = 1 << 13;   // boundary face whose zone state comes per node from a
+                                     // boundary-state function (StageArgs::fstate)
+constexpr int FF_MCURVED = 1 << 14;  // mortar face of a curved edge: BR1 uStar dS n per direction
 
 // Viscous face-flux traces: per element face node, mu times this side's viscous flux from its
 // own Q and gradient traces, (f1, f2, f3, g2, g3) and one pad (f0 = g0 = 0, g1 = f2).
@@ -83,6 +86,10 @@ struct StageArgs {
     const double *mustar;       // BR1 uStar of this side, projected and signed [4]
     const double *mflux;        // (Fi - Fv) dS of this side, projected and signed [4]
     double *mdq;                // this side's gradient trace [8], written by the gradient kernel
+    const double *mgrad;        // interior penalty: this side's gradient flux, projected, times dS [8]
+    // boundary-state functions: per boundary face node [(e*4+f)*P + k][4], the state the ghost
+    // state starts from in place of the zone's (faces flagged FF_FSTATE)
+    const double *fstate;
 };
 
 // p-non-conforming edges of a handle whose elements have several orders: the elements of each
@@ -97,8 +104,9 @@ struct MortarEdge {
     int low;         // the side of lower order (0 LEFT, 1 RIGHT)
     int inverse;     // the RIGHT element runs the edge backwards
     int mat;         // offset of T_forward [P][Plow] and then T_backward [Plow][P] in the matrix pool
-    int pad;
+    int geo;         // curved edge: (nx, ny, dS) per edge node in the geometry pool; -1: straight
     double nx, ny, dS;   // straight edge: normal (LEFT to RIGHT) and length
+    double invh;     // interior penalty: the edge's 1/h
 };
 // A subdivided edge (one element's edge meeting two elements, MeshFile.f90:364-490): its big
 // (LEFT), NORTH and SOUTH elements and the two mortars between them.
@@ -109,6 +117,9 @@ struct SubMortarEdge {
     int PM[2];       // points of the NORTH and SOUTH mortars
     int mat;         // T_LN_F, T_LS_F, T_LN_B, T_LS_B, T_RN_F, T_RS_F, T_RN_B, T_RS_B in the pool, row-major
     int geo;         // (nx, ny, dS) per mortar node in the geometry pool: NORTH then SOUTH
+    int curved;      // curved edge: the BR1 uStar is scaled per mortar node before its return
+    int pad;
+    double invh;     // interior penalty: the edge's 1/h
 };
 struct MortarArgs {
     const MortarEdge *edges;
@@ -119,8 +130,10 @@ struct MortarArgs {
     const double *mats;
     const double *tr[kMaxSub];   // Q traces of the current stage, per sub-handle
     const double *mdq[kMaxSub];  // gradient traces at mortar faces, per sub-handle
-    double *mustar[kMaxSub];
+    double *mustar[kMaxSub];     // per mortar face node [8]: straight edges uStar [0..3]; curved
+                                 // edges uStar dS nx [0..3] and uStar dS ny [4..7]
     double *mflux[kMaxSub];
+    double *mgrad[kMaxSub];      // interior penalty gradient fluxes per mortar face node [8]
     PhysParams ph;
 };
 // BR1 uStar of every mortar edge (from the sub-handles' current Q traces), to each side.

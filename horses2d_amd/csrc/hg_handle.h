@@ -74,6 +74,11 @@ struct hg_handle {
     // p-non-conforming faces of this handle: per face node the mortar kernels' projected flux,
     // BR1 uStar, and this side's gradient trace (null when every edge is conforming)
     double *mflux = nullptr, *mustar = nullptr, *mdq = nullptr;
+    double *mgrad = nullptr;   // interior penalty: the projected gradient fluxes per mortar face node
+    // boundary-state functions: per face node the state (boundary faces of zones given one by
+    // hg_set_boundary_state; null until the first such call)
+    double *fstate = nullptr;
+    std::vector<int> fzone;    // zones given per-node states
     // A mesh with several polynomial orders: one sub-handle per order (its elements in mesh
     // order, sharing this handle's stream) coupled by the mortar kernels; this handle keeps the
     // whole mesh, the time-step scan over all elements in mesh order and the mortar tables.

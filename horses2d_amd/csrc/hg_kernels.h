@@ -104,12 +104,20 @@ __device__ __forceinline__ R qslot(const PhysParams &pp, double v, int c) {
     return (R)(c == 3 ? slot_energy<R>(pp, v) : v);
 }
 
+// The state a boundary face node's ghost state starts from: its zone's record (8 doubles per
+// zone, the state in the first four), or, on a face flagged FF_FSTATE, the node's own value of
+// the zone's boundary-state function (slot = (element*4 + face)*P + node).
+__device__ __forceinline__ const double *zone_state(const StageArgs &a, int flags, int y, int64_t slot) {
+    return (flags & FF_FSTATE) ? a.fstate + slot * 4 : a.zone + 8 * y;
+}
+
 // BR1 solution flux at a face node: uStar n dS in both directions (DGViscous_BR1.f90:313-328;
 // boundary faces use the ghost state's uSB, DGViscousMethods.f90:786-917), stored as
 // out[0..3] = uStar dS and out[4..5] = (nx, ny); the lift forms (uStar dS) nx and (uStar dS) ny.
+// zd: the boundary's zone state (zone_state).
 template <typename R, bool XB>
-__device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> &ph, int flags, int y, const R *qs,
-                                              const R *qo, R nx, R ny, R dS, R *out) {
+__device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> &ph, int flags, const double *zd,
+                                              const R *qs, const R *qo, R nx, R ny, R dS, R *out) {
     const int kind = flags & 15;
     R us[4];
     if (kind <= FK_PERIODIC) {
@@ -121,12 +129,19 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
         }
     } else if (kind == FK_MORTAR) {
         // projected and signed by the mortar kernel (load_face put it in qo)
+        if (flags & FF_MCURVED) {
+            // a curved mortar edge: qo holds uStar dS nx already; the y terms follow separately
+#pragma unroll
+            for (int c = 0; c < 4; ++c) out[c] = qo[c];
+            out[4] = R(1); out[5] = R(0);
+            return;
+        }
 #pragma unroll
         for (int c = 0; c < 4; ++c) us[c] = qo[c];
     } else {
         R uB[4];
         int neu;
-        ghost_state<R, XB>(kind, flags, a.zone + 8 * y, a.ph, ph, qs, nx, ny, uB, us, neu);
+        ghost_state<R, XB>(kind, flags, zd, a.ph, ph, qs, nx, ny, uB, us, neu);
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) out[c] = us[c] * dS;
@@ -139,6 +154,10 @@ __device__ __forceinline__ void br1_face_flux(const StageArgs &a, const Phys<R> 
 // Ja: element metric [n*4], Fu: element uStar fluxes [(f*P + k)*6] (see br1_face_flux).
 // ROLLED keeps the two l loops rolled (one node's values live; the stage kernel, where the
 // registers bound occupancy); the sums are the same either way.
+// Faces whose bit 8 + f is set in ord are curved p-non-conforming or subdivided edges, whose
+// lift terms come per direction from the mortar kernel (uStar dS n projected direction by
+// direction): the face lane stores the x terms with the "normal" (1, 0), and mortar_y_lift adds
+// the y terms afterwards.
 template <int P, typename R, int QS = 4, int JS = 4, bool ROLLED = false>
 __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const R *sHat, const R *Fu, const R *sLbw,
                                                   int ord, int i, int j, R ijac, R *dq) {
@@ -190,6 +209,23 @@ __device__ __forceinline__ void br1_node_gradient(const R *q, const R *Ja, const
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) dq[c] = dq[c] * ijac;
+}
+
+// The y-direction lift terms of an element's curved mortar faces (bit 8 + f of ord), added to
+// the gradient br1_node_gradient formed: (uStar dS ny projected) lw / jac from mu8 (the element's
+// [(f*P + k)*8] block of StageArgs::mustar).  FP64 only (meshes with mortar edges run in it).
+template <int P, typename R>
+__device__ __forceinline__ void mortar_y_lift(const double *mu8, const R *sLbw, int ord, int i, int j, R ijac, R *dq) {
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll 1
+        for (int f = 0; f < 4; ++f) {
+            if (!((ord >> (8 + f)) & 1)) continue;
+            const R lw = (f == 0) ? sLbw[j] : (f == 1) ? sLbw[P + i] : (f == 2) ? sLbw[P + j] : sLbw[i];
+            const double *M = mu8 + (f * P + ((f == 0 || f == 2) ? i : j)) * 8;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) dq[4 + c] += (R)(M[4 + c] * lw) * ijac;
+        }
+    }
 }
 
 // Interior (broken) gradient of the IP method (QuadElement_QuadProcedures.incf:42-88); q as above.
@@ -277,7 +313,7 @@ __device__ __forceinline__ void load_face(const StageArgs &a, int fe, int ff, in
         for (int c = 0; c < 4; ++c) qo[c] = (R)nbt[c];
     } else if ((fr.x & 15) == FK_MORTAR && a.mustar) {
         // p-non-conforming edge: this side's projected uStar in place of a neighbour trace
-        const double *u = a.mustar + ((int64_t)(fe * 4 + ff) * P + fk) * 4;
+        const double *u = a.mustar + ((int64_t)(fe * 4 + ff) * P + fk) * 8;
 #pragma unroll
         for (int c = 0; c < 4; ++c) qo[c] = (R)u[c];
     }
@@ -426,12 +462,18 @@ __device__ __forceinline__ void gradient_body(const StageArgs &a, const int e0, 
         stored_trace<P, R, TR>(a, fe, ff, fk, qs);
     }
     __syncthreads();
-    if (BR1 && face_active) br1_face_flux<R, XB>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sFu + t * 6);
+    if (BR1 && face_active)
+        br1_face_flux<R, XB>(a, ph, fr.x, zone_state(a, fr.x, fr.y, (int64_t)(fe * 4 + ff) * P + fk), qs, qo, nx, ny,
+                             dS, sFu + t * 6);
     if (BR1) __syncthreads();
     R dq[8];
     if (node_active) {
         const int i = n % P, j = n / P;
-        if (BR1) br1_node_gradient<P, R, QS, JS>(sQr + s * PP * QS, sJa + s * PP * JS, sHat, sFu + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
+        if (BR1) {
+            br1_node_gradient<P, R, QS, JS>(sQr + s * PP * QS, sJa + s * PP * JS, sHat, sFu + s * 4 * P * 6, sLbw, ord, i,
+                                            j, ijac, dq);
+            if (ord >> 8) mortar_y_lift<P, R>(a.mustar + (int64_t)e * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        }
         else ip_node_gradient<P, R, QS>(sQr + s * PP * QS, sD, i, j, j11, j12, j21, j22, ijac, dq);
     }
     __syncthreads();   // every lane is done with the state and the metric
@@ -550,7 +592,9 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     __syncthreads();
 
     if (BR1) {
-        if (face_active) br1_face_flux<R, XB>(a, ph, fr.x, fr.y, qs, qo, nx, ny, dS, sF + t * 6);
+        if (face_active)
+            br1_face_flux<R, XB>(a, ph, fr.x, zone_state(a, fr.x, fr.y, (int64_t)(fe * 4 + ff) * P + fk), qs, qo, nx,
+                                 ny, dS, sF + t * 6);
         __syncthreads();
     }
 
@@ -562,7 +606,11 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
     R dqi[LAPG ? 8 : 1];
     if (NS && node_active) {
         const int i = n % P, j = n / P;
-        if (BR1) br1_node_gradient<P, R, 4, 4, true>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 6, sLbw, ord, i, j, ijac, dq);
+        if (BR1) {
+            br1_node_gradient<P, R, 4, 4, true>(sQr + s * PP * 4, sCF + s * PP * 4, sHat, sF + s * 4 * P * 6, sLbw, ord,
+                                                i, j, ijac, dq);
+            if (ord >> 8) mortar_y_lift<P, R>(a.mustar + (int64_t)e * 4 * P * 8, sLbw, ord, i, j, ijac, dq);
+        }
         else ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dq);
         if (LAPG && a.lap) ip_node_gradient<P, R>(sQr + s * PP * 4, sD, i, j, j11, j12, j21, j22, ijac, dqi);
     }
@@ -631,8 +679,16 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
 #pragma unroll
             for (int c = 0; c < 4; ++c) sF[t * 4 + c] = (R)mf[c];
             if (ipgrad) {
+                // this side's IP gradient flux, projected and times dS by the mortar kernel, in
+                // contravariant components (DGWeakIntegrals.f90:268-495)
+                const double *mg = a.mgrad + ((int64_t)(fe * 4 + ff) * P + fk) * 8;
+                const R *fm = (const R *)a.fmet + ((int64_t)(fe * 4 + ff) * P + fk) * 5;
 #pragma unroll
-                for (int c = 0; c < 4; ++c) { sU[t * 4 + c] = R(0); sV[t * 4 + c] = R(0); }
+                for (int c = 0; c < 4; ++c) {
+                    const R gx = (R)mg[c], gy = (R)mg[4 + c];
+                    sU[t * 4 + c] = (gx * fm[0] + gy * fm[1]) * fm[4];
+                    sV[t * 4 + c] = (gx * fm[2] + gy * fm[3]) * fm[4];
+                }
             }
         } else {
             R vs[NS ? VT : 1];   // own viscous flux trace
@@ -645,8 +701,8 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
             R uSB[4] = {R(0), R(0), R(0), R(0)};
             int neumann = 0, prescribed = 0;
             if (kind > FK_PERIODIC) {
-                prescribed = ghost_state<R, XB>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, nx, ny, qo, uSB,
-                                                      neumann);
+                prescribed = ghost_state<R, XB>(kind, fr.x, zone_state(a, fr.x, fr.y, (int64_t)(fe * 4 + ff) * P + fk),
+                                                a.ph, ph, qs, nx, ny, qo, uSB, neumann);
                 const int zr = (fr.x >> FF_RS_SHIFT) & 15;
                 if (zr) rs = zr - 1;
             }
@@ -914,9 +970,11 @@ __device__ __forceinline__ void stage_body(const StageArgs &a, const int e0, con
                 for (int c = 0; c < 8; ++c) sDQ[t * 8 + c] = dq[c];
             }
             __syncthreads();
-            if (face_active)
+            if (face_active) {
+                double *mdq = ((fr.x & 15) == FK_MORTAR && a.mdq) ? a.mdq + ((int64_t)(fe * 4 + ff) * P + fk) * 8 : nullptr;
                 viscous_face_trace<P, 8, R, TR>(a, ph, fr.x, qn, sDQ + fs * PP * 8, sLb, S.sLbR, ff, fk,
-                                                (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT, ph.mu);
+                                                (TR *)a.dtr_out + ((int64_t)(fe * 4 + ff) * P + fk) * VT, ph.mu, mdq);
+            }
           }
         }
     }
@@ -1149,7 +1207,8 @@ __global__ __launch_bounds__(256) void k_art_visc(StageArgs a, const double *evo
         } else {
             double uB[4];
             int neu;
-            ghost_state<double, true>(kind, fr.x, a.zone + 8 * fr.y, a.ph, ph, qs, geo[0], geo[1], uB, qo, neu);
+            ghost_state<double, true>(kind, fr.x, zone_state(a, fr.x, fr.y, (int64_t)(e * 4 + f) * P + k), a.ph, ph, qs,
+                                      geo[0], geo[1], uB, qo, neu);
             dS = a.egeo[3 * (int64_t)fr.z + 2];
         }
         const double nr = 2.0 * (qs[0] - qo[0]) / (qs[0] + qo[0]);

@@ -82,6 +82,7 @@ module horses_gpu_bindings
    public :: hg_create, hg_destroy, hg_ndof, hg_get_node_coords, hg_upload_state, hg_download_state
    public :: hg_download_qdot, hg_time_derivative, hg_rk_step, hg_residuals, hg_estimate_dt
    public :: hg_reset_dt_state, hg_conservation_sums, hg_volume_integrals, hg_surface_integrals
+   public :: hg_boundary_node_count, hg_boundary_nodes, hg_set_boundary_state
    public :: hg_probes_set, hg_probes_update
    public :: hg_synchronize, hg_last_error
    public :: hg_set_tiles, hg_get_tiles, hg_set_graph, hg_profile_enable, hg_profile_read
@@ -187,6 +188,31 @@ module horses_gpu_bindings
          integer(c_int32_t), value    :: marker
          real(c_double), intent(out)  :: vals(9)
       end function hg_surface_integrals
+
+      ! "Use function" zones: the caller evaluates its BoundaryState function at these nodes
+      ! (edge % x of the zone's edges, ascending edge id) and hands the states over
+      integer(c_int) function hg_boundary_node_count(handle, marker, n) bind(C, name="hg_boundary_node_count")
+         import :: c_int, c_ptr, c_int32_t, c_int64_t
+         type(c_ptr), value           :: handle
+         integer(c_int32_t), value    :: marker
+         integer(c_int64_t), intent(out) :: n
+      end function hg_boundary_node_count
+
+      integer(c_int) function hg_boundary_nodes(handle, marker, n, x) bind(C, name="hg_boundary_nodes")
+         import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+         type(c_ptr), value           :: handle
+         integer(c_int32_t), value    :: marker
+         integer(c_int64_t), value    :: n
+         real(c_double), intent(out)  :: x(2, n)
+      end function hg_boundary_nodes
+
+      integer(c_int) function hg_set_boundary_state(handle, marker, n, q) bind(C, name="hg_set_boundary_state")
+         import :: c_int, c_ptr, c_double, c_int32_t, c_int64_t
+         type(c_ptr), value           :: handle
+         integer(c_int32_t), value    :: marker
+         integer(c_int64_t), value    :: n
+         real(c_double), intent(in)   :: q(4, n)
+      end function hg_set_boundary_state
 
       ! Probe_Initialization (IO/Monitors.f90:647-737): elements 0-based (eID - 1), reference
       ! coordinates (xi, eta), variables HG_PROBE_* (rho .. Mach in Probe_Update's order, 0-based)

@@ -60,7 +60,8 @@ EXPORTS = [
     "hg_upload_state", "hg_download_state", "hg_download_qdot", "hg_time_derivative", "hg_rk_step",
     "hg_residuals", "hg_estimate_dt", "hg_reset_dt_state", "hg_conservation_sums", "hg_conservation_sums_noalloc",
     "hg_volume_integrals",
-    "hg_surface_integrals", "hg_probes_set", "hg_probes_update", "hg_synchronize",
+    "hg_surface_integrals", "hg_boundary_node_count", "hg_boundary_nodes", "hg_set_boundary_state",
+    "hg_probes_set", "hg_probes_update", "hg_synchronize",
     "hg_last_error", "hg_profile_enable", "hg_profile_read", "hg_set_tiles", "hg_get_tiles", "hg_set_graph", "hg_batch_create", "hg_batch_rk_step",
     "hg_batch_synchronize", "hg_batch_schedule", "hg_batch_profile_enable", "hg_batch_profile_read",
     "hg_batch_destroy",
@@ -99,6 +100,9 @@ def load_library(path: str = LIB_PATH):
         "hg_conservation_sums_noalloc": ([vp, vp], ctypes.c_int),
         "hg_volume_integrals": ([vp, vp], ctypes.c_int),
         "hg_surface_integrals": ([vp, i32, vp], ctypes.c_int),
+        "hg_boundary_node_count": ([vp, i32, vp], ctypes.c_int),
+        "hg_boundary_nodes": ([vp, i32, i64, vp], ctypes.c_int),
+        "hg_set_boundary_state": ([vp, i32, i64, vp], ctypes.c_int),
         "hg_probes_set": ([vp, i32, vp, vp, vp, vp], ctypes.c_int),
         "hg_probes_update": ([vp, vp], ctypes.c_int),
         "hg_synchronize": ([vp], ctypes.c_int),
@@ -307,6 +311,24 @@ class DGSEM:
         r = np.zeros(9)
         _check(self._L.hg_surface_integrals(self._h, int(marker), _ptr(r)), "hg_surface_integrals")
         return r
+
+    def boundary_nodes(self, marker: int) -> np.ndarray:
+        """Coordinates [n, 2] of zone ``marker``'s edge nodes (ascending edge id, each edge's nodes
+        in its own order): where the caller evaluates a boundary-state function."""
+        n = ctypes.c_int64(0)
+        _check(self._L.hg_boundary_node_count(self._h, int(marker), ctypes.byref(n)), "hg_boundary_node_count")
+        x = np.zeros((n.value, 2))
+        _check(self._L.hg_boundary_nodes(self._h, int(marker), n.value, _ptr(x)), "hg_boundary_nodes")
+        return x
+
+    def set_boundary_state(self, marker: int, q):
+        """A "Use function" zone's dimensionless states [n, 4] at those nodes (None: back to the
+        zone state), as hg_set_boundary_state."""
+        if q is None:
+            _check(self._L.hg_set_boundary_state(self._h, int(marker), 0, None), "hg_set_boundary_state")
+            return
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 4)
+        _check(self._L.hg_set_boundary_state(self._h, int(marker), q.shape[0], _ptr(q)), "hg_set_boundary_state")
 
     PROBE_VARIABLES = ("rho", "rhou", "rhov", "rhoe", "rhot", "rhout", "rhovt", "rhoet", "u", "v", "p", "Mach")
 

@@ -245,6 +245,24 @@ int hg_surface_integrals(hg_handle *h, int32_t marker, double out[9]);
 #define HG_PROBE_P 10
 #define HG_PROBE_MACH 11
 
+/* Boundary states from problem-file functions ("Use function = BC1..BC4" or "Initial condition"
+ * in a zone block; AssociateBoundaryConditionFunction, BoundaryConditionFunctionDrivers.incf:22-63).
+ * The reference evaluates q = BoundaryState(edge % x(:, iXi), time) at every node of the zone's
+ * edges in each _UpdateSolution (far field Farfield.incf:179-180, Dirichlet Dirichlet.incf:
+ * 182-183, newDirichlet newDirichlet.incf:195-196, pressure outlet PressureOutlet.incf:187-188 and
+ * inlet PressureInlet.incf:160-161) in place of the zone state.  Here the caller evaluates the
+ * function at the zone's nodes and hands the values over; the kernels read them wherever the
+ * zone's state is read.  The nodes of zone `marker` (1-based) are those of its edges in ascending
+ * edge id, each edge's N + 1 nodes in the edge's own order (the element side's face nodes).
+ * hg_boundary_node_count gives their number, hg_boundary_nodes their coordinates x[2 i + d]
+ * (edge % x), hg_set_boundary_state the dimensionless states q[4 i + eq] (q = NULL: back to the
+ * zone state).  The values hold until the next call: a function of time is re-evaluated by the
+ * caller between steps (the reference takes each RK stage's time; the GPU path keeps one value
+ * per step).  Zones of other kinds are refused. */
+int hg_boundary_node_count(hg_handle *h, int32_t marker, int64_t *n);
+int hg_boundary_nodes(hg_handle *h, int32_t marker, int64_t n, double *x);
+int hg_set_boundary_state(hg_handle *h, int32_t marker, int64_t n, const double *q);
+
 /* Probes, Probe_Initialization (IO/Monitors.f90:647-737) after findElementWithCoords: n probes,
  * probe k in element elem[k] (0-based) at reference coordinates (xi[k], eta[k]) (the element's
  * [0,1]^2), tracking variable var[k] (HG_PROBE_*).  The library evaluates the Lagrange

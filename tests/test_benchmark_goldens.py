@@ -51,21 +51,45 @@ def benchmark_case(name):
     m = np.load(os.path.join(GOLDEN, d["mesh"]), allow_pickle=False)
     zones = [C.Zone(z["kind"], direction=z["direction"], partner=z["partner"], isothermal=z["isothermal"],
                     q=tuple(z["q"]), Tw=z["Tw"], wall_velocity=tuple(z["wall_velocity"]), riemann=z["riemann"],
-                    outflow=z["outflow"], mode=z["mode"], outflow_pressure=z["outflow_pressure"]) for z in p["zones"]]
+                    outflow=z["outflow"], mode=z["mode"], outflow_pressure=z["outflow_pressure"],
+                    function=z.get("function", "")) for z in p["zones"]]
     mesh = C.HiOMesh(m["points"], m["quads"], m["bedges"], m["bmarker"], zones,
                      m["cedges"] if "cedges" in m else None, m["cxy"] if "cxy" in m else None)
     orders = np.array(p["orders"], np.int32) if p.get("orders") else np.full(mesh.K, p["N"][0], np.int32)
     # the reference's benchmark build solves the Navier-Stokes equations; at Re = 0 without artificial
     # dissipation every viscous term is a product with mu = 0, the Euler form's computation
     ns = p["Re"] > 0.0 or p["artificial_dissipation"] != "none"
-    ic = C.gaussian_vortex(p["Mach"]) if name == "vortex_transport" else C.uniform(C.freestream_state(p["Mach"]))
+    p_ref, T_ref = p.get("p_ref", 101325.0), p.get("T_ref", 273.15)
+    if name == "vortex_transport":
+        ic = C.gaussian_vortex(p["Mach"])
+    elif name == "couette":
+        ic = C.couette_flow(p["Mach"], p["Pr"], p_ref, T_ref, perturb=True)
+    else:
+        ic = C.uniform(C.freestream_state(p["Mach"]))
     case = C.Case(name, mesh, orders, p["Mach"], p["Re"], p["Pr"], ns,
                   riemann=p["riemann"], viscous=p["viscous"], ip_method=p["ip_method"], sigma0=p["sigma0"],
                   cfl=p["cfl"], dt_max=p["dt_max"], steps=p["steps"], scheme=p["scheme"],
                   ic=ic, artificial_dissipation=p["artificial_dissipation"],
                   ad_intensity=p["ad_intensity"], mode=p["mode"], residual_target=p["residual_target"],
-                  t_end=p["t_end"], t0=p["t0"])
+                  t_end=p["t_end"], t0=p["t0"], p_ref=p_ref, T_ref=T_ref)
     return case, d["finalize"]
+
+
+# The problem-file boundary-state functions of the benchmarks ("Use function = BCn" zones):
+# Couette's BC1 is its Couette profile (Couette/SETUP/ProblemFile.f90:194-218).
+BOUNDARY_FUNCTIONS = {
+    ("couette", "BC1"): lambda case: C.couette_flow(case.Mach, case.Pr, case.p_ref, case.T_ref),
+}
+
+
+def apply_boundary_functions(solver, case):
+    """Every "Use function" zone's state, evaluated at its boundary nodes (ascending edge id,
+    each edge's nodes in its own order), given to the solver (oracle or GPU handle)."""
+    for marker, z in enumerate(case.mesh.zones, start=1):
+        if z.function:
+            fn = BOUNDARY_FUNCTIONS[(case.name, z.function)](case)
+            x = solver.boundary_nodes(marker)
+            solver.set_boundary_state(marker, fn(x[:, 0], x[:, 1]))
 
 
 def test_ramp_fixture_matches_the_case_file():
@@ -269,3 +293,114 @@ def test_gpu_reproduces_the_vortex_transport_golden():
     print("gpu: iteration", dg.iter, "t", dg.t, "errors", err, "golden", gold)
     assert dg.iter == gold["final_iteration"] and dg.t == case.t_end
     assert np.max(np.abs(err - np.array(gold["errors"]))) <= gold["residual_tolerance"]
+
+
+def test_cylinder_re1000_fixture_matches_the_case_file():
+    path = os.path.join(REFERENCE, "TestCases/Benchmark/Simulations/CylinderRe1000/SETUP/CylinderRe1000.HiOCase")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    from horses2d_amd import hio
+    c = hio.case_from_hiocase(path, navier_stokes=True)
+    f, gold = benchmark_case("cylinder_re1000")
+    for k in ("Mach", "Re", "viscous", "ip_method", "sigma0", "riemann", "scheme", "mode", "cfl", "dt_max", "t_end"):
+        assert getattr(c, k) == getattr(f, k), k
+    np.testing.assert_array_equal(c.N, f.N)
+    assert sorted(set(int(n) for n in f.N)) == [4, 6, 8, 10] and gold["final_iteration"] == 512
+
+
+def test_oracle_reproduces_the_cylinder_re1000_golden():
+    """CylinderRe1000 (CylinderRe1000/SETUP/ProblemFile.f90:79-99): interior penalty (SIPG, jumps
+    penalty 10) across the p-non-conforming and subdivided edges of 1048 quads of orders 4, 6, 8
+    and 10 (curved around the cylinder), HLL inside and the exact solver on every zone, RK5 to
+    t = 0.013: iteration 512, residuals within Finalize's 1e-10, lift and drag within 1e-12."""
+    from oracle.oracle import Oracle
+    case, gold = benchmark_case("cylinder_re1000")
+    orc = Oracle(case)
+    orc.set_state(case.initial_state(orc.coords()))
+    t, _, it, hist = orc.integrate(case.steps, dt_max=case.dt_max, t_end=case.t_end, scheme=case.scheme,
+                                   mode=case.mode)
+    cl, cd = lift_drag(orc.surface_integrals(5), case.Mach)
+    print("oracle: iteration", it, "t", t, "residuals", hist[-1], "cl", cl, "cd", cd, "golden", gold)
+    assert it == gold["final_iteration"]
+    assert np.max(np.abs(hist[-1] - np.array(gold["residuals"]))) <= gold["residual_tolerance"]
+    assert abs(cl - gold["cl"]) <= 1e-12 and abs(cd - gold["cd"]) <= 1e-12
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_the_cylinder_re1000_golden():
+    """The FP64 GPU path (one sub-handle per order, the IP mortar kernels with their gradient
+    fluxes, RK5 replayed as a graph) through the same 512 iterations: Finalize's bars."""
+    from horses2d_amd import gpu
+    case, gold = benchmark_case("cylinder_re1000")
+    dg = gpu.DGSEM(case, "fp64")
+    hist = dg.integrate(monitor=True)
+    cl, cd = lift_drag(dg.surface_integrals(5), case.Mach)
+    print("gpu: iteration", dg.iter, "t", dg.t, "residuals", hist[-1], "cl", cl, "cd", cd, "golden", gold)
+    assert dg.iter == gold["final_iteration"]
+    assert np.max(np.abs(hist[-1] - np.array(gold["residuals"]))) <= gold["residual_tolerance"]
+    assert abs(cl - gold["cl"]) <= 1e-12 and abs(cd - gold["cd"]) <= 1e-12
+
+
+def test_couette_fixture_matches_the_case_file():
+    path = os.path.join(REFERENCE, "TestCases/Benchmark/Simulations/Couette/SETUP/Couette.HiOCase")
+    if not os.path.exists(path):
+        pytest.skip("reference tree not present")
+    from horses2d_amd import hio
+    c = hio.case_from_hiocase(path, navier_stokes=True)
+    f, gold = benchmark_case("couette")
+    for k in ("Mach", "Re", "Pr", "viscous", "riemann", "scheme", "mode", "cfl", "dt_max", "residual_target",
+              "p_ref", "T_ref"):
+        assert getattr(c, k) == getattr(f, k), k
+    np.testing.assert_array_equal(c.N, f.N)
+    assert [z.function for z in c.mesh.zones] == ["", "", "BC1", ""]
+    assert gold["final_iteration"] == 19585
+
+
+def test_oracle_run_reproduces_the_couette_golden():
+    """Couette (Couette/SETUP/ProblemFile.f90:90-92,156): BR1 and Roe on 144 quads of orders 5, 7
+    and 10 with curved p-non-conforming edges, isothermal walls (the top one moving at Mach
+    0.8), a Dirichlet inflow whose state is the problem file's BoundaryConditionFunction1 at each
+    node, a pressure outlet, RK5 in steady mode to residual 1e-4 from the perturbed Couette
+    profile.  The oracle's whole run (19585 iterations, about two minutes on 6 cores) is
+    recorded by tests/golden/oracle_benchmark_runs.py; Finalize requires iteration 19585 and the
+    residuals within 1e-12."""
+    import json
+    with open(os.path.join(GOLDEN, "oracle_benchmark_runs.json")) as f:
+        run = json.load(f)["couette"]
+    _, gold = benchmark_case("couette")
+    assert run["final_iteration"] == gold["final_iteration"]
+    assert np.max(np.abs(np.array(run["residuals"]) - np.array(gold["residuals"]))) <= gold["residual_tolerance"]
+
+
+def test_oracle_starts_the_couette_run_like_its_record():
+    """The first 20 iterations of the recorded run, from the committed fixtures: the same time
+    step and the same residuals at iteration 20 (so the record belongs to this case and these
+    boundary states)."""
+    import json
+    from oracle.oracle import Oracle
+    with open(os.path.join(GOLDEN, "oracle_benchmark_runs.json")) as f:
+        run = json.load(f)["couette"]
+    case, _ = benchmark_case("couette")
+    orc = Oracle(case)
+    apply_boundary_functions(orc, case)
+    orc.set_state(case.initial_state(orc.coords()))
+    _, dts, it, hist = orc.integrate(20, dt_max=case.dt_max, scheme=case.scheme, mode=case.mode,
+                                     residual_target=case.residual_target)
+    assert it == 20 and dts[0] == run["first_dt"]
+    np.testing.assert_allclose(hist[-1], run["residuals_at_20"], rtol=1e-13)
+
+
+@pytest.mark.gpu
+def test_gpu_reproduces_the_couette_golden():
+    """The FP64 GPU path with the inflow states handed over per node (hg_set_boundary_state), the
+    curved mortar kernels (BR1 uStar dS n per direction), RK5 steady to residual 1e-4:
+    Finalize's bars (iteration 19585, residuals within 1e-12)."""
+    from horses2d_amd import gpu
+    case, gold = benchmark_case("couette")
+    dg = gpu.DGSEM(case, "fp64")
+    apply_boundary_functions(dg, case)
+    dg.upload(case.initial_state(dg.coords()))
+    hist = dg.integrate()
+    print("gpu: iteration", dg.iter, "residuals", hist[-1], "golden", gold)
+    assert dg.iter == gold["final_iteration"]
+    assert np.max(np.abs(hist[-1] - np.array(gold["residuals"]))) <= gold["residual_tolerance"]

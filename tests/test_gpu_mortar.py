@@ -62,12 +62,36 @@ def br1_case(seed=2, riemann="hll"):
                   ic=_flow_ic(0.3, 0.4))
 
 
-def annulus_case():
+def ip_case(seed, method, riemann="hll"):
+    """Interior penalty across orders (ComputeRiemannSolver_InteriorEdge with the IP penalty at
+    the edge's order and, for SIPG and NIPG, the gradient fluxes returned by
+    ProjectGradientFluxes, QuadElement_EdgeProcedures.incf:1208-1339) on the rotated channel
+    (CFL 0.02: the penalty's stiffness, which the time-step estimate does not see)."""
+    c = br1_case(seed, riemann)
+    c.viscous, c.ip_method, c.sigma0, c.cfl = "ip", method, 1.0, 0.02
+    return c
+
+
+def annulus_case(viscous="br1"):
     """The curved annulus (far field, adiabatic wall), BR1, the orders changing between
     azimuthal sectors: the p-non-conforming edges are the straight radial ones."""
     c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
     nr = 4
     c.N = np.array([2 + (e // nr) % 4 for e in range(c.mesh.K)], np.int32)
+    c.viscous = viscous
+    return c
+
+
+def curved_annulus_case(viscous="br1"):
+    """The curved annulus with the orders changing from ring to ring (4, 5, 6, 4 outwards): the
+    p-non-conforming edges are the circumferential arcs (ComputeRiemannSolver_InteriorCurvedEdge;
+    BR1 returns uStar dS n per node and direction, BaseClass_ComputeSolutionRiemann_CurvedInterior).
+    An interior edge's curve takes the order NLow / 2 (QuadMesh_Construct.incf:104-118), so
+    orders of at least 4 keep the arcs curved."""
+    c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
+    c.N = np.array([4 + (e % 4) % 3 for e in range(c.mesh.K)], np.int32)   # elements run radially fastest
+    c.viscous, c.sigma0 = viscous, 1.0
+    c.cfl, c.dt_max = C.stable_cfl(6), C.viscous_dt_cap(c.mesh, 6, 200.0)
     return c
 
 
@@ -77,7 +101,20 @@ CASES = {
     "br1_hll": lambda: br1_case(2, "hll"),
     "br1_roe": lambda: br1_case(7, "roe"),
     "br1_annulus": annulus_case,
+    "sipg_hll": lambda: ip_case(3, "sipg"),
+    "nipg_roe": lambda: ip_case(8, "nipg", "roe"),
+    "iipg_hll": lambda: ip_case(5, "iipg"),
+    "sipg_annulus": lambda: annulus_case("ip"),
+    "br1_curved_annulus": lambda: curved_annulus_case("br1"),
+    "sipg_curved_annulus": lambda: curved_annulus_case("ip"),
+    "euler_curved_annulus": lambda: _euler(curved_annulus_case("br1")),
 }
+
+
+def _euler(c):
+    c.navier_stokes, c.Re, c.viscous = False, 0.0, "none"
+    c.mesh.zones[0] = C.Zone(C.BC_EULERWALL)
+    return c
 
 
 def test_the_cases_hold_every_edge_type():
@@ -146,17 +183,27 @@ def test_graph_and_direct_steps_agree_bit_for_bit():
     np.testing.assert_array_equal(a.residuals(), b.residuals())
 
 
+def test_curved_cases_have_curved_mortars():
+    """The curved annulus cases put their p-non-conforming edges on the curved rings."""
+    from oracle.oracle import Oracle
+    orc = Oracle(curved_annulus_case())
+    edges, _, _ = orc.connectivity()
+    nx, _, _, _ = orc.edge_geometry(16)
+    curved = []
+    for k in range(edges.shape[0]):
+        low, N, _, _, _ = orc.edge_mortar(k)
+        if low >= 0 and np.ptp(nx[k, :N + 1]) > 1e-9:   # the normal turns along the edge
+            curved.append(k)
+    assert len(curved) >= 16
+
+
 def test_unsupported_forms_are_refused():
     from horses2d_amd import gpu
     with pytest.raises(RuntimeError, match="FP64 form only"):
         gpu.DGSEM(br1_case(2), "mixed")
     c = br1_case(2)
-    c.viscous = "ip"
-    with pytest.raises(RuntimeError, match="Euler or the BR1 form"):
-        gpu.DGSEM(c, "fp64")
-    c = C.ns_cylinder_case(4, 16, N=3, Re=200.0)
-    c.N = np.array([2 + (e % 4) % 2 for e in range(c.mesh.K)], np.int32)   # orders change across curved edges
-    with pytest.raises(RuntimeError, match="curved edges between elements of different orders"):
+    c.artificial_dissipation = "physical"
+    with pytest.raises(RuntimeError, match="without artificial dissipation"):
         gpu.DGSEM(c, "fp64")
     c = C.euler_vortex_case(4, 3)
     c.N = np.array([2, 3] * (c.mesh.K // 2), np.int32)
