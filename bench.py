@@ -183,7 +183,7 @@ def ensemble_forms(cases, steps, warmup):
                      "resid_kernel_avg_ms": ms_rs / max(n_rs, 1),
                      "schedule": {"tile": ["narrow", "wide"][tile],
                                   "order": ["case-major", "costliest first", "interleaved",
-                                            "one launch per order"][order]}}
+                                            "one launch per order", "persistent"][order]}}
         states[name] = q
 
     batched(hs, "batched_fp64")

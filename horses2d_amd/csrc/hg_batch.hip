@@ -121,6 +121,7 @@ struct hg_batch {
 namespace {
 
 constexpr int kPerOrder = 3;   // schedule order: one launch per polynomial order
+constexpr int kPersist = 4;    // schedule order: persistent workgroups (one launch per stage)
 
 hg::BatchArgs args_of(const hg_batch *b, int tile, int order, int slot) {
     hg::BatchArgs a{};
@@ -133,6 +134,17 @@ hg::BatchArgs args_of(const hg_batch *b, int tile, int order, int slot) {
     a.dt = b->dt;
     a.dt_cq = b->sch.dt_cq;
     a.dt_cg = b->sch.dt_cg;
+    return a;
+}
+
+// The persistent form over one tile's list: each order's items by their positions.
+hg::BatchArgs args_persist(const hg_batch *b, int tile, int slot) {
+    hg::BatchArgs a = args_of(b, tile, 0, slot);
+    a.persist = 1;
+    for (int P = 0; P < 12; ++P) {
+        a.pn[P] = b->pnitems[tile][P];
+        a.psched[P] = b->psched[tile][P];
+    }
     return a;
 }
 
@@ -173,6 +185,8 @@ hipError_t per_order(hg_batch *b, F &&launch) {
 // or one per polynomial order (all gradient launches before the stage launches, so that each
 // pass is timed as one interval).
 hipError_t gradient_pass(hg_batch *b, int tile, int order, int slot) {
+    if (order == kPersist)
+        return hg::launch_batch_gradient(b->variant, tile, args_persist(b, tile, slot), b->block[tile], b->stream);
     if (order != kPerOrder)
         return hg::launch_batch_gradient(b->variant, tile, args_of(b, tile, order, slot), b->block[tile], b->stream);
     return per_order(b, [&](int P, hipStream_t s) {
@@ -181,6 +195,8 @@ hipError_t gradient_pass(hg_batch *b, int tile, int order, int slot) {
 }
 
 hipError_t stage_pass(hg_batch *b, int tile, int order, int slot) {
+    if (order == kPersist)
+        return hg::launch_batch_stage(b->variant, b->rs, tile, args_persist(b, tile, slot), b->block[tile], b->stream);
     if (order != kPerOrder)
         return hg::launch_batch_stage(b->variant, b->rs, tile, args_of(b, tile, order, slot), b->block[tile], b->stream);
     return per_order(b, [&](int P, hipStream_t s) {
@@ -305,6 +321,18 @@ int autotune(hg_batch *b) {
             if (ms < best) { best = ms; bt = t; bo = o; }
         }
     }
+    // the persistent form, both tiles
+    for (int t = 0; t < 2; ++t) {
+        if (b->opt.tile >= 0 && t != b->opt.tile) continue;
+        if (b->opt.order >= 0 && b->opt.order != kPersist) continue;
+        HB_CHECK(time_of([&]() -> hipError_t {
+            hipError_t e = art_visc_all(b, dslot);
+            if (e == hipSuccess && ns) e = gradient_pass(b, t, kPersist, dslot);
+            if (e == hipSuccess) e = stage_pass(b, t, kPersist, dslot);
+            return e;
+        }, &ms));
+        if (ms < best) { best = ms; bt = t; bo = kPersist; }
+    }
     if (b->opt.order < 0 || b->opt.order == kPerOrder) {
         for (int P : b->porders) {
             float pbest = 1e30f;
@@ -416,7 +444,7 @@ int hg_batch_create(hg_handle *const *cases, int32_t ncases, const hg_batch_opti
     *out = nullptr;
     hg_batch_options o{-1, -1, 1};
     if (opt) o = *opt;
-    if (o.tile < -1 || o.tile > 1 || o.order < -1 || o.order > kPerOrder)
+    if (o.tile < -1 || o.tile > 1 || o.order < -1 || o.order > kPersist)
         return hg::handle_error("hg_batch_create: tile must be -1..1 and order -1..3");
     hg_handle *h0 = cases[0];
     for (int c = 0; c < ncases; ++c) {

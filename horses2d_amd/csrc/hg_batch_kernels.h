@@ -120,6 +120,155 @@ __global__ __launch_bounds__(256) void k_batch_gradient(const StageArgs *__restr
 
 #undef HG_BATCH_CASE
 
+// ---- persistent form: one launch per RK stage whose workgroups stay resident and loop.  The
+// grid is what the GPU holds at once; the host splits it over the orders in proportion to their
+// work, in multiples of eight (pg0 / pgn, filled per launch).  Workgroup r of order P (on XCD
+// r & 7) takes every (pgn / 8)-th item of XCD r & 7's contiguous eighth of P's list, so that the
+// traces a tile reads from its neighbours were mostly written through the same L2.  The loop
+// sits inside each order's case of the switch: a workgroup keeps one order, and the registers
+// are those of the costliest order, as in the one-item-per-workgroup kernel.
+struct PersistGrid {
+    int pg0[12], pgn[12];
+};
+
+template <int P, int TILE, bool NS, bool IP, int RS, typename R, typename TR, typename Smem>
+__device__ __forceinline__ void persist_stage(const StageArgs *__restrict__ slots, const int4 *__restrict__ items,
+                                              const double *__restrict__ dt, const BatchArgs &b, int pgn, int r,
+                                              Smem &S) {
+    constexpr int EPB = TileOf<P, TILE>::EPB;
+    const int n = b.pn[P], per = pgn >> 3, x = r & 7;
+    const int cs = (int)((int64_t)n * x / 8), ce = (int)((int64_t)n * (x + 1) / 8);
+    for (int k = cs + (r >> 3); k < ce; k += per) {
+        if (k != cs + (r >> 3)) __syncthreads();   // the previous item's LDS is read
+        const int4 it = uniform_item(items, b.psched[P], k);
+        const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
+        double cq = a.cq, cg = a.cg;
+        if (dt) {
+            const double d = dt[it.x];
+            if (b.dt_cq) cq *= d;
+            if (b.dt_cg) cg *= d;
+        }
+        stage_body<P, EPB, NS, IP, RS, R, TR>(a, it.y, it.w, cq, cg, S);
+    }
+}
+
+template <int P, int TILE, bool BR1, typename R, typename TR, typename Smem>
+__device__ __forceinline__ void persist_gradient(const StageArgs *__restrict__ slots, const int4 *__restrict__ items,
+                                                 const BatchArgs &b, int pgn, int r, Smem &S) {
+    constexpr int EPB = TileOf<P, TILE>::EPB;
+    const int n = b.pn[P], per = pgn >> 3, x = r & 7;
+    const int cs = (int)((int64_t)n * x / 8), ce = (int)((int64_t)n * (x + 1) / 8);
+    for (int k = cs + (r >> 3); k < ce; k += per) {
+        if (k != cs + (r >> 3)) __syncthreads();
+        const int4 it = uniform_item(items, b.psched[P], k);
+        const StageArgs &a = slots[(int64_t)it.x * b.nslots + b.slot];
+        gradient_body<P, EPB, BR1, R, TR, true>(a, it.y, S);
+    }
+}
+
+// The order of workgroup w and its rank among that order's workgroups (uniform).
+__device__ inline int persist_order(const PersistGrid &g, int w, int &r) {
+    int P = 0;
+    for (int p = 2; p < 12; ++p)
+        if (w >= g.pg0[p] && w < g.pg0[p] + g.pgn[p]) P = p;
+    r = w - g.pg0[P];
+    return P;
+}
+
+#define HG_PERSIST_CASE(PV, CALL) \
+    case PV: CALL(PV, p##PV); break;
+
+// The FP32 form keeps the occupancy of the one-item kernel (eight wavefronts per SIMD, 64
+// VGPRs): without the bound the loop's hoisted lane-index values take it to 86 VGPRs.
+template <typename R>
+struct PersistWaves {
+    static constexpr int value = sizeof(R) == 4 ? 8 : 1;
+};
+
+template <int TILE, bool NS, bool IP, int RS, typename R, typename TR>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PersistWaves<R>::value))) void k_batch_stage_persist(const StageArgs *__restrict__ slots,
+                                                             const int4 *__restrict__ items,
+                                                             const double *__restrict__ dt, BatchArgs b,
+                                                             PersistGrid g) {
+    __shared__ BatchStageSmem<TILE, NS, IP, R> S;
+    int r = 0;
+    const int P = persist_order(g, blockIdx.x, r);
+#define HG_CALL(PV, M) persist_stage<PV, TILE, NS, IP, RS, R, TR>(slots, items, dt, b, g.pgn[PV], r, S.M)
+    switch (P) {
+        HG_PERSIST_CASE(2, HG_CALL)
+        HG_PERSIST_CASE(3, HG_CALL)
+        HG_PERSIST_CASE(4, HG_CALL)
+        HG_PERSIST_CASE(5, HG_CALL)
+        HG_PERSIST_CASE(6, HG_CALL)
+        HG_PERSIST_CASE(7, HG_CALL)
+        HG_PERSIST_CASE(8, HG_CALL)
+        HG_PERSIST_CASE(9, HG_CALL)
+        HG_PERSIST_CASE(10, HG_CALL)
+        HG_PERSIST_CASE(11, HG_CALL)
+        default: break;
+    }
+#undef HG_CALL
+}
+
+template <int TILE, bool BR1, typename R, typename TR>
+__global__ __launch_bounds__(256) void k_batch_gradient_persist(const StageArgs *__restrict__ slots,
+                                                                const int4 *__restrict__ items, BatchArgs b,
+                                                                PersistGrid g) {
+    __shared__ BatchGradSmem<TILE, BR1, R> S;
+    int r = 0;
+    const int P = persist_order(g, blockIdx.x, r);
+#define HG_CALL(PV, M) persist_gradient<PV, TILE, BR1, R, TR>(slots, items, b, g.pgn[PV], r, S.M)
+    switch (P) {
+        HG_PERSIST_CASE(2, HG_CALL)
+        HG_PERSIST_CASE(3, HG_CALL)
+        HG_PERSIST_CASE(4, HG_CALL)
+        HG_PERSIST_CASE(5, HG_CALL)
+        HG_PERSIST_CASE(6, HG_CALL)
+        HG_PERSIST_CASE(7, HG_CALL)
+        HG_PERSIST_CASE(8, HG_CALL)
+        HG_PERSIST_CASE(9, HG_CALL)
+        HG_PERSIST_CASE(10, HG_CALL)
+        HG_PERSIST_CASE(11, HG_CALL)
+        default: break;
+    }
+#undef HG_CALL
+}
+#undef HG_PERSIST_CASE
+
+// The persistent grid of a kernel: the resident workgroups of the whole GPU (occupancy times
+// compute units, once per kernel), split over the orders of b.pn in proportion to their nodes
+// (items x elements per item x P^2: the kernels are bound by memory traffic, which follows the
+// nodes; a P^3 weighting was slower), in multiples of eight, at least eight per order with work.
+template <int TILE, typename K>
+hipError_t persist_grid(K kernel, int block, const BatchArgs &b, PersistGrid &g, int &grid) {
+    static int cap = 0;
+    if (!cap) {
+        int dev = 0, cus = 0, per = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0);
+        if (e != hipSuccess) return e;
+        cap = std::max(8, cus * std::max(per, 1));
+    }
+    double work[12] = {0}, tot = 0.0;
+    for (int p = 2; p < 12; ++p) {
+        const int epb = TILE == 0 ? tile_elems(p, 0) : tile_elems(p, 1);
+        work[p] = (double)b.pn[p] * epb * p * p;
+        tot += work[p];
+    }
+    grid = 0;
+    for (int p = 0; p < 12; ++p) {
+        g.pg0[p] = grid;
+        g.pgn[p] = 0;
+        if (p < 2 || b.pn[p] <= 0) continue;
+        int n8 = (int)(cap / 8 * work[p] / tot + 0.5);
+        n8 = std::max(1, std::min(n8, (b.pn[p] + 7) / 8));   // no more workgroups than items
+        g.pgn[p] = 8 * n8;
+        grid += g.pgn[p];
+    }
+    return hipSuccess;
+}
+
 // One order per launch: the work items of the cases of order P only, with that order's own LDS
 // tile and register allocation (the union kernels above reserve the largest order's LDS and the
 // register count of the costliest order for every item).  Same stage_body / gradient_body, so
@@ -212,6 +361,21 @@ hipError_t batch_gradient_p_variant(int P, int tile, const BatchArgs &b, hipStre
 // Launchers of one physics/precision variant (instantiated by hg_batch_inst.hip); grid = nitems.
 template <bool NS, bool IP, int RS, typename R, typename TR>
 hipError_t batch_stage_variant(int tile, const BatchArgs &b, int block, hipStream_t st) {
+    if (b.persist) {
+        PersistGrid g{};
+        int grid = 0;
+        hipError_t e = tile == 0 ? persist_grid<0>(k_batch_stage_persist<0, NS, IP, RS, R, TR>, block, b, g, grid)
+                                 : persist_grid<1>(k_batch_stage_persist<1, NS, IP, RS, R, TR>, block, b, g, grid);
+        if (e != hipSuccess) return e;
+        if (grid == 0) return hipSuccess;
+        if (tile == 0)
+            hipLaunchKernelGGL((k_batch_stage_persist<0, NS, IP, RS, R, TR>), dim3(grid), dim3(block), 0, st, b.slots,
+                               b.items, b.dt, b, g);
+        else
+            hipLaunchKernelGGL((k_batch_stage_persist<1, NS, IP, RS, R, TR>), dim3(grid), dim3(block), 0, st, b.slots,
+                               b.items, b.dt, b, g);
+        return hipGetLastError();
+    }
     if (tile == 0)
         hipLaunchKernelGGL((k_batch_stage<0, NS, IP, RS, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
                            b.sched, b.dt, b);
@@ -223,6 +387,21 @@ hipError_t batch_stage_variant(int tile, const BatchArgs &b, int block, hipStrea
 
 template <bool BR1, typename R, typename TR>
 hipError_t batch_gradient_variant(int tile, const BatchArgs &b, int block, hipStream_t st) {
+    if (b.persist) {
+        PersistGrid g{};
+        int grid = 0;
+        hipError_t e = tile == 0 ? persist_grid<0>(k_batch_gradient_persist<0, BR1, R, TR>, block, b, g, grid)
+                                 : persist_grid<1>(k_batch_gradient_persist<1, BR1, R, TR>, block, b, g, grid);
+        if (e != hipSuccess) return e;
+        if (grid == 0) return hipSuccess;
+        if (tile == 0)
+            hipLaunchKernelGGL((k_batch_gradient_persist<0, BR1, R, TR>), dim3(grid), dim3(block), 0, st, b.slots,
+                               b.items, b, g);
+        else
+            hipLaunchKernelGGL((k_batch_gradient_persist<1, BR1, R, TR>), dim3(grid), dim3(block), 0, st, b.slots,
+                               b.items, b, g);
+        return hipGetLastError();
+    }
     if (tile == 0)
         hipLaunchKernelGGL((k_batch_gradient<0, BR1, R, TR>), dim3(b.nitems), dim3(block), 0, st, b.slots, b.items,
                            b.sched, b);

@@ -155,6 +155,12 @@ struct BatchArgs {
     int nitems;
     const double *dt;        // per case time step
     int dt_cq, dt_cg;        // multiply the slot's cq / cg by the case's dt
+    // persistent form (persist = 1): as many workgroups as the GPU holds at once, a multiple
+    // of eight per polynomial order; the workgroups of order P loop over its items
+    // (psched[P][0..pn[P]), positions in `items`), each XCD over a contiguous eighth of them
+    int persist;
+    int pn[12];
+    const int *psched[12];
 };
 
 // variant = 2 * physics (0 Euler, 1 Navier-Stokes BR1, 2 Navier-Stokes IP) + (1 if mixed)

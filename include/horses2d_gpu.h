@@ -318,7 +318,8 @@ int hg_get_tiles(hg_handle *h, int32_t *stage_tile, int32_t *gradient_tile);
 typedef struct {
     int32_t tile;      /* -1: autotune, 0: narrow, 1: wide workgroup tile */
     int32_t order;     /* -1: autotune, 0: case-major, 1: costliest tiles first, 2: cases interleaved,
-                          3: one launch per polynomial order (case-major within it) */
+                          3: one launch per polynomial order (case-major within it),
+                          4: persistent (resident workgroups looping over each order's tiles) */
     int32_t use_graph; /* 1: replay the captured step as a HIP graph, 0: launch kernel by kernel */
 } hg_batch_options;
 

@@ -52,10 +52,10 @@ def test_euler_ensemble_fp64_bitwise(use_graph):
     single, batched, b = run_pair(cases, "fp64", 4, use_graph=use_graph)
     assert_same(single, batched)
     tile, order = b.schedule()
-    assert tile in (0, 1) and order in (0, 1, 2, 3)
+    assert tile in (0, 1) and order in (0, 1, 2, 3, 4)
 
 
-@pytest.mark.parametrize("tile,order", [(0, 0), (1, 1), (0, 2), (1, 0), (0, 3), (1, 3)])
+@pytest.mark.parametrize("tile,order", [(0, 0), (1, 1), (0, 2), (1, 0), (0, 3), (1, 3), (0, 4), (1, 4)])
 def test_every_schedule_gives_the_same_bits(tile, order):
     cases = small_ensemble(4)
     single, batched, b = run_pair(cases, "mixed", 3, tile=tile, order=order)
@@ -107,6 +107,21 @@ def test_navier_stokes_one_launch_per_order(viscous):
         single, batched, b = run_pair(cases, precision, 3, order=3)
         assert_same(single, batched)
         assert b.schedule()[1] == 3
+
+
+@pytest.mark.parametrize("viscous", ["br1", "sipg"])
+def test_navier_stokes_persistent(viscous):
+    """The persistent form: resident workgroups looping over each order's tiles, the gradient
+    pass (BR1) persistent too; bit for bit the single-handle steps, both tiles."""
+    if viscous == "br1":
+        cases = [C.ns_tgv_case(4, 3), C.ns_tgv_case(5, 5, riemann="roe"), C.ns_cylinder_case(3, 12, N=4),
+                 C.ns_tgv_case(3, 7)]
+    else:
+        cases = [C.ns_kh_case(4, 4), C.ns_kh_case(3, 6), C.ns_kh_case(5, 4)]
+    for precision, tile in (("fp64", 1), ("mixed", 0)):
+        single, batched, b = run_pair(cases, precision, 3, tile=tile, order=4)
+        assert_same(single, batched)
+        assert b.schedule() == (tile, 4)
 
 
 def test_boundary_condition_kinds_in_a_batch():
