@@ -302,9 +302,11 @@ hg::MortarArgs mortar_args(hg_handle *h) {
     a.sgeo = h->sgeo;
     a.mats = h->mmats;
     a.ph = h->sub[0]->ph;
+    a.mixed = h->sub[0]->mixed;
+    a.Eref = h->sub[0]->mixed ? h->sub[0]->ph.Eref : 0.0;
     for (size_t g = 0; g < h->sub.size(); ++g) {
         hg_handle *s = h->sub[g];
-        a.tr[g] = (const double *)s->tr[s->cur];
+        a.tr[g] = s->tr[s->cur];
         a.mdq[g] = s->mdq;
         a.mustar[g] = s->mustar;
         a.mflux[g] = s->mflux;
@@ -756,12 +758,12 @@ int setup_handle(hg_handle *h, const hg::Mesh &m, const hg_physics *phys, const 
 // A mesh with several polynomial orders (p-refinement zones, MeshFile.f90:143-180): one
 // sub-handle per order over its elements, the p-non-conforming edges between them (edge order
 // the higher of its two elements', QuadMesh_Construct.incf:104-118) in the mortar tables, and
-// the whole mesh's time-step scan.  The FP64 form (Euler, BR1 or interior penalty) without
-// artificial dissipation; straight and curved mortar edges.
+// the whole mesh's time-step scan.  The FP64 and mixed forms (Euler, BR1 or interior penalty)
+// without artificial dissipation; straight and curved mortar edges (curved ones under BR1 in the
+// FP64 form only).  The mixed form's sub-handles share one reference pressure, so that the
+// mortar kernels add one energy shift back to both sides' FP32 traces.
 int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt, const std::vector<int> &orders) {
     const hg::Mesh &m = h->mesh;
-    if (opt->precision != HG_PRECISION_FP64)
-        return set_err("hg_create: a mesh with several polynomial orders or subdivided edges runs in the FP64 form only");
     if (opt->navier_stokes && opt->artificial_dissipation != HG_AD_NONE)
         return set_err("hg_create: a mesh with several polynomial orders or subdivided edges runs without artificial dissipation");
     if ((int)orders.size() > hg::kMaxSub) return set_err("hg_create: too many polynomial orders in one mesh");
@@ -854,6 +856,14 @@ int create_composite(hg_handle *h, const hg_physics *phys, const hg_options *opt
         std::vector<int> lg(K, -1);
         for (int e : h->sub_els[g]) lg[e] = loc[e];
         if (setup_handle(s, m, phys, opt, h->sub_els[g], lg, mort)) return 1;
+    }
+    if (opt->precision == HG_PRECISION_MIXED && opt->navier_stokes && opt->viscous == HG_VISCOUS_BR1) {
+        // the y lift of a curved mortar face's uStar is added in the FP64 gradient only
+        bool curved = false;
+        for (const hg::MortarEdge &r : medges) curved = curved || r.geo >= 0;
+        for (const hg::SubMortarEdge &r : sedges) curved = curved || r.curved;
+        if (curved)
+            return set_err("hg_create: curved mortar edges under BR1 run in the FP64 form only");
     }
     h->opt = *opt;
     h->phys = *phys;
@@ -1047,11 +1057,32 @@ int hg_build_geometry(const hg_mesh_desc *mesh, int64_t nnodes, double *X, doubl
     return 0;
 }
 
+// The mean nodal pressure of a state (0 when it is not finite).
+static double mean_pressure(double gm1, const double *Q, int64_t nodes) {
+    double sum = 0.0;
+    for (int64_t n = 0; n < nodes; ++n) {
+        const double *q = Q + 4 * n;
+        sum += gm1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+    }
+    const double pref = nodes > 0 ? sum / (double)nodes : 0.0;
+    return std::isfinite(pref) ? pref : 0.0;
+}
+
 int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     if (!h || !Q) return set_err("hg_upload_state: null argument");
     if (ndof != h->ndof) return set_err("hg_upload_state: ndof mismatch");
     if (!h->sub.empty()) {
-        // each order's elements, from the reference layout, to its sub-handle
+        // each order's elements, from the reference layout, to its sub-handle (the mixed form's
+        // reference pressure over the whole mesh, common to all of them)
+        if (h->sub[0]->mixed) {
+            const double pref = mean_pressure(h->sub[0]->ph.gm1, Q, h->nodes);
+            // the captured mortar kernels carry the energy shift by value
+            if (pref != h->sub[0]->ph.pref) drop_graphs(h);
+            for (hg_handle *s : h->sub) {
+                s->has_pref = true;
+                s->pref_in = pref;
+            }
+        }
         for (size_t g = 0; g < h->sub.size(); ++g) {
             hg_handle *s = h->sub[g];
             const size_t n = 4 * (size_t)s->P * s->P;
@@ -1065,14 +1096,9 @@ int hg_upload_state(hg_handle *h, const double *Q, int64_t ndof) {
     }
     const double pref_old = h->ph.pref;
     if (h->mixed) {
-        // Reference pressure of the FP32 form: the mean nodal pressure of the uploaded state.
-        double sum = 0.0;
-        for (int64_t n = 0; n < h->nodes; ++n) {
-            const double *q = Q + 4 * n;
-            sum += h->ph.gm1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
-        }
-        const double pref = h->nodes > 0 ? sum / (double)h->nodes : 0.0;
-        h->ph.pref = std::isfinite(pref) ? pref : 0.0;
+        // Reference pressure of the FP32 form: the mean nodal pressure of the uploaded state
+        // (of the whole mesh, for a sub-handle of a mesh with several orders).
+        h->ph.pref = h->has_pref ? h->pref_in : mean_pressure(h->ph.gm1, Q, h->nodes);
         h->ph.Eref = h->ph.pref / h->ph.gm1;
     }
     // the captured kernels carry the physics parameters (the FP32 form's reference pressure)

@@ -128,7 +128,10 @@ struct MortarArgs {
     int nsubs;
     const double *sgeo;
     const double *mats;
-    const double *tr[kMaxSub];   // Q traces of the current stage, per sub-handle
+    const void *tr[kMaxSub];     // Q traces of the current stage, per sub-handle (FP64, or FP32
+                                 // in the mixed form with the energy slot less Eref)
+    int mixed;                   // the traces are FP32
+    double Eref;                 // the mixed form's energy shift, common to all sub-handles
     const double *mdq[kMaxSub];  // gradient traces at mortar faces, per sub-handle
     double *mustar[kMaxSub];     // per mortar face node [8]: straight edges uStar [0..3]; curved
                                  // edges uStar dS nx [0..3] and uStar dS ny [4..7]

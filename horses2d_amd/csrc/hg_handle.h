@@ -24,6 +24,8 @@ struct hg_handle {
     int K = 0, N = 0, P = 0;
     int64_t nodes = 0, ndof = 0;
     bool mixed = false;
+    bool has_pref = false;   // the mixed form's reference pressure is given (pref_in), not the
+    double pref_in = 0.0;    // uploaded state's mean (sub-handles of a mesh with several orders)
     int device = -1;   // the HIP device current at hg_create (the handle's stream and buffers)
     hg_options opt{};
     hg_physics phys{};

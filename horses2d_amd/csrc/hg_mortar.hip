@@ -22,22 +22,45 @@ constexpr int kEdgeMax = 16;   // edge points per direction held in LDS (orders 
 // side's own face nodes): the low side interpolated with T_forward (rows in ascending l, as
 // the reference accumulates them), a RIGHT side of an inverse edge read backwards (Types 2, 4)
 // or interpolated into the reversed node (Type 6).
-template <int NV>
-__device__ void side_values(const MortarEdge &me, const double *mats, int side, const double *src, int i, double *out) {
+template <int NV, typename T>
+__device__ void side_values(const MortarEdge &me, const double *mats, int side, const T *src, int i, double *out) {
     const int P = me.P, Pl = me.Ps[me.low];
     const bool rev = side == 1 && me.inverse;
     if (side != me.low) {
-        const double *v = src + (rev ? P - 1 - i : i) * NV;
+        const T *v = src + (rev ? P - 1 - i : i) * NV;
 #pragma unroll
-        for (int c = 0; c < NV; ++c) out[c] = v[c];
+        for (int c = 0; c < NV; ++c) out[c] = (double)v[c];
         return;
     }
     const double *Tf = mats + me.mat + (rev ? P - 1 - i : i) * Pl;
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
         double acc = 0.0;
-        for (int l = 0; l < Pl; ++l) acc = acc + Tf[l] * src[l * NV + c];
+        for (int l = 0; l < Pl; ++l) acc = acc + Tf[l] * (double)src[l * NV + c];
         out[c] = acc;
+    }
+}
+
+// A side's Q traces at its face slot, in the traces' type.
+template <typename TR>
+__device__ inline const TR *q_trace(const MortarArgs &m, int sub, int slot, int P) {
+    return (const TR *)m.tr[sub] + (int64_t)slot * P * 4;
+}
+
+// The mixed form keeps the energy slot of its traces less Eref: the fluxes need it back.
+template <typename TR>
+__device__ inline void unshift(const MortarArgs &m, double *q) {
+    if constexpr (sizeof(TR) == 4) q[3] += m.Eref;
+}
+
+// The mixed form's momentum fluxes carry the pressure less pref (a uniform pressure adds nothing
+// to an element's balance, so its volume and face terms both leave it out): the FP64 Riemann
+// flux of a mortar node, in the solver's scaling, follows that convention.
+template <typename TR>
+__device__ inline void shift_flux(const MortarArgs &m, const Phys<double> &ph, double nx, double ny, double *F) {
+    if constexpr (sizeof(TR) == 4) {
+        F[1] -= m.ph.pref * ph.invSGM * nx;
+        F[2] -= m.ph.pref * ph.invSGM * ny;
     }
 }
 
@@ -81,12 +104,17 @@ __device__ inline const double *side_trace(const MortarArgs &m, const MortarEdge
                                            int nv) {
     return buf[me.sub[side]] + (int64_t)me.slot[side] * me.Ps[side] * nv;
 }
+template <typename TR>
+__device__ inline const TR *side_q(const MortarArgs &m, const MortarEdge &me, int side) {
+    return q_trace<TR>(m, me.sub[side], me.slot[side], me.Ps[side]);
+}
 
 // BaseClass_ComputeSolutionRiemann_Interior / _CurvedInterior (DGViscousMethods.f90:538-616):
 // uStar = (QL + QR)/2 on the edge's nodes (BR1_SolutionRiemannSolver, DGViscous_BR1.f90:311-328).
 // A straight edge projects uStar to each side (the gradient kernels form uStar dS n from it); a
 // curved edge scales each node's uStar by its dS n first and projects each direction, which the
 // gradient lift of the side then takes as they are.
+template <typename TR>
 __global__ __launch_bounds__(64) void k_mortar_ustar(MortarArgs m) {
     __shared__ double sU[kEdgeMax * 8];
     const MortarEdge me = m.edges[blockIdx.x];
@@ -94,8 +122,10 @@ __global__ __launch_bounds__(64) void k_mortar_ustar(MortarArgs m) {
     const bool curved = me.geo >= 0;
     if (t < me.P) {
         double qL[4], qR[4];
-        side_values<4>(me, m.mats, 0, side_trace(m, me, 0, m.tr, 4), t, qL);
-        side_values<4>(me, m.mats, 1, side_trace(m, me, 1, m.tr, 4), t, qR);
+        // (the mixed form's uStar stays in the traces' shifted energy, as its gradient kernels
+        // lift it against the shifted state)
+        side_values<4>(me, m.mats, 0, side_q<TR>(m, me, 0), t, qL);
+        side_values<4>(me, m.mats, 1, side_q<TR>(m, me, 1), t, qR);
         if (!curved) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) sU[t * 4 + c] = 0.5 * (qL[c] + qR[c]);
@@ -177,6 +207,7 @@ __device__ inline void ip_gradient_flux(const MortarArgs &m, const Phys<double> 
 // projected to each side.  Interior penalty with gradient fluxes: each side's IP gradient flux,
 // returned by ProjectGradientFluxes (straight: projected, then times dS; curved: times each
 // node's dS, then projected).
+template <typename TR>
 __global__ __launch_bounds__(64) void k_mortar_flux(MortarArgs m) {
     __shared__ double sF[kEdgeMax * 4];
     __shared__ double sG[2][kEdgeMax * 8];
@@ -188,9 +219,12 @@ __global__ __launch_bounds__(64) void k_mortar_flux(MortarArgs m) {
     if (t < me.P) {
         double qL[4], qR[4], Fi[4], F[4], nx, ny, dS;
         edge_geo(m, me, t, nx, ny, dS);
-        side_values<4>(me, m.mats, 0, side_trace(m, me, 0, m.tr, 4), t, qL);
-        side_values<4>(me, m.mats, 1, side_trace(m, me, 1, m.tr, 4), t, qR);
+        side_values<4>(me, m.mats, 0, side_q<TR>(m, me, 0), t, qL);
+        side_values<4>(me, m.mats, 1, side_q<TR>(m, me, 1), t, qR);
+        unshift<TR>(m, qL);
+        unshift<TR>(m, qR);
         riemann<-1, double>(m.ph.riemann, ph, qL, qR, nx, ny, Fi);
+        shift_flux<TR>(m, ph, nx, ny, Fi);
         if (m.ph.ns) {
             double dL[8], dR[8], Fv[4];
             side_values<8>(me, m.mats, 0, side_trace(m, me, 0, m.mdq, 8), t, dL);
@@ -264,21 +298,25 @@ __device__ inline SubMats sub_mats(const MortarArgs &m, const SubMortarEdge &se)
 __device__ inline const double *sub_trace(const double *const *buf, const SubMortarEdge &se, int side, int nv) {
     return buf[se.sub[side]] + (int64_t)se.slot[side] * se.Ps[side] * nv;
 }
+template <typename TR>
+__device__ inline const TR *sub_q(const MortarArgs &m, const SubMortarEdge &se, int side) {
+    return q_trace<TR>(m, se.sub[side], se.slot[side], se.Ps[side]);
+}
 
 // Mortar node i of half h: the big side's (a) and the small side's (b) NV values there, sums
 // over the sides' nodes in ascending order (as the reference accumulates them).
-template <int NV>
-__device__ void sub_values(const SubMortarEdge &se, const SubMats &T, int h, int i, const double *big,
-                           const double *small, double *a, double *b) {
+template <int NV, typename T>
+__device__ void sub_values(const SubMortarEdge &se, const SubMats &Tm, int h, int i, const T *big,
+                           const T *small, double *a, double *b) {
     const int PL = se.Ps[0], Pe = se.Ps[1 + h];
-    const double *Tl = T.LF[h] + i * PL, *Tr = T.RF[h] + i * Pe;
+    const double *Tl = Tm.LF[h] + i * PL, *Tr = Tm.RF[h] + i * Pe;
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
         double acc = 0.0;
-        for (int l = 0; l < PL; ++l) acc = acc + Tl[l] * big[l * NV + c];
+        for (int l = 0; l < PL; ++l) acc = acc + Tl[l] * (double)big[l * NV + c];
         a[c] = acc;
         acc = 0.0;
-        for (int l = 0; l < Pe; ++l) acc = acc + Tr[l] * small[l * NV + c];
+        for (int l = 0; l < Pe; ++l) acc = acc + Tr[l] * (double)small[l * NV + c];
         b[c] = acc;
     }
 }
@@ -316,6 +354,7 @@ __device__ void sub_to_side(const SubMortarEdge &se, const SubMats &T, int side,
 // (lifted with its mortar's dS / 2 and n).  Curved edges: each mortar node's uStar times its dS n,
 // each direction returned (big side: T_BKW of both; small sides: -T_BKW of their own) and lifted
 // as it is.
+template <typename TR>
 __global__ __launch_bounds__(64) void k_sub_ustar(MortarArgs m) {
     __shared__ double sU[2][kEdgeMax * 8];
     const SubMortarEdge se = m.subs[blockIdx.x];
@@ -323,7 +362,7 @@ __global__ __launch_bounds__(64) void k_sub_ustar(MortarArgs m) {
     const int t = threadIdx.x, h = t >> 5, i = t & 31;
     if (i < se.PM[h]) {
         double a[4], b[4];
-        sub_values<4>(se, T, h, i, sub_trace(m.tr, se, 0, 4), sub_trace(m.tr, se, 1 + h, 4), a, b);
+        sub_values<4>(se, T, h, i, sub_q<TR>(m, se, 0), sub_q<TR>(m, se, 1 + h), a, b);
         if (!se.curved) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) sU[h][i * 4 + c] = 0.5 * (a[c] + b[c]);
@@ -361,6 +400,7 @@ __global__ __launch_bounds__(64) void k_sub_ustar(MortarArgs m) {
 // to each side.  Interior penalty with gradient fluxes: each side's IP gradient flux on each
 // mortar times the mortar node's dS, returned with T_BKW to all three sides, not negated
 // (:906-940, :1093-1136).
+template <typename TR>
 __global__ __launch_bounds__(64) void k_sub_flux(MortarArgs m) {
     __shared__ double sF[2][kEdgeMax * 8];
     __shared__ double sGa[2][kEdgeMax * 8], sGb[2][kEdgeMax * 8];   // [mortar]: big side, small side
@@ -373,8 +413,11 @@ __global__ __launch_bounds__(64) void k_sub_flux(MortarArgs m) {
         const double *g = m.sgeo + 3 * ((int64_t)se.geo + (h ? se.PM[0] : 0) + i);
         const double nx = g[0], ny = g[1], dS = g[2];
         double qa[4], qb[4], Fi[4], F[4];
-        sub_values<4>(se, T, h, i, sub_trace(m.tr, se, 0, 4), sub_trace(m.tr, se, 1 + h, 4), qa, qb);
+        sub_values<4>(se, T, h, i, sub_q<TR>(m, se, 0), sub_q<TR>(m, se, 1 + h), qa, qb);
+        unshift<TR>(m, qa);
+        unshift<TR>(m, qb);
         riemann<-1, double>(m.ph.riemann, ph, qa, qb, nx, ny, Fi);
+        shift_flux<TR>(m, ph, nx, ny, Fi);
         if (m.ph.ns) {
             double da[8], db[8], Fv[4];
             sub_values<8>(se, T, h, i, sub_trace(m.mdq, se, 0, 8), sub_trace(m.mdq, se, 1 + h, 8), da, db);
@@ -422,25 +465,29 @@ __global__ void k_scatter(const double *src, const int *map, int n, double *dst)
 
 hipError_t launch_mortar_ustar(const MortarArgs &m, hipStream_t st) {
     if (m.nedges <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mortar_ustar, dim3(m.nedges), dim3(64), 0, st, m);
+    if (m.mixed) hipLaunchKernelGGL(k_mortar_ustar<float>, dim3(m.nedges), dim3(64), 0, st, m);
+    else hipLaunchKernelGGL(k_mortar_ustar<double>, dim3(m.nedges), dim3(64), 0, st, m);
     return hipGetLastError();
 }
 
 hipError_t launch_mortar_flux(const MortarArgs &m, hipStream_t st) {
     if (m.nedges <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mortar_flux, dim3(m.nedges), dim3(64), 0, st, m);
+    if (m.mixed) hipLaunchKernelGGL(k_mortar_flux<float>, dim3(m.nedges), dim3(64), 0, st, m);
+    else hipLaunchKernelGGL(k_mortar_flux<double>, dim3(m.nedges), dim3(64), 0, st, m);
     return hipGetLastError();
 }
 
 hipError_t launch_sub_ustar(const MortarArgs &m, hipStream_t st) {
     if (m.nsubs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sub_ustar, dim3(m.nsubs), dim3(64), 0, st, m);
+    if (m.mixed) hipLaunchKernelGGL(k_sub_ustar<float>, dim3(m.nsubs), dim3(64), 0, st, m);
+    else hipLaunchKernelGGL(k_sub_ustar<double>, dim3(m.nsubs), dim3(64), 0, st, m);
     return hipGetLastError();
 }
 
 hipError_t launch_sub_flux(const MortarArgs &m, hipStream_t st) {
     if (m.nsubs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sub_flux, dim3(m.nsubs), dim3(64), 0, st, m);
+    if (m.mixed) hipLaunchKernelGGL(k_sub_flux<float>, dim3(m.nsubs), dim3(64), 0, st, m);
+    else hipLaunchKernelGGL(k_sub_flux<double>, dim3(m.nsubs), dim3(64), 0, st, m);
     return hipGetLastError();
 }
 

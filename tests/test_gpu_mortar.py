@@ -199,8 +199,6 @@ def test_curved_cases_have_curved_mortars():
 
 def test_unsupported_forms_are_refused():
     from horses2d_amd import gpu
-    with pytest.raises(RuntimeError, match="FP64 form only"):
-        gpu.DGSEM(br1_case(2), "mixed")
     c = br1_case(2)
     c.artificial_dissipation = "physical"
     with pytest.raises(RuntimeError, match="without artificial dissipation"):

@@ -161,5 +161,7 @@ def test_graph_and_direct_steps_agree_bit_for_bit():
 
 def test_unsupported_forms_are_refused():
     from horses2d_amd import gpu
-    with pytest.raises(RuntimeError, match="FP64 form only"):
-        gpu.DGSEM(euler_case(0, 4, 5), "mixed")
+    c = br1_case(2)
+    c.artificial_dissipation = "physical"
+    with pytest.raises(RuntimeError, match="without artificial dissipation"):
+        gpu.DGSEM(c, "fp64")
